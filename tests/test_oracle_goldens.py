@@ -1,0 +1,177 @@
+"""The CPU oracle against the values printed in the reference's executed notebooks.
+
+These pin the oracle's order rule, step-size rule (absolute and relative
+branches), Horner update, double-length time and propagate_until bookkeeping
+(nsteps, min_h / max_h excluding clamped steps) to heyoka's own outputs
+(tests/golden/notebook_goldens.json; each entry names its notebook file:line).
+"""
+
+import numpy as np
+import pytest
+
+import heyoka_amd as hy
+import pyoracle as O
+
+
+def ulp_diff(a, b):
+    a, b = np.float64(a), np.float64(b)
+    ia = np.frombuffer(a.tobytes(), dtype=np.int64)[0]
+    ib = np.frombuffer(b.tobytes(), dtype=np.int64)[0]
+    return abs(int(ia) - int(ib))
+
+
+@pytest.fixture(scope="module")
+def pend():
+    x, v = hy.make_vars("x", "v")
+    return hy.taylor_decompose([(x, v), (v, -9.8 * hy.sin(x))])
+
+
+@pytest.fixture(scope="module")
+def fdp():
+    x, v = hy.make_vars("x", "v")
+    return hy.taylor_decompose([(x, v), (v, hy.cos(hy.time) - hy.par[0] * v - hy.sin(x))])
+
+
+def test_order_rule(goldens):
+    assert hy.order_from_tol(np.finfo(float).eps) == 20
+    assert hy.order_from_tol(1e-9) == goldens["pend_tol1e9"]["order"] == 12
+    assert hy.order_from_tol(1e-15) == 19
+
+
+def test_pendulum_single_steps(pend, goldens):
+    g = goldens["pend_first_step"]
+    r = O.step_batch(pend, 20, [[0.05], [0.025]])
+    assert r["outcome"][0] == O.SUCCESS
+    assert ulp_diff(r["h"][0], g["h"]) <= 2
+    np.testing.assert_allclose(r["state"][:, 0], g["state"], rtol=0, atol=5e-17)
+    # Backward step from there.
+    r2 = O.step_batch(pend, 20, r["state"], t_hi=r["t_hi"], t_lo=r["t_lo"], max_dt=-np.inf)
+    assert ulp_diff(r2["h"][0], goldens["pend_step_backward"]["h"]) <= 2
+    # Clamped steps: +0.01 then -0.02, both time_limit.
+    r3 = O.step_batch(pend, 20, r2["state"], t_hi=r2["t_hi"], t_lo=r2["t_lo"], max_dt=0.01)
+    assert r3["outcome"][0] == O.TIME_LIMIT and r3["h"][0] == 0.01
+    r4 = O.step_batch(pend, 20, r3["state"], t_hi=r3["t_hi"], t_lo=r3["t_lo"], max_dt=-0.02)
+    assert r4["outcome"][0] == O.TIME_LIMIT and r4["h"][0] == -0.02
+    g = goldens["pend_after_clamped"]
+    # The first step differs from the golden by at most 2 ulp (of 0.216), which the
+    # summed time carries along.
+    assert abs(r4["t_hi"][0] - g["time"]) <= 6e-17
+    np.testing.assert_allclose(r4["state"][:, 0], g["state"], rtol=0, atol=5e-9)
+
+
+def test_pendulum_propagate(pend, goldens):
+    st = np.array([[0.05], [0.025]])
+    r = O.propagate_until(pend, 20, st, 5.0)
+    for key, (rr) in (("pend_propagate_for_5", r),):
+        g = goldens[key]
+        assert rr["outcome"][0] == O.TIME_LIMIT
+        assert int(rr["nsteps"][0]) == g["nsteps"]
+        assert ulp_diff(rr["min_h"][0], g["min_h"]) <= 4
+        assert ulp_diff(rr["max_h"][0], g["max_h"]) <= 4
+        assert rr["t_hi"][0] == g["time"]
+    r2 = O.propagate_until(pend, 20, r["state"], 20.0, t_hi=r["t_hi"], t_lo=r["t_lo"])
+    g = goldens["pend_propagate_until_20"]
+    assert int(r2["nsteps"][0]) == g["nsteps"]
+    assert ulp_diff(r2["min_h"][0], g["min_h"]) <= 4 and ulp_diff(r2["max_h"][0], g["max_h"]) <= 4
+    assert r2["t_hi"][0] == 20.0
+    r3 = O.propagate_until(pend, 20, r2["state"], 0.0, t_hi=r2["t_hi"], t_lo=r2["t_lo"])
+    g = goldens["pend_propagate_until_0"]
+    assert int(r3["nsteps"][0]) == g["nsteps"]
+    assert ulp_diff(r3["min_h"][0], g["min_h"]) <= 4 and ulp_diff(r3["max_h"][0], g["max_h"]) <= 4
+    assert r3["t_hi"][0] == 0.0
+    # Round trip back to t=0 reproduces the printed 17-digit state to a few ulp.
+    np.testing.assert_allclose(r3["state"][:, 0], g["state"], rtol=0, atol=2e-16)
+
+
+def test_pendulum_clamped_propagation_times(pend, goldens):
+    # propagate_until(0.5, max_delta_t=0.1): every step is clamped; the times the
+    # step callback printed follow from double-length time accumulation.
+    st = np.array([[0.05], [0.025]])
+    t_hi, t_lo = np.zeros(1), np.zeros(1)
+    times = []
+    for _ in range(5):
+        rem = (0.5 - t_hi[0]) - t_lo[0]
+        r = O.step_batch(pend, 20, st, t_hi=t_hi, t_lo=t_lo, max_dt=min(0.1, rem))
+        st, t_hi, t_lo = r["state"], r["t_hi"], r["t_lo"]
+        times.append(t_hi[0])
+    assert times == goldens["pend_callback_times"]["times"]
+
+
+def test_pendulum_tol_1e9_round_trip(goldens):
+    x, v = hy.make_vars("x", "v")
+    dec = hy.taylor_decompose([(x, v), (v, -9.8 * hy.sin(x))])
+    p = hy.order_from_tol(1e-9)
+    r = O.propagate_until(dec, p, [[0.05], [0.025]], 10.0)
+    r = O.propagate_until(dec, p, r["state"], 0.0, t_hi=r["t_hi"], t_lo=r["t_lo"])
+    # The round-trip error (~1e-12) is a property of the order-12 truncation; it
+    # must be reproduced, not just bounded.
+    np.testing.assert_allclose(r["state"][:, 0], goldens["pend_tol1e9"]["state"], rtol=0, atol=3e-15)
+
+
+def test_fdp_batch_step(fdp, goldens):
+    s = goldens["fdp_setup"]
+    r = O.step_batch(fdp, 20, s["state"], pars=s["pars"])
+    assert np.all(r["outcome"] == goldens["fdp_step"]["outcome"])
+    assert r["h"].tolist() == goldens["fdp_step"]["h"]  # bit-exact in the relative-error branch
+    g = goldens["fdp_step_state"]
+    np.testing.assert_allclose(r["state"], g["state"], rtol=0, atol=5e-9)
+    np.testing.assert_allclose(r["t_hi"], g["time"], rtol=0, atol=5e-9)
+    r2 = O.step_batch(fdp, 20, r["state"], pars=s["pars"], t_hi=r["t_hi"], t_lo=r["t_lo"],
+                      max_dt=[0.010, 0.011, 0.012, 0.013])
+    g = goldens["fdp_clamped"]
+    assert np.all(r2["outcome"] == g["outcome"]) and r2["h"].tolist() == g["h"]
+    np.testing.assert_allclose(r2["state"], g["state"], rtol=0, atol=5e-9)
+    np.testing.assert_allclose(r2["t_hi"], g["time"], rtol=0, atol=5e-9)
+
+
+def test_fdp_batch_propagate(fdp, goldens):
+    s = goldens["fdp_setup"]
+    r = O.step_batch(fdp, 20, s["state"], pars=s["pars"])
+    r = O.step_batch(fdp, 20, r["state"], pars=s["pars"], t_hi=r["t_hi"], t_lo=r["t_lo"],
+                     max_dt=[0.010, 0.011, 0.012, 0.013])
+    # propagate_for(delta_t): final time = current time + delta_t in double length.
+    dt = np.array(goldens["fdp_propagate_for"]["delta_t"])
+    tf_hi = r["t_hi"] + dt
+    tf_lo = (dt - (tf_hi - r["t_hi"])) + (r["t_hi"] - (tf_hi - (tf_hi - r["t_hi"]))) + r["t_lo"]
+    p = O.propagate_until(fdp, 20, r["state"], tf_hi, tf_lo=tf_lo, pars=s["pars"], t_hi=r["t_hi"], t_lo=r["t_lo"],
+                          lane_batch=4)
+    g = goldens["fdp_propagate_for"]
+    for b in range(4):
+        oc, mn, mx, ns = g["res"][b]
+        assert p["outcome"][b] == oc and int(p["nsteps"][b]) == ns
+        # ~40 steps of ulp-level differences in the state move h by a few ulp.
+        np.testing.assert_allclose([p["min_h"][b], p["max_h"][b]], [mn, mx], rtol=1e-14, atol=0)
+    np.testing.assert_allclose(p["state"], g["state"], rtol=0, atol=5e-8)
+    np.testing.assert_allclose(p["t_hi"], g["time"], rtol=0, atol=5e-8)
+    q = O.propagate_until(fdp, 20, p["state"], goldens["fdp_propagate_until"]["t"], pars=s["pars"], t_hi=p["t_hi"],
+                          t_lo=p["t_lo"], lane_batch=4)
+    g = goldens["fdp_propagate_until"]
+    for b in range(4):
+        oc, mn, mx, ns = g["res"][b]
+        assert q["outcome"][b] == oc and int(q["nsteps"][b]) == ns
+        np.testing.assert_allclose([q["min_h"][b], q["max_h"][b]], [mn, mx], rtol=1e-13, atol=0)
+    np.testing.assert_allclose(q["state"], g["state"], rtol=0, atol=5e-8)
+    assert q["t_hi"].tolist() == g["t"]
+
+
+def test_fdp_taylor_coefficients(fdp, goldens):
+    s = goldens["fdp_setup"]
+    r = O.step_batch(fdp, 20, s["state"], pars=s["pars"], want_tc=True)
+    tc = np.array(goldens["fdp_tc"]["tc"])  # [2][21][4], printed with 9 significant digits
+    np.testing.assert_allclose(r["tc"], tc, rtol=6e-9, atol=1e-15)
+
+
+def test_fdp_dense_output_from_tc(fdp, goldens):
+    # update_d_output(t) after one step is the Taylor polynomial of the step
+    # evaluated at t - t0 (here t0 = 0).
+    s = goldens["fdp_setup"]
+    r = O.step_batch(fdp, 20, s["state"], pars=s["pars"], want_tc=True)
+    g = goldens["fdp_d_output"]
+    t = np.array(g["t"])
+    out = np.zeros((2, 4))
+    for i in range(2):
+        acc = r["tc"][i, 20, :].copy()
+        for k in range(19, -1, -1):
+            acc = acc * t + r["tc"][i, k, :]
+        out[i] = acc
+    np.testing.assert_allclose(out, g["out"], rtol=0, atol=5e-9)
