@@ -1,0 +1,193 @@
+"""Throughput benchmark: FP64 instance-steps/s of the batch/ensemble Taylor integrator.
+
+Workload (default): the 6-body ensemble of 65,536 perturbed copies (36 states,
+FP64, tol = 1e-15, order 19), integrated from t = 0 by consecutive
+propagate_until slices of `--dt` time units. One bench step = one slice for the
+whole ensemble, device-resident (inputs already in HBM when the timer starts;
+the state carries over from slice to slice). instance-steps = the sum over
+instances of nsteps in propagate_res (zero-length steps are not counted).
+
+`--config cr3bp` runs the ragged planar CR3BP ensemble (131,072 instances,
+horizons 10**U(1, 4)) once per step in the persistent form, and reports the
+plain one-launch-per-step form beside it.
+
+Prints one JSON line (see the keys below). The CPU baseline is the oracle
+(a C restatement of the same algorithm) timed on this machine's cores over a
+bounded sample of the same workload.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP64 instance-steps/sec at tol=1e-15 vs plain GPU & CPU; p50/p99 request latency"
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+
+
+def _cpu_baseline(dec, order, state, pars, t0, t1, n_sample, threads, lane_batch=4):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+
+    st = np.ascontiguousarray(state[:, :n_sample])
+    pa = None if pars is None else np.ascontiguousarray(pars[:, :n_sample])
+    tf = np.broadcast_to(np.asarray(t1, dtype=float), (st.shape[1],)).copy() if np.ndim(t1) == 0 else t1[:n_sample]
+    t = time.perf_counter()
+    r = O.propagate_until(dec, order, st, tf, pars=pa, t_hi=np.full(st.shape[1], t0), nthreads=threads,
+                          lane_batch=lane_batch)
+    dt = time.perf_counter() - t
+    return float(r["nsteps"].astype(np.float64).sum()) / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="nbody6", choices=["nbody6", "cr3bp"])
+    ap.add_argument("--n", type=int, default=None, help="instances (default: the config's)")
+    ap.add_argument("--dt", type=float, default=25.0, help="nbody6: time units per bench step")
+    ap.add_argument("--code-form", default=None, choices=[None, "compact", "unrolled"])
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-sample", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (instances are independent; "
+              "a multi-GPU run would shard them, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import heyoka_amd as hy
+    from heyoka_amd import _lib
+    from heyoka_amd import workloads as W
+    from heyoka_amd.integrator import get_system
+
+    if args.config == "nbody6":
+        wl = W.nbody6(args.n or 65536)
+    else:
+        wl = W.cr3bp(args.n or 131072)
+    sys_ = wl.sys.sys if hasattr(wl.sys, "n_orig_sv") else wl.sys
+    dec = hy.taylor_decompose(sys_)
+    order = hy.order_from_tol(wl.tol)
+    compact = {"compact": True, "unrolled": False, None: None}[args.code_form]
+    t_c = time.perf_counter()
+    system = get_system(dec, order, compact=compact)
+    t_compile = time.perf_counter() - t_c
+    N = wl.state.shape[1]
+    flops = dec.algorithmic_flops(order)["total"]
+    batch = _lib.Batch(system, N)
+
+    def load():
+        batch.write(_lib.FIELD_STATE, wl.state)
+        if wl.pars is not None:
+            batch.write(_lib.FIELD_PARS, wl.pars)
+        batch.write(_lib.FIELD_TIME_HI, np.zeros(N))
+        batch.write(_lib.FIELD_TIME_LO, np.zeros(N))
+
+    def one_step(i, mode=_lib.MODE_PERSISTENT):
+        if args.config == "nbody6":
+            tf = np.full(N, args.dt * (i + 1))
+        else:
+            tf = wl.t_final  # cr3bp: each step is the whole ragged propagation from the initial conditions
+        r = batch.propagate_until(tf, None, None, 0, mode)
+        ms, launches = batch.last_timing()
+        return r, ms, launches
+
+    load()
+    for i in range(args.warmup):
+        if args.config == "cr3bp":
+            load()
+        one_step(i)
+        print(f"[bench] warmup step {i} done", file=sys.stderr, flush=True)
+    load()
+    batch.sync()
+    total_steps = 0
+    kernel_ms = 0.0
+    launches = 0
+    wall = 0.0
+    for i in range(args.steps):
+        if args.config == "cr3bp":
+            load()
+        batch.sync()
+        t = time.perf_counter()
+        r, ms, nl = one_step(i)
+        batch.sync()
+        wall += time.perf_counter() - t
+        total_steps += int(r["nsteps"].astype(np.int64).sum())
+        kernel_ms += ms
+        launches += nl
+        bad = int(np.sum(r["outcome"] != -4294967299))
+        print(f"[bench] step {i}: {r['nsteps'].astype(np.float64).sum():.3e} instance-steps in {ms:.1f} ms "
+              f"({nl} launches), outcomes not time_limit: {bad}", file=sys.stderr, flush=True)
+    value = total_steps / wall
+    avg_launch_s = kernel_ms / 1e3 / max(launches, 1)
+    flops_per_launch = flops * total_steps / max(launches, 1)
+    achieved_tf = flops_per_launch / avg_launch_s / 1e12
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "instance-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"nbody6 ensemble: {N} instances, 36 states, tol 1e-15 (order {order}), propagate_until in "
+                         f"slices of dt={args.dt} from t=0" if args.config == "nbody6" else
+                         f"cr3bp ragged ensemble: {N} instances, 4 states + par[0], tol 1e-15, horizons 10**U(1,4)"),
+            "instances": N,
+            "order": order,
+            "code_form": "compact" if system.compact else "unrolled",
+            "form": "persistent",
+            "algorithmic_flops_per_instance_step": flops,
+            "hiprtc_compile_s": round(t_compile, 3),
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved_tf,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tf / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "hyx_persist_k",
+            "avg_launch_ms": avg_launch_s * 1e3,
+        },
+        "cpu_baseline": None,
+    }
+    if args.config == "cr3bp" and not args.no_plain:
+        load()
+        t = time.perf_counter()
+        r, ms, nl = one_step(0, _lib.MODE_PLAIN)
+        batch.sync()
+        plain_wall = time.perf_counter() - t
+        plain_value = float(r["nsteps"].astype(np.float64).sum()) / plain_wall
+        out["plain_gpu"] = {"value": plain_value, "unit": "instance-steps/s", "launches": nl,
+                            "persistent_over_plain": value / plain_value}
+    if not args.no_cpu:
+        ns = args.cpu_sample or (1024 if args.config == "nbody6" else 2048)
+        if args.config == "nbody6":
+            v, secs = _cpu_baseline(dec, order, wl.state, wl.pars, 0.0, args.dt, ns, args.cpu_threads)
+            sample = f"{ns} instances of the same ensemble, t in [0, {args.dt}], {args.cpu_threads} threads"
+        else:
+            v, secs = _cpu_baseline(dec, order, wl.state, wl.pars, 0.0, wl.t_final, ns, args.cpu_threads)
+            sample = f"first {ns} instances of the same ensemble to their own horizons, {args.cpu_threads} threads"
+        out["cpu_baseline"] = {"value": v, "unit": "instance-steps/s", "cores": args.cpu_threads, "kind": "port",
+                               "sample": sample + f" ({secs:.1f} s)"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

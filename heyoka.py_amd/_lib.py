@@ -1,0 +1,254 @@
+"""ctypes binding of libhyx.so (the C-ABI declared in include/hyx.h).
+
+ctypes releases the GIL for the duration of every foreign call, as the
+reference's binding does around step/propagate
+(heyoka/expose_batch_integrators.cpp:253-255, 289-291, 300-302).
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+import threading
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libhyx.so")
+CSRC = os.path.join(PKG_DIR, "csrc")
+
+FIELD_STATE, FIELD_PARS, FIELD_TIME_HI, FIELD_TIME_LO = 0, 1, 2, 3
+MODE_PERSISTENT, MODE_PLAIN, MODE_GRAPH = 0, 1, 2
+FLAG_HIGH_ACCURACY, FLAG_COMPILE_ONLY, FLAG_FORCE_COMPACT, FLAG_FORCE_UNROLLED, FLAG_SOURCE_ONLY = 1, 2, 4, 8, 16
+
+
+class prop_res(ct.Structure):
+    _fields_ = [("outcome", ct.c_int64), ("min_h", ct.c_double), ("max_h", ct.c_double), ("nsteps", ct.c_uint64)]
+
+
+class HyxError(RuntimeError):
+    pass
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def build(force: bool = False) -> str:
+    """Compile libhyx.so in-tree with its Makefile (hipcc, gfx950)."""
+    cmd = ["make", "-C", CSRC] + (["-B"] if force else [])
+    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    return LIB_PATH
+
+
+def lib():
+    """Load libhyx.so. It is never built implicitly: a missing library is an error."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')"
+            )
+        L = ct.CDLL(LIB_PATH)
+        P = ct.POINTER
+        u32, u64, i64, d, vp, cp = ct.c_uint32, ct.c_uint64, ct.c_int64, ct.c_double, ct.c_void_p, ct.c_char_p
+        sig = {
+            "hyx_sys_create": (ct.c_int, [P(u32), u32, P(d), u32, u32, u32, P(vp)]),
+            "hyx_sys_precompile": (ct.c_int, [P(u32), u32, P(d), u32, u32, u32, cp]),
+            "hyx_sys_order": (ct.c_int, [vp]),
+            "hyx_sys_dim": (ct.c_int, [vp]),
+            "hyx_sys_npar": (ct.c_int, [vp]),
+            "hyx_sys_is_compact": (ct.c_int, [vp]),
+            "hyx_sys_source": (ct.c_size_t, [vp, ct.c_char_p, ct.c_size_t]),
+            "hyx_sys_kernel_info": (ct.c_int, [vp, P(ct.c_int), P(ct.c_int)]),
+            "hyx_sys_destroy": (None, [vp]),
+            "hyx_batch_create": (ct.c_int, [vp, u64, P(vp)]),
+            "hyx_batch_destroy": (None, [vp]),
+            "hyx_batch_size": (u64, [vp]),
+            "hyx_batch_write": (ct.c_int, [vp, ct.c_int, P(d)]),
+            "hyx_batch_read": (ct.c_int, [vp, ct.c_int, P(d)]),
+            "hyx_batch_device_ptr": (ct.c_int, [vp, ct.c_int, P(vp)]),
+            "hyx_step": (ct.c_int, [vp, P(d), ct.c_int, P(i64), P(d)]),
+            "hyx_read_tc": (ct.c_int, [vp, P(d)]),
+            "hyx_propagate_until": (ct.c_int, [vp, P(d), P(d), P(d), u64, ct.c_int, P(prop_res)]),
+            "hyx_batch_last_timing": (ct.c_int, [vp, P(d), P(u64)]),
+            "hyx_batch_set_persistent_config": (ct.c_int, [vp, u32, u32]),
+            "hyx_batch_set_queue": (ct.c_int, [vp, P(u32)]),
+            "hyx_batch_set_iter_budget": (ct.c_int, [vp, u64]),
+            "hyx_batch_sync": (ct.c_int, [vp]),
+            "hyx_last_error": (ct.c_char_p, []),
+            "hyx_device_count": (ct.c_int, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+        return _lib
+
+
+EXPORTED = (
+    "hyx_sys_create", "hyx_sys_precompile", "hyx_sys_order", "hyx_sys_dim", "hyx_sys_npar", "hyx_sys_is_compact",
+    "hyx_sys_source",
+    "hyx_sys_kernel_info", "hyx_sys_destroy", "hyx_batch_create", "hyx_batch_destroy", "hyx_batch_size",
+    "hyx_batch_write", "hyx_batch_read", "hyx_batch_device_ptr", "hyx_step", "hyx_read_tc", "hyx_propagate_until",
+    "hyx_batch_last_timing", "hyx_batch_set_persistent_config", "hyx_batch_set_queue", "hyx_batch_set_iter_budget",
+    "hyx_batch_sync",
+    "hyx_last_error", "hyx_device_count",
+)
+
+
+def check(rc: int):
+    if rc != 0:
+        msg = lib().hyx_last_error().decode(errors="replace")
+        raise HyxError(f"hyx error {rc}: {msg}")
+
+
+def dptr(a: np.ndarray, ctype=ct.c_double):
+    return None if a is None else a.ctypes.data_as(ct.POINTER(ctype))
+
+
+def oplist_args(dec):
+    words = np.ascontiguousarray(dec.to_words(), dtype=np.uint32)
+    consts = np.ascontiguousarray(dec.consts_array(), dtype=np.float64)
+    return words, consts
+
+
+def _flags(high_accuracy=False, compact=None):
+    f = FLAG_HIGH_ACCURACY if high_accuracy else 0
+    if compact is True:
+        f |= FLAG_FORCE_COMPACT
+    elif compact is False:
+        f |= FLAG_FORCE_UNROLLED
+    return f
+
+
+def precompile(dec, order: int, high_accuracy: bool = False, arch: str = "gfx950", compact=None):
+    """Compile the system's kernels into the code-object cache (no GPU needed)."""
+    words, consts = oplist_args(dec)
+    flags = _flags(high_accuracy, compact)
+    check(lib().hyx_sys_precompile(dptr(words, ct.c_uint32), len(words), dptr(consts), len(dec.consts), order, flags,
+                                   arch.encode()))
+
+
+def device_count() -> int:
+    return int(lib().hyx_device_count())
+
+
+class System:
+    """Owner of a hyx_sys handle."""
+
+    def __init__(self, dec, order: int, high_accuracy: bool = False, compile_only: bool = False, compact=None,
+                 source_only: bool = False):
+        L = lib()
+        words, consts = oplist_args(dec)
+        flags = (_flags(high_accuracy, compact) | (FLAG_COMPILE_ONLY if compile_only else 0)
+                 | (FLAG_SOURCE_ONLY if source_only else 0))
+        h = ct.c_void_p()
+        check(L.hyx_sys_create(dptr(words, ct.c_uint32), len(words), dptr(consts), len(dec.consts), order, flags,
+                               ct.byref(h)))
+        self.h = h
+        self.order = order
+        self.n = dec.n_state
+        self.npar = dec.n_par
+        self._L = L
+        self.compact = bool(L.hyx_sys_is_compact(self.h))
+
+    def source(self) -> str:
+        n = self._L.hyx_sys_source(self.h, None, 0)
+        buf = ct.create_string_buffer(n + 1)
+        self._L.hyx_sys_source(self.h, buf, n + 1)
+        return buf.value.decode()
+
+    def kernel_info(self):
+        v, s = ct.c_int(), ct.c_int()
+        check(self._L.hyx_sys_kernel_info(self.h, ct.byref(v), ct.byref(s)))
+        return {"vgprs": v.value, "scratch_bytes": s.value}
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            self._L.hyx_sys_destroy(h)
+            self.h = None
+
+
+class Batch:
+    """Owner of a hyx_batch handle: N device-resident instances of one System."""
+
+    def __init__(self, system: System, n: int):
+        self.sys = system
+        self.N = int(n)
+        self._L = system._L
+        h = ct.c_void_p()
+        check(self._L.hyx_batch_create(system.h, self.N, ct.byref(h)))
+        self.h = h
+
+    def write(self, field: int, arr: np.ndarray):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        check(self._L.hyx_batch_write(self.h, field, dptr(a)))
+
+    def read(self, field: int, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=np.float64)
+        if out.size:
+            check(self._L.hyx_batch_read(self.h, field, dptr(out)))
+        return out
+
+    def device_ptr(self, field: int) -> int:
+        p = ct.c_void_p()
+        check(self._L.hyx_batch_device_ptr(self.h, field, ct.byref(p)))
+        return p.value
+
+    def step(self, max_dt: np.ndarray | None, write_tc: bool):
+        oc = np.empty(self.N, dtype=np.int64)
+        h = np.empty(self.N, dtype=np.float64)
+        md = None if max_dt is None else np.ascontiguousarray(max_dt, dtype=np.float64)
+        check(self._L.hyx_step(self.h, dptr(md), int(write_tc), dptr(oc, ct.c_int64), dptr(h)))
+        return oc, h
+
+    def read_tc(self) -> np.ndarray:
+        out = np.empty((self.sys.n, self.sys.order + 1, self.N), dtype=np.float64)
+        check(self._L.hyx_read_tc(self.h, dptr(out)))
+        return out
+
+    def propagate_until(self, tf_hi, tf_lo=None, max_delta_t=None, max_steps=0, mode=MODE_PERSISTENT,
+                        want_res=True):
+        th = np.ascontiguousarray(tf_hi, dtype=np.float64)
+        tl = None if tf_lo is None else np.ascontiguousarray(tf_lo, dtype=np.float64)
+        md = None if max_delta_t is None else np.ascontiguousarray(max_delta_t, dtype=np.float64)
+        res = (prop_res * self.N)() if want_res else None
+        check(self._L.hyx_propagate_until(self.h, dptr(th), dptr(tl), dptr(md), int(max_steps), int(mode), res))
+        if not want_res:
+            return None
+        arr = np.frombuffer(res, dtype=np.dtype([("outcome", "<i8"), ("min_h", "<f8"), ("max_h", "<f8"),
+                                                 ("nsteps", "<u8")]), count=self.N)
+        return arr.copy()
+
+    def last_timing(self):
+        ms, n = ct.c_double(), ct.c_uint64()
+        check(self._L.hyx_batch_last_timing(self.h, ct.byref(ms), ct.byref(n)))
+        return ms.value, n.value
+
+    def set_persistent_config(self, waves_per_cu: int = 0, refill_min: int = 1):
+        check(self._L.hyx_batch_set_persistent_config(self.h, int(waves_per_cu), int(refill_min)))
+
+    def set_queue(self, order):
+        if order is None:
+            check(self._L.hyx_batch_set_queue(self.h, None))
+        else:
+            o = np.ascontiguousarray(order, dtype=np.uint32)
+            check(self._L.hyx_batch_set_queue(self.h, dptr(o, ct.c_uint32)))
+
+    def set_iter_budget(self, iters: int):
+        check(self._L.hyx_batch_set_iter_budget(self.h, int(iters)))
+
+    def sync(self):
+        check(self._L.hyx_batch_sync(self.h))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            self._L.hyx_batch_destroy(h)
+            self.h = None

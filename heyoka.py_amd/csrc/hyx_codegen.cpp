@@ -1,0 +1,799 @@
+// Per-system code generation: op-list -> HIP source of hyx_taylor_step<WTC>().
+//
+// The generated function runs, for one instance held by the calling lane, the
+// normalised-derivative recurrences of every node of the Taylor decomposition
+// (doc/tut_taylor_method.rst:22-81; Jorba & Zou 2005), picks the step size
+// from the infinity norms of the last two orders with heyoka's absolute /
+// relative switch (doc/notebooks/Customising the adaptive integrator.ipynb:23-31)
+// and advances the state by Horner's rule (compensated summation in
+// high-accuracy mode).
+//
+// Unrolled form: every coefficient is a scalar SSA value and every loop is
+// unrolled, so the register allocator keeps exactly the histories that later
+// orders read. Operands whose series is known to stop early (numbers,
+// parameters, time, nodes built only from those) contribute no terms past
+// their degree, so e.g. a product with a parameter costs one multiply per order.
+// Divisions inside the recurrences become multiplications by reciprocals
+// computed once per step; 1/k and j/k factors are literals.
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <set>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/hyx_oplist.h"
+#include "hyx_internal.h"
+
+namespace hyx
+{
+
+static const int INF_DEG = 1 << 20;
+
+int parse_oplist(const uint32_t *w, uint32_t n_words, const double *consts, uint32_t n_consts, Sys &S,
+                 std::string &err)
+{
+    if (n_words < 6 || w[0] != HYX_OPLIST_MAGIC) {
+        err = "invalid op-list: bad magic or size";
+        return -1;
+    }
+    S.n = w[1];
+    S.npar = w[2];
+    S.nnodes = w[3];
+    S.nconsts = w[4];
+    if (S.n == 0 || S.nnodes < S.n || S.nconsts != n_consts) {
+        err = "invalid op-list header";
+        return -1;
+    }
+    S.consts.assign(consts, consts + n_consts);
+    S.op.assign(S.nnodes, 0);
+    S.aux.assign(S.nnodes, 0);
+    S.args.assign(S.nnodes, {});
+    uint32_t off = 6;
+    for (uint32_t j = S.n; j < S.nnodes; ++j) {
+        if (off + 3 > n_words) {
+            err = "invalid op-list: truncated node record";
+            return -1;
+        }
+        S.op[j] = w[off];
+        const uint32_t na = w[off + 1];
+        S.aux[j] = w[off + 2];
+        if (off + 3 + na > n_words) {
+            err = "invalid op-list: truncated arguments";
+            return -1;
+        }
+        for (uint32_t q = 0; q < na; ++q) {
+            const uint32_t a = w[off + 3 + q];
+            const uint32_t kind = HYX_OPERAND_KIND(a), idx = HYX_OPERAND_INDEX(a);
+            if ((kind == HYX_KIND_NODE && idx >= j) || (kind == HYX_KIND_CONST && idx >= S.nconsts)
+                || (kind == HYX_KIND_PAR && idx >= S.npar) || kind > HYX_KIND_PAR) {
+                err = "invalid op-list: operand out of range";
+                return -1;
+            }
+            S.args[j].push_back(a);
+        }
+        const uint32_t op = S.op[j];
+        const size_t need = (op == HYX_OP_TIME) ? 0
+                            : (op == HYX_OP_ADD || op == HYX_OP_SUB || op == HYX_OP_MUL || op == HYX_OP_DIV) ? 2
+                            : (op == HYX_OP_SUM) ? S.args[j].size()
+                                                 : 1;
+        if (op < HYX_OP_ADD || op > HYX_OP_SUM || S.args[j].size() != need || (op == HYX_OP_SUM && need < 2)) {
+            err = "invalid op-list: bad opcode or arity at node " + std::to_string(j);
+            return -1;
+        }
+        if (op == HYX_OP_POW && S.aux[j] >= S.nconsts) {
+            err = "invalid op-list: pow exponent out of range";
+            return -1;
+        }
+        if ((op == HYX_OP_SIN || op == HYX_OP_COS)) {
+            const uint32_t p = S.aux[j];
+            if (p < S.n || p >= S.nnodes) {
+                err = "invalid op-list: sin/cos partner out of range";
+                return -1;
+            }
+        }
+        off += 3 + na;
+    }
+    if (off != w[5] || off + S.n != n_words) {
+        err = "invalid op-list: rhs offset mismatch";
+        return -1;
+    }
+    S.rhs.assign(w + off, w + off + S.n);
+    for (uint32_t i = 0; i < S.n; ++i) {
+        const uint32_t kind = HYX_OPERAND_KIND(S.rhs[i]), idx = HYX_OPERAND_INDEX(S.rhs[i]);
+        if ((kind == HYX_KIND_NODE && idx >= S.nnodes) || (kind == HYX_KIND_CONST && idx >= S.nconsts)
+            || (kind == HYX_KIND_PAR && idx >= S.npar) || kind > HYX_KIND_PAR) {
+            err = "invalid op-list: rhs operand out of range";
+            return -1;
+        }
+    }
+    // sin/cos partners must point at each other with the same argument.
+    for (uint32_t j = S.n; j < S.nnodes; ++j) {
+        if (S.op[j] == HYX_OP_SIN || S.op[j] == HYX_OP_COS) {
+            const uint32_t p = S.aux[j];
+            const uint32_t want = (S.op[j] == HYX_OP_SIN) ? HYX_OP_COS : HYX_OP_SIN;
+            if (S.op[p] != want || S.aux[p] != j || S.args[p] != S.args[j]) {
+                err = "invalid op-list: inconsistent sin/cos pair";
+                return -1;
+            }
+        }
+    }
+    // Degrees: highest order with a possibly non-zero coefficient.
+    S.deg.assign(S.nnodes, INF_DEG);
+    auto od = [&](uint32_t a) {
+        return HYX_OPERAND_KIND(a) == HYX_KIND_NODE ? S.deg[HYX_OPERAND_INDEX(a)] : 0;
+    };
+    for (uint32_t j = S.n; j < S.nnodes; ++j) {
+        const auto &A = S.args[j];
+        int d = 0;
+        switch (S.op[j]) {
+        case HYX_OP_TIME: d = 1; break;
+        case HYX_OP_ADD:
+        case HYX_OP_SUB:
+        case HYX_OP_SUM:
+        case HYX_OP_NEG:
+            for (auto a : A) d = std::max(d, od(a));
+            break;
+        case HYX_OP_MUL: d = std::min(INF_DEG, od(A[0]) + od(A[1])); break;
+        case HYX_OP_SQUARE: d = std::min(INF_DEG, 2 * od(A[0])); break;
+        case HYX_OP_DIV: d = (od(A[1]) == 0) ? od(A[0]) : INF_DEG; break;
+        default: d = (od(A[0]) == 0) ? 0 : INF_DEG;
+        }
+        S.deg[j] = d;
+    }
+    return 0;
+}
+
+std::string lit(double v)
+{
+    if (std::isnan(v)) return "(__builtin_nan(\"\"))";
+    if (std::isinf(v)) return v > 0 ? "(__builtin_inf())" : "(-__builtin_inf())";
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "(%a)", v);
+    return buf;
+}
+
+namespace
+{
+
+struct Gen {
+    const Sys &S;
+    const int P;
+    const bool high_acc;
+    std::ostringstream o;
+    std::set<std::pair<uint32_t, int>> zset; // (node, k) known to be exactly zero
+    std::set<std::pair<uint32_t, int>> dvars; // generated j * a^[j] values
+
+    Gen(const Sys &s, int p, bool ha) : S(s), P(p), high_acc(ha) {}
+
+    static std::string cv(uint32_t node, int k) { return "c" + std::to_string(node) + "_" + std::to_string(k); }
+
+    bool zero(uint32_t w, int k) const
+    {
+        const uint32_t kind = HYX_OPERAND_KIND(w), idx = HYX_OPERAND_INDEX(w);
+        if (kind != HYX_KIND_NODE) return k > 0 || (kind == HYX_KIND_CONST && S.consts[idx] == 0.0);
+        if (k > S.deg[idx]) return true;
+        return zset.count({idx, k}) != 0;
+    }
+
+    // Degree-0 operand: a number, a parameter or a node whose series is constant.
+    bool scalar(uint32_t w) const
+    {
+        const uint32_t kind = HYX_OPERAND_KIND(w);
+        return kind != HYX_KIND_NODE || S.deg[HYX_OPERAND_INDEX(w)] == 0;
+    }
+
+    std::string val(uint32_t w, int k) const
+    {
+        const uint32_t kind = HYX_OPERAND_KIND(w), idx = HYX_OPERAND_INDEX(w);
+        if (kind == HYX_KIND_NODE) return cv(idx, k);
+        if (k > 0) return "0.0";
+        if (kind == HYX_KIND_CONST) return lit(S.consts[idx]);
+        return "pv[" + std::to_string(idx) + "]";
+    }
+
+    // j * a^[j], shared by the exp / sin / cos recurrences over the same argument.
+    std::string dval(uint32_t w, int j)
+    {
+        if (j == 1) return val(w, 1);
+        const uint32_t idx = HYX_OPERAND_INDEX(w);
+        const std::string nm = "d" + std::to_string(idx) + "_" + std::to_string(j);
+        if (!dvars.count({idx, j})) {
+            dvars.insert({idx, j});
+            o << "    const double " << nm << " = " << lit((double)j) << " * " << val(w, j) << ";\n";
+        }
+        return nm;
+    }
+
+    void set_zero(uint32_t node, int k)
+    {
+        zset.insert({node, k});
+        o << "    const double " << cv(node, k) << " = 0.0;\n";
+    }
+
+    // acc = sum of products (terms), as an fma chain; "" when empty.
+    static std::string fma_chain(const std::vector<std::pair<std::string, std::string>> &t, const std::string &nm,
+                                 std::ostringstream &o)
+    {
+        if (t.empty()) return "";
+        o << "    double " << nm << " = " << t[0].first << " * " << t[0].second << ";\n";
+        for (size_t q = 1; q < t.size(); ++q)
+            o << "    " << nm << " = __builtin_fma(" << t[q].first << ", " << t[q].second << ", " << nm << ");\n";
+        return nm;
+    }
+
+    void order0(uint32_t j)
+    {
+        const auto &A = S.args[j];
+        const std::string c = cv(j, 0);
+        auto v = [&](size_t q) { return val(A[q], 0); };
+        switch (S.op[j]) {
+        case HYX_OP_ADD: o << "    const double " << c << " = " << v(0) << " + " << v(1) << ";\n"; break;
+        case HYX_OP_SUB: o << "    const double " << c << " = " << v(0) << " - " << v(1) << ";\n"; break;
+        case HYX_OP_NEG: o << "    const double " << c << " = -" << v(0) << ";\n"; break;
+        case HYX_OP_SUM: {
+            o << "    const double " << c << " = " << v(0);
+            for (size_t q = 1; q < A.size(); ++q) o << " + " << v(q);
+            o << ";\n";
+            break;
+        }
+        case HYX_OP_MUL: o << "    const double " << c << " = " << v(0) << " * " << v(1) << ";\n"; break;
+        case HYX_OP_DIV:
+            o << "    const double " << c << " = " << v(0) << " / " << v(1) << ";\n";
+            if (S.deg[j] > 0) o << "    const double rb" << j << " = 1.0 / " << v(1) << ";\n";
+            break;
+        case HYX_OP_SQUARE: o << "    const double " << c << " = " << v(0) << " * " << v(0) << ";\n"; break;
+        case HYX_OP_SQRT:
+            o << "    const double " << c << " = sqrt(" << v(0) << ");\n";
+            if (S.deg[j] > 0) o << "    const double rs" << j << " = 0.5 / " << c << ";\n";
+            break;
+        case HYX_OP_POW:
+            o << "    const double " << c << " = pow(" << v(0) << ", " << lit(S.consts[S.aux[j]]) << ");\n";
+            if (S.deg[j] > 0) o << "    const double ra" << j << " = 1.0 / " << v(0) << ";\n";
+            break;
+        case HYX_OP_EXP: o << "    const double " << c << " = exp(" << v(0) << ");\n"; break;
+        case HYX_OP_LOG:
+            o << "    const double " << c << " = log(" << v(0) << ");\n";
+            if (S.deg[j] > 0) o << "    const double ra" << j << " = 1.0 / " << v(0) << ";\n";
+            break;
+        case HYX_OP_SIN: {
+            // The pair shares one sincos() call, issued at the sin node.
+            const uint32_t p = S.aux[j];
+            if (p > j) {
+                o << "    double " << c << ", " << cv(p, 0) << ";\n";
+                o << "    sincos(" << v(0) << ", &" << c << ", &" << cv(p, 0) << ");\n";
+            }
+            break;
+        }
+        case HYX_OP_COS: {
+            const uint32_t p = S.aux[j];
+            if (p > j) {
+                o << "    double " << cv(p, 0) << ", " << c << ";\n";
+                o << "    sincos(" << v(0) << ", &" << cv(p, 0) << ", &" << c << ");\n";
+            }
+            break;
+        }
+        case HYX_OP_TIME: o << "    const double " << c << " = t0;\n"; break;
+        }
+    }
+
+    void orderk(uint32_t j, int k)
+    {
+        if (k > S.deg[j]) return; // zero by degree; never referenced
+        const auto &A = S.args[j];
+        const std::string c = cv(j, k);
+        const std::string ks = std::to_string(k);
+        std::vector<std::pair<std::string, std::string>> t;
+        switch (S.op[j]) {
+        case HYX_OP_ADD:
+        case HYX_OP_SUB:
+        case HYX_OP_NEG:
+        case HYX_OP_SUM: {
+            std::string e;
+            bool first = true;
+            for (size_t q = 0; q < A.size(); ++q) {
+                if (zero(A[q], k)) continue;
+                const bool neg = (S.op[j] == HYX_OP_NEG) || (S.op[j] == HYX_OP_SUB && q == 1);
+                if (first)
+                    e = (neg ? "-" : "") + val(A[q], k);
+                else
+                    e += (neg ? " - " : " + ") + val(A[q], k);
+                first = false;
+            }
+            if (first)
+                set_zero(j, k);
+            else
+                o << "    const double " << c << " = " << e << ";\n";
+            break;
+        }
+        case HYX_OP_MUL: {
+            for (int q = 0; q <= k; ++q)
+                if (!zero(A[0], q) && !zero(A[1], k - q)) t.push_back({val(A[0], q), val(A[1], k - q)});
+            if (t.empty())
+                set_zero(j, k);
+            else if (t.size() == 1)
+                o << "    const double " << c << " = " << t[0].first << " * " << t[0].second << ";\n";
+            else {
+                fma_chain(t, "m" + std::to_string(j) + "_" + ks, o);
+                o << "    const double " << c << " = m" << j << "_" << ks << ";\n";
+            }
+            break;
+        }
+        case HYX_OP_DIV: {
+            // c^[k] = (a^[k] - sum_{i=1}^{k} b^[i] c^[k-i]) / b^[0]
+            for (int q = 1; q <= k; ++q)
+                if (!zero(A[1], q) && !zero(HYX_OPERAND(HYX_KIND_NODE, j), k - q)) t.push_back({val(A[1], q), cv(j, k - q)});
+            const std::string acc = fma_chain(t, "s" + std::to_string(j) + "_" + ks, o);
+            const std::string a = zero(A[0], k) ? "0.0" : val(A[0], k);
+            if (acc.empty() && a == "0.0")
+                set_zero(j, k);
+            else
+                o << "    const double " << c << " = (" << a << (acc.empty() ? "" : " - " + acc) << ") * rb" << j << ";\n";
+            break;
+        }
+        case HYX_OP_SQUARE: {
+            for (int q = 0; 2 * q < k; ++q)
+                if (!zero(A[0], q) && !zero(A[0], k - q)) t.push_back({val(A[0], q), val(A[0], k - q)});
+            const std::string acc = fma_chain(t, "q" + std::to_string(j) + "_" + ks, o);
+            const bool mid = (k % 2 == 0) && !zero(A[0], k / 2);
+            if (acc.empty() && !mid)
+                set_zero(j, k);
+            else if (acc.empty())
+                o << "    const double " << c << " = " << val(A[0], k / 2) << " * " << val(A[0], k / 2) << ";\n";
+            else if (!mid)
+                o << "    const double " << c << " = " << acc << " + " << acc << ";\n";
+            else
+                o << "    const double " << c << " = __builtin_fma(" << val(A[0], k / 2) << ", " << val(A[0], k / 2)
+                  << ", " << acc << " + " << acc << ");\n";
+            break;
+        }
+        case HYX_OP_SQRT: {
+            // c^[k] = (a^[k] - sum_{i=1}^{k-1} c^[i] c^[k-i]) / (2 c^[0])
+            for (int q = 1; 2 * q < k; ++q) t.push_back({cv(j, q), cv(j, k - q)});
+            std::string acc = fma_chain(t, "r" + std::to_string(j) + "_" + ks, o);
+            std::string tot;
+            if (!acc.empty()) tot = acc + " + " + acc;
+            if (k % 2 == 0 && k >= 2) {
+                const std::string m = cv(j, k / 2);
+                tot = tot.empty() ? (m + " * " + m) : ("__builtin_fma(" + m + ", " + m + ", " + tot + ")");
+            }
+            const std::string a = zero(A[0], k) ? "0.0" : val(A[0], k);
+            o << "    const double " << c << " = (" << a << (tot.empty() ? "" : " - (" + tot + ")") << ") * rs" << j
+              << ";\n";
+            break;
+        }
+        case HYX_OP_POW: {
+            // c^[k] = 1/(k a^[0]) sum_{i=0}^{k-1} (k alpha - i (alpha + 1)) a^[k-i] c^[i]
+            const double al = S.consts[S.aux[j]];
+            for (int q = 0; q < k; ++q) {
+                if (zero(A[0], k - q)) continue;
+                const double cf = ((double)k * al - (double)q * (al + 1.0)) / (double)k;
+                if (cf == 0.0) continue;
+                t.push_back({lit(cf), "(" + val(A[0], k - q) + " * " + cv(j, q) + ")"});
+            }
+            const std::string acc = fma_chain(t, "w" + std::to_string(j) + "_" + ks, o);
+            if (acc.empty())
+                set_zero(j, k);
+            else
+                o << "    const double " << c << " = " << acc << " * ra" << j << ";\n";
+            break;
+        }
+        case HYX_OP_EXP: {
+            // c^[k] = (1/k) sum_{i=1}^{k} i a^[i] c^[k-i]
+            for (int q = 1; q <= k; ++q)
+                if (!zero(A[0], q)) t.push_back({dval(A[0], q), cv(j, k - q)});
+            const std::string acc = fma_chain(t, "e" + std::to_string(j) + "_" + ks, o);
+            if (acc.empty())
+                set_zero(j, k);
+            else
+                o << "    const double " << c << " = " << acc << " * " << lit(1.0 / k) << ";\n";
+            break;
+        }
+        case HYX_OP_LOG: {
+            // c^[k] = (a^[k] - (1/k) sum_{i=1}^{k-1} i c^[i] a^[k-i]) / a^[0]
+            for (int q = 1; q < k; ++q)
+                if (!zero(A[0], k - q)) t.push_back({lit((double)q / k), "(" + cv(j, q) + " * " + val(A[0], k - q) + ")"});
+            const std::string acc = fma_chain(t, "l" + std::to_string(j) + "_" + ks, o);
+            const std::string a = zero(A[0], k) ? "0.0" : val(A[0], k);
+            o << "    const double " << c << " = (" << a << (acc.empty() ? "" : " - " + acc) << ") * ra" << j << ";\n";
+            break;
+        }
+        case HYX_OP_SIN:
+        case HYX_OP_COS: {
+            // s^[k] = (1/k) sum i a^[i] c^[k-i];  c^[k] = -(1/k) sum i a^[i] s^[k-i]
+            const uint32_t p = S.aux[j];
+            for (int q = 1; q <= k; ++q)
+                if (!zero(A[0], q)) t.push_back({dval(A[0], q), cv(p, k - q)});
+            const std::string acc = fma_chain(t, "g" + std::to_string(j) + "_" + ks, o);
+            if (acc.empty())
+                set_zero(j, k);
+            else
+                o << "    const double " << c << " = " << acc << " * " << lit(S.op[j] == HYX_OP_SIN ? 1.0 / k : -1.0 / k)
+                  << ";\n";
+            break;
+        }
+        case HYX_OP_TIME:
+            if (k == 1)
+                o << "    const double " << c << " = 1.0;\n";
+            else
+                set_zero(j, k);
+            break;
+        }
+    }
+
+    std::string run()
+    {
+        const uint32_t n = S.n;
+        o << "#define HYX_N " << n << "\n";
+        o << "#define HYX_NPAR " << std::max<uint32_t>(S.npar, 1) << "\n";
+        o << "#define HYX_P " << P << "\n";
+        o << "typedef unsigned long long hyx_u64;\n";
+        o << "template <bool WTC>\n"
+             "__device__ __forceinline__ void hyx_taylor_step(double (&x)[HYX_N], const double (&pv)[HYX_NPAR], "
+             "double t0, double max_dt, double &h_out, bool &nf_out, double *__restrict__ tc, hyx_u64 tcs, "
+             "double *__restrict__ tape, hyx_u64 tps)\n{\n";
+        o << "    (void)pv; (void)t0; (void)tape; (void)tps; (void)tc; (void)tcs;\n";
+        // order 0
+        for (uint32_t i = 0; i < n; ++i) o << "    const double " << cv(i, 0) << " = x[" << i << "];\n";
+        for (uint32_t j = n; j < S.nnodes; ++j) order0(j);
+        for (int k = 1; k <= P; ++k) {
+            o << "    // order " << k << "\n";
+            for (uint32_t i = 0; i < n; ++i) {
+                const uint32_t r = S.rhs[i];
+                if (zero(r, k - 1))
+                    set_zero(i, k);
+                else
+                    o << "    const double " << cv(i, k) << " = " << val(r, k - 1) << " * " << lit(1.0 / k) << ";\n";
+            }
+            if (k < P)
+                for (uint32_t j = n; j < S.nnodes; ++j) orderk(j, k);
+        }
+        // Taylor coefficients of the state, when asked for.
+        o << "    if constexpr (WTC) {\n";
+        for (uint32_t i = 0; i < n; ++i)
+            for (int k = 0; k <= P; ++k)
+                o << "        tc[(hyx_u64)" << (i * (P + 1) + k) << " * tcs] = " << cv(i, k) << ";\n";
+        o << "    }\n";
+        // Step size: infinity norms of the state and of the last two orders.
+        o << "    double mx = 0.0, mo = 0.0, mom1 = 0.0, ab;\n";
+        for (uint32_t i = 0; i < n; ++i) {
+            o << "    ab = fabs(" << cv(i, 0) << "); mx = (ab > mx) ? ab : mx;\n";
+            o << "    ab = fabs(" << cv(i, P) << "); mo = (ab > mo) ? ab : mo;\n";
+            o << "    ab = fabs(" << cv(i, P - 1) << "); mom1 = (ab > mom1) ? ab : mom1;\n";
+        }
+        o << "    const double num = (mx <= 1.0) ? 1.0 : mx;\n";
+        o << "    const double rho_o = pow(num / mo, " << lit(1.0 / P) << ");\n";
+        o << "    const double rho_om1 = pow(num / mom1, " << lit(1.0 / (P - 1)) << ");\n";
+        o << "    const double rho_m = (rho_om1 < rho_o) ? rho_om1 : rho_o;\n";
+        const double e1 = std::exp(1.0);
+        const double rhofac = std::exp(-7.0 / 10.0 / (double)(P - 1)) / (e1 * e1);
+        o << "    double h = rho_m * " << lit(rhofac) << ";\n";
+        o << "    const double lim = fabs(max_dt);\n";
+        o << "    h = (h < lim) ? h : lim;\n";
+        o << "    if (max_dt < 0.0) h = -h;\n";
+        o << "    bool nf = false;\n";
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!high_acc) {
+                o << "    { double r = " << cv(i, P) << ";\n";
+                for (int k = P - 1; k >= 0; --k) o << "      r = __builtin_fma(r, h, " << cv(i, k) << ");\n";
+                o << "      x[" << i << "] = r; nf = nf || !isfinite(r); }\n";
+            } else {
+                // Kahan-compensated sum of x^[k] h^k, k = 0..p.
+                o << "    { double sm = " << cv(i, 0) << ", cp = 0.0, hk = 1.0;\n";
+                for (int k = 1; k <= P; ++k)
+                    o << "      { hk *= h; const double y = " << cv(i, k)
+                      << " * hk - cp; const double tt = sm + y; cp = (tt - sm) - y; sm = tt; }\n";
+                o << "      x[" << i << "] = sm; nf = nf || !isfinite(sm); }\n";
+            }
+        }
+        o << "    h_out = h;\n    nf_out = nf;\n}\n";
+        return o.str();
+    }
+};
+
+
+// Compact form: a runtime loop over the orders with the node list unrolled
+// inside it and the Taylor tape in device memory, laid out
+// tape[(node * (P + 1) + k) * tps + lane] so each access of a wave is one
+// contiguous 512-byte row. Code size grows with the node count only, not with
+// node count x order^2, which keeps hipRTC compile times to seconds for
+// systems with hundreds of nodes.
+struct CompactGen {
+    const Sys &S;
+    const int P;
+    const bool high_acc;
+    std::ostringstream o;
+
+    CompactGen(const Sys &s, int p, bool ha) : S(s), P(p), high_acc(ha) {}
+
+    static std::string T(uint32_t node, const std::string &k)
+    {
+        return "T(" + std::to_string(node) + ", " + k + ")";
+    }
+
+    bool is_node(uint32_t w) const { return HYX_OPERAND_KIND(w) == HYX_KIND_NODE; }
+    uint32_t idx(uint32_t w) const { return HYX_OPERAND_INDEX(w); }
+
+    // Order-0 value of an operand.
+    std::string v0(uint32_t w) const
+    {
+        const uint32_t kind = HYX_OPERAND_KIND(w), i = HYX_OPERAND_INDEX(w);
+        if (kind == HYX_KIND_NODE) return T(i, "0");
+        if (kind == HYX_KIND_CONST) return lit(S.consts[i]);
+        return "pv[" + std::to_string(i) + "]";
+    }
+
+    // Order-k (k >= 1, runtime) value of an operand: 0 for numbers/parameters.
+    std::string vk(uint32_t w, const std::string &k) const
+    {
+        if (!is_node(w)) return "0.0";
+        return T(idx(w), k);
+    }
+
+    bool scalar(uint32_t w) const { return !is_node(w) || S.deg[idx(w)] == 0; }
+
+    void order0(uint32_t j)
+    {
+        const auto &A = S.args[j];
+        const std::string c = T(j, "0");
+        switch (S.op[j]) {
+        case HYX_OP_ADD: o << "    " << c << " = " << v0(A[0]) << " + " << v0(A[1]) << ";\n"; break;
+        case HYX_OP_SUB: o << "    " << c << " = " << v0(A[0]) << " - " << v0(A[1]) << ";\n"; break;
+        case HYX_OP_NEG: o << "    " << c << " = -" << v0(A[0]) << ";\n"; break;
+        case HYX_OP_SUM:
+            o << "    " << c << " = " << v0(A[0]);
+            for (size_t q = 1; q < A.size(); ++q) o << " + " << v0(A[q]);
+            o << ";\n";
+            break;
+        case HYX_OP_MUL: o << "    " << c << " = " << v0(A[0]) << " * " << v0(A[1]) << ";\n"; break;
+        case HYX_OP_DIV:
+            o << "    " << c << " = " << v0(A[0]) << " / " << v0(A[1]) << ";\n";
+            o << "    const double rb" << j << " = 1.0 / " << v0(A[1]) << ";\n";
+            break;
+        case HYX_OP_SQUARE: o << "    { const double a = " << v0(A[0]) << "; " << c << " = a * a; }\n"; break;
+        case HYX_OP_SQRT:
+            o << "    " << c << " = sqrt(" << v0(A[0]) << ");\n";
+            o << "    const double rs" << j << " = 0.5 / " << c << ";\n";
+            break;
+        case HYX_OP_POW:
+            o << "    " << c << " = pow(" << v0(A[0]) << ", " << lit(S.consts[S.aux[j]]) << ");\n";
+            o << "    const double ra" << j << " = 1.0 / " << v0(A[0]) << ";\n";
+            break;
+        case HYX_OP_EXP: o << "    " << c << " = exp(" << v0(A[0]) << ");\n"; break;
+        case HYX_OP_LOG:
+            o << "    " << c << " = log(" << v0(A[0]) << ");\n";
+            o << "    const double ra" << j << " = 1.0 / " << v0(A[0]) << ";\n";
+            break;
+        case HYX_OP_SIN:
+        case HYX_OP_COS: {
+            const uint32_t p = S.aux[j];
+            if (p > j) {
+                const uint32_t si = S.op[j] == HYX_OP_SIN ? j : p, co = S.op[j] == HYX_OP_SIN ? p : j;
+                o << "    { double s_, c_; sincos(" << v0(A[0]) << ", &s_, &c_); " << T(si, "0") << " = s_; "
+                  << T(co, "0") << " = c_; }\n";
+            }
+            break;
+        }
+        case HYX_OP_TIME: o << "    " << c << " = t0;\n"; break;
+        }
+    }
+
+    void orderk(uint32_t j)
+    {
+        const auto &A = S.args[j];
+        const std::string c = T(j, "k");
+        const int d = S.deg[j];
+        if (d == 0) {
+            o << "        " << c << " = 0.0;\n";
+            return;
+        }
+        std::string guard_open, guard_close;
+        if (d < INF_DEG) {
+            o << "        if (k > " << d << ") { " << c << " = 0.0; } else {\n";
+            guard_close = "        }\n";
+        }
+        switch (S.op[j]) {
+        case HYX_OP_ADD:
+        case HYX_OP_SUB:
+        case HYX_OP_NEG:
+        case HYX_OP_SUM: {
+            std::string e;
+            bool first = true;
+            for (size_t q = 0; q < A.size(); ++q) {
+                if (!is_node(A[q]) || S.deg[idx(A[q])] == 0) continue;
+                const bool neg = (S.op[j] == HYX_OP_NEG) || (S.op[j] == HYX_OP_SUB && q == 1);
+                e += first ? (neg ? "-" : "") + vk(A[q], "k") : (neg ? " - " : " + ") + vk(A[q], "k");
+                first = false;
+            }
+            o << "        " << c << " = " << (first ? "0.0" : e) << ";\n";
+            break;
+        }
+        case HYX_OP_MUL: {
+            const bool s0 = scalar(A[0]), s1 = scalar(A[1]);
+            if (s0 && s1)
+                o << "        " << c << " = 0.0;\n";
+            else if (s0)
+                o << "        " << c << " = " << vk(A[1], "k") << " * " << v0(A[0]) << ";\n";
+            else if (s1)
+                o << "        " << c << " = " << vk(A[0], "k") << " * " << v0(A[1]) << ";\n";
+            else
+                o << "        { double acc = 0.0; for (int q = 0; q <= k; ++q) acc = __builtin_fma(" << T(idx(A[0]), "q")
+                  << ", " << T(idx(A[1]), "k - q") << ", acc); " << c << " = acc; }\n";
+            break;
+        }
+        case HYX_OP_DIV: {
+            const std::string a = scalar(A[0]) ? "0.0" : vk(A[0], "k");
+            if (scalar(A[1]))
+                o << "        " << c << " = " << a << " * rb" << j << ";\n";
+            else
+                o << "        { double acc = 0.0; for (int q = 1; q <= k; ++q) acc = __builtin_fma(" << T(idx(A[1]), "q")
+                  << ", " << T(j, "k - q") << ", acc); " << c << " = (" << a << " - acc) * rb" << j << "; }\n";
+            break;
+        }
+        case HYX_OP_SQUARE: {
+            const uint32_t a = idx(A[0]);
+            o << "        { double acc = 0.0; for (int q = 0; 2 * q < k; ++q) acc = __builtin_fma(" << T(a, "q") << ", "
+              << T(a, "k - q") << ", acc); acc = acc + acc; if ((k & 1) == 0) { const double m = " << T(a, "k >> 1")
+              << "; acc = __builtin_fma(m, m, acc); } " << c << " = acc; }\n";
+            break;
+        }
+        case HYX_OP_SQRT: {
+            const std::string a = scalar(A[0]) ? "0.0" : vk(A[0], "k");
+            o << "        { double acc = 0.0; for (int q = 1; 2 * q < k; ++q) acc = __builtin_fma(" << T(j, "q") << ", "
+              << T(j, "k - q") << ", acc); acc = acc + acc; if ((k & 1) == 0) { const double m = " << T(j, "k >> 1")
+              << "; acc = __builtin_fma(m, m, acc); } " << c << " = (" << a << " - acc) * rs" << j << "; }\n";
+            break;
+        }
+        case HYX_OP_POW: {
+            const double al = S.consts[S.aux[j]];
+            const uint32_t a = idx(A[0]);
+            o << "        { double acc = 0.0; const double ka = (double)k * " << lit(al)
+              << "; for (int q = 0; q < k; ++q) acc = __builtin_fma(__builtin_fma(-(double)q, " << lit(al + 1.0)
+              << ", ka) * " << T(a, "k - q") << ", " << T(j, "q") << ", acc); " << c << " = acc * (ra" << j
+              << " * hyx_inv[k]); }\n";
+            break;
+        }
+        case HYX_OP_EXP: {
+            const uint32_t a = idx(A[0]);
+            o << "        { double acc = 0.0; for (int q = 1; q <= k; ++q) acc = __builtin_fma((double)q * " << T(a, "q")
+              << ", " << T(j, "k - q") << ", acc); " << c << " = acc * hyx_inv[k]; }\n";
+            break;
+        }
+        case HYX_OP_LOG: {
+            const uint32_t a = idx(A[0]);
+            o << "        { double acc = 0.0; for (int q = 1; q < k; ++q) acc = __builtin_fma((double)q * " << T(j, "q")
+              << ", " << T(a, "k - q") << ", acc); " << c << " = (" << T(a, "k") << " - acc * hyx_inv[k]) * ra" << j
+              << "; }\n";
+            break;
+        }
+        case HYX_OP_SIN:
+        case HYX_OP_COS: {
+            const uint32_t a = idx(A[0]), p = S.aux[j];
+            o << "        { double acc = 0.0; for (int q = 1; q <= k; ++q) acc = __builtin_fma((double)q * " << T(a, "q")
+              << ", " << T(p, "k - q") << ", acc); " << c << " = acc * " << (S.op[j] == HYX_OP_SIN ? "" : "-")
+              << "hyx_inv[k]; }\n";
+            break;
+        }
+        case HYX_OP_TIME: o << "        " << c << " = (k == 1) ? 1.0 : 0.0;\n"; break;
+        }
+        o << guard_close;
+    }
+
+    std::string run()
+    {
+        const uint32_t n = S.n;
+        o << "#define HYX_N " << n << "\n";
+        o << "#define HYX_NPAR " << std::max<uint32_t>(S.npar, 1) << "\n";
+        o << "#define HYX_P " << P << "\n";
+        o << "#define HYX_COMPACT 1\n";
+        o << "#define HYX_TAPE_ROWS " << (S.nnodes * (uint64_t)(P + 1)) << "ull\n";
+        o << "typedef unsigned long long hyx_u64;\n";
+        o << "__constant__ double hyx_inv[" << (P + 1) << "] = {0.0";
+        for (int k = 1; k <= P; ++k) o << ", " << lit(1.0 / k);
+        o << "};\n";
+        o << "#define T(node, kk) tape[((hyx_u64)(node) * (HYX_P + 1) + (hyx_u64)(kk)) * tps]\n";
+        o << "template <bool WTC>\n"
+             "__device__ __forceinline__ void hyx_taylor_step(double (&x)[HYX_N], const double (&pv)[HYX_NPAR], "
+             "double t0, double max_dt, double &h_out, bool &nf_out, double *__restrict__ tc, hyx_u64 tcs, "
+             "double *__restrict__ tape, hyx_u64 tps)\n{\n";
+        o << "    (void)pv; (void)t0; (void)tc; (void)tcs;\n";
+        for (uint32_t i = 0; i < n; ++i) o << "    " << T(i, "0") << " = x[" << i << "];\n";
+        for (uint32_t j = n; j < S.nnodes; ++j) order0(j);
+        o << "    for (int k = 1; k <= HYX_P; ++k) {\n";
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t r = S.rhs[i];
+            if (HYX_OPERAND_KIND(r) == HYX_KIND_NODE)
+                o << "        " << T(i, "k") << " = " << T(idx(r), "k - 1") << " * hyx_inv[k];\n";
+            else
+                o << "        " << T(i, "k") << " = (k == 1) ? " << v0(r) << " : 0.0;\n";
+        }
+        o << "        if (k == HYX_P) break;\n";
+        for (uint32_t j = n; j < S.nnodes; ++j) orderk(j);
+        o << "    }\n";
+        o << "    if constexpr (WTC) {\n";
+        o << "        for (int i = 0; i < HYX_N; ++i)\n"
+             "            for (int k = 0; k <= HYX_P; ++k) tc[(hyx_u64)(i * (HYX_P + 1) + k) * tcs] = T(i, k);\n";
+        o << "    }\n";
+        o << "    double mx = 0.0, mo = 0.0, mom1 = 0.0;\n";
+        o << "    for (int i = 0; i < HYX_N; ++i) {\n"
+             "        double ab = fabs(T(i, 0)); mx = (ab > mx) ? ab : mx;\n"
+             "        ab = fabs(T(i, HYX_P)); mo = (ab > mo) ? ab : mo;\n"
+             "        ab = fabs(T(i, HYX_P - 1)); mom1 = (ab > mom1) ? ab : mom1;\n"
+             "    }\n";
+        o << "    const double num = (mx <= 1.0) ? 1.0 : mx;\n";
+        o << "    const double rho_o = pow(num / mo, " << lit(1.0 / P) << ");\n";
+        o << "    const double rho_om1 = pow(num / mom1, " << lit(1.0 / (P - 1)) << ");\n";
+        o << "    const double rho_m = (rho_om1 < rho_o) ? rho_om1 : rho_o;\n";
+        const double e1 = std::exp(1.0);
+        const double rhofac = std::exp(-7.0 / 10.0 / (double)(P - 1)) / (e1 * e1);
+        o << "    double h = rho_m * " << lit(rhofac) << ";\n";
+        o << "    const double lim = fabs(max_dt);\n";
+        o << "    h = (h < lim) ? h : lim;\n";
+        o << "    if (max_dt < 0.0) h = -h;\n";
+        o << "    bool nf = false;\n";
+        o << "#pragma unroll\n    for (int i = 0; i < HYX_N; ++i) {\n";
+        if (!high_acc) {
+            o << "        double r = T(i, HYX_P);\n"
+                 "        for (int k = HYX_P - 1; k >= 0; --k) r = __builtin_fma(r, h, T(i, k));\n";
+        } else {
+            o << "        double r = T(i, 0), cp = 0.0, hk = 1.0;\n"
+                 "        for (int k = 1; k <= HYX_P; ++k) { hk *= h; const double y = T(i, k) * hk - cp; "
+                 "const double tt = r + y; cp = (tt - r) - y; r = tt; }\n";
+        }
+        o << "        x[i] = r; nf = nf || !isfinite(r);\n    }\n";
+        o << "    h_out = h;\n    nf_out = nf;\n}\n#undef T\n";
+        return o.str();
+    }
+};
+
+} // namespace
+
+int history_nodes(const Sys &S)
+{
+    // Nodes whose coefficient history later orders read: state variables, the
+    // node operands of convolutions and the self-referencing recurrences.
+    std::vector<char> h(S.nnodes, 0);
+    for (uint32_t i = 0; i < S.n; ++i) h[i] = 1;
+    for (uint32_t j = S.n; j < S.nnodes; ++j) {
+        if (S.deg[j] == 0) continue;
+        const uint32_t op = S.op[j];
+        const bool conv = op == HYX_OP_MUL || op == HYX_OP_DIV || op == HYX_OP_SQUARE || op == HYX_OP_SQRT
+                          || op == HYX_OP_POW || op == HYX_OP_EXP || op == HYX_OP_LOG || op == HYX_OP_SIN
+                          || op == HYX_OP_COS;
+        if (!conv) continue;
+        int nn = 0;
+        for (auto a : S.args[j])
+            if (HYX_OPERAND_KIND(a) == HYX_KIND_NODE && S.deg[HYX_OPERAND_INDEX(a)] > 0) ++nn;
+        if (op == HYX_OP_MUL && nn < 2) continue;
+        if (op == HYX_OP_DIV && !(HYX_OPERAND_KIND(S.args[j][1]) == HYX_KIND_NODE && S.deg[HYX_OPERAND_INDEX(S.args[j][1])] > 0)) continue;
+        for (auto a : S.args[j])
+            if (HYX_OPERAND_KIND(a) == HYX_KIND_NODE) h[HYX_OPERAND_INDEX(a)] = 1;
+        if (op != HYX_OP_MUL && op != HYX_OP_SQUARE) h[j] = 1;
+    }
+    int c = 0;
+    for (auto v : h) c += v;
+    return c;
+}
+
+bool choose_compact(const Sys &S, int order)
+{
+    // Unrolled code keeps every history in registers (256 doubles per lane, spilling a little beyond).
+    return (long)history_nodes(S) * (order + 1) > 320;
+}
+
+std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compact)
+{
+    if (order < 2) throw std::invalid_argument("the Taylor order must be at least 2");
+    if (compact) {
+        CompactGen g(S, order, high_acc);
+        return g.run();
+    }
+    Gen g(S, order, high_acc);
+    return g.run();
+}
+
+} // namespace hyx

@@ -1,0 +1,313 @@
+// Hand-written CDNA4 kernels of the batch/ensemble Taylor integrator.
+//
+// This file is compiled at run time with hipRTC, appended to a per-system
+// prelude that the runtime generates from the op-list (hyx_codegen.cpp). The
+// prelude defines:
+//   HYX_N, HYX_NPAR (>= 1), HYX_P        state size, parameter count, order
+//   hyx_taylor_step<WTC>(x, pv, t0, max_dt, h, nf, tc, tcs, tape, tps)
+//       one adaptive Taylor step of one instance held by the calling lane:
+//       coefficient recurrences, step size from the last two orders, Horner
+//       update of x in place; WTC writes the state's Taylor coefficients.
+//
+// Layout in HBM: every per-instance array is structure-of-arrays with the
+// instance index fastest (state[i * ld + inst]), so the 64 lanes of a wave
+// touch 512 contiguous bytes per access.
+//
+// Semantics follow heyoka's batch integrator (heyoka/expose_batch_integrators.cpp):
+//   - step(max_dt): |h| clamped to |max_dt|, sign of max_dt; outcome
+//     time_limit when h == max_dt, err_nf_state on a non-finite state/time
+//     (:233-242, doc/notebooks/Batch mode overview.ipynb:242-245,320);
+//   - propagate_until(t): steps clamped to the remaining time and to
+//     max_delta_t, double-length time, per-instance (outcome, min_h, max_h,
+//     nsteps) with min_h/max_h over unclamped steps only (:256-314, :393).
+// Each instance is independent: one instance's error never stops another.
+
+typedef unsigned long long hyx_u64;
+typedef long long hyx_i64;
+
+#define HYX_SUCCESS (-4294967297LL)
+#define HYX_STEP_LIMIT (-4294967298LL)
+#define HYX_TIME_LIMIT (-4294967299LL)
+#define HYX_ERR_NF_STATE (-4294967300LL)
+
+// Kernel argument block. The host fills the identical struct (hyx_runtime.cpp).
+struct hyx_kargs {
+    double *state;          // [n][ld]
+    const double *pars;     // [npar][ld]
+    double *t_hi;           // [N]
+    double *t_lo;           // [N]
+    const double *tf_hi;    // [N] final time (propagate)
+    const double *tf_lo;    // [N]
+    const double *max_dt;   // [N] step: signed clamp; propagate: max_delta_t > 0
+    double *h_out;          // [N] step: h taken
+    hyx_i64 *outcome;       // [N]
+    double *min_h;          // [N]
+    double *max_h;          // [N]
+    hyx_u64 *nsteps;        // [N]
+    hyx_u64 *iters;         // [N] plain form: iterations taken so far
+    unsigned *status;       // [N] plain form: 0 active forward, 2 active backward, 1 done
+    double *tc;             // [n][P+1][ld] Taylor coefficients (step with write_tc)
+    double *tape;           // per-lane scratch tape for compact code (may be null)
+    hyx_u64 *work_counter;  // persistent form: next queue position
+    unsigned *active_count; // plain form: active instances after this launch
+    const unsigned *queue;  // persistent form: instance order (null = identity)
+    hyx_u64 N;
+    hyx_u64 ld;
+    hyx_u64 max_steps;      // 0 = unlimited
+    hyx_u64 tape_stride;    // lanes per tape row
+    hyx_u64 iter_budget;    // persistent form: iterations per instance per launch
+    unsigned *unfinished;   // persistent form: instances suspended by the budget
+    unsigned refill_min;    // persistent form: refill when this many lanes idle
+    unsigned pad_;
+};
+
+// ---- double-length time -----------------------------------------------------------
+// Knuth's error-free sum (TAOCP vol. 2, 4.2.2).
+__device__ __forceinline__ void hyx_two_sum(double a, double b, double &s, double &e)
+{
+    double x = a + b;
+    double z = x - a;
+    e = (a - (x - z)) + (b - z);
+    s = x;
+}
+
+// Dekker's fast error-free sum, |a| >= |b|.
+__device__ __forceinline__ void hyx_fast_two_sum(double a, double b, double &s, double &e)
+{
+    double x = a + b;
+    e = b - (x - a);
+    s = x;
+}
+
+// (hi, lo) += (bh, bl), renormalised.
+__device__ __forceinline__ void hyx_dfl_add(double &hi, double &lo, double bh, double bl)
+{
+    double s, e, t, f;
+    hyx_two_sum(hi, bh, s, e);
+    hyx_two_sum(lo, bl, t, f);
+    e += t;
+    hyx_fast_two_sum(s, e, s, e);
+    e += f;
+    hyx_fast_two_sum(s, e, hi, lo);
+}
+
+// hi part of (ah, al) - (bh, bl).
+__device__ __forceinline__ double hyx_dfl_sub_hi(double ah, double al, double bh, double bl)
+{
+    double hi = ah, lo = al;
+    hyx_dfl_add(hi, lo, -bh, -bl);
+    return hi;
+}
+
+// ---- one instance's state held in registers ------------------------------------------
+struct hyx_inst {
+    double x[HYX_N];
+    double pv[HYX_NPAR];
+    double thi, tlo, tfhi, tflo, mdt, minh, maxh;
+    hyx_u64 iter, nst;
+    int tdir;
+};
+
+__device__ __forceinline__ void hyx_load(hyx_inst &s, const hyx_kargs &a, hyx_u64 i)
+{
+#pragma unroll
+    for (int q = 0; q < HYX_N; ++q) s.x[q] = a.state[(hyx_u64)q * a.ld + i];
+#pragma unroll
+    for (int q = 0; q < HYX_NPAR; ++q) s.pv[q] = a.pars ? a.pars[(hyx_u64)q * a.ld + i] : 0.0;
+    s.thi = a.t_hi[i];
+    s.tlo = a.t_lo[i];
+}
+
+__device__ __forceinline__ void hyx_store_state(const hyx_inst &s, const hyx_kargs &a, hyx_u64 i)
+{
+#pragma unroll
+    for (int q = 0; q < HYX_N; ++q) a.state[(hyx_u64)q * a.ld + i] = s.x[q];
+    a.t_hi[i] = s.thi;
+    a.t_lo[i] = s.tlo;
+}
+
+// One iteration of propagate_until for one instance. Returns true when the
+// instance is done; *res then holds its outcome.
+__device__ __forceinline__ bool hyx_prop_iter(hyx_inst &s, const hyx_kargs &a, hyx_u64 lane_tape, hyx_i64 *res)
+{
+    // Clamp: the remaining time, or max_delta_t if smaller.
+    const double rem = hyx_dfl_sub_hi(s.tfhi, s.tflo, s.thi, s.tlo);
+    const double lim = s.tdir ? ((s.mdt < rem) ? s.mdt : rem) : ((-s.mdt > rem) ? -s.mdt : rem);
+    double h;
+    bool nf;
+    hyx_taylor_step<false>(s.x, s.pv, s.thi, lim, h, nf, nullptr, 0, a.tape ? a.tape + lane_tape : nullptr,
+                           a.tape_stride);
+    hyx_dfl_add(s.thi, s.tlo, h, 0.0);
+    nf = nf || !isfinite(s.thi);
+    s.iter += 1;
+    s.nst += (h != 0.0) ? 1 : 0;
+    const hyx_i64 oc = nf ? HYX_ERR_NF_STATE : ((h == lim) ? HYX_TIME_LIMIT : HYX_SUCCESS);
+    if (oc == HYX_SUCCESS) {
+        const double ah = fabs(h);
+        s.minh = (ah < s.minh) ? ah : s.minh;
+        s.maxh = (ah > s.maxh) ? ah : s.maxh;
+    }
+    if (oc == HYX_ERR_NF_STATE) {
+        *res = HYX_ERR_NF_STATE;
+        return true;
+    }
+    if (oc == HYX_TIME_LIMIT && h == rem) {
+        *res = HYX_TIME_LIMIT;
+        return true;
+    }
+    if (a.max_steps != 0 && s.iter >= a.max_steps) {
+        *res = HYX_STEP_LIMIT;
+        return true;
+    }
+    return false;
+}
+
+// Resume an instance of a propagation set up by hyx_iter_init_k.
+__device__ __forceinline__ void hyx_resume(hyx_inst &s, const hyx_kargs &a, hyx_u64 i, unsigned st)
+{
+    hyx_load(s, a, i);
+    s.tfhi = a.tf_hi[i];
+    s.tflo = a.tf_lo[i];
+    s.mdt = a.max_dt[i];
+    s.tdir = (st == 0u);
+    s.minh = a.min_h[i];
+    s.maxh = a.max_h[i];
+    s.iter = a.iters[i];
+    s.nst = a.nsteps[i];
+}
+
+// Write an instance back; done instances also record their outcome.
+__device__ __forceinline__ void hyx_suspend(const hyx_inst &s, const hyx_kargs &a, hyx_u64 i, bool done, hyx_i64 res)
+{
+    hyx_store_state(s, a, i);
+    a.min_h[i] = s.minh;
+    a.max_h[i] = s.maxh;
+    a.iters[i] = s.iter;
+    a.nsteps[i] = s.nst;
+    if (done) {
+        a.status[i] = 1u;
+        a.outcome[i] = res;
+    }
+}
+
+// ---- step(): one adaptive step per instance ----------------------------------------------
+template <bool WTC>
+__device__ __forceinline__ void hyx_step_body(const hyx_kargs &a)
+{
+    const hyx_u64 i = (hyx_u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.N) return;
+    hyx_inst s;
+    hyx_load(s, a, i);
+    const double mdt = a.max_dt[i];
+    double h;
+    bool nf;
+    hyx_taylor_step<WTC>(s.x, s.pv, s.thi, mdt, h, nf, WTC ? a.tc + i : nullptr, a.ld, a.tape ? a.tape + i : nullptr,
+                         a.tape_stride);
+    hyx_dfl_add(s.thi, s.tlo, h, 0.0);
+    nf = nf || !isfinite(s.thi);
+    hyx_store_state(s, a, i);
+    a.h_out[i] = h;
+    a.outcome[i] = nf ? HYX_ERR_NF_STATE : ((h == mdt) ? HYX_TIME_LIMIT : HYX_SUCCESS);
+}
+
+extern "C" __global__ void __launch_bounds__(64) hyx_step_k(hyx_kargs a) { hyx_step_body<false>(a); }
+
+extern "C" __global__ void __launch_bounds__(64) hyx_step_tc_k(hyx_kargs a) { hyx_step_body<true>(a); }
+
+// ---- plain form: one propagate iteration per launch ----------------------------------------
+extern "C" __global__ void __launch_bounds__(64) hyx_iter_init_k(hyx_kargs a)
+{
+    const hyx_u64 i = (hyx_u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.N) return;
+    const double rem = hyx_dfl_sub_hi(a.tf_hi[i], a.tf_lo[i], a.t_hi[i], a.t_lo[i]);
+    a.status[i] = (rem >= 0.0) ? 0u : 2u;
+    a.iters[i] = 0;
+    a.nsteps[i] = 0;
+    a.min_h[i] = __builtin_inf();
+    a.max_h[i] = 0.0;
+    a.outcome[i] = HYX_TIME_LIMIT;
+}
+
+extern "C" __global__ void __launch_bounds__(64) hyx_iter_k(hyx_kargs a)
+{
+    const hyx_u64 i = (hyx_u64)blockIdx.x * blockDim.x + threadIdx.x;
+    bool still = false;
+    if (i < a.N) {
+        const unsigned st = a.status[i];
+        if (st != 1u) {
+            hyx_inst s;
+            hyx_resume(s, a, i, st);
+            hyx_i64 res = HYX_TIME_LIMIT;
+            const bool done = hyx_prop_iter(s, a, i, &res);
+            hyx_suspend(s, a, i, done, res);
+            still = !done;
+        }
+    }
+    const hyx_u64 m = __ballot(still);
+    if (threadIdx.x == 0 && m != 0) atomicAdd(a.active_count, (unsigned)__popcll(m));
+}
+
+// Clears the plain form's per-launch active counters (a kernel rather than a
+// memset so that it can sit inside a captured graph).
+extern "C" __global__ void hyx_clear_u32_k(unsigned *p, unsigned n)
+{
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+extern "C" __global__ void hyx_clear_u64_k(hyx_u64 *p, unsigned n)
+{
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// ---- persistent form: per-wavefront work stealing -----------------------------------------
+// Each wave keeps 64 instances in registers. Lanes whose instance is done
+// claim new queue positions with one atomic per wave (ballot + prefix count),
+// so a ragged ensemble keeps every lane busy until the queue drains. An
+// instance that reaches iter_budget iterations in this launch is written back
+// unfinished (counted in *unfinished) and the host launches again; so every
+// lane leaves the loop after a bounded number of iterations.
+extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
+{
+    const int lane = (int)__lane_id();
+    const hyx_u64 gl = (hyx_u64)blockIdx.x * 64 + lane;
+    hyx_inst s;
+    hyx_u64 inst = 0, budget = 0;
+    bool active = false;
+    bool exhausted = false;
+    for (;;) {
+        const hyx_u64 idle = __ballot(!active);
+        const unsigned nidle = (unsigned)__popcll(idle);
+        if (!exhausted && nidle != 0 && (nidle >= a.refill_min || nidle == 64u)) {
+            const int leader = __ffsll((long long)idle) - 1;
+            hyx_u64 base = 0;
+            if (lane == leader) base = atomicAdd(a.work_counter, (hyx_u64)nidle);
+            base = __shfl(base, leader);
+            if (base + nidle >= a.N) exhausted = true;
+            if (!active) {
+                const hyx_u64 pos = base + (hyx_u64)__popcll(idle & __lanemask_lt());
+                if (pos < a.N) {
+                    inst = a.queue ? (hyx_u64)a.queue[pos] : pos;
+                    const unsigned st = a.status[inst];
+                    if (st != 1u) {
+                        hyx_resume(s, a, inst, st);
+                        budget = 0;
+                        active = true;
+                    }
+                }
+            }
+        }
+        if (!__any(active)) break;
+        if (active) {
+            hyx_i64 res = HYX_TIME_LIMIT;
+            const bool done = hyx_prop_iter(s, a, gl, &res);
+            budget += 1;
+            if (done || budget >= a.iter_budget) {
+                hyx_suspend(s, a, inst, done, res);
+                if (!done) atomicAdd(a.unfinished, 1u);
+                active = false;
+            }
+        }
+    }
+}

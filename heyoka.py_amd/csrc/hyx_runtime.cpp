@@ -1,0 +1,754 @@
+// Host runtime of the MI355X Taylor integrator: the C-ABI of include/hyx.h.
+//
+// hyx_sys:   a parsed op-list, its generated HIP source (hyx_codegen.cpp +
+//            hyx_kernels.hip) and the hipRTC-compiled module. Code objects are
+//            cached in memory and on disk, keyed by a hash of source, options
+//            and target, as heyoka caches its LLVM JIT output
+//            (heyoka/core.cpp:148-149, 191-297).
+// hyx_batch: N device-resident instances (SoA, instance index fastest) plus the
+//            per-instance bookkeeping of propagate_until, on one HIP stream.
+
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <dlfcn.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/hyx.h"
+#include "hyx_internal.h"
+
+#include "build/hyx_kernels_embed.inc" // kKernelsSrc: the text of hyx_kernels.hip
+
+namespace
+{
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                                      \
+    do {                                                                                                               \
+        hipError_t e_ = (x);                                                                                           \
+        if (e_ != hipSuccess) return fail(HYX_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));                \
+    } while (0)
+
+// Kernel argument block; must match struct hyx_kargs in hyx_kernels.hip.
+struct KArgs {
+    double *state;
+    const double *pars;
+    double *t_hi;
+    double *t_lo;
+    const double *tf_hi;
+    const double *tf_lo;
+    const double *max_dt;
+    double *h_out;
+    int64_t *outcome;
+    double *min_h;
+    double *max_h;
+    uint64_t *nsteps;
+    uint64_t *iters;
+    unsigned *status;
+    double *tc;
+    double *tape;
+    uint64_t *work_counter;
+    unsigned *active_count;
+    const unsigned *queue;
+    uint64_t N;
+    uint64_t ld;
+    uint64_t max_steps;
+    uint64_t tape_stride;
+    uint64_t iter_budget;
+    unsigned *unfinished;
+    unsigned refill_min;
+    unsigned pad_;
+};
+
+// ---------------------------------------------------------------------------------------------
+// Code-object cache.
+
+std::mutex g_cache_mu;
+std::unordered_map<std::string, std::vector<char>> g_mem_cache;
+
+uint64_t fnv1a(const std::string &s, uint64_t h = 1469598103934665603ull)
+{
+    for (unsigned char c : s) {
+        h ^= c;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+std::string lib_dir()
+{
+    Dl_info info;
+    if (dladdr((void *)&fnv1a, &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        auto pos = p.rfind('/');
+        if (pos != std::string::npos) return p.substr(0, pos);
+    }
+    return ".";
+}
+
+std::string cache_dir()
+{
+    const char *e = std::getenv("HYX_CACHE_DIR");
+    if (e && *e) return e;
+    return lib_dir() + "/code_cache";
+}
+
+std::vector<std::string> rtc_options(const std::string &arch)
+{
+    return {"--offload-arch=" + arch, "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+}
+
+int get_code(const std::string &src, const std::string &arch, std::vector<char> &code)
+{
+    const auto opts = rtc_options(arch);
+    std::string keysrc = src;
+    for (auto &o : opts) keysrc += "\n" + o;
+    char kbuf[32];
+    std::snprintf(kbuf, sizeof(kbuf), "%016llx", (unsigned long long)fnv1a(keysrc));
+    const std::string key = arch + "-" + kbuf;
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto it = g_mem_cache.find(key);
+        if (it != g_mem_cache.end()) {
+            code = it->second;
+            return 0;
+        }
+    }
+    const std::string dir = cache_dir();
+    const std::string path = dir + "/" + key + ".co";
+    {
+        std::ifstream f(path, std::ios::binary);
+        if (f) {
+            code.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+            if (!code.empty()) {
+                std::lock_guard<std::mutex> lk(g_cache_mu);
+                g_mem_cache[key] = code;
+                return 0;
+            }
+        }
+    }
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "hyx_system.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+        return fail(HYX_ERTC, "hiprtcCreateProgram failed");
+    std::vector<const char *> copts;
+    for (auto &o : opts) copts.push_back(o.c_str());
+    const hiprtcResult rc = hiprtcCompileProgram(prog, (int)copts.size(), copts.data());
+    if (rc != HIPRTC_SUCCESS) {
+        size_t ls = 0;
+        hiprtcGetProgramLogSize(prog, &ls);
+        std::string log(ls + 1, '\0');
+        hiprtcGetProgramLog(prog, &log[0]);
+        hiprtcDestroyProgram(&prog);
+        return fail(HYX_ERTC, "hipRTC compilation failed: " + log.substr(0, 4000));
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    code.resize(cs);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        g_mem_cache[key] = code;
+    }
+    // Best-effort disk cache: write to a temporary name, then rename.
+    ::mkdir(dir.c_str(), 0755);
+    const std::string tmp = path + ".tmp" + std::to_string((long)::getpid());
+    {
+        std::ofstream f(tmp, std::ios::binary);
+        if (f) f.write(code.data(), (std::streamsize)code.size());
+    }
+    std::rename(tmp.c_str(), path.c_str());
+    return 0;
+}
+
+std::string device_arch(int dev)
+{
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) return "gfx950";
+    std::string a = p.gcnArchName;
+    auto pos = a.find(':');
+    return pos == std::string::npos ? a : a.substr(0, pos);
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------------------------
+
+struct hyx_sys {
+    hyx::Sys S;
+    uint32_t order = 0;
+    uint32_t flags = 0;
+    bool compact = false;
+    std::string src;
+    std::vector<char> code;
+    int device = -1;
+    hipModule_t mod = nullptr;
+    hipFunction_t k_step = nullptr, k_step_tc = nullptr, k_init = nullptr, k_iter = nullptr, k_persist = nullptr,
+                  k_clr32 = nullptr, k_clr64 = nullptr;
+    int vgprs = -1, scratch = -1;
+};
+
+struct hyx_batch {
+    hyx_sys *sys = nullptr;
+    uint64_t N = 0;
+    int device = 0;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    double *state = nullptr, *pars = nullptr, *thi = nullptr, *tlo = nullptr, *tfhi = nullptr, *tflo = nullptr,
+           *mdt = nullptr, *hout = nullptr, *minh = nullptr, *maxh = nullptr, *tc = nullptr;
+    int64_t *oc = nullptr;
+    uint64_t *nsteps = nullptr, *iters = nullptr, *work = nullptr;
+    unsigned *status = nullptr, *active = nullptr, *unfinished = nullptr, *queue = nullptr;
+    double *tape = nullptr;     // compact code: [nodes * (p + 1)][tape_stride]
+    uint64_t tape_stride = 0;
+    uint32_t waves_per_cu = 0, refill_min = 1;
+    uint64_t iter_budget = 1ull << 16;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    uint64_t last_launches = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+};
+
+namespace
+{
+
+const int kPlainGroup = 16; // plain form: launches between host checks
+
+int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint32_t n_consts, uint32_t order,
+              uint32_t flags, hyx_sys &s)
+{
+    if (!ops) return fail(HYX_EINVAL, "null op-list");
+    std::string err;
+    if (hyx::parse_oplist(ops, n_words, consts, n_consts, s.S, err) != 0) return fail(HYX_EINVAL, err);
+    if (order < 2 || order > 200) return fail(HYX_EINVAL, "the Taylor order must be in [2, 200]");
+    s.order = order;
+    s.flags = flags;
+    try {
+        s.compact = (flags & HYX_FLAG_FORCE_COMPACT)    ? true
+                    : (flags & HYX_FLAG_FORCE_UNROLLED) ? false
+                                                        : hyx::choose_compact(s.S, (int)order);
+        s.src = hyx::generate_prelude(s.S, (int)order, (flags & HYX_FLAG_HIGH_ACCURACY) != 0, s.compact) + "\n"
+                + kKernelsSrc;
+    } catch (const std::exception &e) {
+        return fail(HYX_EINVAL, e.what());
+    }
+    return 0;
+}
+
+KArgs make_args(hyx_batch *b)
+{
+    KArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.state = b->state;
+    a.pars = b->pars;
+    a.t_hi = b->thi;
+    a.t_lo = b->tlo;
+    a.tf_hi = b->tfhi;
+    a.tf_lo = b->tflo;
+    a.max_dt = b->mdt;
+    a.h_out = b->hout;
+    a.outcome = b->oc;
+    a.min_h = b->minh;
+    a.max_h = b->maxh;
+    a.nsteps = b->nsteps;
+    a.iters = b->iters;
+    a.status = b->status;
+    a.tc = b->tc;
+    a.tape = b->tape;
+    a.work_counter = b->work;
+    a.active_count = b->active;
+    a.queue = b->queue;
+    a.N = b->N;
+    a.ld = b->N;
+    a.max_steps = 0;
+    a.tape_stride = b->tape_stride;
+    a.iter_budget = b->iter_budget;
+    a.unfinished = b->unfinished;
+    a.refill_min = b->refill_min;
+    return a;
+}
+
+hipError_t launch(hipFunction_t f, uint64_t grid, hipStream_t st, KArgs *a)
+{
+    void *params[] = {a};
+    return hipModuleLaunchKernel(f, (unsigned)grid, 1, 1, 64, 1, 1, 0, st, params, nullptr);
+}
+
+hipError_t clear32(hyx_batch *b, unsigned *p, unsigned n)
+{
+    void *params[] = {&p, &n};
+    return hipModuleLaunchKernel(b->sys->k_clr32, (n + 63) / 64, 1, 1, 64, 1, 1, 0, b->stream, params, nullptr);
+}
+
+hipError_t clear64(hyx_batch *b, uint64_t *p, unsigned n)
+{
+    void *params[] = {&p, &n};
+    return hipModuleLaunchKernel(b->sys->k_clr64, (n + 63) / 64, 1, 1, 64, 1, 1, 0, b->stream, params, nullptr);
+}
+
+uint64_t blocks_for(uint64_t N) { return (N + 63) / 64; }
+
+// Compact code keeps its Taylor tape in device memory, one column per lane:
+// lanes are instances for the step / plain kernels and persistent-grid lanes
+// for the persistent kernel.
+int ensure_tape(hyx_batch *b, uint64_t lanes)
+{
+    if (!b->sys->compact) return 0;
+    const uint64_t want = std::max<uint64_t>(lanes, b->N);
+    if (b->tape && b->tape_stride >= want) return 0;
+    if (b->tape) HIPCHK(hipFree(b->tape));
+    b->tape = nullptr;
+    const uint64_t rows = (uint64_t)b->sys->S.nnodes * (b->sys->order + 1);
+    if (hipMalloc((void **)&b->tape, rows * want * 8) != hipSuccess) {
+        b->tape = nullptr;
+        b->tape_stride = 0;
+        return fail(HYX_ENOMEM, "hipMalloc of the Taylor tape failed (" + std::to_string(rows * want * 8) + " bytes)");
+    }
+    b->tape_stride = want;
+    if (b->graph_exec) {
+        hipGraphExecDestroy(b->graph_exec);
+        hipGraphDestroy(b->graph);
+        b->graph_exec = nullptr;
+        b->graph = nullptr;
+    }
+    return 0;
+}
+
+int upload(hyx_batch *b, double *dst, const double *src, uint64_t count, double fill)
+{
+    if (src) {
+        HIPCHK(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    } else {
+        std::vector<double> tmp(count, fill);
+        HIPCHK(hipMemcpyAsync(dst, tmp.data(), count * sizeof(double), hipMemcpyHostToDevice, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+    }
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *hyx_last_error(void) { return g_err.c_str(); }
+
+int hyx_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int hyx_sys_precompile(const uint32_t *ops, uint32_t n_words, const double *consts, uint32_t n_consts,
+                       uint32_t order, uint32_t flags, const char *arch)
+{
+    hyx_sys s;
+    int rc = build_sys(ops, n_words, consts, n_consts, order, flags, s);
+    if (rc) return rc;
+    std::vector<char> code;
+    return get_code(s.src, arch ? arch : "gfx950", code);
+}
+
+int hyx_sys_create(const uint32_t *ops, uint32_t n_words, const double *consts, uint32_t n_consts, uint32_t order,
+                   uint32_t flags, hyx_sys **out)
+{
+    if (!out) return fail(HYX_EINVAL, "null output pointer");
+    *out = nullptr;
+    auto *s = new hyx_sys;
+    int rc = build_sys(ops, n_words, consts, n_consts, order, flags, *s);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    if (flags & HYX_FLAG_SOURCE_ONLY) {
+        *out = s;
+        return 0;
+    }
+    if (flags & HYX_FLAG_COMPILE_ONLY) {
+        rc = get_code(s->src, "gfx950", s->code);
+        if (rc) {
+            delete s;
+            return rc;
+        }
+        *out = s;
+        return 0;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        delete s;
+        return fail(HYX_ENODEV, "no HIP device is available; the hyx GPU integrator needs an MI355X (gfx950)");
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        delete s;
+        return fail(HYX_EHIP, "hipGetDevice failed");
+    }
+    s->device = dev;
+    rc = get_code(s->src, device_arch(dev), s->code);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    hipError_t e = hipModuleLoadData(&s->mod, s->code.data());
+    if (e != hipSuccess) {
+        delete s;
+        return fail(HYX_EHIP, std::string("hipModuleLoadData: ") + hipGetErrorString(e));
+    }
+    struct {
+        hipFunction_t *f;
+        const char *name;
+    } fns[] = {{&s->k_step, "hyx_step_k"},       {&s->k_step_tc, "hyx_step_tc_k"}, {&s->k_init, "hyx_iter_init_k"},
+               {&s->k_iter, "hyx_iter_k"},       {&s->k_persist, "hyx_persist_k"}, {&s->k_clr32, "hyx_clear_u32_k"},
+               {&s->k_clr64, "hyx_clear_u64_k"}};
+    for (auto &f : fns) {
+        e = hipModuleGetFunction(f.f, s->mod, f.name);
+        if (e != hipSuccess) {
+            hipModuleUnload(s->mod);
+            delete s;
+            return fail(HYX_EHIP, std::string("hipModuleGetFunction(") + f.name + "): " + hipGetErrorString(e));
+        }
+    }
+    int v = -1, sc = -1;
+    if (hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_NUM_REGS, s->k_persist) == hipSuccess) s->vgprs = v;
+    if (hipFuncGetAttribute(&sc, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, s->k_persist) == hipSuccess) s->scratch = sc;
+    *out = s;
+    return 0;
+}
+
+int hyx_sys_order(const hyx_sys *s) { return s ? (int)s->order : HYX_EINVAL; }
+int hyx_sys_dim(const hyx_sys *s) { return s ? (int)s->S.n : HYX_EINVAL; }
+int hyx_sys_npar(const hyx_sys *s) { return s ? (int)s->S.npar : HYX_EINVAL; }
+int hyx_sys_is_compact(const hyx_sys *s) { return s ? (int)s->compact : HYX_EINVAL; }
+
+size_t hyx_sys_source(const hyx_sys *s, char *buf, size_t cap)
+{
+    if (!s) return 0;
+    if (buf && cap) {
+        const size_t n = std::min(cap - 1, s->src.size());
+        std::memcpy(buf, s->src.data(), n);
+        buf[n] = '\0';
+    }
+    return s->src.size();
+}
+
+int hyx_sys_kernel_info(const hyx_sys *s, int *vgprs, int *scratch_bytes)
+{
+    if (!s) return fail(HYX_EINVAL, "null system");
+    if (vgprs) *vgprs = s->vgprs;
+    if (scratch_bytes) *scratch_bytes = s->scratch;
+    return 0;
+}
+
+void hyx_sys_destroy(hyx_sys *s)
+{
+    if (!s) return;
+    if (s->mod) hipModuleUnload(s->mod);
+    delete s;
+}
+
+int hyx_batch_create(hyx_sys *s, uint64_t n_inst, hyx_batch **out)
+{
+    if (!s || !out) return fail(HYX_EINVAL, "null argument");
+    if (!s->mod) return fail(HYX_EINVAL, "the system was created compile-only");
+    if (n_inst == 0 || n_inst > (1ull << 32)) return fail(HYX_EINVAL, "the batch size must be in [1, 2^32]");
+    *out = nullptr;
+    auto *b = new hyx_batch;
+    b->sys = s;
+    b->N = n_inst;
+    b->device = s->device;
+    auto cleanup = [&](int code, const std::string &m) {
+        hyx_batch_destroy(b);
+        return fail(code, m);
+    };
+    if (hipSetDevice(b->device) != hipSuccess) return cleanup(HYX_EHIP, "hipSetDevice failed");
+    hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, b->device);
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(HYX_EHIP, "hipStreamCreate failed");
+    hipEventCreate(&b->ev0);
+    hipEventCreate(&b->ev1);
+    const uint64_t N = n_inst, n = s->S.n, np = std::max<uint32_t>(s->S.npar, 1);
+    struct {
+        void **p;
+        size_t bytes;
+    } allocs[] = {
+        {(void **)&b->state, n * N * 8},  {(void **)&b->pars, np * N * 8}, {(void **)&b->thi, N * 8},
+        {(void **)&b->tlo, N * 8},        {(void **)&b->tfhi, N * 8},      {(void **)&b->tflo, N * 8},
+        {(void **)&b->mdt, N * 8},        {(void **)&b->hout, N * 8},      {(void **)&b->minh, N * 8},
+        {(void **)&b->maxh, N * 8},       {(void **)&b->oc, N * 8},        {(void **)&b->nsteps, N * 8},
+        {(void **)&b->iters, N * 8},      {(void **)&b->status, N * 4},    {(void **)&b->work, 64},
+        {(void **)&b->active, kPlainGroup * 4}, {(void **)&b->unfinished, 64},
+    };
+    for (auto &al : allocs) {
+        if (hipMalloc(al.p, al.bytes) != hipSuccess) return cleanup(HYX_ENOMEM, "hipMalloc failed (device memory)");
+        if (hipMemsetAsync(*al.p, 0, al.bytes, b->stream) != hipSuccess) return cleanup(HYX_EHIP, "hipMemset failed");
+    }
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return cleanup(HYX_EHIP, "hipStreamSynchronize failed");
+    *out = b;
+    return 0;
+}
+
+void hyx_batch_destroy(hyx_batch *b)
+{
+    if (!b) return;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    void *ptrs[] = {b->state, b->pars, b->thi,    b->tlo,   b->tfhi,   b->tflo,  b->mdt,
+                    b->hout,  b->minh, b->maxh,   b->tc,    b->oc,     b->nsteps, b->iters,
+                    b->work,  b->status, b->active, b->unfinished, b->queue, b->tape};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+uint64_t hyx_batch_size(const hyx_batch *b) { return b ? b->N : 0; }
+
+static int field_info(hyx_batch *b, int field, double **p, uint64_t *count)
+{
+    const uint64_t N = b->N;
+    switch (field) {
+    case HYX_FIELD_STATE: *p = b->state; *count = (uint64_t)b->sys->S.n * N; return 0;
+    case HYX_FIELD_PARS: *p = b->pars; *count = (uint64_t)b->sys->S.npar * N; return 0;
+    case HYX_FIELD_TIME_HI: *p = b->thi; *count = N; return 0;
+    case HYX_FIELD_TIME_LO: *p = b->tlo; *count = N; return 0;
+    default: return fail(HYX_EINVAL, "unknown field");
+    }
+}
+
+int hyx_batch_write(hyx_batch *b, int field, const double *src)
+{
+    if (!b || !src) return fail(HYX_EINVAL, "null argument");
+    double *p;
+    uint64_t count;
+    if (int rc = field_info(b, field, &p, &count)) return rc;
+    if (count == 0) return 0;
+    HIPCHK(hipMemcpyAsync(p, src, count * 8, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int hyx_batch_read(hyx_batch *b, int field, double *dst)
+{
+    if (!b || !dst) return fail(HYX_EINVAL, "null argument");
+    double *p;
+    uint64_t count;
+    if (int rc = field_info(b, field, &p, &count)) return rc;
+    if (count == 0) return 0;
+    HIPCHK(hipMemcpyAsync(dst, p, count * 8, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int hyx_batch_device_ptr(hyx_batch *b, int field, void **out)
+{
+    if (!b || !out) return fail(HYX_EINVAL, "null argument");
+    double *p;
+    uint64_t count;
+    if (int rc = field_info(b, field, &p, &count)) return rc;
+    *out = p;
+    return 0;
+}
+
+int hyx_step(hyx_batch *b, const double *max_dt, int write_tc, int64_t *outcome, double *h)
+{
+    if (!b) return fail(HYX_EINVAL, "null batch");
+    const uint64_t N = b->N;
+    if (write_tc && !b->tc) {
+        const size_t bytes = (size_t)b->sys->S.n * (b->sys->order + 1) * N * 8;
+        HIPCHK(hipMalloc((void **)&b->tc, bytes));
+    }
+    if (int rc = upload(b, b->mdt, max_dt, N, std::numeric_limits<double>::infinity())) return rc;
+    if (int rc = ensure_tape(b, N)) return rc;
+    KArgs a = make_args(b);
+    HIPCHK(hipEventRecord(b->ev0, b->stream));
+    HIPCHK(launch(write_tc ? b->sys->k_step_tc : b->sys->k_step, blocks_for(N), b->stream, &a));
+    HIPCHK(hipEventRecord(b->ev1, b->stream));
+    if (outcome) HIPCHK(hipMemcpyAsync(outcome, b->oc, N * 8, hipMemcpyDeviceToHost, b->stream));
+    if (h) HIPCHK(hipMemcpyAsync(h, b->hout, N * 8, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    b->last_ms = ms;
+    b->last_launches = 1;
+    return 0;
+}
+
+int hyx_read_tc(hyx_batch *b, double *dst)
+{
+    if (!b || !dst) return fail(HYX_EINVAL, "null argument");
+    if (!b->tc) return fail(HYX_EINVAL, "no Taylor coefficients: call hyx_step with write_tc first");
+    const size_t bytes = (size_t)b->sys->S.n * (b->sys->order + 1) * b->N * 8;
+    HIPCHK(hipMemcpyAsync(dst, b->tc, bytes, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int hyx_propagate_until(hyx_batch *b, const double *tf_hi, const double *tf_lo, const double *max_delta_t,
+                        uint64_t max_steps, int mode, hyx_prop_res *res)
+{
+    if (!b || !tf_hi) return fail(HYX_EINVAL, "null argument");
+    if (mode != HYX_MODE_PERSISTENT && mode != HYX_MODE_PLAIN && mode != HYX_MODE_GRAPH)
+        return fail(HYX_EINVAL, "unknown propagation mode");
+    const uint64_t N = b->N;
+    if (int rc = upload(b, b->tfhi, tf_hi, N, 0.0)) return rc;
+    if (int rc = upload(b, b->tflo, tf_lo, N, 0.0)) return rc;
+    if (int rc = upload(b, b->mdt, max_delta_t, N, std::numeric_limits<double>::infinity())) return rc;
+    uint64_t pgrid = 0;
+    if (mode == HYX_MODE_PERSISTENT) {
+        int occ = 0;
+        HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, b->sys->k_persist, 64, 0));
+        const uint64_t per_cu = b->waves_per_cu ? b->waves_per_cu : (uint64_t)std::max(occ, 1);
+        pgrid = std::max<uint64_t>(1, std::min<uint64_t>(per_cu * (uint64_t)b->n_cu, blocks_for(N)));
+    }
+    if (int rc = ensure_tape(b, pgrid * 64)) return rc;
+    KArgs a = make_args(b);
+    a.max_steps = max_steps;
+    HIPCHK(launch(b->sys->k_init, blocks_for(N), b->stream, &a));
+    uint64_t launches = 0;
+    HIPCHK(hipEventRecord(b->ev0, b->stream));
+    if (mode == HYX_MODE_PERSISTENT) {
+        const uint64_t grid = pgrid;
+        for (;;) {
+            HIPCHK(clear64(b, b->work, 1));
+            HIPCHK(clear32(b, b->unfinished, 1));
+            HIPCHK(launch(b->sys->k_persist, grid, b->stream, &a));
+            ++launches;
+            unsigned unf = 0;
+            HIPCHK(hipMemcpyAsync(&unf, b->unfinished, 4, hipMemcpyDeviceToHost, b->stream));
+            HIPCHK(hipStreamSynchronize(b->stream));
+            if (unf == 0) break;
+        }
+    } else {
+        const uint64_t grid = blocks_for(N);
+        if (mode == HYX_MODE_GRAPH && !b->graph_exec) {
+            HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = clear32(b, b->active, kPlainGroup);
+            for (int i = 0; i < kPlainGroup && e == hipSuccess; ++i) {
+                KArgs ai = a;
+                ai.active_count = b->active + i;
+                ai.max_steps = 0; // the graph is replayed only when max_steps == 0
+                e = launch(b->sys->k_iter, grid, b->stream, &ai);
+            }
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+            if (e != hipSuccess || e2 != hipSuccess)
+                return fail(HYX_EHIP, std::string("graph capture failed: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+            b->graph = g;
+            HIPCHK(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+        }
+        for (;;) {
+            if (mode == HYX_MODE_GRAPH && max_steps == 0) {
+                HIPCHK(hipGraphLaunch(b->graph_exec, b->stream));
+            } else {
+                HIPCHK(clear32(b, b->active, kPlainGroup));
+                for (int i = 0; i < kPlainGroup; ++i) {
+                    KArgs ai = a;
+                    ai.active_count = b->active + i;
+                    HIPCHK(launch(b->sys->k_iter, grid, b->stream, &ai));
+                }
+            }
+            launches += kPlainGroup;
+            unsigned act = 0;
+            HIPCHK(hipMemcpyAsync(&act, b->active + (kPlainGroup - 1), 4, hipMemcpyDeviceToHost, b->stream));
+            HIPCHK(hipStreamSynchronize(b->stream));
+            if (act == 0) break;
+        }
+    }
+    HIPCHK(hipEventRecord(b->ev1, b->stream));
+    HIPCHK(hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    b->last_ms = ms;
+    b->last_launches = launches;
+    if (res) {
+        std::vector<int64_t> oc(N);
+        std::vector<double> mn(N), mx(N);
+        std::vector<uint64_t> ns(N);
+        HIPCHK(hipMemcpyAsync(oc.data(), b->oc, N * 8, hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipMemcpyAsync(mn.data(), b->minh, N * 8, hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipMemcpyAsync(mx.data(), b->maxh, N * 8, hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipMemcpyAsync(ns.data(), b->nsteps, N * 8, hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        for (uint64_t i = 0; i < N; ++i) res[i] = hyx_prop_res{oc[i], mn[i], mx[i], ns[i]};
+    }
+    return 0;
+}
+
+int hyx_batch_last_timing(const hyx_batch *b, double *kernel_ms, uint64_t *launches)
+{
+    if (!b) return fail(HYX_EINVAL, "null batch");
+    if (kernel_ms) *kernel_ms = b->last_ms;
+    if (launches) *launches = b->last_launches;
+    return 0;
+}
+
+int hyx_batch_set_persistent_config(hyx_batch *b, uint32_t waves_per_cu, uint32_t refill_min)
+{
+    if (!b) return fail(HYX_EINVAL, "null batch");
+    if (refill_min < 1 || refill_min > 64) return fail(HYX_EINVAL, "refill_min must be in [1, 64]");
+    if (waves_per_cu > 64) return fail(HYX_EINVAL, "waves_per_cu must be in [0, 64]");
+    b->waves_per_cu = waves_per_cu;
+    b->refill_min = refill_min;
+    return 0;
+}
+
+int hyx_batch_set_iter_budget(hyx_batch *b, uint64_t iters)
+{
+    if (!b) return fail(HYX_EINVAL, "null batch");
+    if (iters == 0) return fail(HYX_EINVAL, "the iteration budget must be positive");
+    b->iter_budget = iters;
+    return 0;
+}
+
+int hyx_batch_set_queue(hyx_batch *b, const uint32_t *order)
+{
+    if (!b) return fail(HYX_EINVAL, "null batch");
+    if (!order) {
+        if (b->queue) hipFree(b->queue);
+        b->queue = nullptr;
+        return 0;
+    }
+    std::vector<char> seen(b->N, 0);
+    for (uint64_t i = 0; i < b->N; ++i) {
+        if (order[i] >= b->N || seen[order[i]]) return fail(HYX_EINVAL, "the queue must be a permutation of 0..N-1");
+        seen[order[i]] = 1;
+    }
+    if (!b->queue) HIPCHK(hipMalloc((void **)&b->queue, b->N * 4));
+    HIPCHK(hipMemcpyAsync(b->queue, order, b->N * 4, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int hyx_batch_sync(hyx_batch *b)
+{
+    if (!b) return fail(HYX_EINVAL, "null batch");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+} // extern "C"

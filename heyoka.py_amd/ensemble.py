@@ -1,0 +1,155 @@
+"""Ensemble propagation: ``ensemble_propagate_until/for/grid[_batch]``.
+
+Same signature, argument validation and return structure as the reference
+(``heyoka/__init__.py:251-336``, ``heyoka/_ensemble_impl.py:13-138``):
+``gen(deepcopy(ta), i)`` builds member i and the result is
+``[(ta_i, *ret_i) for i in range(n_iter)]``.
+
+On the GPU the members are not run one by one. Their states, parameters and
+times are packed side by side into one batch of ``n_iter * batch_size``
+instances, propagated in one persistent launch, and unpacked into the member
+integrators. Instances are independent and their arithmetic does not depend on
+where they sit in the batch, so every member ends bit-for-bit where its own
+``propagate_until`` would have taken it (tests/test_gpu_ensemble.py checks this).
+Members with a step callback are propagated one at a time instead.
+"""
+
+from __future__ import annotations
+
+from collections.abc import Iterable
+from copy import copy, deepcopy
+
+import numpy as np
+
+from . import _lib
+from .integrator import taylor_adaptive_batch, dfl_add, _outcome, _MODES
+
+
+def _ensemble_generic(tp, ta, arg, n_iter, gen, **kwargs):
+    if not isinstance(n_iter, int):
+        raise TypeError(
+            f"The n_iter parameter must be an integer, but an object of type {type(n_iter)} was provided instead")
+    if n_iter < 0:
+        raise ValueError(f"The n_iter parameter must be non-negative, but it is {n_iter} instead")
+    if tp in ("until", "for"):
+        if isinstance(arg, Iterable):
+            raise TypeError("Cannot perform an ensemble propagate_until/for(): the final epoch/time interval must be "
+                            "a scalar, not an iterable object")
+    else:
+        arg = np.array(arg)
+        if arg.ndim != 1:
+            raise ValueError("Cannot perform an ensemble propagate_grid(): the input time grid must be "
+                             f"one-dimensional, but instead it has {arg.ndim} dimensions")
+    if "max_delta_t" in kwargs and isinstance(kwargs["max_delta_t"], Iterable):
+        raise TypeError('Cannot perform an ensemble propagate_until/for/grid(): the "max_delta_t" argument must be a '
+                        "scalar, not an iterable object")
+    algo = kwargs.pop("algorithm", "thread")
+    allowed = ["thread", "process"]
+    if algo not in allowed:
+        raise ValueError(f"The parallelisation algorithm must be one of {allowed}, but '{algo}' was provided instead")
+    kwargs.pop("max_workers", None)
+    if algo == "process":
+        kwargs.pop("chunksize", None)
+    if "chunksize" in kwargs:
+        raise TypeError("propagate_until() got an unexpected keyword argument 'chunksize'")
+    if tp == "grid":
+        raise NotImplementedError("Ensemble propagate_grid() is not supported yet by the GPU batch integrator")
+    if kwargs.get("c_output"):
+        raise NotImplementedError("Continuous output is not supported by the GPU batch integrator")
+    if "callback" in kwargs and kwargs["callback"] is not None:
+        return _ensemble_sequential(tp, ta, arg, n_iter, gen, kwargs)
+    return _ensemble_packed(tp, ta, arg, n_iter, gen, kwargs)
+
+
+def _ensemble_sequential(tp, ta, arg, n_iter, gen, kwargs):
+    out = []
+    for i in range(n_iter):
+        kw = copy(kwargs)
+        kw["callback"] = deepcopy(kwargs["callback"])
+        local = gen(deepcopy(ta), i)
+        ret = local.propagate_until(arg, **kw) if tp == "until" else local.propagate_for(arg, **kw)
+        out.append((local,) + tuple(ret))
+    return out
+
+
+def _ensemble_packed(tp, ta, arg, n_iter, gen, kwargs):
+    if not isinstance(ta, taylor_adaptive_batch):
+        raise TypeError("The template integrator must be a taylor_adaptive_batch")
+    max_steps = int(kwargs.pop("max_steps", 0))
+    mdt = kwargs.pop("max_delta_t", None)
+    kwargs.pop("callback", None)
+    kwargs.pop("c_output", None)
+    if kwargs.pop("write_tc", False):
+        raise NotImplementedError("write_tc is not supported by ensemble propagations on the GPU")
+    if kwargs:
+        raise TypeError(f"Unexpected keyword argument(s): {sorted(kwargs)}")
+    members = [gen(deepcopy(ta), i) for i in range(n_iter)]
+    if not members:
+        return []
+    B = ta.batch_size
+    n = ta.dim
+    for m in members:
+        if m.batch_size != B or m.dim != n or m._hyx_system is not ta._hyx_system:
+            raise ValueError("The generator must return integrators of the same system and batch size as the template")
+    N = n_iter * B
+    st = np.concatenate([m._hyx_state for m in members], axis=1)
+    npar = ta._hyx_system.npar
+    pa = np.concatenate([m._hyx_pars[:npar] for m in members], axis=1) if npar else None
+    thi = np.concatenate([m._hyx_thi for m in members])
+    tlo = np.concatenate([m._hyx_tlo for m in members])
+    if tp == "until":
+        tf_hi, tf_lo = np.full(N, float(arg)), np.zeros(N)
+    else:
+        tf_hi, tf_lo = dfl_add(thi, tlo, np.full(N, float(arg)), np.zeros(N))
+    if mdt is None:
+        md = np.full(N, np.inf)
+    else:
+        md = np.full(N, float(mdt))
+        if np.isnan(md).any() or (md <= 0).any():
+            raise ValueError("A non-positive or nan max_delta_t was passed to an ensemble propagation")
+    if not np.all(np.isfinite(tf_hi)):
+        raise ValueError("A non-finite time was passed to an ensemble propagation")
+    batch = _lib.Batch(ta._hyx_system, N)
+    batch.write(_lib.FIELD_STATE, st)
+    if npar:
+        batch.write(_lib.FIELD_PARS, pa)
+    batch.write(_lib.FIELD_TIME_HI, thi)
+    batch.write(_lib.FIELD_TIME_LO, tlo)
+    r = batch.propagate_until(tf_hi, tf_lo, md, max_steps, _MODES[ta._hyx_flags[3]])
+    st = batch.read(_lib.FIELD_STATE, (n, N))
+    thi = batch.read(_lib.FIELD_TIME_HI, (N,))
+    tlo = batch.read(_lib.FIELD_TIME_LO, (N,))
+    out = []
+    for i, m in enumerate(members):
+        sl = slice(i * B, (i + 1) * B)
+        m._hyx_state[...] = st[:, sl]
+        m._hyx_thi[...] = thi[sl]
+        m._hyx_tlo[...] = tlo[sl]
+        m._hyx_prop_res = [(_outcome(o), float(a), float(b), int(c))
+                           for o, a, b, c in zip(r["outcome"][sl], r["min_h"][sl], r["max_h"][sl], r["nsteps"][sl])]
+        out.append((m, None, None))
+    return out
+
+
+def ensemble_propagate_until(ta, t, n_iter, gen, **kwargs):
+    return _ensemble_generic("until", ta, t, n_iter, gen, **kwargs)
+
+
+def ensemble_propagate_for(ta, delta_t, n_iter, gen, **kwargs):
+    return _ensemble_generic("for", ta, delta_t, n_iter, gen, **kwargs)
+
+
+def ensemble_propagate_grid(ta, grid, n_iter, gen, **kwargs):
+    return _ensemble_generic("grid", ta, grid, n_iter, gen, **kwargs)
+
+
+def ensemble_propagate_until_batch(ta, t, n_iter, gen, **kwargs):
+    return _ensemble_generic("until", ta, t, n_iter, gen, **kwargs)
+
+
+def ensemble_propagate_for_batch(ta, delta_t, n_iter, gen, **kwargs):
+    return _ensemble_generic("for", ta, delta_t, n_iter, gen, **kwargs)
+
+
+def ensemble_propagate_grid_batch(ta, grid, n_iter, gen, **kwargs):
+    return _ensemble_generic("grid", ta, grid, n_iter, gen, **kwargs)

@@ -1,0 +1,541 @@
+"""``taylor_adaptive_batch``: the drop-in batch integrator backed by libhyx.so.
+
+Mirrors the Python surface of ``heyoka.core.taylor_adaptive_batch_dbl``
+(``heyoka/expose_batch_integrators.cpp:214-673``): constructor arguments and
+their validation texts, ``state`` / ``pars`` writable arrays, read-only
+``time`` / ``dtime`` / ``last_h`` / ``tc``, ``step`` / ``step_backward`` /
+``step_res``, ``propagate_until`` / ``propagate_for`` / ``propagate_res``,
+``update_d_output``, variational accessors, copy, deepcopy and pickling.
+
+The batch lives on the GPU. The host arrays are mirrors: they are uploaded
+before every device call and refreshed after it, so writing ``ta.state[:]``
+and then propagating behaves as in the reference.
+
+Lanes are independent instances: a lane that finishes or fails stops on its
+own without affecting the others' results.
+"""
+
+from __future__ import annotations
+
+import copy as _copy
+import enum
+import threading
+
+import numpy as np
+
+from . import _lib
+from .decompose import taylor_decompose, order_from_tol
+from .expr import _ex
+from .var_ode_sys import var_ode_sys as _var_ode_sys
+
+
+class taylor_outcome(enum.IntEnum):
+    """Outcome codes of heyoka (``heyoka/core.cpp:324-329``)."""
+
+    success = -4294967297
+    step_limit = -4294967298
+    time_limit = -4294967299
+    err_nf_state = -4294967300
+    cb_stop = -4294967301
+
+
+def _outcome(v):
+    try:
+        return taylor_outcome(int(v))
+    except ValueError:
+        return int(v)
+
+
+_MODES = {"persistent": _lib.MODE_PERSISTENT, "plain": _lib.MODE_PLAIN, "graph": _lib.MODE_GRAPH}
+
+# Compiled systems keyed by (op-list, constants, order, flags): the analogue of
+# heyoka's in-memory JIT cache (heyoka/core.cpp:148-149). Copies of an
+# integrator share one compiled system.
+_sys_cache: dict = {}
+_sys_cache_lock = threading.Lock()
+
+
+def get_system(dec, order, high_accuracy=False, compact=None):
+    words, consts = _lib.oplist_args(dec)
+    key = (words.tobytes(), consts.tobytes(), int(order), bool(high_accuracy), compact)
+    with _sys_cache_lock:
+        s = _sys_cache.get(key)
+        if s is None:
+            s = _lib.System(dec, order, high_accuracy=high_accuracy, compact=compact)
+            _sys_cache[key] = s
+        return s
+
+
+# ---- double-length arithmetic on the host (same algorithm as the kernels) ----------------
+def _two_sum(a, b):
+    x = a + b
+    z = x - a
+    return x, (a - (x - z)) + (b - z)
+
+
+def dfl_add(ah, al, bh, bl):
+    s, e = _two_sum(ah, bh)
+    t, f = _two_sum(al, bl)
+    e = e + t
+    x = s + e
+    e = e - (x - s)
+    s = x
+    e = e + f
+    x = s + e
+    return x, e - (x - s)
+
+
+def _dfl_norm(hi, lo):
+    x = hi + lo
+    return x, lo - (x - hi)
+
+
+class taylor_adaptive_batch:
+    """Adaptive Taylor integrator over a batch of independent instances (FP64, GPU)."""
+
+    _internal = frozenset()
+
+    def __init__(self, sys, state, time=None, pars=None, tol=0.0, high_accuracy=False, compact_mode=False,
+                 t_events=None, nt_events=None, parallel_mode=False, **kwargs):
+        if t_events or nt_events:
+            raise NotImplementedError("Event detection is not supported by the GPU batch integrator")
+        gpu_mode = kwargs.pop("gpu_mode", "persistent")
+        code_form = kwargs.pop("code_form", None)  # None (automatic), "compact" or "unrolled"
+        # LLVM-specific options of the reference are accepted and ignored.
+        for k in list(kwargs):
+            if k in ("opt_level", "force_avx512", "slp_vectorize", "fast_math", "code_model", "parjit", "fp_type"):
+                kwargs.pop(k)
+        if kwargs:
+            raise TypeError(f"Unexpected keyword argument(s): {sorted(kwargs)}")
+        if gpu_mode not in _MODES:
+            raise ValueError(f"gpu_mode must be one of {sorted(_MODES)}")
+        self._hyx_init(sys, state, time, pars, tol, high_accuracy, compact_mode, parallel_mode, gpu_mode, code_form)
+
+    # ------------------------------------------------------------------------------------
+    def _hyx_init(self, sys, state, time, pars, tol, high_accuracy, compact_mode, parallel_mode, gpu_mode,
+                  code_form, restore=None):
+        d = self.__dict__
+        vsys = sys if isinstance(sys, _var_ode_sys) else None
+        pairs = vsys.sys if vsys is not None else [(_ex(a), _ex(b)) for a, b in sys]
+        st = np.asarray(state)
+        if st.ndim != 2:
+            raise ValueError(
+                "Invalid state vector passed to the constructor of a batch integrator: the expected number of "
+                f"dimensions is 2, but the input array has a dimension of {st.ndim}")
+        if st.dtype != np.float64:
+            if st.size and not np.can_cast(st.dtype, np.float64, casting="safe"):
+                raise TypeError(f"Cannot cast the state array from {st.dtype} to float64 according to the rule 'safe'")
+            st = st.astype(np.float64)
+        B = int(st.shape[1])
+        if B == 0:
+            raise ValueError("The batch size in an adaptive Taylor integrator cannot be zero")
+        dec = taylor_decompose(pairs)
+        n = dec.n_state
+        if st.shape[0] == 0:
+            st = np.zeros((n, B))
+        elif vsys is not None and st.shape[0] == vsys.n_orig_sv:
+            ext = np.repeat(np.array(vsys.initial_derivatives(), dtype=np.float64)[:, None], B, axis=1)
+            st = np.concatenate([st, ext], axis=0)
+        if st.shape[0] != n:
+            raise ValueError(
+                f"Inconsistent sizes detected in the initialization of an adaptive Taylor integrator: the state "
+                f"vector has a dimension of {st.shape[0]}, while the number of equations is {n}")
+        npar = dec.n_par
+        if pars is not None:
+            pa = np.asarray(pars)
+            if pa.ndim != 2 or pa.shape[1] != B:
+                raise ValueError(
+                    "Invalid parameter vector passed to the constructor of a batch integrator: the expected array "
+                    f"shape is (n, {B}), but the input array has either the wrong number of dimensions or the wrong "
+                    "shape")
+            pa = pa.astype(np.float64)
+            if pa.shape[0] < npar:
+                raise ValueError(
+                    f"Inconsistent sizes detected in the initialization of an adaptive Taylor integrator: the number "
+                    f"of parameter rows is {pa.shape[0]}, while the number of parameters in the system is {npar}")
+            npar = max(npar, pa.shape[0])
+        else:
+            pa = np.zeros((npar, B))
+        if time is not None:
+            tm = np.asarray(time, dtype=np.float64)
+            if tm.ndim != 1 or tm.shape[0] != B:
+                raise ValueError(
+                    "Invalid time vector passed to the constructor of a batch integrator: the expected array shape is "
+                    f"({B}), but the input array has either the wrong number of dimensions or the wrong shape")
+        else:
+            tm = np.zeros(B)
+        tol = float(tol)
+        if tol == 0.0:
+            tol = float(np.finfo(np.float64).eps)
+        order = order_from_tol(tol)
+        compact = {"compact": True, "unrolled": False, None: None}[code_form]
+        system = get_system(dec, order, high_accuracy=bool(high_accuracy), compact=compact)
+        d["_hyx_pairs"] = pairs
+        d["_hyx_vsys"] = vsys
+        d["_hyx_dec"] = dec
+        d["_hyx_order"] = order
+        d["_hyx_tol"] = tol
+        d["_hyx_flags"] = (bool(high_accuracy), bool(compact_mode), bool(parallel_mode), gpu_mode, code_form)
+        d["_hyx_system"] = system
+        d["_hyx_B"] = B
+        d["_hyx_npar"] = npar
+        d["_hyx_batch"] = _lib.Batch(system, B)
+        d["_hyx_state"] = np.ascontiguousarray(st, dtype=np.float64).copy()
+        d["_hyx_pars"] = np.ascontiguousarray(pa, dtype=np.float64).copy()
+        d["_hyx_thi"] = tm.copy()
+        d["_hyx_tlo"] = np.zeros(B)
+        d["_hyx_last_h"] = np.zeros(B)
+        d["_hyx_prop_res"] = [(taylor_outcome.success, 0.0, 0.0, 0)] * B
+        d["_hyx_step_res"] = [(taylor_outcome.success, 0.0)] * B
+        d["_hyx_tc"] = np.zeros((n, order + 1, B))
+        d["_hyx_dout"] = np.zeros((n, B))
+        if restore is not None:
+            for k, v in restore.items():
+                d[k] = v
+
+    # ---- host <-> device ------------------------------------------------------------------
+    def _push(self):
+        b = self._hyx_batch
+        b.write(_lib.FIELD_STATE, self._hyx_state)
+        if self._hyx_system.npar:
+            b.write(_lib.FIELD_PARS, self._hyx_pars[: self._hyx_system.npar])
+        b.write(_lib.FIELD_TIME_HI, self._hyx_thi)
+        b.write(_lib.FIELD_TIME_LO, self._hyx_tlo)
+
+    def _pull(self):
+        b = self._hyx_batch
+        self._hyx_state[...] = b.read(_lib.FIELD_STATE, self._hyx_state.shape)
+        self._hyx_thi[...] = b.read(_lib.FIELD_TIME_HI, (self._hyx_B,))
+        self._hyx_tlo[...] = b.read(_lib.FIELD_TIME_LO, (self._hyx_B,))
+
+    def _lanes(self, v, what):
+        a = np.asarray(v, dtype=np.float64)
+        if a.ndim == 0:
+            return np.full(self._hyx_B, float(a))
+        if a.ndim != 1 or a.shape[0] != self._hyx_B:
+            raise ValueError(
+                f"Invalid {what} passed to a batch integrator: the expected size is {self._hyx_B}, but a vector of "
+                f"shape {a.shape} was provided instead")
+        return a.copy()
+
+    # ---- stepping -----------------------------------------------------------------------
+    def _step_impl(self, mdt, write_tc):
+        self._push()
+        oc, h = self._hyx_batch.step(mdt, write_tc)
+        self._pull()
+        self._hyx_last_h[...] = h
+        self._hyx_step_res = [(_outcome(o), float(x)) for o, x in zip(oc, h)]
+        if write_tc:
+            self._hyx_tc[...] = self._hyx_batch.read_tc()
+        return oc, h
+
+    def step(self, max_delta_t=None, write_tc=False):
+        """One adaptive step per lane; ``max_delta_t`` (per lane) clamps |h| and sets its sign."""
+        if isinstance(max_delta_t, (bool, np.bool_)):
+            write_tc, max_delta_t = bool(max_delta_t), None
+        mdt = None if max_delta_t is None else self._lanes(max_delta_t, "max_delta_t vector")
+        if mdt is not None and np.isnan(mdt).any():
+            raise ValueError("A NaN max_delta_t was passed to the step() function of an adaptive Taylor integrator")
+        self._step_impl(mdt, write_tc)
+
+    def step_backward(self, write_tc=False):
+        self._step_impl(np.full(self._hyx_B, -np.inf), write_tc)
+
+    # ---- propagation --------------------------------------------------------------------
+    def _check_cb(self, callback):
+        if callback is None:
+            return None
+        if not callable(callback):
+            raise TypeError(
+                f"The object of type '{type(callback)}' cannot be used as a step callback because it is not callable")
+        return callback
+
+    def _prop_impl(self, tf_hi, tf_lo, max_steps, max_delta_t, callback, write_tc, c_output):
+        if c_output:
+            raise NotImplementedError("Continuous output is not supported by the GPU batch integrator")
+        if not isinstance(max_steps, (int, np.integer)) or max_steps < 0:
+            raise ValueError("max_steps must be a non-negative integer")
+        B = self._hyx_B
+        if max_delta_t is None or (not np.isscalar(max_delta_t) and len(max_delta_t) == 0):
+            mdt = np.full(B, np.inf)
+        else:
+            mdt = self._lanes(max_delta_t, "max_delta_t vector")
+        if np.isnan(mdt).any():
+            raise ValueError("A nan max_delta_t was passed to the propagate_until() function of an adaptive Taylor "
+                             "integrator")
+        if (mdt <= 0).any():
+            raise ValueError("A non-positive max_delta_t was passed to the propagate_until() function of an adaptive "
+                             "Taylor integrator")
+        if not np.all(np.isfinite(tf_hi)) or not np.all(np.isfinite(self._hyx_thi)):
+            raise ValueError("A non-finite time was passed to the propagate_until() function of an adaptive Taylor "
+                             "integrator")
+        cb = self._check_cb(callback)
+        if cb is None and not write_tc:
+            self._push()
+            r = self._hyx_batch.propagate_until(tf_hi, tf_lo, mdt, int(max_steps),
+                                                _MODES[self._hyx_flags[3]])
+            self._pull()
+            self._hyx_prop_res = [(_outcome(o), float(a), float(b), int(c))
+                                  for o, a, b, c in zip(r["outcome"], r["min_h"], r["max_h"], r["nsteps"])]
+            return None, None
+        return self._prop_stepwise(tf_hi, tf_lo, int(max_steps), mdt, cb, write_tc)
+
+    def _prop_stepwise(self, tf_hi, tf_lo, max_steps, mdt, cb, write_tc):
+        # Host-driven loop, one device step per iteration, so that a step
+        # callback can observe and modify the integrator between steps.
+        B = self._hyx_B
+        rem0 = dfl_add(tf_hi, tf_lo, -self._hyx_thi, -self._hyx_tlo)[0]
+        tdir = rem0 >= 0
+        done = np.zeros(B, dtype=bool)
+        res_oc = np.full(B, int(taylor_outcome.time_limit), dtype=np.int64)
+        min_h = np.full(B, np.inf)
+        max_h = np.zeros(B)
+        nst = np.zeros(B, dtype=np.int64)
+        if cb is not None and hasattr(cb, "pre_hook"):
+            cb.pre_hook(self)
+        it = 0
+        while True:
+            rem = dfl_add(tf_hi, tf_lo, -self._hyx_thi, -self._hyx_tlo)[0]
+            lim = np.where(tdir, np.minimum(mdt, rem), np.maximum(-mdt, rem))
+            lim = np.where(done, 0.0, lim)
+            oc, h = self._step_impl(lim, write_tc)
+            it += 1
+            act = ~done
+            nst += (act & (h != 0)).astype(np.int64)
+            succ = act & (oc == int(taylor_outcome.success))
+            min_h = np.where(succ, np.minimum(min_h, np.abs(h)), min_h)
+            max_h = np.where(succ, np.maximum(max_h, np.abs(h)), max_h)
+            nf = act & (oc == int(taylor_outcome.err_nf_state))
+            res_oc[nf] = int(taylor_outcome.err_nf_state)
+            done |= nf
+            fin = act & ~nf & (oc == int(taylor_outcome.time_limit)) & (h == rem)
+            done |= fin
+            if max_steps and it >= max_steps:
+                res_oc[act & ~done] = int(taylor_outcome.step_limit)
+                done[:] = True
+            stop = False
+            if cb is not None:
+                r = cb(self)
+                if not isinstance(r, (bool, np.bool_)):
+                    raise TypeError("The call operator of a step callback is expected to return a boolean, but a "
+                                    f"value of type {type(r)} was returned instead")
+                if not r:
+                    res_oc[~done] = int(taylor_outcome.cb_stop)
+                    stop = True
+            if stop or done.all():
+                break
+        self._hyx_prop_res = [(_outcome(o), float(a), float(b), int(c)) for o, a, b, c in zip(res_oc, min_h, max_h, nst)]
+        return None, cb
+
+    def propagate_until(self, t, max_steps=0, max_delta_t=None, callback=None, write_tc=False, c_output=False):
+        tf = self._lanes(t, "final time vector")
+        ret = self._prop_impl(tf, np.zeros(self._hyx_B), max_steps, max_delta_t, callback, write_tc, c_output)
+        return ret
+
+    def propagate_for(self, delta_t, max_steps=0, max_delta_t=None, callback=None, write_tc=False, c_output=False):
+        dt = self._lanes(delta_t, "time interval vector")
+        if not np.all(np.isfinite(dt)):
+            raise ValueError("A non-finite time interval was passed to the propagate_for() function of an adaptive "
+                             "Taylor integrator")
+        hi, lo = dfl_add(self._hyx_thi, self._hyx_tlo, dt, np.zeros_like(dt))
+        return self._prop_impl(hi, lo, max_steps, max_delta_t, callback, write_tc, c_output)
+
+    def propagate_grid(self, grid, max_steps=0, max_delta_t=None, callback=None):
+        g = np.asarray(grid)
+        if g.ndim != 2:
+            raise ValueError(
+                "Invalid grid passed to the propagate_grid() method of a batch integrator: the expected number of "
+                f"dimensions is 2, but the input array has a dimension of {g.ndim}")
+        if g.shape[1] != self._hyx_B:
+            raise ValueError(
+                "Invalid grid passed to the propagate_grid() method of a batch integrator: the shape must be "
+                f"(n, {self._hyx_B}) but the number of columns is {g.shape[1]} instead")
+        raise NotImplementedError("propagate_grid() is not supported yet by the GPU batch integrator")
+
+    # ---- dense output from the last step's Taylor coefficients ----------------------------
+    def update_d_output(self, t, rel_time=False):
+        tt = self._lanes(t, "time vector")
+        if rel_time:
+            delta = tt + self._hyx_last_h
+        else:
+            delta = ((tt - self._hyx_thi) - self._hyx_tlo) + self._hyx_last_h
+        tc = self._hyx_tc
+        acc = tc[:, -1, :].copy()
+        for k in range(tc.shape[1] - 2, -1, -1):
+            acc = acc * delta + tc[:, k, :]
+        self._hyx_dout[...] = acc
+        return self.d_output
+
+    # ---- properties -----------------------------------------------------------------------
+    @property
+    def state(self):
+        return self._hyx_state
+
+    @property
+    def pars(self):
+        return self._hyx_pars
+
+    @staticmethod
+    def _ro(a):
+        v = a.view()
+        v.flags.writeable = False
+        return v
+
+    @property
+    def time(self):
+        return self._ro(self._hyx_thi)
+
+    @property
+    def dtime(self):
+        return self._ro(self._hyx_thi), self._ro(self._hyx_tlo)
+
+    def set_time(self, time):
+        t = self._lanes(time, "time vector")
+        self._hyx_thi[...] = t
+        self._hyx_tlo[...] = 0.0
+
+    def set_dtime(self, hi_time, lo_time):
+        if np.isscalar(hi_time) != np.isscalar(lo_time):
+            raise TypeError("The two arguments to the set_dtime() method must be of the same type")
+        hi = self._lanes(hi_time, "time vector")
+        lo = self._lanes(lo_time, "time vector")
+        h, l = _dfl_norm(hi, lo)
+        self._hyx_thi[...] = h
+        self._hyx_tlo[...] = l
+
+    @property
+    def last_h(self):
+        return self._ro(self._hyx_last_h)
+
+    @property
+    def tc(self):
+        return self._ro(self._hyx_tc)
+
+    @property
+    def d_output(self):
+        return self._ro(self._hyx_dout)
+
+    @property
+    def propagate_res(self):
+        return list(self._hyx_prop_res)
+
+    @property
+    def step_res(self):
+        return list(self._hyx_step_res)
+
+    @property
+    def order(self):
+        return self._hyx_order
+
+    @property
+    def tol(self):
+        return self._hyx_tol
+
+    @property
+    def dim(self):
+        return self._hyx_dec.n_state
+
+    @property
+    def batch_size(self):
+        return self._hyx_B
+
+    @property
+    def decomposition(self):
+        return self._hyx_dec.as_expressions()
+
+    @property
+    def sys(self):
+        return list(self._hyx_pairs)
+
+    @property
+    def high_accuracy(self):
+        return self._hyx_flags[0]
+
+    @property
+    def compact_mode(self):
+        return self._hyx_flags[1]
+
+    @property
+    def with_events(self):
+        return False
+
+    # ---- variational --------------------------------------------------------------------
+    @property
+    def is_variational(self):
+        return self._hyx_vsys is not None
+
+    @property
+    def n_orig_sv(self):
+        return self._hyx_vsys.n_orig_sv if self._hyx_vsys is not None else self.dim
+
+    @property
+    def vargs(self):
+        return self._hyx_vsys.vargs if self._hyx_vsys is not None else []
+
+    @property
+    def vorder(self):
+        return self._hyx_vsys.order if self._hyx_vsys is not None else 0
+
+    def _need_var(self):
+        if self._hyx_vsys is None:
+            raise ValueError("The function cannot be invoked on a non-variational integrator")
+
+    def get_vslice(self, order, component=None):
+        self._need_var()
+        return self._hyx_vsys.get_vslice(order, component)
+
+    def get_mindex(self, i):
+        self._need_var()
+        return self._hyx_vsys.get_mindex(i)
+
+    # ---- GPU specifics ------------------------------------------------------------------
+    @property
+    def code_form(self):
+        return "compact" if self._hyx_system.compact else "unrolled"
+
+    def last_kernel_timing(self):
+        """(milliseconds on the device, kernel launches) of the last step/propagate call."""
+        return self._hyx_batch.last_timing()
+
+    # ---- copy / pickle --------------------------------------------------------------------
+    _STATE_KEYS = ("_hyx_state", "_hyx_pars", "_hyx_thi", "_hyx_tlo", "_hyx_last_h", "_hyx_prop_res",
+                   "_hyx_step_res", "_hyx_tc", "_hyx_dout")
+
+    def _user_attrs(self):
+        return {k: v for k, v in self.__dict__.items() if not k.startswith("_hyx_")}
+
+    def _clone(self, user):
+        new = taylor_adaptive_batch.__new__(taylor_adaptive_batch)
+        f = self._hyx_flags
+        restore = {k: _copy.deepcopy(self.__dict__[k]) for k in self._STATE_KEYS}
+        sysarg = self._hyx_vsys if self._hyx_vsys is not None else self._hyx_pairs
+        new._hyx_init(sysarg, self._hyx_state, self._hyx_thi, self._hyx_pars, self._hyx_tol, f[0], f[1], f[2], f[3],
+                      f[4], restore=restore)
+        new.__dict__.update(user)
+        return new
+
+    def __copy__(self):
+        return self._clone(self._user_attrs())
+
+    def __deepcopy__(self, memo):
+        return self._clone(_copy.deepcopy(self._user_attrs(), memo))
+
+    def __getstate__(self):
+        st = {k: self.__dict__[k] for k in self._STATE_KEYS}
+        st["_hyx_ctor"] = (self._hyx_vsys if self._hyx_vsys is not None else self._hyx_pairs, self._hyx_tol,
+                           self._hyx_flags)
+        st["_hyx_user"] = self._user_attrs()
+        return st
+
+    def __setstate__(self, st):
+        sysarg, tol, f = st["_hyx_ctor"]
+        restore = {k: st[k] for k in self._STATE_KEYS}
+        self._hyx_init(sysarg, restore["_hyx_state"], restore["_hyx_thi"], restore["_hyx_pars"], tol, f[0], f[1], f[2],
+                       f[3], f[4], restore=restore)
+        self.__dict__.update(st["_hyx_user"])
+
+    def __repr__(self):
+        return (f"GPU batch Taylor integrator (FP64)\nTolerance               : {self._hyx_tol!r}\n"
+                f"High accuracy           : {str(self.high_accuracy).lower()}\nTaylor order            : "
+                f"{self._hyx_order}\nDimension               : {self.dim}\nBatch size              : {self._hyx_B}\n"
+                f"Code form               : {self.code_form}\n")

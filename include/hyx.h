@@ -1,0 +1,146 @@
+/* C-ABI of the MI355X batch/ensemble Taylor integrator (libhyx.so).
+ *
+ * This is the drop-in boundary for heyoka.py's batch integrator path. Each entry
+ * point replaces one piece of what the pybind11 class
+ * heyoka.core.taylor_adaptive_batch_dbl binds from heyoka C++
+ * (heyoka/expose_batch_integrators.cpp:78-673); the comment on each cites it.
+ * Signatures use plain pointers and sizes only. Every function returns 0 on
+ * success and a negative HYX_E* code on failure; hyx_last_error() then gives a
+ * message (thread-local). One batch object must not be used by two threads at
+ * once; distinct objects may be used concurrently.
+ *
+ * Layouts (identical to the reference's, :101-119): state is [n][N] and
+ * parameters [n_par][N], row-major doubles with the instance index fastest;
+ * per-instance vectors are [N].
+ */
+#ifndef HYX_H
+#define HYX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "hyx_oplist.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hyx_sys hyx_sys;
+typedef struct hyx_batch hyx_batch;
+
+/* Per-instance result of a propagation, as in propagate_res
+ * (heyoka/expose_batch_integrators.cpp:393): outcome is a taylor_outcome value
+ * (heyoka/core.cpp:324-329). */
+typedef struct {
+    int64_t outcome;
+    double min_h;
+    double max_h;
+    uint64_t nsteps;
+} hyx_prop_res;
+
+/* taylor_outcome values. */
+#define HYX_OUTCOME_SUCCESS (-4294967297LL)
+#define HYX_OUTCOME_STEP_LIMIT (-4294967298LL)
+#define HYX_OUTCOME_TIME_LIMIT (-4294967299LL)
+#define HYX_OUTCOME_ERR_NF_STATE (-4294967300LL)
+
+/* Error codes. */
+#define HYX_OK 0
+#define HYX_EINVAL (-1)
+#define HYX_EHIP (-2)
+#define HYX_ERTC (-3)
+#define HYX_ENOMEM (-4)
+#define HYX_ENODEV (-5)
+
+/* hyx_sys_create flags. */
+#define HYX_FLAG_HIGH_ACCURACY 1u /* compensated summation in the state update */
+#define HYX_FLAG_COMPILE_ONLY 2u  /* compile into the code-object cache; no device needed */
+#define HYX_FLAG_FORCE_COMPACT 4u /* loop over orders, Taylor tape in device memory */
+#define HYX_FLAG_FORCE_UNROLLED 8u /* fully unrolled, histories in registers */
+#define HYX_FLAG_SOURCE_ONLY 16u   /* generate the HIP source only (hyx_sys_source); no compile, no device */
+
+/* Fields for hyx_batch_write / hyx_batch_read / hyx_batch_device_ptr. */
+#define HYX_FIELD_STATE 0   /* [n][N] */
+#define HYX_FIELD_PARS 1    /* [n_par][N] */
+#define HYX_FIELD_TIME_HI 2 /* [N] */
+#define HYX_FIELD_TIME_LO 3 /* [N] */
+
+/* Propagation forms for hyx_propagate_until. */
+#define HYX_MODE_PERSISTENT 0 /* one persistent work-stealing launch (resumed if the budget runs out) */
+#define HYX_MODE_PLAIN 1      /* one kernel launch per integration step */
+#define HYX_MODE_GRAPH 2      /* the plain form's launches replayed from a captured hipGraph */
+
+/* Build the integrator for one ODE system: parse the op-list (hyx_oplist.h),
+ * generate its Taylor-step kernels and compile them for the current device's
+ * architecture (gfx950) with hipRTC, through an in-memory and on-disk
+ * code-object cache. `order` is the Taylor order p = ceil(-ln(tol)/2 + 1).
+ * Replaces the construction of heyoka::taylor_adaptive_batch<double>
+ * (heyoka/expose_batch_integrators.cpp:91-212): taylor_decompose + LLVM JIT,
+ * whose caches are enabled at heyoka/core.cpp:148-149. */
+int hyx_sys_create(const uint32_t *ops, uint32_t n_words, const double *consts, uint32_t n_consts, uint32_t order,
+                   uint32_t flags, hyx_sys **out);
+/* As hyx_sys_create with HYX_FLAG_COMPILE_ONLY, for an explicit target such as "gfx950". */
+int hyx_sys_precompile(const uint32_t *ops, uint32_t n_words, const double *consts, uint32_t n_consts,
+                       uint32_t order, uint32_t flags, const char *arch);
+/* order / dim properties (expose_batch_integrators.cpp:542-544). */
+int hyx_sys_order(const hyx_sys *s);
+int hyx_sys_dim(const hyx_sys *s);
+int hyx_sys_npar(const hyx_sys *s);
+/* 1 if the compact code form (tape in device memory) was generated, 0 if unrolled. */
+int hyx_sys_is_compact(const hyx_sys *s);
+/* The generated HIP source (for inspection); returns its length, copies up to cap-1 bytes. */
+size_t hyx_sys_source(const hyx_sys *s, char *buf, size_t cap);
+/* Register usage of the compiled Taylor-step kernels: VGPRs, scratch bytes per lane. */
+int hyx_sys_kernel_info(const hyx_sys *s, int *vgprs, int *scratch_bytes);
+void hyx_sys_destroy(hyx_sys *s);
+
+/* N device-resident instances of one system (the batch_size of
+ * taylor_adaptive_batch, :116). State, parameters and time start at zero. */
+int hyx_batch_create(hyx_sys *s, uint64_t n_inst, hyx_batch **out);
+void hyx_batch_destroy(hyx_batch *b);
+uint64_t hyx_batch_size(const hyx_batch *b);
+
+/* Copy a field host -> device / device -> host; the state / pars / time
+ * accessors of the reference (:394-472) map onto these. */
+int hyx_batch_write(hyx_batch *b, int field, const double *src);
+int hyx_batch_read(hyx_batch *b, int field, double *dst);
+/* Device pointer of a field (device-resident use, no copies). */
+int hyx_batch_device_ptr(hyx_batch *b, int field, void **out);
+
+/* One adaptive step of every instance: step() / step(max_delta_t)
+ * (:233-242). max_dt: [N] signed clamps, or NULL for +inf (step_backward passes
+ * -inf). outcome / h: [N] host outputs (step_res), may be NULL. write_tc keeps the
+ * Taylor coefficients for hyx_read_tc (tc property, :473-489). */
+int hyx_step(hyx_batch *b, const double *max_dt, int write_tc, int64_t *outcome, double *h);
+/* Taylor coefficients of the last step with write_tc: [n][p+1][N]. */
+int hyx_read_tc(hyx_batch *b, double *dst);
+
+/* propagate_until (:279-314) for every instance: tf_hi / tf_lo ([N], tf_lo may be
+ * NULL) is the double-length final time; max_delta_t: [N] positive values or NULL
+ * (no limit); max_steps: 0 = unlimited; res: [N] host output, may be NULL. */
+int hyx_propagate_until(hyx_batch *b, const double *tf_hi, const double *tf_lo, const double *max_delta_t,
+                        uint64_t max_steps, int mode, hyx_prop_res *res);
+
+/* Kernel time (ms, HIP events on the batch's stream) and launch count of the
+ * last hyx_propagate_until / hyx_step call. */
+int hyx_batch_last_timing(const hyx_batch *b, double *kernel_ms, uint64_t *launches);
+/* Persistent-form tuning: waves per CU (0 = occupancy limit) and the idle-lane
+ * count that triggers a refill (1..64). */
+int hyx_batch_set_persistent_config(hyx_batch *b, uint32_t waves_per_cu, uint32_t refill_min);
+/* Persistent form: iterations one instance may take per launch before it is
+ * written back and the kernel relaunched (default 65536). Bounds the duration of
+ * every launch; results do not depend on it. */
+int hyx_batch_set_iter_budget(hyx_batch *b, uint64_t iters);
+/* Persistent-form instance order (a permutation of 0..N-1), or NULL for identity. */
+int hyx_batch_set_queue(hyx_batch *b, const uint32_t *order);
+int hyx_batch_sync(hyx_batch *b);
+
+const char *hyx_last_error(void);
+/* Number of HIP devices visible (0 without a GPU). */
+int hyx_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

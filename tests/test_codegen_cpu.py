@@ -1,0 +1,112 @@
+"""The generated Taylor-step function, compiled for the CPU, against the oracle.
+
+libhyx.so generates HIP source for hyx_taylor_step<WTC>() from the op-list. The
+same function body is plain C++ apart from a few device qualifiers, so it is
+compiled here with g++ (qualifiers defined away, fma via the C library) and one
+step is compared with the oracle's: Taylor coefficients, h and the new state.
+This checks the code generator on the CPU, for both code forms, without a GPU.
+"""
+
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import heyoka_amd as hy
+import pyoracle as O
+from heyoka_amd import _lib
+from heyoka_amd import workloads as W
+
+HARNESS = r"""
+#include <cmath>
+#include <cstdint>
+#include <math.h>  // sincos (GNU)
+#define __device__
+#define __forceinline__ inline
+#define __constant__ static const
+using std::sqrt; using std::pow; using std::exp; using std::log; using std::fabs; using std::isfinite;
+
+#include "prelude.inc"
+extern "C" int step_cpu(const double *x_in, const double *pv_in, double t0, double max_dt, double *x_out,
+                        double *h, int *nf, double *tc, double *tape)
+{
+    double x[HYX_N];
+    double pv[HYX_NPAR];
+    for (int i = 0; i < HYX_N; ++i) x[i] = x_in[i];
+    for (int i = 0; i < HYX_NPAR; ++i) pv[i] = pv_in[i];
+    bool nfb = false;
+    hyx_taylor_step<true>(x, pv, t0, max_dt, *h, nfb, tc, 1, tape, 1);
+    for (int i = 0; i < HYX_N; ++i) x_out[i] = x[i];
+    *nf = nfb;
+    return 0;
+}
+"""
+
+
+def _build(src, tmp_path, tag):
+    prelude = src.split("// Hand-written CDNA4 kernels")[0]
+    prelude = prelude.replace("typedef unsigned long long hyx_u64;", "typedef unsigned long long hyx_u64;\n")
+    d = tmp_path / tag
+    d.mkdir()
+    (d / "prelude.inc").write_text(prelude)
+    (d / "h.cpp").write_text(HARNESS)
+    so = d / "h.so"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-ffp-contract=off", "-shared", "-fPIC", "-o", str(so), str(d / "h.cpp")],
+                   check=True, capture_output=True)
+    L = ct.CDLL(str(so))
+    P = ct.POINTER(ct.c_double)
+    L.step_cpu.argtypes = [P, P, ct.c_double, ct.c_double, P, P, ct.POINTER(ct.c_int), P, P]
+    return L
+
+
+CASES = {
+    "pendulum": (lambda: W.pendulum(), 0),
+    "fdp": (None, 0),
+    "cr3bp": (lambda: W.cr3bp(4), 0),
+    "nbody6": (lambda: W.nbody6(2), 0),
+    "j2var": (lambda: W.j2var(2), 0),
+}
+
+
+def _case(name):
+    if name == "fdp":
+        x, v = hy.make_vars("x", "v")
+        sys_ = [(x, v), (v, hy.cos(hy.time) - hy.par[0] * v - hy.sin(x))]
+        return sys_, np.array([[0.03], [1.88]]), np.array([[0.13]]), 0.7, 2.220446049250313e-16
+    wl = CASES[name][0]()
+    sys_ = wl.sys.sys if hasattr(wl.sys, "n_orig_sv") else wl.sys
+    pars = None if wl.pars is None else wl.pars[:, :1]
+    return sys_, wl.state[:, :1], pars, 0.0, wl.tol
+
+
+@pytest.mark.parametrize("compact", [False, True])
+@pytest.mark.parametrize("name", list(CASES))
+def test_generated_step_matches_oracle(name, compact, tmp_path):
+    if name in ("nbody6", "j2var") and not compact:
+        pytest.skip("large systems use the compact form; their unrolled step takes minutes to compile")
+    sys_, st, pars, t0, tol = _case(name)
+    dec = hy.taylor_decompose(sys_)
+    p = hy.order_from_tol(tol)
+    s = _lib.System(dec, p, compact=compact, source_only=True)
+    L = _build(s.source(), tmp_path, f"{name}_{int(compact)}")
+    n = dec.n_state
+    xin = np.ascontiguousarray(st[:, 0], dtype=np.float64)
+    pv = np.ascontiguousarray(pars[:, 0] if pars is not None else np.zeros(1), dtype=np.float64)
+    xout = np.zeros(n)
+    h = ct.c_double()
+    nf = ct.c_int()
+    tc = np.zeros(n * (p + 1))
+    tape = np.zeros(dec.n_nodes * (p + 1))
+    ptr = lambda a: a.ctypes.data_as(ct.POINTER(ct.c_double))
+    L.step_cpu(ptr(xin), ptr(pv), t0, np.inf, ptr(xout), ct.byref(h), ct.byref(nf), ptr(tc), ptr(tape))
+    r = O.step_batch(dec, p, st, pars=pars, t_hi=[t0], want_tc=True)
+    tco = r["tc"][:, :, 0]
+    tcg = tc.reshape(n, p + 1)
+    scale = np.maximum(np.abs(tco), np.max(np.abs(tco), axis=1, keepdims=True) * 1e-3) + 1e-300
+    rel = np.max(np.abs(tcg - tco) / scale)
+    assert rel < 1e-11, (name, compact, rel, np.unravel_index(np.argmax(np.abs(tcg - tco) / scale), tco.shape))
+    assert abs(h.value - r["h"][0]) <= 1e-13 * abs(r["h"][0])
+    np.testing.assert_allclose(xout, r["state"][:, 0], rtol=1e-13, atol=1e-15)
+    assert nf.value == 0

@@ -175,3 +175,28 @@ def test_fdp_dense_output_from_tc(fdp, goldens):
             acc = acc * t + r["tc"][i, k, :]
         out[i] = acc
     np.testing.assert_allclose(out, g["out"], rtol=0, atol=5e-9)
+
+
+def test_variational_order2_goldens(goldens):
+    # var_ode_sys(FDP, var_args.vars, order=2) from [0.2, 0.3], par 0.4, to t = 3
+    # (doc/notebooks/var_ode_sys.ipynb:299,361,529): pins the variational
+    # formulation (layout + equations) together with the propagate loop.
+    x, v = hy.make_vars("x", "v")
+    vs = hy.var_ode_sys([(x, v), (v, hy.cos(hy.time) - hy.par[0] * v - hy.sin(x))], hy.var_args.vars, order=2)
+    dec = hy.taylor_decompose(vs.sys)
+    g = goldens["var2_state_t3"]
+    st = np.array(g["ic"] + vs.initial_derivatives())[:, None]
+    r = O.propagate_until(dec, 20, st, 3.0, pars=[[0.4]])
+    np.testing.assert_allclose(r["state"][:, 0], g["state"], rtol=0, atol=5e-9)
+    sl = vs.get_vslice(2)
+    assert (sl.start, sl.stop) == (6, 12)
+    assert [vs.get_mindex(i) for i in range(sl.start, sl.stop)] == goldens["var2_order2"]["mindex"]
+    np.testing.assert_allclose(r["state"][sl, 0], goldens["var2_order2"]["values"], rtol=1e-13, atol=0)
+    # The order-1 + order-2 Taylor map reproduces the displaced non-variational run
+    # (var_ode_sys.ipynb:650-651, 707: error ~6.6e-13).
+    dx = np.array([goldens["var2_displaced"]["dx"], goldens["var2_displaced"]["dv"]])
+    sv = r["state"][:, 0]
+    approx = sv[:2] + sv[2:6].reshape(2, 2) @ dx
+    o2 = sv[6:12].reshape(2, 3)
+    approx = approx + o2 @ np.array([dx[0] ** 2 / 2, dx[0] * dx[1], dx[1] ** 2 / 2])
+    np.testing.assert_allclose(approx, goldens["var2_displaced"]["nonvar_state"], rtol=0, atol=5e-9)
