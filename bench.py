@@ -150,7 +150,7 @@ def main():
                          f"cr3bp ragged ensemble: {N} instances, 4 states + par[0], tol 1e-15, horizons 10**U(1,4)"),
             "instances": N,
             "order": order,
-            "code_form": "compact" if system.compact else "unrolled",
+            "code_form": system.form,
             "form": "persistent",
             "algorithmic_flops_per_instance_step": flops,
             "hiprtc_compile_s": round(t_compile, 3),

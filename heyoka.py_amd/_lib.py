@@ -21,6 +21,8 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 FIELD_STATE, FIELD_PARS, FIELD_TIME_HI, FIELD_TIME_LO = 0, 1, 2, 3
 MODE_PERSISTENT, MODE_PLAIN, MODE_GRAPH = 0, 1, 2
 FLAG_HIGH_ACCURACY, FLAG_COMPILE_ONLY, FLAG_FORCE_COMPACT, FLAG_FORCE_UNROLLED, FLAG_SOURCE_ONLY = 1, 2, 4, 8, 16
+FLAG_FORCE_GROUP = 32
+FORMS = {0: "unrolled", 1: "compact", 2: "group"}
 
 
 class prop_res(ct.Structure):
@@ -62,6 +64,7 @@ def lib():
             "hyx_sys_dim": (ct.c_int, [vp]),
             "hyx_sys_npar": (ct.c_int, [vp]),
             "hyx_sys_is_compact": (ct.c_int, [vp]),
+            "hyx_sys_code_form": (ct.c_int, [vp]),
             "hyx_sys_source": (ct.c_size_t, [vp, ct.c_char_p, ct.c_size_t]),
             "hyx_sys_kernel_info": (ct.c_int, [vp, P(ct.c_int), P(ct.c_int)]),
             "hyx_sys_destroy": (None, [vp]),
@@ -92,6 +95,7 @@ def lib():
 
 EXPORTED = (
     "hyx_sys_create", "hyx_sys_precompile", "hyx_sys_order", "hyx_sys_dim", "hyx_sys_npar", "hyx_sys_is_compact",
+    "hyx_sys_code_form",
     "hyx_sys_source",
     "hyx_sys_kernel_info", "hyx_sys_destroy", "hyx_batch_create", "hyx_batch_destroy", "hyx_batch_size",
     "hyx_batch_write", "hyx_batch_read", "hyx_batch_device_ptr", "hyx_step", "hyx_read_tc", "hyx_propagate_until",
@@ -118,11 +122,14 @@ def oplist_args(dec):
 
 
 def _flags(high_accuracy=False, compact=None):
+    """compact: None (automatic), True / "compact", False / "unrolled", or "group"."""
     f = FLAG_HIGH_ACCURACY if high_accuracy else 0
-    if compact is True:
+    if compact is True or compact == "compact":
         f |= FLAG_FORCE_COMPACT
-    elif compact is False:
+    elif compact is False or compact == "unrolled":
         f |= FLAG_FORCE_UNROLLED
+    elif compact == "group":
+        f |= FLAG_FORCE_GROUP
     return f
 
 
@@ -156,6 +163,7 @@ class System:
         self.npar = dec.n_par
         self._L = L
         self.compact = bool(L.hyx_sys_is_compact(self.h))
+        self.form = FORMS[int(L.hyx_sys_code_form(self.h))]
 
     def source(self) -> str:
         n = self._L.hyx_sys_source(self.h, None, 0)

@@ -34,6 +34,33 @@ namespace hyx
 
 static const int INF_DEG = 1 << 20;
 
+void compute_degrees(Sys &S)
+{
+    // Highest order with a possibly non-zero coefficient; state variables are full series.
+    S.deg.assign(S.nnodes, INF_DEG);
+    auto od = [&](uint32_t a) {
+        return HYX_OPERAND_KIND(a) == HYX_KIND_NODE ? S.deg[HYX_OPERAND_INDEX(a)] : 0;
+    };
+    for (uint32_t j = S.n; j < S.nnodes; ++j) {
+        const auto &A = S.args[j];
+        int d = 0;
+        switch (S.op[j]) {
+        case HYX_OP_TIME: d = 1; break;
+        case HYX_OP_ADD:
+        case HYX_OP_SUB:
+        case HYX_OP_SUM:
+        case HYX_OP_NEG:
+            for (auto a : A) d = std::max(d, od(a));
+            break;
+        case HYX_OP_MUL: d = std::min(INF_DEG, od(A[0]) + od(A[1])); break;
+        case HYX_OP_SQUARE: d = std::min(INF_DEG, 2 * od(A[0])); break;
+        case HYX_OP_DIV: d = (od(A[1]) == 0) ? od(A[0]) : INF_DEG; break;
+        default: d = (od(A[0]) == 0) ? 0 : INF_DEG;
+        }
+        S.deg[j] = d;
+    }
+}
+
 int parse_oplist(const uint32_t *w, uint32_t n_words, const double *consts, uint32_t n_consts, Sys &S,
                  std::string &err)
 {
@@ -122,29 +149,7 @@ int parse_oplist(const uint32_t *w, uint32_t n_words, const double *consts, uint
             }
         }
     }
-    // Degrees: highest order with a possibly non-zero coefficient.
-    S.deg.assign(S.nnodes, INF_DEG);
-    auto od = [&](uint32_t a) {
-        return HYX_OPERAND_KIND(a) == HYX_KIND_NODE ? S.deg[HYX_OPERAND_INDEX(a)] : 0;
-    };
-    for (uint32_t j = S.n; j < S.nnodes; ++j) {
-        const auto &A = S.args[j];
-        int d = 0;
-        switch (S.op[j]) {
-        case HYX_OP_TIME: d = 1; break;
-        case HYX_OP_ADD:
-        case HYX_OP_SUB:
-        case HYX_OP_SUM:
-        case HYX_OP_NEG:
-            for (auto a : A) d = std::max(d, od(a));
-            break;
-        case HYX_OP_MUL: d = std::min(INF_DEG, od(A[0]) + od(A[1])); break;
-        case HYX_OP_SQUARE: d = std::min(INF_DEG, 2 * od(A[0])); break;
-        case HYX_OP_DIV: d = (od(A[1]) == 0) ? od(A[0]) : INF_DEG; break;
-        default: d = (od(A[0]) == 0) ? 0 : INF_DEG;
-        }
-        S.deg[j] = d;
-    }
+    compute_degrees(S);
     return 0;
 }
 
@@ -167,15 +172,18 @@ struct Gen {
     std::ostringstream o;
     std::set<std::pair<uint32_t, int>> zset; // (node, k) known to be exactly zero
     std::set<std::pair<uint32_t, int>> dvars; // generated j * a^[j] values
+    std::vector<char> perlane;                // constants held per lane (kc_prefix[]) instead of literals
+    std::string kc_prefix = "kc";
 
-    Gen(const Sys &s, int p, bool ha) : S(s), P(p), high_acc(ha) {}
+    Gen(const Sys &s, int p, bool ha) : S(s), P(p), high_acc(ha), perlane(s.nconsts, 0) {}
 
     static std::string cv(uint32_t node, int k) { return "c" + std::to_string(node) + "_" + std::to_string(k); }
 
     bool zero(uint32_t w, int k) const
     {
         const uint32_t kind = HYX_OPERAND_KIND(w), idx = HYX_OPERAND_INDEX(w);
-        if (kind != HYX_KIND_NODE) return k > 0 || (kind == HYX_KIND_CONST && S.consts[idx] == 0.0);
+        if (kind != HYX_KIND_NODE)
+            return k > 0 || (kind == HYX_KIND_CONST && !perlane[idx] && S.consts[idx] == 0.0);
         if (k > S.deg[idx]) return true;
         return zset.count({idx, k}) != 0;
     }
@@ -192,7 +200,8 @@ struct Gen {
         const uint32_t kind = HYX_OPERAND_KIND(w), idx = HYX_OPERAND_INDEX(w);
         if (kind == HYX_KIND_NODE) return cv(idx, k);
         if (k > 0) return "0.0";
-        if (kind == HYX_KIND_CONST) return lit(S.consts[idx]);
+        if (kind == HYX_KIND_CONST)
+            return perlane[idx] ? kc_prefix + "[" + std::to_string(idx) + "]" : lit(S.consts[idx]);
         return "pv[" + std::to_string(idx) + "]";
     }
 
@@ -751,6 +760,385 @@ struct CompactGen {
 };
 
 } // namespace
+
+
+// ---------------------------------------------------------------------------------------------
+// Group form planning.
+//
+// The right-hand side of every state variable is flattened, through linear
+// nodes (add, sub, neg, sum), into signed terms: state variables, numbers and
+// "output" nodes. The remaining nodes are split into clusters, the connected
+// components of the dependency graph restricted to them. If all clusters are
+// isomorphic (same ops in the same order, same exponent constants, same output
+// positions; inputs and other constants may differ), one template cluster is
+// generated and each lane of a group evaluates one cluster with its own input
+// and constant bindings.
+
+namespace
+{
+
+struct UF {
+    std::vector<uint32_t> p;
+    explicit UF(uint32_t n) : p(n)
+    {
+        for (uint32_t i = 0; i < n; ++i) p[i] = i;
+    }
+    uint32_t find(uint32_t x)
+    {
+        while (p[x] != x) x = p[x] = p[p[x]];
+        return x;
+    }
+    void join(uint32_t a, uint32_t b) { p[find(a)] = find(b); }
+};
+
+bool is_linear(uint32_t op) { return op == HYX_OP_ADD || op == HYX_OP_SUB || op == HYX_OP_NEG || op == HYX_OP_SUM; }
+
+} // namespace
+
+bool plan_groups(const Sys &S, GroupPlan &gp, std::string &why)
+{
+    const uint32_t n = S.n, NN = S.nnodes;
+    if (NN == n) {
+        why = "no nodes";
+        return false;
+    }
+    std::vector<char> internal(NN, 0);
+    std::vector<std::vector<std::pair<uint32_t, double>>> terms(n);
+    std::vector<double> rconst(n, 0.0);
+    for (uint32_t i = 0; i < n; ++i) {
+        std::vector<std::pair<uint32_t, double>> stack{{S.rhs[i], 1.0}};
+        while (!stack.empty()) {
+            auto [w, sg] = stack.back();
+            stack.pop_back();
+            const uint32_t kind = HYX_OPERAND_KIND(w), idx = HYX_OPERAND_INDEX(w);
+            if (kind == HYX_KIND_CONST) {
+                rconst[i] += sg * S.consts[idx];
+                continue;
+            }
+            if (kind == HYX_KIND_PAR) {
+                why = "a runtime parameter appears as a right-hand-side term";
+                return false;
+            }
+            if (idx < n || !is_linear(S.op[idx]) || S.deg[idx] == 0) {
+                terms[i].push_back({w, sg});
+                continue;
+            }
+            internal[idx] = 1;
+            const auto &A = S.args[idx];
+            for (size_t q = A.size(); q-- > 0;) {
+                const bool neg = S.op[idx] == HYX_OP_NEG || (S.op[idx] == HYX_OP_SUB && q == 1);
+                stack.push_back({A[q], neg ? -sg : sg});
+            }
+        }
+    }
+    for (uint32_t j = n; j < NN; ++j) {
+        if (internal[j]) continue;
+        for (auto a : S.args[j])
+            if (HYX_OPERAND_KIND(a) == HYX_KIND_NODE && internal[HYX_OPERAND_INDEX(a)]) {
+                why = "a linear right-hand-side node is also used inside a cluster";
+                return false;
+            }
+    }
+    UF uf(NN);
+    for (uint32_t j = n; j < NN; ++j) {
+        if (internal[j]) continue;
+        for (auto a : S.args[j]) {
+            const uint32_t idx = HYX_OPERAND_INDEX(a);
+            if (HYX_OPERAND_KIND(a) == HYX_KIND_NODE && idx >= n) uf.join(j, idx);
+        }
+        if (S.op[j] == HYX_OP_SIN || S.op[j] == HYX_OP_COS) uf.join(j, S.aux[j]);
+    }
+    // Clusters, ordered by their first node.
+    std::vector<int> cid(NN, -1);
+    std::vector<std::vector<uint32_t>> cl;
+    for (uint32_t j = n; j < NN; ++j) {
+        if (internal[j]) continue;
+        const uint32_t r = uf.find(j);
+        if (cid[r] < 0) {
+            cid[r] = (int)cl.size();
+            cl.push_back({});
+        }
+        cid[j] = cid[r];
+        cl[cid[j]].push_back(j);
+    }
+    const int ncl = (int)cl.size();
+    if (ncl < 2) {
+        why = "fewer than two clusters";
+        return false;
+    }
+    // Output nodes: node terms of any right-hand side.
+    std::vector<char> is_out(NN, 0);
+    for (uint32_t i = 0; i < n; ++i)
+        for (auto &t : terms[i]) {
+            const uint32_t idx = HYX_OPERAND_INDEX(t.first);
+            if (HYX_OPERAND_KIND(t.first) == HYX_KIND_NODE && idx >= n) is_out[idx] = 1;
+        }
+    // Signature and bindings of each cluster.
+    struct Bind {
+        std::vector<uint32_t> inputs;   // state variables by input slot
+        std::vector<uint32_t> kconsts;  // const indices by constant slot
+        std::vector<uint32_t> outs;     // output nodes by output slot
+        std::string sig;
+    };
+    std::vector<Bind> B(ncl);
+    for (int c = 0; c < ncl; ++c) {
+        auto &b = B[c];
+        std::vector<int> pos(NN, -1);
+        for (size_t q = 0; q < cl[c].size(); ++q) pos[cl[c][q]] = (int)q;
+        std::ostringstream sg;
+        for (uint32_t j : cl[c]) {
+            sg << S.op[j] << ":";
+            for (auto a : S.args[j]) {
+                const uint32_t kind = HYX_OPERAND_KIND(a), idx = HYX_OPERAND_INDEX(a);
+                if (kind == HYX_KIND_NODE && idx >= n) {
+                    sg << "L" << pos[idx];
+                } else if (kind == HYX_KIND_NODE) {
+                    size_t s2 = std::find(b.inputs.begin(), b.inputs.end(), idx) - b.inputs.begin();
+                    if (s2 == b.inputs.size()) b.inputs.push_back(idx);
+                    sg << "I" << s2;
+                } else if (kind == HYX_KIND_CONST) {
+                    size_t s2 = std::find(b.kconsts.begin(), b.kconsts.end(), idx) - b.kconsts.begin();
+                    if (s2 == b.kconsts.size()) b.kconsts.push_back(idx);
+                    sg << "K" << s2;
+                } else {
+                    sg << "P" << idx;
+                }
+                sg << ",";
+            }
+            if (S.op[j] == HYX_OP_POW) sg << "E" << lit(S.consts[S.aux[j]]);
+            if (S.op[j] == HYX_OP_SIN || S.op[j] == HYX_OP_COS) sg << "S" << pos[S.aux[j]];
+            sg << "|";
+        }
+        sg << "O";
+        for (uint32_t j : cl[c])
+            if (is_out[j]) {
+                b.outs.push_back(j);
+                sg << pos[j] << ",";
+            }
+        b.sig = sg.str();
+    }
+    for (int c = 1; c < ncl; ++c)
+        if (B[c].sig != B[0].sig) {
+            why = "the clusters are not isomorphic";
+            return false;
+        }
+    int G = 1;
+    while (G < ncl) G <<= 1;
+    if (G > 64) {
+        why = "more than 64 clusters";
+        return false;
+    }
+    const int own = (int)((n + G - 1) / G);
+    if (own > 8) {
+        why = "too many state variables per lane";
+        return false;
+    }
+    gp = GroupPlan{};
+    gp.G = G;
+    gp.ncl = ncl;
+    gp.nin = (int)B[0].inputs.size();
+    gp.nout = (int)B[0].outs.size();
+    gp.own = own;
+    // Template: cluster 0 with inputs as state variables 0..nin-1.
+    {
+        Sys &T = gp.tmpl;
+        const auto &c0 = cl[0];
+        const uint32_t nin = gp.nin;
+        T.n = nin;
+        T.npar = S.npar;
+        T.nnodes = nin + (uint32_t)c0.size();
+        T.op.assign(T.nnodes, 0);
+        T.aux.assign(T.nnodes, 0);
+        T.args.assign(T.nnodes, {});
+        std::vector<int> pos(NN, -1);
+        for (size_t q = 0; q < c0.size(); ++q) pos[c0[q]] = (int)q;
+        // Constants: per-lane slots first, then the pow exponents.
+        T.consts.clear();
+        for (size_t q = 0; q < B[0].kconsts.size(); ++q) T.consts.push_back(S.consts[B[0].kconsts[q]]);
+        gp.perlane.assign(T.consts.size(), 1);
+        for (size_t q = 0; q < c0.size(); ++q) {
+            const uint32_t j = c0[q], tj = nin + (uint32_t)q;
+            T.op[tj] = S.op[j];
+            for (auto a : S.args[j]) {
+                const uint32_t kind = HYX_OPERAND_KIND(a), idx = HYX_OPERAND_INDEX(a);
+                if (kind == HYX_KIND_NODE && idx >= n)
+                    T.args[tj].push_back(HYX_OPERAND(HYX_KIND_NODE, nin + pos[idx]));
+                else if (kind == HYX_KIND_NODE)
+                    T.args[tj].push_back(HYX_OPERAND(
+                        HYX_KIND_NODE, std::find(B[0].inputs.begin(), B[0].inputs.end(), idx) - B[0].inputs.begin()));
+                else if (kind == HYX_KIND_CONST)
+                    T.args[tj].push_back(HYX_OPERAND(
+                        HYX_KIND_CONST,
+                        std::find(B[0].kconsts.begin(), B[0].kconsts.end(), idx) - B[0].kconsts.begin()));
+                else
+                    T.args[tj].push_back(a);
+            }
+            if (S.op[j] == HYX_OP_POW) {
+                T.aux[tj] = (uint32_t)T.consts.size();
+                T.consts.push_back(S.consts[S.aux[j]]);
+                gp.perlane.push_back(0);
+            }
+            if (S.op[j] == HYX_OP_SIN || S.op[j] == HYX_OP_COS) T.aux[tj] = nin + pos[S.aux[j]];
+        }
+        T.nconsts = (uint32_t)T.consts.size();
+        compute_degrees(T);
+        for (uint32_t j : B[0].outs) gp.outs.push_back(nin + pos[j]);
+    }
+    gp.nkc = std::max<int>(1, (int)gp.tmpl.nconsts);
+    // Per-lane bindings.
+    gp.iv.assign((size_t)G * std::max(gp.nin, 1), 0);
+    gp.kc.assign((size_t)G * gp.nkc, 0.0);
+    for (int c = 0; c < ncl; ++c) {
+        for (int q = 0; q < gp.nin; ++q) gp.iv[(size_t)c * std::max(gp.nin, 1) + q] = (int)B[c].inputs[q];
+        for (size_t q = 0; q < B[c].kconsts.size(); ++q) gp.kc[(size_t)c * gp.nkc + q] = S.consts[B[c].kconsts[q]];
+        for (size_t q = B[c].kconsts.size(); q < gp.tmpl.nconsts; ++q) gp.kc[(size_t)c * gp.nkc + q] = gp.tmpl.consts[q];
+    }
+    // Flat LDS layout per instance: [0, n) state coefficients, then G x nout outputs, then a zero slot.
+    gp.flat = (int)n + G * gp.nout + 1;
+    const int zero_slot = gp.flat - 1;
+    std::vector<int> outslot(NN, -1);
+    for (int c = 0; c < ncl; ++c)
+        for (size_t q = 0; q < B[c].outs.size(); ++q) outslot[B[c].outs[q]] = (int)n + c * gp.nout + (int)q;
+    int maxt = 1;
+    for (uint32_t i = 0; i < n; ++i) maxt = std::max<int>(maxt, (int)terms[i].size());
+    gp.maxt = maxt;
+    gp.ti.assign((size_t)G * own * maxt, zero_slot);
+    gp.ts.assign((size_t)G * own * maxt, 1.0);
+    gp.rc.assign((size_t)G * own, 0.0);
+    gp.ov.assign((size_t)G * own, 0);
+    for (uint32_t v = 0; v < n; ++v) {
+        const int g = (int)(v % G), o = (int)(v / G);
+        gp.ov[(size_t)g * own + o] = 1;
+        gp.rc[(size_t)g * own + o] = rconst[v];
+        for (size_t t = 0; t < terms[v].size(); ++t) {
+            const uint32_t idx = HYX_OPERAND_INDEX(terms[v][t].first);
+            const int fi = idx < n ? (int)idx : outslot[idx];
+            if (fi < 0) {
+                why = "a right-hand-side term is not a cluster output";
+                return false;
+            }
+            gp.ti[((size_t)g * own + o) * maxt + t] = fi;
+            gp.ts[((size_t)g * own + o) * maxt + t] = terms[v][t].second;
+        }
+    }
+    return true;
+}
+
+std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, bool high_acc)
+{
+    const int G = gp.G, own = gp.own, nin = gp.nin, nout = gp.nout, maxt = gp.maxt;
+    std::ostringstream h;
+    h << "#define HYX_N " << S.n << "\n#define HYX_NPAR " << std::max<uint32_t>(S.npar, 1) << "\n#define HYX_P " << P
+      << "\n#define HYX_GROUP 1\n#define HYX_G " << G << "\n#define HYX_I " << (64 / G) << "\n#define HYX_NIN "
+      << std::max(nin, 1) << "\n#define HYX_NOUT " << std::max(nout, 1) << "\n#define HYX_NKC " << gp.nkc
+      << "\n#define HYX_OWN " << own << "\n#define HYX_MAXT " << maxt << "\n#define HYX_FLAT " << gp.flat << "\n";
+    h << "typedef unsigned long long hyx_u64;\n";
+    auto itab = [&](const char *nm, const std::vector<int> &v) {
+        h << "__constant__ int " << nm << "[" << v.size() << "] = {";
+        for (size_t q = 0; q < v.size(); ++q) h << (q ? "," : "") << v[q];
+        h << "};\n";
+    };
+    auto dtab = [&](const char *nm, const std::vector<double> &v) {
+        h << "__constant__ double " << nm << "[" << v.size() << "] = {";
+        for (size_t q = 0; q < v.size(); ++q) h << (q ? "," : "") << lit(v[q]);
+        h << "};\n";
+    };
+    itab("hyx_tab_iv", gp.iv);
+    dtab("hyx_tab_kc", gp.kc);
+    itab("hyx_tab_ti", gp.ti);
+    dtab("hyx_tab_ts", gp.ts);
+    dtab("hyx_tab_rc", gp.rc);
+    itab("hyx_tab_ov", gp.ov);
+
+    Gen gen(gp.tmpl, P, high_acc);
+    gen.perlane = gp.perlane;
+    gen.kc_prefix = "tab.kc";
+    auto &o = gen.o;
+    auto xv = [](int oo, int k) { return "x" + std::to_string(oo) + "_" + std::to_string(k); };
+    o << "template <bool WTC, class TAB>\n"
+         "__device__ __forceinline__ void hyx_gstep(double (&xo)[HYX_OWN], const double (&pv)[HYX_NPAR], double t0, "
+         "double max_dt, double &h_out, bool &nf_out, int g, double *fl, const TAB &tab, double *__restrict__ tc, "
+         "hyx_u64 tcs)\n{\n";
+    o << "    (void)pv; (void)t0; (void)tc; (void)tcs;\n";
+    o << "    __syncthreads();\n";
+    for (int oo = 0; oo < own; ++oo) {
+        o << "    const double " << xv(oo, 0) << " = xo[" << oo << "];\n";
+        o << "    if (tab.ov[" << oo << "]) fl[g + " << oo * G << "] = " << xv(oo, 0) << ";\n";
+    }
+    o << "    __syncthreads();\n";
+    for (int k = 0; k < P; ++k) {
+        const std::string base = (k & 1) ? "fl[HYX_FLAT + " : "fl[";
+        const std::string next = ((k + 1) & 1) ? "fl[HYX_FLAT + " : "fl[";
+        o << "    // order " << k << "\n";
+        for (int q = 0; q < nin; ++q)
+            o << "    const double " << Gen::cv((uint32_t)q, k) << " = " << base << "tab.iv[" << q << "]];\n";
+        for (uint32_t j = gp.tmpl.n; j < gp.tmpl.nnodes; ++j) {
+            if (k == 0)
+                gen.order0(j);
+            else
+                gen.orderk(j, k);
+        }
+        for (int q = 0; q < nout; ++q) {
+            const uint32_t w = HYX_OPERAND(HYX_KIND_NODE, gp.outs[q]);
+            o << "    " << base << "HYX_N + g * HYX_NOUT + " << q << "] = " << (gen.zero(w, k) ? "0.0" : gen.val(w, k))
+              << ";\n";
+        }
+        o << "    __syncthreads();\n";
+        for (int oo = 0; oo < own; ++oo) {
+            o << "    double r" << oo << "_" << k << " = " << (k == 0 ? "tab.rc[" + std::to_string(oo) + "]" : "0.0")
+              << ";\n";
+            for (int t = 0; t < maxt; ++t)
+                o << "    r" << oo << "_" << k << " = __builtin_fma(tab.ts[" << oo * maxt + t << "], " << base << "tab.ti["
+                  << oo * maxt + t << "]], r" << oo << "_" << k << ");\n";
+            o << "    const double " << xv(oo, k + 1) << " = r" << oo << "_" << k << " * " << lit(1.0 / (k + 1)) << ";\n";
+            if (k + 1 < P) o << "    if (tab.ov[" << oo << "]) " << next << "g + " << oo * G << "] = " << xv(oo, k + 1) << ";\n";
+        }
+        if (k + 1 < P) o << "    __syncthreads();\n";
+    }
+    o << "    if constexpr (WTC) if (tc) {\n";
+    for (int oo = 0; oo < own; ++oo) {
+        o << "        if (tab.ov[" << oo << "]) {\n";
+        for (int k = 0; k <= P; ++k)
+            o << "            tc[((hyx_u64)(g + " << oo * G << ") * " << (P + 1) << " + " << k << ") * tcs] = " << xv(oo, k)
+              << ";\n";
+        o << "        }\n";
+    }
+    o << "    }\n";
+    o << "    double mx = 0.0, mo = 0.0, mom1 = 0.0, ab;\n";
+    for (int oo = 0; oo < own; ++oo) {
+        o << "    ab = fabs(" << xv(oo, 0) << "); mx = (ab > mx) ? ab : mx;\n";
+        o << "    ab = fabs(" << xv(oo, P) << "); mo = (ab > mo) ? ab : mo;\n";
+        o << "    ab = fabs(" << xv(oo, P - 1) << "); mom1 = (ab > mom1) ? ab : mom1;\n";
+    }
+    o << "#pragma unroll\n    for (int off = 1; off < HYX_G; off <<= 1) {\n"
+         "        ab = __shfl_xor(mx, off); mx = (ab > mx) ? ab : mx;\n"
+         "        ab = __shfl_xor(mo, off); mo = (ab > mo) ? ab : mo;\n"
+         "        ab = __shfl_xor(mom1, off); mom1 = (ab > mom1) ? ab : mom1;\n    }\n";
+    o << "    const double num = (mx <= 1.0) ? 1.0 : mx;\n";
+    o << "    const double rho_o = pow(num / mo, " << lit(1.0 / P) << ");\n";
+    o << "    const double rho_om1 = pow(num / mom1, " << lit(1.0 / (P - 1)) << ");\n";
+    o << "    const double rho_m = (rho_om1 < rho_o) ? rho_om1 : rho_o;\n";
+    const double e1 = std::exp(1.0);
+    const double rhofac = std::exp(-7.0 / 10.0 / (double)(P - 1)) / (e1 * e1);
+    o << "    double h = rho_m * " << lit(rhofac) << ";\n";
+    o << "    const double lim = fabs(max_dt);\n    h = (h < lim) ? h : lim;\n    if (max_dt < 0.0) h = -h;\n";
+    o << "    int nf = 0;\n";
+    for (int oo = 0; oo < own; ++oo) {
+        if (!high_acc) {
+            o << "    { double r = " << xv(oo, P) << ";\n";
+            for (int k = P - 1; k >= 0; --k) o << "      r = __builtin_fma(r, h, " << xv(oo, k) << ");\n";
+        } else {
+            o << "    { double r = " << xv(oo, 0) << ", cp = 0.0, hk = 1.0;\n";
+            for (int k = 1; k <= P; ++k)
+                o << "      { hk *= h; const double y = " << xv(oo, k)
+                  << " * hk - cp; const double tt = r + y; cp = (tt - r) - y; r = tt; }\n";
+        }
+        o << "      xo[" << oo << "] = r; nf |= (tab.ov[" << oo << "] && !isfinite(r)) ? 1 : 0; }\n";
+    }
+    o << "#pragma unroll\n    for (int off = 1; off < HYX_G; off <<= 1) nf |= __shfl_xor(nf, off);\n";
+    o << "    h_out = h;\n    nf_out = nf != 0;\n}\n";
+    return h.str() + o.str();
+}
 
 int history_nodes(const Sys &S)
 {

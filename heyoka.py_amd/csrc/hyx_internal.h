@@ -19,6 +19,8 @@ struct Sys {
     std::vector<int> deg;                        // highest possibly non-zero order, per node
 };
 
+void compute_degrees(Sys &S);
+
 int parse_oplist(const uint32_t *w, uint32_t n_words, const double *consts, uint32_t n_consts, Sys &S,
                  std::string &err);
 
@@ -28,6 +30,24 @@ std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compac
 int history_nodes(const Sys &S);
 // Default choice of code form: compact when the histories exceed the register file.
 bool choose_compact(const Sys &S, int order);
+
+// Group form: a template cluster evaluated by each lane of a group of G lanes
+// per instance, with per-lane bindings (see plan_groups in hyx_codegen.cpp).
+struct GroupPlan {
+    int G = 0, ncl = 0, nin = 0, nout = 0, nkc = 1, own = 0, maxt = 1, flat = 0;
+    Sys tmpl;                       // cluster template: inputs are its state variables 0..nin-1
+    std::vector<char> perlane;      // template constant held per lane
+    std::vector<uint32_t> outs;     // template nodes feeding right-hand sides, by output slot
+    std::vector<int> iv;            // [G][nin] state variable of each input
+    std::vector<double> kc;         // [G][nkc] constants
+    std::vector<int> ti;            // [G][own][maxt] flat LDS index of each rhs term
+    std::vector<double> ts;         // [G][own][maxt] term signs
+    std::vector<double> rc;         // [G][own] constant rhs parts
+    std::vector<int> ov;            // [G][own] owned variable exists
+};
+
+bool plan_groups(const Sys &S, GroupPlan &gp, std::string &why);
+std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int order, bool high_acc);
 
 // Exact hexadecimal double literal.
 std::string lit(double v);

@@ -1,0 +1,96 @@
+// Hand-written CDNA4 kernels of the batch/ensemble Taylor integrator: the part
+// shared by the lane form (hyx_klane.hip) and the group form (hyx_kgroup.hip).
+//
+// These files are compiled at run time with hipRTC, after a per-system prelude
+// that the runtime generates from the op-list (hyx_codegen.cpp).
+//
+// Layout in HBM: every per-instance array is structure-of-arrays with the
+// instance index fastest (state[i * ld + inst]), so the 64 lanes of a wave
+// touch 512 contiguous bytes per access.
+//
+// Semantics follow heyoka's batch integrator (heyoka/expose_batch_integrators.cpp):
+//   - step(max_dt): |h| clamped to |max_dt|, sign of max_dt; outcome
+//     time_limit when h == max_dt, err_nf_state on a non-finite state/time
+//     (:233-242, doc/notebooks/Batch mode overview.ipynb:242-245,320);
+//   - propagate_until(t): steps clamped to the remaining time and to
+//     max_delta_t, double-length time, per-instance (outcome, min_h, max_h,
+//     nsteps) with min_h/max_h over unclamped steps only (:256-314, :393).
+// Each instance is independent: one instance's error never stops another.
+
+typedef unsigned long long hyx_u64;
+typedef long long hyx_i64;
+
+#define HYX_SUCCESS (-4294967297LL)
+#define HYX_STEP_LIMIT (-4294967298LL)
+#define HYX_TIME_LIMIT (-4294967299LL)
+#define HYX_ERR_NF_STATE (-4294967300LL)
+
+// Kernel argument block. The host fills the identical struct (hyx_runtime.cpp).
+struct hyx_kargs {
+    double *state;          // [n][ld]
+    const double *pars;     // [npar][ld]
+    double *t_hi;           // [N]
+    double *t_lo;           // [N]
+    const double *tf_hi;    // [N] final time (propagate)
+    const double *tf_lo;    // [N]
+    const double *max_dt;   // [N] step: signed clamp; propagate: max_delta_t > 0
+    double *h_out;          // [N] step: h taken
+    hyx_i64 *outcome;       // [N]
+    double *min_h;          // [N]
+    double *max_h;          // [N]
+    hyx_u64 *nsteps;        // [N]
+    hyx_u64 *iters;         // [N] plain form: iterations taken so far
+    unsigned *status;       // [N] plain form: 0 active forward, 2 active backward, 1 done
+    double *tc;             // [n][P+1][ld] Taylor coefficients (step with write_tc)
+    double *tape;           // per-lane scratch tape for compact code (may be null)
+    hyx_u64 *work_counter;  // persistent form: next queue position
+    unsigned *active_count; // plain form: active instances after this launch
+    const unsigned *queue;  // persistent form: instance order (null = identity)
+    hyx_u64 N;
+    hyx_u64 ld;
+    hyx_u64 max_steps;      // 0 = unlimited
+    hyx_u64 tape_stride;    // lanes per tape row
+    hyx_u64 iter_budget;    // persistent form: iterations per instance per launch
+    unsigned *unfinished;   // persistent form: instances suspended by the budget
+    unsigned refill_min;    // persistent form: refill when this many lanes idle
+    unsigned pad_;
+};
+
+// ---- double-length time -----------------------------------------------------------
+// Knuth's error-free sum (TAOCP vol. 2, 4.2.2).
+__device__ __forceinline__ void hyx_two_sum(double a, double b, double &s, double &e)
+{
+    double x = a + b;
+    double z = x - a;
+    e = (a - (x - z)) + (b - z);
+    s = x;
+}
+
+// Dekker's fast error-free sum, |a| >= |b|.
+__device__ __forceinline__ void hyx_fast_two_sum(double a, double b, double &s, double &e)
+{
+    double x = a + b;
+    e = b - (x - a);
+    s = x;
+}
+
+// (hi, lo) += (bh, bl), renormalised.
+__device__ __forceinline__ void hyx_dfl_add(double &hi, double &lo, double bh, double bl)
+{
+    double s, e, t, f;
+    hyx_two_sum(hi, bh, s, e);
+    hyx_two_sum(lo, bl, t, f);
+    e += t;
+    hyx_fast_two_sum(s, e, s, e);
+    e += f;
+    hyx_fast_two_sum(s, e, hi, lo);
+}
+
+// hi part of (ah, al) - (bh, bl).
+__device__ __forceinline__ double hyx_dfl_sub_hi(double ah, double al, double bh, double bl)
+{
+    double hi = ah, lo = al;
+    hyx_dfl_add(hi, lo, -bh, -bl);
+    return hi;
+}
+

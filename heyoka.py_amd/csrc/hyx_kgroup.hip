@@ -1,0 +1,285 @@
+// Group form: HYX_G lanes cooperate on one instance, HYX_I = 64 / HYX_G
+// instances per wavefront (hand-written CDNA4 kernels, compiled with hipRTC
+// after hyx_kcommon.hip).
+//
+// The generated prelude defines the group plan and
+//   hyx_gstep<WTC>(xo, pv, t0, max_dt, h, nf, g, fl, tab, tc, tcs)
+// which advances one instance by one adaptive step. Lane g of the group
+// evaluates cluster g of the decomposition (one of the isomorphic subgraphs
+// the code generator found, e.g. one body pair of an N-body system) with its
+// coefficient histories in registers, and owns the state variables
+// v = g + o * HYX_G (o < HYX_OWN). Per order, coefficients are exchanged
+// through this instance's LDS block `fl` (two buffers of HYX_FLAT doubles:
+// state coefficients, cluster outputs, a zero slot).
+//
+// Every lane of the wave executes every step, also lanes whose group has no
+// instance (their results are not stored): hyx_gstep contains workgroup
+// barriers, and a block is exactly one wavefront.
+
+struct hyx_gtab {
+    int iv[HYX_NIN];                    // state variable behind each cluster input
+    double kc[HYX_NKC];                 // per-lane constants of the cluster
+    int ti[HYX_OWN * HYX_MAXT];         // right-hand-side terms of the owned variables (flat indices)
+    double ts[HYX_OWN * HYX_MAXT];      // their signs
+    double rc[HYX_OWN];                 // constant part of the owned right-hand sides
+    int ov[HYX_OWN];                    // owned variable exists (v < HYX_N)
+};
+
+__device__ __forceinline__ void hyx_gtab_load(hyx_gtab &t, int g)
+{
+#pragma unroll
+    for (int q = 0; q < HYX_NIN; ++q) t.iv[q] = hyx_tab_iv[g * HYX_NIN + q];
+#pragma unroll
+    for (int q = 0; q < HYX_NKC; ++q) t.kc[q] = hyx_tab_kc[g * HYX_NKC + q];
+#pragma unroll
+    for (int q = 0; q < HYX_OWN * HYX_MAXT; ++q) {
+        t.ti[q] = hyx_tab_ti[g * HYX_OWN * HYX_MAXT + q];
+        t.ts[q] = hyx_tab_ts[g * HYX_OWN * HYX_MAXT + q];
+    }
+#pragma unroll
+    for (int q = 0; q < HYX_OWN; ++q) {
+        t.rc[q] = hyx_tab_rc[g * HYX_OWN + q];
+        t.ov[q] = hyx_tab_ov[g * HYX_OWN + q];
+    }
+}
+
+struct hyx_ginst {
+    double xo[HYX_OWN];
+    double pv[HYX_NPAR];
+    double thi, tlo, tfhi, tflo, mdt, minh, maxh;
+    hyx_u64 iter, nst;
+    int tdir;
+};
+
+__device__ __forceinline__ void hyx_gload(hyx_ginst &s, const hyx_kargs &a, hyx_u64 i, int g, const hyx_gtab &t)
+{
+#pragma unroll
+    for (int o = 0; o < HYX_OWN; ++o)
+        s.xo[o] = t.ov[o] ? a.state[(hyx_u64)(g + o * HYX_G) * a.ld + i] : 0.0;
+#pragma unroll
+    for (int q = 0; q < HYX_NPAR; ++q) s.pv[q] = a.pars ? a.pars[(hyx_u64)q * a.ld + i] : 0.0;
+    s.thi = a.t_hi[i];
+    s.tlo = a.t_lo[i];
+}
+
+__device__ __forceinline__ void hyx_gstore_state(const hyx_ginst &s, const hyx_kargs &a, hyx_u64 i, int g,
+                                                 const hyx_gtab &t)
+{
+#pragma unroll
+    for (int o = 0; o < HYX_OWN; ++o)
+        if (t.ov[o]) a.state[(hyx_u64)(g + o * HYX_G) * a.ld + i] = s.xo[o];
+    if (g == 0) {
+        a.t_hi[i] = s.thi;
+        a.t_lo[i] = s.tlo;
+    }
+}
+
+__device__ __forceinline__ void hyx_gresume(hyx_ginst &s, const hyx_kargs &a, hyx_u64 i, unsigned st, int g,
+                                            const hyx_gtab &t)
+{
+    hyx_gload(s, a, i, g, t);
+    s.tfhi = a.tf_hi[i];
+    s.tflo = a.tf_lo[i];
+    s.mdt = a.max_dt[i];
+    s.tdir = (st == 0u);
+    s.minh = a.min_h[i];
+    s.maxh = a.max_h[i];
+    s.iter = a.iters[i];
+    s.nst = a.nsteps[i];
+}
+
+__device__ __forceinline__ void hyx_gsuspend(const hyx_ginst &s, const hyx_kargs &a, hyx_u64 i, int g,
+                                             const hyx_gtab &t, bool done, hyx_i64 res)
+{
+    hyx_gstore_state(s, a, i, g, t);
+    if (g == 0) {
+        a.min_h[i] = s.minh;
+        a.max_h[i] = s.maxh;
+        a.iters[i] = s.iter;
+        a.nsteps[i] = s.nst;
+        if (done) {
+            a.status[i] = 1u;
+            a.outcome[i] = res;
+        }
+    }
+}
+
+// One propagate_until iteration of the group's instance; every lane of the
+// group computes the same bookkeeping (h and the non-finite flag are reduced
+// across the group inside hyx_gstep).
+__device__ __forceinline__ bool hyx_gprop_iter(hyx_ginst &s, const hyx_kargs &a, int g, double *fl,
+                                               const hyx_gtab &t, hyx_i64 *res)
+{
+    const double rem = hyx_dfl_sub_hi(s.tfhi, s.tflo, s.thi, s.tlo);
+    const double lim = s.tdir ? ((s.mdt < rem) ? s.mdt : rem) : ((-s.mdt > rem) ? -s.mdt : rem);
+    double h;
+    bool nf;
+    hyx_gstep<false>(s.xo, s.pv, s.thi, lim, h, nf, g, fl, t, nullptr, 0);
+    hyx_dfl_add(s.thi, s.tlo, h, 0.0);
+    nf = nf || !isfinite(s.thi);
+    s.iter += 1;
+    s.nst += (h != 0.0) ? 1 : 0;
+    const hyx_i64 oc = nf ? HYX_ERR_NF_STATE : ((h == lim) ? HYX_TIME_LIMIT : HYX_SUCCESS);
+    if (oc == HYX_SUCCESS) {
+        const double ah = fabs(h);
+        s.minh = (ah < s.minh) ? ah : s.minh;
+        s.maxh = (ah > s.maxh) ? ah : s.maxh;
+    }
+    if (oc == HYX_ERR_NF_STATE) {
+        *res = HYX_ERR_NF_STATE;
+        return true;
+    }
+    if (oc == HYX_TIME_LIMIT && h == rem) {
+        *res = HYX_TIME_LIMIT;
+        return true;
+    }
+    if (a.max_steps != 0 && s.iter >= a.max_steps) {
+        *res = HYX_STEP_LIMIT;
+        return true;
+    }
+    return false;
+}
+
+#define HYX_GROUP_SETUP                                                                                               \
+    __shared__ double hyx_lds[HYX_I][2 * HYX_FLAT];                                                                  \
+    const int lane = (int)__lane_id();                                                                               \
+    const int grp = lane / HYX_G;                                                                                    \
+    const int g = lane % HYX_G;                                                                                      \
+    double *fl = &hyx_lds[grp][0];                                                                                   \
+    for (int q = g; q < 2 * HYX_FLAT; q += HYX_G) fl[q] = 0.0;                                                       \
+    hyx_gtab tab;                                                                                                    \
+    hyx_gtab_load(tab, g);                                                                                           \
+    __syncthreads();
+
+// ---- step(): one adaptive step per instance ----------------------------------------------
+template <bool WTC>
+__device__ __forceinline__ void hyx_gstep_body(const hyx_kargs &a)
+{
+    HYX_GROUP_SETUP
+    const hyx_u64 i = (hyx_u64)blockIdx.x * HYX_I + grp;
+    const bool valid = i < a.N;
+    const hyx_u64 ii = valid ? i : 0; // groups past the end step instance 0 and store nothing
+    hyx_ginst s;
+    hyx_gload(s, a, ii, g, tab);
+    const double mdt = a.max_dt[ii];
+    double h;
+    bool nf;
+    hyx_gstep<WTC>(s.xo, s.pv, s.thi, mdt, h, nf, g, fl, tab, (WTC && valid) ? a.tc + i : nullptr, a.ld);
+    hyx_dfl_add(s.thi, s.tlo, h, 0.0);
+    nf = nf || !isfinite(s.thi);
+    if (valid) {
+        hyx_gstore_state(s, a, i, g, tab);
+        if (g == 0) {
+            a.h_out[i] = h;
+            a.outcome[i] = nf ? HYX_ERR_NF_STATE : ((h == mdt) ? HYX_TIME_LIMIT : HYX_SUCCESS);
+        }
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(64) hyx_step_k(hyx_kargs a) { hyx_gstep_body<false>(a); }
+
+extern "C" __global__ void __launch_bounds__(64) hyx_step_tc_k(hyx_kargs a) { hyx_gstep_body<true>(a); }
+
+// ---- plain form: one propagate iteration per launch ----------------------------------------
+extern "C" __global__ void __launch_bounds__(64) hyx_iter_init_k(hyx_kargs a)
+{
+    const hyx_u64 i = (hyx_u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.N) return;
+    const double rem = hyx_dfl_sub_hi(a.tf_hi[i], a.tf_lo[i], a.t_hi[i], a.t_lo[i]);
+    a.status[i] = (rem >= 0.0) ? 0u : 2u;
+    a.iters[i] = 0;
+    a.nsteps[i] = 0;
+    a.min_h[i] = __builtin_inf();
+    a.max_h[i] = 0.0;
+    a.outcome[i] = HYX_TIME_LIMIT;
+}
+
+extern "C" __global__ void __launch_bounds__(64) hyx_iter_k(hyx_kargs a)
+{
+    HYX_GROUP_SETUP
+    const hyx_u64 i = (hyx_u64)blockIdx.x * HYX_I + grp;
+    const unsigned st = (i < a.N) ? a.status[i] : 1u;
+    const bool valid = st != 1u;
+    const hyx_u64 ii = (i < a.N) ? i : 0;
+    hyx_ginst s;
+    hyx_gresume(s, a, ii, st, g, tab);
+    hyx_i64 res = HYX_TIME_LIMIT;
+    const bool done = hyx_gprop_iter(s, a, g, fl, tab, &res);
+    if (valid) hyx_gsuspend(s, a, i, g, tab, done, res);
+    const hyx_u64 m = __ballot(valid && !done && g == 0);
+    if (lane == 0 && m != 0) atomicAdd(a.active_count, (unsigned)__popcll(m));
+}
+
+extern "C" __global__ void hyx_clear_u32_k(unsigned *p, unsigned n)
+{
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+extern "C" __global__ void hyx_clear_u64_k(hyx_u64 *p, unsigned n)
+{
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// ---- persistent form: per-group work stealing ----------------------------------------------
+// Each wave holds HYX_I instances. Groups whose instance is done claim new
+// queue positions together, with one atomic per wave (ballot over the group
+// leaders + prefix count). An instance that reaches iter_budget iterations in
+// this launch is written back unfinished (counted in *unfinished) and the host
+// launches again, so every wave leaves the loop after a bounded number of
+// iterations once the queue is drained.
+extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
+{
+    HYX_GROUP_SETUP
+    hyx_ginst s;
+    hyx_gload(s, a, 0, g, tab);
+    s.tfhi = s.thi;
+    s.tflo = s.tlo;
+    s.mdt = 0.0;
+    s.tdir = 1;
+    s.minh = 0.0;
+    s.maxh = 0.0;
+    s.iter = 0;
+    s.nst = 0;
+    hyx_u64 inst = 0, budget = 0;
+    bool active = false;
+    bool exhausted = false;
+    for (;;) {
+        const hyx_u64 idle = __ballot(g == 0 && !active);
+        const unsigned nidle = (unsigned)__popcll(idle);
+        if (!exhausted && nidle != 0 && (nidle * HYX_G >= a.refill_min || nidle == HYX_I)) {
+            const int leader = __ffsll((long long)idle) - 1;
+            hyx_u64 base = 0;
+            if (lane == leader) base = atomicAdd(a.work_counter, (hyx_u64)nidle);
+            base = __shfl(base, leader);
+            if (base + nidle >= a.N) exhausted = true;
+            if (!active) {
+                // Rank of this group among the idle groups: idle leaders below its leader lane.
+                const int lead_lane = grp * HYX_G;
+                const hyx_u64 below_lead = (lead_lane == 0) ? 0ull : ((~0ull) >> (64 - lead_lane));
+                const hyx_u64 gpos = base + (hyx_u64)__popcll(idle & below_lead);
+                if (gpos < a.N) {
+                    inst = a.queue ? (hyx_u64)a.queue[gpos] : gpos;
+                    const unsigned st = a.status[inst];
+                    if (st != 1u) {
+                        hyx_gresume(s, a, inst, st, g, tab);
+                        budget = 0;
+                        active = true;
+                    }
+                }
+            }
+        }
+        if (!__any(active)) break;
+        hyx_i64 res = HYX_TIME_LIMIT;
+        const bool done = hyx_gprop_iter(s, a, g, fl, tab, &res);
+        if (active) {
+            budget += 1;
+            if (done || budget >= a.iter_budget) {
+                hyx_gsuspend(s, a, inst, g, tab, done, res);
+                if (!done && g == 0) atomicAdd(a.unfinished, 1u);
+                active = false;
+            }
+        }
+    }
+}

@@ -31,7 +31,7 @@
 #include "../../include/hyx.h"
 #include "hyx_internal.h"
 
-#include "build/hyx_kernels_embed.inc" // kKernelsSrc: the text of hyx_kernels.hip
+#include "build/hyx_kernels_embed.inc" // k_hyx_kcommon, k_hyx_klane, k_hyx_kgroup: the kernel templates
 
 namespace
 {
@@ -200,6 +200,8 @@ struct hyx_sys {
     uint32_t order = 0;
     uint32_t flags = 0;
     bool compact = false;
+    int form = 0;                 // HYX_FORM_UNROLLED / COMPACT / GROUP
+    uint32_t inst_per_block = 64; // instances handled by one 64-lane block
     std::string src;
     std::vector<char> code;
     int device = -1;
@@ -246,11 +248,29 @@ int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint3
     s.order = order;
     s.flags = flags;
     try {
-        s.compact = (flags & HYX_FLAG_FORCE_COMPACT)    ? true
-                    : (flags & HYX_FLAG_FORCE_UNROLLED) ? false
-                                                        : hyx::choose_compact(s.S, (int)order);
-        s.src = hyx::generate_prelude(s.S, (int)order, (flags & HYX_FLAG_HIGH_ACCURACY) != 0, s.compact) + "\n"
-                + kKernelsSrc;
+        const bool ha = (flags & HYX_FLAG_HIGH_ACCURACY) != 0;
+        hyx::GroupPlan gp;
+        std::string why;
+        const bool lane_compact = hyx::choose_compact(s.S, (int)order);
+        if (flags & HYX_FLAG_FORCE_COMPACT)
+            s.form = HYX_FORM_COMPACT;
+        else if (flags & HYX_FLAG_FORCE_UNROLLED)
+            s.form = HYX_FORM_UNROLLED;
+        else if (flags & HYX_FLAG_FORCE_GROUP) {
+            if (!hyx::plan_groups(s.S, gp, why)) return fail(HYX_EINVAL, "the group form does not apply: " + why);
+            s.form = HYX_FORM_GROUP;
+        } else if (lane_compact && hyx::plan_groups(s.S, gp, why))
+            s.form = HYX_FORM_GROUP;
+        else
+            s.form = lane_compact ? HYX_FORM_COMPACT : HYX_FORM_UNROLLED;
+        s.compact = s.form == HYX_FORM_COMPACT;
+        if (s.form == HYX_FORM_GROUP) {
+            s.inst_per_block = 64u / (uint32_t)gp.G;
+            s.src = hyx::generate_group_prelude(s.S, gp, (int)order, ha) + "\n" + k_hyx_kcommon + "\n" + k_hyx_kgroup;
+        } else {
+            s.inst_per_block = 64;
+            s.src = hyx::generate_prelude(s.S, (int)order, ha, s.compact) + "\n" + k_hyx_kcommon + "\n" + k_hyx_klane;
+        }
     } catch (const std::exception &e) {
         return fail(HYX_EINVAL, e.what());
     }
@@ -309,6 +329,10 @@ hipError_t clear64(hyx_batch *b, uint64_t *p, unsigned n)
 }
 
 uint64_t blocks_for(uint64_t N) { return (N + 63) / 64; }
+
+// Blocks of the step / plain / persistent kernels: one block holds 64 instances
+// in the lane form and 64 / G in the group form.
+uint64_t blocks_of(const hyx_batch *b) { return (b->N + b->sys->inst_per_block - 1) / b->sys->inst_per_block; }
 
 // Compact code keeps its Taylor tape in device memory, one column per lane:
 // lanes are instances for the step / plain kernels and persistent-grid lanes
@@ -441,6 +465,7 @@ int hyx_sys_order(const hyx_sys *s) { return s ? (int)s->order : HYX_EINVAL; }
 int hyx_sys_dim(const hyx_sys *s) { return s ? (int)s->S.n : HYX_EINVAL; }
 int hyx_sys_npar(const hyx_sys *s) { return s ? (int)s->S.npar : HYX_EINVAL; }
 int hyx_sys_is_compact(const hyx_sys *s) { return s ? (int)s->compact : HYX_EINVAL; }
+int hyx_sys_code_form(const hyx_sys *s) { return s ? s->form : HYX_EINVAL; }
 
 size_t hyx_sys_source(const hyx_sys *s, char *buf, size_t cap)
 {
@@ -586,7 +611,7 @@ int hyx_step(hyx_batch *b, const double *max_dt, int write_tc, int64_t *outcome,
     if (int rc = ensure_tape(b, N)) return rc;
     KArgs a = make_args(b);
     HIPCHK(hipEventRecord(b->ev0, b->stream));
-    HIPCHK(launch(write_tc ? b->sys->k_step_tc : b->sys->k_step, blocks_for(N), b->stream, &a));
+    HIPCHK(launch(write_tc ? b->sys->k_step_tc : b->sys->k_step, blocks_of(b), b->stream, &a));
     HIPCHK(hipEventRecord(b->ev1, b->stream));
     if (outcome) HIPCHK(hipMemcpyAsync(outcome, b->oc, N * 8, hipMemcpyDeviceToHost, b->stream));
     if (h) HIPCHK(hipMemcpyAsync(h, b->hout, N * 8, hipMemcpyDeviceToHost, b->stream));
@@ -623,7 +648,7 @@ int hyx_propagate_until(hyx_batch *b, const double *tf_hi, const double *tf_lo, 
         int occ = 0;
         HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, b->sys->k_persist, 64, 0));
         const uint64_t per_cu = b->waves_per_cu ? b->waves_per_cu : (uint64_t)std::max(occ, 1);
-        pgrid = std::max<uint64_t>(1, std::min<uint64_t>(per_cu * (uint64_t)b->n_cu, blocks_for(N)));
+        pgrid = std::max<uint64_t>(1, std::min<uint64_t>(per_cu * (uint64_t)b->n_cu, blocks_of(b)));
     }
     if (int rc = ensure_tape(b, pgrid * 64)) return rc;
     KArgs a = make_args(b);
@@ -644,7 +669,7 @@ int hyx_propagate_until(hyx_batch *b, const double *tf_hi, const double *tf_lo, 
             if (unf == 0) break;
         }
     } else {
-        const uint64_t grid = blocks_for(N);
+        const uint64_t grid = blocks_of(b);
         if (mode == HYX_MODE_GRAPH && !b->graph_exec) {
             HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             hipError_t e = clear32(b, b->active, kPlainGroup);

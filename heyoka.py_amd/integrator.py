@@ -168,7 +168,7 @@ class taylor_adaptive_batch:
         if tol == 0.0:
             tol = float(np.finfo(np.float64).eps)
         order = order_from_tol(tol)
-        compact = {"compact": True, "unrolled": False, None: None}[code_form]
+        compact = {"compact": True, "unrolled": False, "group": "group", None: None}[code_form]
         system = get_system(dec, order, high_accuracy=bool(high_accuracy), compact=compact)
         d["_hyx_pairs"] = pairs
         d["_hyx_vsys"] = vsys
@@ -491,7 +491,7 @@ class taylor_adaptive_batch:
     # ---- GPU specifics ------------------------------------------------------------------
     @property
     def code_form(self):
-        return "compact" if self._hyx_system.compact else "unrolled"
+        return self._hyx_system.form
 
     def last_kernel_timing(self):
         """(milliseconds on the device, kernel launches) of the last step/propagate call."""

@@ -58,6 +58,12 @@ typedef struct {
 #define HYX_FLAG_FORCE_COMPACT 4u /* loop over orders, Taylor tape in device memory */
 #define HYX_FLAG_FORCE_UNROLLED 8u /* fully unrolled, histories in registers */
 #define HYX_FLAG_SOURCE_ONLY 16u   /* generate the HIP source only (hyx_sys_source); no compile, no device */
+#define HYX_FLAG_FORCE_GROUP 32u   /* group form: G lanes per instance over isomorphic clusters */
+
+/* Code forms (hyx_sys_code_form). */
+#define HYX_FORM_UNROLLED 0 /* one instance per lane, fully unrolled, histories in registers */
+#define HYX_FORM_COMPACT 1  /* one instance per lane, loop over orders, Taylor tape in device memory */
+#define HYX_FORM_GROUP 2    /* G lanes per instance, one cluster per lane, histories in registers, LDS exchange */
 
 /* Fields for hyx_batch_write / hyx_batch_read / hyx_batch_device_ptr. */
 #define HYX_FIELD_STATE 0   /* [n][N] */
@@ -86,8 +92,10 @@ int hyx_sys_precompile(const uint32_t *ops, uint32_t n_words, const double *cons
 int hyx_sys_order(const hyx_sys *s);
 int hyx_sys_dim(const hyx_sys *s);
 int hyx_sys_npar(const hyx_sys *s);
-/* 1 if the compact code form (tape in device memory) was generated, 0 if unrolled. */
+/* 1 if the compact code form (tape in device memory) was generated, 0 otherwise. */
 int hyx_sys_is_compact(const hyx_sys *s);
+/* The generated code form: HYX_FORM_UNROLLED, HYX_FORM_COMPACT or HYX_FORM_GROUP. */
+int hyx_sys_code_form(const hyx_sys *s);
 /* The generated HIP source (for inspection); returns its length, copies up to cap-1 bytes. */
 size_t hyx_sys_source(const hyx_sys *s, char *buf, size_t cap);
 /* Register usage of the compiled Taylor-step kernels: VGPRs, scratch bytes per lane. */

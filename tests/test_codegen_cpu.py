@@ -110,3 +110,101 @@ def test_generated_step_matches_oracle(name, compact, tmp_path):
     assert abs(h.value - r["h"][0]) <= 1e-13 * abs(r["h"][0])
     np.testing.assert_allclose(xout, r["state"][:, 0], rtol=1e-13, atol=1e-15)
     assert nf.value == 0
+
+
+GROUP_HARNESS = r"""
+#include <cmath>
+#include <cstdint>
+#include <math.h>
+#include <pthread.h>
+#define __device__
+#define __forceinline__ inline
+#define __constant__ static const
+using std::sqrt; using std::pow; using std::exp; using std::log; using std::fabs; using std::isfinite;
+static pthread_barrier_t bar;
+static thread_local int tl_g;
+static void __syncthreads() { pthread_barrier_wait(&bar); }
+static double xch_d[64];
+static int xch_i[64];
+static double __shfl_xor(double v, int off) { xch_d[tl_g] = v; __syncthreads(); double r = xch_d[tl_g ^ off]; __syncthreads(); return r; }
+static int __shfl_xor(int v, int off) { xch_i[tl_g] = v; __syncthreads(); int r = xch_i[tl_g ^ off]; __syncthreads(); return r; }
+#include "prelude.inc"
+struct Tab { int iv[HYX_NIN]; double kc[HYX_NKC]; int ti[HYX_OWN * HYX_MAXT]; double ts[HYX_OWN * HYX_MAXT];
+             double rc[HYX_OWN]; int ov[HYX_OWN]; };
+static double fl[2 * HYX_FLAT];
+struct Arg { int g; const double *x; const double *pv; double t0, mdt; double *xout; double *h; int *nf; double *tc; };
+static void *run(void *p)
+{
+    Arg *a = (Arg *)p;
+    const int g = a->g;
+    tl_g = g;
+    Tab t;
+    for (int q = 0; q < HYX_NIN; ++q) t.iv[q] = hyx_tab_iv[g * HYX_NIN + q];
+    for (int q = 0; q < HYX_NKC; ++q) t.kc[q] = hyx_tab_kc[g * HYX_NKC + q];
+    for (int q = 0; q < HYX_OWN * HYX_MAXT; ++q) { t.ti[q] = hyx_tab_ti[g * HYX_OWN * HYX_MAXT + q]; t.ts[q] = hyx_tab_ts[g * HYX_OWN * HYX_MAXT + q]; }
+    for (int q = 0; q < HYX_OWN; ++q) { t.rc[q] = hyx_tab_rc[g * HYX_OWN + q]; t.ov[q] = hyx_tab_ov[g * HYX_OWN + q]; }
+    double xo[HYX_OWN], pv[HYX_NPAR];
+    for (int o = 0; o < HYX_OWN; ++o) xo[o] = t.ov[o] ? a->x[g + o * HYX_G] : 0.0;
+    for (int q = 0; q < HYX_NPAR; ++q) pv[q] = a->pv[q];
+    double h; bool nf;
+    hyx_gstep<true>(xo, pv, a->t0, a->mdt, h, nf, g, fl, t, a->tc, 1);
+    for (int o = 0; o < HYX_OWN; ++o) if (t.ov[o]) a->xout[g + o * HYX_G] = xo[o];
+    a->h[g] = h; a->nf[g] = nf;
+    return nullptr;
+}
+extern "C" int gstep_cpu(const double *x, const double *pv, double t0, double mdt, double *xout, double *h, int *nf, double *tc)
+{
+    for (int q = 0; q < 2 * HYX_FLAT; ++q) fl[q] = 0.0;
+    pthread_barrier_init(&bar, nullptr, HYX_G);
+    pthread_t th[HYX_G]; Arg args[HYX_G];
+    for (int g = 0; g < HYX_G; ++g) { args[g] = Arg{g, x, pv, t0, mdt, xout, h, nf, tc}; pthread_create(&th[g], nullptr, run, &args[g]); }
+    for (int g = 0; g < HYX_G; ++g) pthread_join(th[g], nullptr);
+    pthread_barrier_destroy(&bar);
+    return HYX_G;
+}
+"""
+
+
+@pytest.mark.parametrize("name", ["nbody6", "nbody3"])
+def test_group_step_matches_oracle(name, tmp_path):
+    # The group form emulated on the CPU: one thread per lane of a group, a
+    # barrier for __syncthreads and a shared array for __shfl_xor.
+    if name == "nbody6":
+        wl = W.nbody6(2)
+        sys_, st = wl.sys, wl.state[:, :1]
+    else:
+        sys_ = hy.model.nbody(3, masses=[1.0, 0.3, 0.01])
+        st = np.array([0, 0, 0, 0, 0, 0, 1, 0, 0.1, 0, 1.1, 0, -2, 0.5, 0, 0.2, -0.6, 0.05], dtype=float)[:, None]
+    dec = hy.taylor_decompose(sys_)
+    p = 19
+    s = _lib.System(dec, p, compact="group", source_only=True)
+    assert s.form == "group"
+    src = s.source()
+    prelude = src.split("// Hand-written CDNA4 kernels")[0]
+    d = tmp_path / name
+    d.mkdir()
+    (d / "prelude.inc").write_text(prelude)
+    (d / "g.cpp").write_text(GROUP_HARNESS)
+    so = d / "g.so"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-ffp-contract=off", "-shared", "-fPIC", "-pthread", "-o", str(so),
+                    str(d / "g.cpp")], check=True, capture_output=True)
+    L = ct.CDLL(str(so))
+    P = ct.POINTER(ct.c_double)
+    L.gstep_cpu.argtypes = [P, P, ct.c_double, ct.c_double, P, P, ct.POINTER(ct.c_int), P]
+    n = dec.n_state
+    x = np.ascontiguousarray(st[:, 0])
+    pv = np.zeros(1)
+    xout = np.zeros(n)
+    h = np.zeros(64)
+    nf = np.zeros(64, dtype=np.int32)
+    tc = np.zeros(n * (p + 1))
+    ptr = lambda a: a.ctypes.data_as(ct.POINTER(ct.c_double))
+    G = L.gstep_cpu(ptr(x), ptr(pv), 0.0, np.inf, ptr(xout), ptr(h), nf.ctypes.data_as(ct.POINTER(ct.c_int)), ptr(tc))
+    assert np.all(h[:G] == h[0]) and not nf[:G].any()
+    r = O.step_batch(dec, p, st, want_tc=True)
+    tco = r["tc"][:, :, 0]
+    tcg = tc.reshape(n, p + 1)
+    scale = np.max(np.abs(tco), axis=1, keepdims=True) + 1e-300
+    assert np.max(np.abs(tcg - tco) / scale) < 1e-11
+    assert abs(h[0] - r["h"][0]) <= 1e-13 * abs(r["h"][0])
+    np.testing.assert_allclose(xout, r["state"][:, 0], rtol=1e-13, atol=1e-15)

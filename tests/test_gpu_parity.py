@@ -140,9 +140,45 @@ def test_cr3bp_vs_oracle(code_form):
     _gpu_vs_oracle(W.cr3bp(256), 256, 20.0, 1e-10, code_form=code_form)
 
 
-def test_nbody6_short_vs_oracle():
-    print("[gpu] nbody6 128 instances t=2", flush=True)
-    _gpu_vs_oracle(W.nbody6(128), 128, 2.0, 1e-10)
+@pytest.mark.parametrize("code_form", ["group", "compact"])
+def test_nbody6_short_vs_oracle(code_form):
+    print(f"[gpu] nbody6 128 instances t=2 code_form={code_form}", flush=True)
+    ta, _, _ = _gpu_vs_oracle(W.nbody6(128), 128, 2.0, 1e-10, code_form=code_form)
+    assert ta.code_form == code_form
+
+
+def _nbody3():
+    sys_ = hy.model.nbody(3, masses=[1.0, 0.3, 0.01])
+    base = np.array([0, 0, 0, 0, 0, 0, 1, 0, 0.1, 0, 1.1, 0, -2, 0.5, 0, 0.2, -0.6, 0.05], dtype=float)
+    rng = np.random.default_rng(7)
+    st = base[:, None] + rng.normal(0, 1e-3, (18, 37))
+    return W.Workload("nbody3", sys_, st, None, np.full(37, 3.0), 1e-15)
+
+
+@pytest.mark.parametrize("mode", ["persistent", "plain", "graph"])
+def test_group_form_nbody3_modes(mode):
+    # G = 4 lanes per instance, 16 instances per wave, a batch that is not a multiple of 16.
+    print(f"[gpu] nbody3 group form mode={mode}", flush=True)
+    ta, _, _ = _gpu_vs_oracle(_nbody3(), 37, 3.0, 1e-10, code_form="group", mode=mode)
+    assert ta.code_form == "group"
+
+
+def test_group_form_step_tc_and_lane_failure():
+    print("[gpu] group form step(write_tc) and a non-finite instance", flush=True)
+    wl = _nbody3()
+    st = wl.state[:, :5].copy()
+    ta = hy.taylor_adaptive_batch(wl.sys, st, tol=1e-15, code_form="group")
+    ta.step(write_tc=True)
+    dec = hy.taylor_decompose(wl.sys)
+    r = O.step_batch(dec, ta.order, st, want_tc=True)
+    np.testing.assert_allclose(ta.tc, r["tc"], rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose([h for _, h in ta.step_res], r["h"], rtol=1e-13)
+    st[3, 2] = np.inf
+    tb = hy.taylor_adaptive_batch(wl.sys, st, tol=1e-15, code_form="group")
+    tb.propagate_until(1.0)
+    oc = [p[0] for p in tb.propagate_res]
+    assert oc[2] == hy.taylor_outcome.err_nf_state
+    assert all(o == hy.taylor_outcome.time_limit for i, o in enumerate(oc) if i != 2)
 
 
 def test_j2var_short_vs_oracle():
