@@ -82,6 +82,13 @@ def lib():
             "hyx_batch_set_queue": (ct.c_int, [vp, P(u32)]),
             "hyx_batch_set_iter_budget": (ct.c_int, [vp, u64]),
             "hyx_batch_sync": (ct.c_int, [vp]),
+            "hyx_slots_reset": (ct.c_int, [vp]),
+            "hyx_slots_write": (ct.c_int, [vp, ct.c_int, P(u32), u32, P(d)]),
+            "hyx_slots_read": (ct.c_int, [vp, ct.c_int, P(u32), u32, P(d)]),
+            "hyx_slots_arm": (ct.c_int, [vp, P(u32), u32, P(d), P(d), P(d)]),
+            "hyx_slots_launch": (ct.c_int, [vp, u64]),
+            "hyx_slots_wait": (ct.c_int, [vp, P(u32)]),
+            "hyx_slots_results": (ct.c_int, [vp, P(u32), u32, P(u32), P(prop_res)]),
             "hyx_last_error": (ct.c_char_p, []),
             "hyx_device_count": (ct.c_int, []),
         }
@@ -100,7 +107,8 @@ EXPORTED = (
     "hyx_sys_kernel_info", "hyx_sys_destroy", "hyx_batch_create", "hyx_batch_destroy", "hyx_batch_size",
     "hyx_batch_write", "hyx_batch_read", "hyx_batch_device_ptr", "hyx_step", "hyx_read_tc", "hyx_propagate_until",
     "hyx_batch_last_timing", "hyx_batch_set_persistent_config", "hyx_batch_set_queue", "hyx_batch_set_iter_budget",
-    "hyx_batch_sync",
+    "hyx_batch_sync", "hyx_slots_reset", "hyx_slots_write", "hyx_slots_read", "hyx_slots_arm", "hyx_slots_launch",
+    "hyx_slots_wait", "hyx_slots_results",
     "hyx_last_error", "hyx_device_count",
 )
 
@@ -254,6 +262,46 @@ class Batch:
 
     def sync(self):
         check(self._L.hyx_batch_sync(self.h))
+
+    # ---- slots (serving) ------------------------------------------------------------------
+    def slots_reset(self):
+        check(self._L.hyx_slots_reset(self.h))
+
+    def slots_write(self, field: int, slots, src):
+        sl = np.ascontiguousarray(slots, dtype=np.uint32)
+        a = np.ascontiguousarray(src, dtype=np.float64)
+        check(self._L.hyx_slots_write(self.h, field, dptr(sl, ct.c_uint32), len(sl), dptr(a)))
+
+    def slots_read(self, field: int, slots, rows: int) -> np.ndarray:
+        sl = np.ascontiguousarray(slots, dtype=np.uint32)
+        out = np.empty((rows, len(sl)), dtype=np.float64)
+        if out.size:
+            check(self._L.hyx_slots_read(self.h, field, dptr(sl, ct.c_uint32), len(sl), dptr(out)))
+        return out
+
+    def slots_arm(self, slots, tf_hi, tf_lo=None, max_delta_t=None):
+        sl = np.ascontiguousarray(slots, dtype=np.uint32)
+        th = np.ascontiguousarray(tf_hi, dtype=np.float64)
+        tl = None if tf_lo is None else np.ascontiguousarray(tf_lo, dtype=np.float64)
+        md = None if max_delta_t is None else np.ascontiguousarray(max_delta_t, dtype=np.float64)
+        check(self._L.hyx_slots_arm(self.h, dptr(sl, ct.c_uint32), len(sl), dptr(th), dptr(tl), dptr(md)))
+
+    def slots_launch(self, iter_budget: int):
+        check(self._L.hyx_slots_launch(self.h, int(iter_budget)))
+
+    def slots_wait(self) -> int:
+        n = ct.c_uint32()
+        check(self._L.hyx_slots_wait(self.h, ct.byref(n)))
+        return n.value
+
+    def slots_results(self, slots):
+        sl = np.ascontiguousarray(slots, dtype=np.uint32)
+        done = np.zeros(len(sl), dtype=np.uint32)
+        res = (prop_res * max(len(sl), 1))()
+        check(self._L.hyx_slots_results(self.h, dptr(sl, ct.c_uint32), len(sl), dptr(done, ct.c_uint32), res))
+        arr = np.frombuffer(res, dtype=np.dtype([("outcome", "<i8"), ("min_h", "<f8"), ("max_h", "<f8"),
+                                                 ("nsteps", "<u8")]), count=len(sl)).copy()
+        return done.astype(bool), arr
 
     def __del__(self):
         h = getattr(self, "h", None)

@@ -94,3 +94,56 @@ __device__ __forceinline__ double hyx_dfl_sub_hi(double ah, double al, double bh
     return hi;
 }
 
+
+// ---- slot operations (serving: continuous batching over the instances of a batch) ----------
+// 64-bit moves, so outcome / nsteps (integers) and doubles travel bit-exactly.
+extern "C" __global__ void hyx_scatter_k(hyx_u64 *dst, hyx_u64 ld, unsigned rows, const hyx_u64 *src,
+                                         const unsigned *idx, unsigned count)
+{
+    const hyx_u64 t = (hyx_u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (hyx_u64)rows * count) return;
+    const hyx_u64 r = t / count, c = t % count;
+    dst[r * ld + idx[c]] = src[r * count + c];
+}
+
+extern "C" __global__ void hyx_gather_k(const hyx_u64 *srcm, hyx_u64 ld, unsigned rows, hyx_u64 *dst,
+                                        const unsigned *idx, unsigned count)
+{
+    const hyx_u64 t = (hyx_u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (hyx_u64)rows * count) return;
+    const hyx_u64 r = t / count, c = t % count;
+    dst[r * count + c] = srcm[r * ld + idx[c]];
+}
+
+extern "C" __global__ void hyx_gather32_k(const unsigned *srcm, unsigned *dst, const unsigned *idx, unsigned count)
+{
+    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < count) dst[c] = srcm[idx[c]];
+}
+
+extern "C" __global__ void hyx_fill_u32_k(unsigned *p, unsigned v, hyx_u64 n)
+{
+    const hyx_u64 i = (hyx_u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Arm slots for a propagation: final time, max_delta_t, direction and zeroed
+// bookkeeping, exactly as hyx_iter_init_k does for a whole batch.
+// stage = [tf_hi(count), tf_lo(count), max_dt(count)].
+extern "C" __global__ void hyx_slots_arm_k(hyx_kargs a, const unsigned *idx, unsigned count, const double *stage)
+{
+    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= count) return;
+    const hyx_u64 i = idx[c];
+    const double th = stage[c], tl = stage[count + c];
+    ((double *)a.tf_hi)[i] = th;
+    ((double *)a.tf_lo)[i] = tl;
+    ((double *)a.max_dt)[i] = stage[2 * (hyx_u64)count + c];
+    const double rem = hyx_dfl_sub_hi(th, tl, a.t_hi[i], a.t_lo[i]);
+    a.status[i] = (rem >= 0.0) ? 0u : 2u;
+    a.iters[i] = 0;
+    a.nsteps[i] = 0;
+    a.min_h[i] = __builtin_inf();
+    a.max_h[i] = 0.0;
+    a.outcome[i] = HYX_TIME_LIMIT;
+}

@@ -207,7 +207,8 @@ struct hyx_sys {
     int device = -1;
     hipModule_t mod = nullptr;
     hipFunction_t k_step = nullptr, k_step_tc = nullptr, k_init = nullptr, k_iter = nullptr, k_persist = nullptr,
-                  k_clr32 = nullptr, k_clr64 = nullptr;
+                  k_clr32 = nullptr, k_clr64 = nullptr, k_scatter = nullptr, k_gather = nullptr, k_gather32 = nullptr,
+                  k_fill32 = nullptr, k_arm = nullptr;
     int vgprs = -1, scratch = -1;
 };
 
@@ -231,6 +232,11 @@ struct hyx_batch {
     uint64_t last_launches = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    void *stage = nullptr;       // serving: device staging for slot data
+    size_t stage_bytes = 0;
+    unsigned *stage_idx = nullptr;
+    size_t stage_idx_n = 0;
+    bool slots_running = false;
 };
 
 namespace
@@ -445,7 +451,8 @@ int hyx_sys_create(const uint32_t *ops, uint32_t n_words, const double *consts, 
         const char *name;
     } fns[] = {{&s->k_step, "hyx_step_k"},       {&s->k_step_tc, "hyx_step_tc_k"}, {&s->k_init, "hyx_iter_init_k"},
                {&s->k_iter, "hyx_iter_k"},       {&s->k_persist, "hyx_persist_k"}, {&s->k_clr32, "hyx_clear_u32_k"},
-               {&s->k_clr64, "hyx_clear_u64_k"}};
+               {&s->k_clr64, "hyx_clear_u64_k"}, {&s->k_scatter, "hyx_scatter_k"}, {&s->k_gather, "hyx_gather_k"},
+               {&s->k_gather32, "hyx_gather32_k"}, {&s->k_fill32, "hyx_fill_u32_k"}, {&s->k_arm, "hyx_slots_arm_k"}};
     for (auto &f : fns) {
         e = hipModuleGetFunction(f.f, s->mod, f.name);
         if (e != hipSuccess) {
@@ -542,7 +549,7 @@ void hyx_batch_destroy(hyx_batch *b)
     if (b->graph) hipGraphDestroy(b->graph);
     void *ptrs[] = {b->state, b->pars, b->thi,    b->tlo,   b->tfhi,   b->tflo,  b->mdt,
                     b->hout,  b->minh, b->maxh,   b->tc,    b->oc,     b->nsteps, b->iters,
-                    b->work,  b->status, b->active, b->unfinished, b->queue, b->tape};
+                    b->work,  b->status, b->active, b->unfinished, b->queue, b->tape, b->stage, b->stage_idx};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (b->ev0) hipEventDestroy(b->ev0);
@@ -766,6 +773,199 @@ int hyx_batch_set_queue(hyx_batch *b, const uint32_t *order)
     if (!b->queue) HIPCHK(hipMalloc((void **)&b->queue, b->N * 4));
     HIPCHK(hipMemcpyAsync(b->queue, order, b->N * 4, hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+
+// ---- serving: slots ------------------------------------------------------------------------
+
+static int stage_reserve(hyx_batch *b, size_t bytes, size_t nidx)
+{
+    if (bytes > b->stage_bytes) {
+        if (b->stage) HIPCHK(hipFree(b->stage));
+        b->stage = nullptr;
+        b->stage_bytes = 0;
+        HIPCHK(hipMalloc(&b->stage, bytes));
+        b->stage_bytes = bytes;
+    }
+    if (nidx > b->stage_idx_n) {
+        if (b->stage_idx) HIPCHK(hipFree(b->stage_idx));
+        b->stage_idx = nullptr;
+        b->stage_idx_n = 0;
+        HIPCHK(hipMalloc((void **)&b->stage_idx, nidx * 4));
+        b->stage_idx_n = nidx;
+    }
+    return 0;
+}
+
+static int check_slots(hyx_batch *b, const uint32_t *slots, uint32_t count)
+{
+    if (!slots && count) return fail(HYX_EINVAL, "null slot list");
+    for (uint32_t c = 0; c < count; ++c)
+        if (slots[c] >= b->N) return fail(HYX_EINVAL, "slot index out of range");
+    if (b->slots_running) return fail(HYX_EINVAL, "a slot epoch is running: call hyx_slots_wait first");
+    return 0;
+}
+
+static hipError_t launch_raw(hyx_batch *b, hipFunction_t f, uint64_t threads, void **params)
+{
+    return hipModuleLaunchKernel(f, (unsigned)((threads + 63) / 64), 1, 1, 64, 1, 1, 0, b->stream, params, nullptr);
+}
+
+int hyx_slots_reset(hyx_batch *b)
+{
+    if (!b) return fail(HYX_EINVAL, "null batch");
+    if (b->slots_running) return fail(HYX_EINVAL, "a slot epoch is running: call hyx_slots_wait first");
+    unsigned *p = b->status;
+    unsigned v = 1u;
+    uint64_t n = b->N;
+    void *params[] = {&p, &v, &n};
+    HIPCHK(launch_raw(b, b->sys->k_fill32, n, params));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int hyx_slots_write(hyx_batch *b, int field, const uint32_t *slots, uint32_t count, const double *src)
+{
+    if (!b || (!src && count)) return fail(HYX_EINVAL, "null argument");
+    if (int rc = check_slots(b, slots, count)) return rc;
+    if (count == 0) return 0;
+    double *p;
+    uint64_t total;
+    if (int rc = field_info(b, field, &p, &total)) return rc;
+    const unsigned rows = (unsigned)(total / b->N);
+    if (rows == 0) return 0;
+    if (int rc = stage_reserve(b, (size_t)rows * count * 8, count)) return rc;
+    HIPCHK(hipMemcpyAsync(b->stage_idx, slots, (size_t)count * 4, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->stage, src, (size_t)rows * count * 8, hipMemcpyHostToDevice, b->stream));
+    uint64_t ld = b->N;
+    void *dst = p;
+    void *params[] = {&dst, &ld, (void *)&rows, &b->stage, &b->stage_idx, &count};
+    HIPCHK(launch_raw(b, b->sys->k_scatter, (uint64_t)rows * count, params));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+static int gather64(hyx_batch *b, const void *srcm, unsigned rows, const uint32_t *slots, uint32_t count, void *dst)
+{
+    if (int rc = stage_reserve(b, (size_t)rows * count * 8, count)) return rc;
+    HIPCHK(hipMemcpyAsync(b->stage_idx, slots, (size_t)count * 4, hipMemcpyHostToDevice, b->stream));
+    uint64_t ld = b->N;
+    const void *sm = srcm;
+    void *params[] = {(void *)&sm, &ld, &rows, &b->stage, &b->stage_idx, &count};
+    HIPCHK(launch_raw(b, b->sys->k_gather, (uint64_t)rows * count, params));
+    HIPCHK(hipMemcpyAsync(dst, b->stage, (size_t)rows * count * 8, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int hyx_slots_read(hyx_batch *b, int field, const uint32_t *slots, uint32_t count, double *dst)
+{
+    if (!b || (!dst && count)) return fail(HYX_EINVAL, "null argument");
+    if (int rc = check_slots(b, slots, count)) return rc;
+    if (count == 0) return 0;
+    double *p;
+    uint64_t total;
+    if (int rc = field_info(b, field, &p, &total)) return rc;
+    const unsigned rows = (unsigned)(total / b->N);
+    if (rows == 0) return 0;
+    return gather64(b, p, rows, slots, count, dst);
+}
+
+int hyx_slots_arm(hyx_batch *b, const uint32_t *slots, uint32_t count, const double *tf_hi, const double *tf_lo,
+                  const double *max_delta_t)
+{
+    if (!b || (!tf_hi && count)) return fail(HYX_EINVAL, "null argument");
+    if (int rc = check_slots(b, slots, count)) return rc;
+    if (count == 0) return 0;
+    std::vector<double> st((size_t)3 * count);
+    for (uint32_t c = 0; c < count; ++c) {
+        st[c] = tf_hi[c];
+        st[count + c] = tf_lo ? tf_lo[c] : 0.0;
+        st[2 * (size_t)count + c] = max_delta_t ? max_delta_t[c] : std::numeric_limits<double>::infinity();
+        if (!(st[2 * (size_t)count + c] > 0.0)) return fail(HYX_EINVAL, "max_delta_t must be positive");
+    }
+    if (int rc = stage_reserve(b, st.size() * 8, count)) return rc;
+    HIPCHK(hipMemcpyAsync(b->stage_idx, slots, (size_t)count * 4, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->stage, st.data(), st.size() * 8, hipMemcpyHostToDevice, b->stream));
+    KArgs a = make_args(b);
+    void *params[] = {&a, &b->stage_idx, &count, &b->stage};
+    HIPCHK(launch_raw(b, b->sys->k_arm, count, params));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int hyx_slots_launch(hyx_batch *b, uint64_t iter_budget)
+{
+    if (!b) return fail(HYX_EINVAL, "null batch");
+    if (b->slots_running) return fail(HYX_EINVAL, "a slot epoch is already running");
+    if (iter_budget == 0) return fail(HYX_EINVAL, "the iteration budget must be positive");
+    int occ = 0;
+    HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, b->sys->k_persist, 64, 0));
+    const uint64_t per_cu = b->waves_per_cu ? b->waves_per_cu : (uint64_t)std::max(occ, 1);
+    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>(per_cu * (uint64_t)b->n_cu, blocks_of(b)));
+    if (int rc = ensure_tape(b, grid * 64)) return rc;
+    KArgs a = make_args(b);
+    a.iter_budget = iter_budget;
+    HIPCHK(clear64(b, b->work, 1));
+    HIPCHK(clear32(b, b->unfinished, 1));
+    HIPCHK(hipEventRecord(b->ev0, b->stream));
+    HIPCHK(launch(b->sys->k_persist, grid, b->stream, &a));
+    HIPCHK(hipEventRecord(b->ev1, b->stream));
+    b->slots_running = true;
+    return 0;
+}
+
+int hyx_slots_wait(hyx_batch *b, uint32_t *n_active)
+{
+    if (!b) return fail(HYX_EINVAL, "null batch");
+    if (!b->slots_running) {
+        if (n_active) *n_active = 0;
+        return 0;
+    }
+    unsigned unf = 0;
+    hipError_t e = hipMemcpyAsync(&unf, b->unfinished, 4, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    b->slots_running = false;
+    if (e != hipSuccess) return fail(HYX_EHIP, std::string("hyx_slots_wait: ") + hipGetErrorString(e));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    b->last_ms = ms;
+    b->last_launches = 1;
+    if (n_active) *n_active = unf;
+    return 0;
+}
+
+int hyx_slots_results(hyx_batch *b, const uint32_t *slots, uint32_t count, uint32_t *done, hyx_prop_res *res)
+{
+    if (!b) return fail(HYX_EINVAL, "null batch");
+    if (int rc = check_slots(b, slots, count)) return rc;
+    if (count == 0) return 0;
+    if (done) {
+        if (int rc = stage_reserve(b, (size_t)count * 8, count)) return rc;
+        HIPCHK(hipMemcpyAsync(b->stage_idx, slots, (size_t)count * 4, hipMemcpyHostToDevice, b->stream));
+        const unsigned *srcm = b->status;
+        void *params[] = {(void *)&srcm, &b->stage, &b->stage_idx, &count};
+        HIPCHK(launch_raw(b, b->sys->k_gather32, count, params));
+        HIPCHK(hipMemcpyAsync(done, b->stage, (size_t)count * 4, hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        for (uint32_t c = 0; c < count; ++c) done[c] = (done[c] == 1u) ? 1u : 0u;
+    }
+    if (res) {
+        std::vector<uint64_t> tmp((size_t)count);
+        const void *cols[4] = {b->oc, b->minh, b->maxh, b->nsteps};
+        std::vector<uint64_t> all((size_t)4 * count);
+        for (int q = 0; q < 4; ++q) {
+            if (int rc = gather64(b, cols[q], 1, slots, count, tmp.data())) return rc;
+            std::memcpy(all.data() + (size_t)q * count, tmp.data(), (size_t)count * 8);
+        }
+        for (uint32_t c = 0; c < count; ++c) {
+            std::memcpy(&res[c].outcome, &all[c], 8);
+            std::memcpy(&res[c].min_h, &all[count + c], 8);
+            std::memcpy(&res[c].max_h, &all[2 * (size_t)count + c], 8);
+            std::memcpy(&res[c].nsteps, &all[3 * (size_t)count + c], 8);
+        }
+    }
     return 0;
 }
 

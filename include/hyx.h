@@ -143,6 +143,27 @@ int hyx_batch_set_iter_budget(hyx_batch *b, uint64_t iters);
 int hyx_batch_set_queue(hyx_batch *b, const uint32_t *order);
 int hyx_batch_sync(hyx_batch *b);
 
+/* Continuous batching over the instance slots of one batch (serving). No
+ * counterpart in the reference, which has no serving loop: these split
+ * hyx_propagate_until into arm / bounded epochs / harvest so that slots can be
+ * armed and harvested while others are still propagating. Every slot's
+ * arithmetic is that of hyx_propagate_until, so results do not depend on what
+ * else shares the batch. One epoch at a time per batch; distinct batches
+ * (streams) may run epochs concurrently. */
+int hyx_slots_reset(hyx_batch *b); /* mark every slot idle */
+/* Scatter / gather one field for a list of slots: src / dst are [rows][count]. */
+int hyx_slots_write(hyx_batch *b, int field, const uint32_t *slots, uint32_t count, const double *src);
+int hyx_slots_read(hyx_batch *b, int field, const uint32_t *slots, uint32_t count, double *dst);
+/* Arm slots for propagate_until(tf) (tf_lo, max_delta_t may be NULL). */
+int hyx_slots_arm(hyx_batch *b, const uint32_t *slots, uint32_t count, const double *tf_hi, const double *tf_lo,
+                  const double *max_delta_t);
+/* Start one epoch (asynchronous): every armed slot advances by at most iter_budget steps. */
+int hyx_slots_launch(hyx_batch *b, uint64_t iter_budget);
+/* Wait for the epoch; n_active = armed slots still unfinished. */
+int hyx_slots_wait(hyx_batch *b, uint32_t *n_active);
+/* done[c] = 1 once slot c finished (then res[c] holds its propagate result). */
+int hyx_slots_results(hyx_batch *b, const uint32_t *slots, uint32_t count, uint32_t *done, hyx_prop_res *res);
+
 const char *hyx_last_error(void);
 /* Number of HIP devices visible (0 without a GPU). */
 int hyx_device_count(void);
