@@ -218,6 +218,18 @@ struct Gen {
         return nm;
     }
 
+    // i * c^[i] of node j's own history (pow recurrence), formed once.
+    std::string sval(uint32_t j, int i)
+    {
+        if (i == 1) return cv(j, 1);
+        const std::string nm = "z" + std::to_string(j) + "_" + std::to_string(i);
+        if (!dvars.count({j | 0x80000000u, i})) {
+            dvars.insert({j | 0x80000000u, i});
+            o << "    const double " << nm << " = " << lit((double)i) << " * " << cv(j, i) << ";\n";
+        }
+        return nm;
+    }
+
     void set_zero(uint32_t node, int k)
     {
         zset.insert({node, k});
@@ -262,7 +274,12 @@ struct Gen {
             break;
         case HYX_OP_POW:
             o << "    const double " << c << " = pow(" << v(0) << ", " << lit(S.consts[S.aux[j]]) << ");\n";
-            if (S.deg[j] > 0) o << "    const double ra" << j << " = 1.0 / " << v(0) << ";\n";
+            if (S.deg[j] > 0) {
+                const double al = S.consts[S.aux[j]];
+                o << "    const double ra" << j << " = 1.0 / " << v(0) << ";\n";
+                o << "    const double raa" << j << " = ra" << j << " * " << lit(al) << ";\n";
+                o << "    const double rab" << j << " = ra" << j << " * " << lit(al + 1.0) << ";\n";
+            }
             break;
         case HYX_OP_EXP: o << "    const double " << c << " = exp(" << v(0) << ");\n"; break;
         case HYX_OP_LOG:
@@ -377,18 +394,24 @@ struct Gen {
         }
         case HYX_OP_POW: {
             // c^[k] = 1/(k a^[0]) sum_{i=0}^{k-1} (k alpha - i (alpha + 1)) a^[k-i] c^[i]
-            const double al = S.consts[S.aux[j]];
+            //       = (alpha / a^[0]) S1 - ((alpha + 1) / a^[0]) (1/k) S2,
+            // S1 = sum a^[k-i] c^[i], S2 = sum a^[k-i] (i c^[i]): two fma chains without
+            // per-term coefficients (i c^[i] is formed once per order).
+            std::vector<std::pair<std::string, std::string>> t2;
             for (int q = 0; q < k; ++q) {
                 if (zero(A[0], k - q)) continue;
-                const double cf = ((double)k * al - (double)q * (al + 1.0)) / (double)k;
-                if (cf == 0.0) continue;
-                t.push_back({lit(cf), "(" + val(A[0], k - q) + " * " + cv(j, q) + ")"});
+                t.push_back({val(A[0], k - q), cv(j, q)});
+                if (q > 0) t2.push_back({val(A[0], k - q), sval(j, q)});
             }
-            const std::string acc = fma_chain(t, "w" + std::to_string(j) + "_" + ks, o);
-            if (acc.empty())
+            const std::string s1 = fma_chain(t, "w" + std::to_string(j) + "_" + ks, o);
+            const std::string s2 = fma_chain(t2, "u" + std::to_string(j) + "_" + ks, o);
+            if (s1.empty())
                 set_zero(j, k);
+            else if (s2.empty())
+                o << "    const double " << c << " = " << s1 << " * raa" << j << ";\n";
             else
-                o << "    const double " << c << " = " << acc << " * ra" << j << ";\n";
+                o << "    const double " << c << " = __builtin_fma(" << s1 << ", raa" << j << ", -(" << s2 << " * rab" << j
+                  << ") * " << lit(1.0 / k) << ");\n";
             break;
         }
         case HYX_OP_EXP: {
@@ -993,85 +1016,159 @@ bool plan_groups(const Sys &S, GroupPlan &gp, std::string &why)
         for (size_t q = 0; q < B[c].kconsts.size(); ++q) gp.kc[(size_t)c * gp.nkc + q] = S.consts[B[c].kconsts[q]];
         for (size_t q = B[c].kconsts.size(); q < gp.tmpl.nconsts; ++q) gp.kc[(size_t)c * gp.nkc + q] = gp.tmpl.consts[q];
     }
-    // Flat LDS layout per instance: [0, n) state coefficients, then G x nout outputs, then a zero slot.
-    gp.flat = (int)n + G * gp.nout + 1;
-    const int zero_slot = gp.flat - 1;
+    // Outputs of lane c live at LDS index c * nout + q; index G * nout reads zero.
     std::vector<int> outslot(NN, -1);
     for (int c = 0; c < ncl; ++c)
-        for (size_t q = 0; q < B[c].outs.size(); ++q) outslot[B[c].outs[q]] = (int)n + c * gp.nout + (int)q;
-    int maxt = 1;
-    for (uint32_t i = 0; i < n; ++i) maxt = std::max<int>(maxt, (int)terms[i].size());
-    gp.maxt = maxt;
-    gp.ti.assign((size_t)G * own * maxt, zero_slot);
-    gp.ts.assign((size_t)G * own * maxt, 1.0);
-    gp.rc.assign((size_t)G * own, 0.0);
-    gp.ov.assign((size_t)G * own, 0);
+        for (size_t q = 0; q < B[c].outs.size(); ++q) outslot[B[c].outs[q]] = c * gp.nout + (int)q;
+    // Ownership: variables sorted by (output terms, state terms), dealt to lanes slot
+    // by slot, so that each slot's term counts are as uniform as possible.
+    std::vector<uint32_t> order(n);
+    std::vector<int> n_out(n, 0), n_st(n, 0);
     for (uint32_t v = 0; v < n; ++v) {
-        const int g = (int)(v % G), o = (int)(v / G);
-        gp.ov[(size_t)g * own + o] = 1;
-        gp.rc[(size_t)g * own + o] = rconst[v];
-        for (size_t t = 0; t < terms[v].size(); ++t) {
-            const uint32_t idx = HYX_OPERAND_INDEX(terms[v][t].first);
-            const int fi = idx < n ? (int)idx : outslot[idx];
-            if (fi < 0) {
-                why = "a right-hand-side term is not a cluster output";
-                return false;
-            }
-            gp.ti[((size_t)g * own + o) * maxt + t] = fi;
-            gp.ts[((size_t)g * own + o) * maxt + t] = terms[v][t].second;
-        }
+        order[v] = v;
+        for (auto &t : terms[v]) (HYX_OPERAND_INDEX(t.first) < n ? n_st[v] : n_out[v])++;
     }
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return std::make_pair(n_out[a], n_st[a]) > std::make_pair(n_out[b], n_st[b]);
+    });
+    gp.vo.assign((size_t)G * own, -1);
+    gp.rc.assign((size_t)G * own, 0.0);
+    gp.nto.assign(own, 0);
+    gp.nts.assign(own, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t v = order[i];
+        const int g = (int)(i % G), o = (int)(i / G);
+        gp.vo[(size_t)g * own + o] = (int)v;
+        gp.rc[(size_t)g * own + o] = rconst[v];
+        gp.nto[o] = std::max(gp.nto[o], n_out[v]);
+        gp.nts[o] = std::max(gp.nts[o], n_st[v]);
+    }
+    gp.tio.assign(own, {});
+    gp.tis.assign(own, {});
+    gp.tso.assign(own, {});
+    gp.tss.assign(own, {});
+    for (int o = 0; o < own; ++o) {
+        gp.tio[o].assign((size_t)G * gp.nto[o], G * gp.nout);
+        gp.tso[o].assign((size_t)G * gp.nto[o], 1.0);
+        gp.tis[o].assign((size_t)G * gp.nts[o], (int)n);
+        gp.tss[o].assign((size_t)G * gp.nts[o], 1.0);
+    }
+    for (int g = 0; g < G; ++g)
+        for (int o = 0; o < own; ++o) {
+            const int v = gp.vo[(size_t)g * own + o];
+            if (v < 0) continue;
+            int a = 0, b = 0;
+            for (auto &t : terms[v]) {
+                const uint32_t idx = HYX_OPERAND_INDEX(t.first);
+                if (idx < n) {
+                    gp.tis[o][(size_t)g * gp.nts[o] + b] = (int)idx;
+                    gp.tss[o][(size_t)g * gp.nts[o] + b] = t.second;
+                    ++b;
+                } else {
+                    if (outslot[idx] < 0) {
+                        why = "a right-hand-side term is not a cluster output";
+                        return false;
+                    }
+                    gp.tio[o][(size_t)g * gp.nto[o] + a] = outslot[idx];
+                    gp.tso[o][(size_t)g * gp.nto[o] + a] = t.second;
+                    ++a;
+                }
+            }
+        }
     return true;
 }
 
 std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, bool high_acc)
 {
-    const int G = gp.G, own = gp.own, nin = gp.nin, nout = gp.nout, maxt = gp.maxt;
+    const int G = gp.G, own = gp.own, nin = gp.nin, nout = gp.nout;
+    const int n = (int)S.n, nx = n + 1;
+    const int xb = G * std::max(nout, 1) + 1; // LDS: outputs, zero slot, then x[P+1][n+1]
+    const int flat = xb + (P + 1) * nx;
     std::ostringstream h;
-    h << "#define HYX_N " << S.n << "\n#define HYX_NPAR " << std::max<uint32_t>(S.npar, 1) << "\n#define HYX_P " << P
+    h << "#define HYX_N " << n << "\n#define HYX_NPAR " << std::max<uint32_t>(S.npar, 1) << "\n#define HYX_P " << P
       << "\n#define HYX_GROUP 1\n#define HYX_G " << G << "\n#define HYX_I " << (64 / G) << "\n#define HYX_NIN "
       << std::max(nin, 1) << "\n#define HYX_NOUT " << std::max(nout, 1) << "\n#define HYX_NKC " << gp.nkc
-      << "\n#define HYX_OWN " << own << "\n#define HYX_MAXT " << maxt << "\n#define HYX_FLAT " << gp.flat << "\n";
+      << "\n#define HYX_OWN " << own << "\n#define HYX_NX " << nx << "\n#define HYX_XB " << xb << "\n#define HYX_FLAT "
+      << flat << "\n";
     h << "typedef unsigned long long hyx_u64;\n";
-    auto itab = [&](const char *nm, const std::vector<int> &v) {
-        h << "__constant__ int " << nm << "[" << v.size() << "] = {";
+    auto itab = [&](const std::string &nm, const std::vector<int> &v) {
+        h << "__constant__ int " << nm << "[" << std::max<size_t>(v.size(), 1) << "] = {";
         for (size_t q = 0; q < v.size(); ++q) h << (q ? "," : "") << v[q];
+        if (v.empty()) h << "0";
         h << "};\n";
     };
-    auto dtab = [&](const char *nm, const std::vector<double> &v) {
-        h << "__constant__ double " << nm << "[" << v.size() << "] = {";
+    auto dtab = [&](const std::string &nm, const std::vector<double> &v) {
+        h << "__constant__ double " << nm << "[" << std::max<size_t>(v.size(), 1) << "] = {";
         for (size_t q = 0; q < v.size(); ++q) h << (q ? "," : "") << lit(v[q]);
+        if (v.empty()) h << "0.0";
         h << "};\n";
     };
     itab("hyx_tab_iv", gp.iv);
     dtab("hyx_tab_kc", gp.kc);
-    itab("hyx_tab_ti", gp.ti);
-    dtab("hyx_tab_ts", gp.ts);
+    itab("hyx_tab_vo", gp.vo);
     dtab("hyx_tab_rc", gp.rc);
-    itab("hyx_tab_ov", gp.ov);
+    // Per-slot term tables; a sign uniform over the lanes becomes a literal.
+    std::vector<std::vector<std::string>> sgn_o(own), sgn_s(own);
+    for (int o = 0; o < own; ++o) {
+        itab("hyx_tab_tio" + std::to_string(o), gp.tio[o]);
+        itab("hyx_tab_tis" + std::to_string(o), gp.tis[o]);
+        dtab("hyx_tab_tso" + std::to_string(o), gp.tso[o]);
+        dtab("hyx_tab_tss" + std::to_string(o), gp.tss[o]);
+        auto uni = [&](const std::vector<double> &v, int cnt, int t, const std::string &arr) {
+            bool same = true;
+            for (int g = 1; g < G; ++g) same = same && v[(size_t)g * cnt + t] == v[t];
+            return same ? (v[t] == 1.0 ? std::string("+") : v[t] == -1.0 ? std::string("-") : lit(v[t]))
+                        : "tab." + arr + "[" + std::to_string(t) + "]";
+        };
+        for (int t = 0; t < gp.nto[o]; ++t) sgn_o[o].push_back(uni(gp.tso[o], gp.nto[o], t, "tso" + std::to_string(o)));
+        for (int t = 0; t < gp.nts[o]; ++t) sgn_s[o].push_back(uni(gp.tss[o], gp.nts[o], t, "tss" + std::to_string(o)));
+    }
+    // The per-lane table struct and its loader.
+    h << "struct hyx_gtab {\n    int iv[HYX_NIN];\n    double kc[HYX_NKC];\n    int vo[HYX_OWN];\n    double rc[HYX_OWN];\n";
+    for (int o = 0; o < own; ++o)
+        h << "    int tio" << o << "[" << std::max(gp.nto[o], 1) << "];\n    int tis" << o << "["
+          << std::max(gp.nts[o], 1) << "];\n    double tso" << o << "[" << std::max(gp.nto[o], 1) << "];\n    double tss"
+          << o << "[" << std::max(gp.nts[o], 1) << "];\n";
+    h << "};\n__device__ __forceinline__ void hyx_gtab_load(hyx_gtab &t, int g)\n{\n";
+    h << "    for (int q = 0; q < HYX_NIN; ++q) t.iv[q] = hyx_tab_iv[g * HYX_NIN + q];\n";
+    h << "    for (int q = 0; q < HYX_NKC; ++q) t.kc[q] = hyx_tab_kc[g * HYX_NKC + q];\n";
+    h << "    for (int q = 0; q < HYX_OWN; ++q) { t.vo[q] = hyx_tab_vo[g * HYX_OWN + q]; t.rc[q] = hyx_tab_rc[g * HYX_OWN + q]; }\n";
+    for (int o = 0; o < own; ++o) {
+        if (gp.nto[o])
+            h << "    for (int q = 0; q < " << gp.nto[o] << "; ++q) { t.tio" << o << "[q] = hyx_tab_tio" << o << "[g * "
+              << gp.nto[o] << " + q]; t.tso" << o << "[q] = hyx_tab_tso" << o << "[g * " << gp.nto[o] << " + q]; }\n";
+        if (gp.nts[o])
+            h << "    for (int q = 0; q < " << gp.nts[o] << "; ++q) { t.tis" << o << "[q] = hyx_tab_tis" << o << "[g * "
+              << gp.nts[o] << " + q]; t.tss" << o << "[q] = hyx_tab_tss" << o << "[g * " << gp.nts[o] << " + q]; }\n";
+    }
+    h << "}\n";
 
     Gen gen(gp.tmpl, P, high_acc);
     gen.perlane = gp.perlane;
     gen.kc_prefix = "tab.kc";
     auto &o = gen.o;
     auto xv = [](int oo, int k) { return "x" + std::to_string(oo) + "_" + std::to_string(k); };
+    auto X = [&](int k, const std::string &col) { return "fl[" + std::to_string(xb + k * nx) + " + " + col + "]"; };
+    auto vcol = [&](int oo) { return "vc" + std::to_string(oo); };
     o << "template <bool WTC, class TAB>\n"
          "__device__ __forceinline__ void hyx_gstep(double (&xo)[HYX_OWN], const double (&pv)[HYX_NPAR], double t0, "
          "double max_dt, double &h_out, bool &nf_out, int g, double *fl, const TAB &tab, double *__restrict__ tc, "
          "hyx_u64 tcs)\n{\n";
-    o << "    (void)pv; (void)t0; (void)tc; (void)tcs;\n";
+    o << "    (void)pv; (void)t0; (void)tc; (void)tcs; (void)g;\n";
+    // Column of each owned variable in the x rows (column n reads zero for empty slots).
+    for (int oo = 0; oo < own; ++oo)
+        o << "    const int " << vcol(oo) << " = (tab.vo[" << oo << "] >= 0) ? tab.vo[" << oo << "] : HYX_N;\n";
     o << "    __syncthreads();\n";
     for (int oo = 0; oo < own; ++oo) {
         o << "    const double " << xv(oo, 0) << " = xo[" << oo << "];\n";
-        o << "    if (tab.ov[" << oo << "]) fl[g + " << oo * G << "] = " << xv(oo, 0) << ";\n";
+        // Empty slots hold +0.0 and write it into the zero column: no branch needed.
+        o << "    " << X(0, vcol(oo)) << " = " << xv(oo, 0) << ";\n";
     }
     o << "    __syncthreads();\n";
     for (int k = 0; k < P; ++k) {
-        const std::string base = (k & 1) ? "fl[HYX_FLAT + " : "fl[";
-        const std::string next = ((k + 1) & 1) ? "fl[HYX_FLAT + " : "fl[";
         o << "    // order " << k << "\n";
         for (int q = 0; q < nin; ++q)
-            o << "    const double " << Gen::cv((uint32_t)q, k) << " = " << base << "tab.iv[" << q << "]];\n";
+            o << "    const double " << Gen::cv((uint32_t)q, k) << " = " << X(k, "tab.iv[" + std::to_string(q) + "]") << ";\n";
         for (uint32_t j = gp.tmpl.n; j < gp.tmpl.nnodes; ++j) {
             if (k == 0)
                 gen.order0(j);
@@ -1080,28 +1177,34 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
         }
         for (int q = 0; q < nout; ++q) {
             const uint32_t w = HYX_OPERAND(HYX_KIND_NODE, gp.outs[q]);
-            o << "    " << base << "HYX_N + g * HYX_NOUT + " << q << "] = " << (gen.zero(w, k) ? "0.0" : gen.val(w, k))
-              << ";\n";
+            o << "    fl[g * " << nout << " + " << q << "] = " << (gen.zero(w, k) ? "0.0" : gen.val(w, k)) << ";\n";
         }
         o << "    __syncthreads();\n";
         for (int oo = 0; oo < own; ++oo) {
-            o << "    double r" << oo << "_" << k << " = " << (k == 0 ? "tab.rc[" + std::to_string(oo) + "]" : "0.0")
-              << ";\n";
-            for (int t = 0; t < maxt; ++t)
-                o << "    r" << oo << "_" << k << " = __builtin_fma(tab.ts[" << oo * maxt + t << "], " << base << "tab.ti["
-                  << oo * maxt + t << "]], r" << oo << "_" << k << ");\n";
-            o << "    const double " << xv(oo, k + 1) << " = r" << oo << "_" << k << " * " << lit(1.0 / (k + 1)) << ";\n";
-            if (k + 1 < P) o << "    if (tab.ov[" << oo << "]) " << next << "g + " << oo * G << "] = " << xv(oo, k + 1) << ";\n";
+            std::string r = (k == 0) ? "tab.rc[" + std::to_string(oo) + "]" : "";
+            auto add = [&](const std::string &sg, const std::string &val) {
+                if (sg == "+")
+                    r = r.empty() ? val : "(" + r + " + " + val + ")";
+                else if (sg == "-")
+                    r = r.empty() ? "(-" + val + ")" : "(" + r + " - " + val + ")";
+                else
+                    r = r.empty() ? "(" + sg + " * " + val + ")" : "__builtin_fma(" + sg + ", " + val + ", " + r + ")";
+            };
+            for (int t = 0; t < gp.nto[oo]; ++t)
+                add(sgn_o[oo][t], "fl[tab.tio" + std::to_string(oo) + "[" + std::to_string(t) + "]]");
+            for (int t = 0; t < gp.nts[oo]; ++t)
+                add(sgn_s[oo][t], X(k, "tab.tis" + std::to_string(oo) + "[" + std::to_string(t) + "]"));
+            if (r.empty()) r = "0.0";
+            o << "    const double " << xv(oo, k + 1) << " = " << r << " * " << lit(1.0 / (k + 1)) << ";\n";
+            o << "    " << X(k + 1, vcol(oo)) << " = " << xv(oo, k + 1) << ";\n";
         }
-        if (k + 1 < P) o << "    __syncthreads();\n";
+        o << "    __syncthreads();\n";
     }
     o << "    if constexpr (WTC) if (tc) {\n";
     for (int oo = 0; oo < own; ++oo) {
-        o << "        if (tab.ov[" << oo << "]) {\n";
-        for (int k = 0; k <= P; ++k)
-            o << "            tc[((hyx_u64)(g + " << oo * G << ") * " << (P + 1) << " + " << k << ") * tcs] = " << xv(oo, k)
-              << ";\n";
-        o << "        }\n";
+        o << "        if (tab.vo[" << oo << "] >= 0)\n";
+        o << "            for (int k = 0; k <= HYX_P; ++k) tc[((hyx_u64)tab.vo[" << oo << "] * " << (P + 1)
+          << " + k) * tcs] = fl[HYX_XB + k * HYX_NX + " << vcol(oo) << "];\n";
     }
     o << "    }\n";
     o << "    double mx = 0.0, mo = 0.0, mom1 = 0.0, ab;\n";
@@ -1123,17 +1226,18 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
     o << "    double h = rho_m * " << lit(rhofac) << ";\n";
     o << "    const double lim = fabs(max_dt);\n    h = (h < lim) ? h : lim;\n    if (max_dt < 0.0) h = -h;\n";
     o << "    int nf = 0;\n";
+    // State update from the coefficient rows in LDS.
     for (int oo = 0; oo < own; ++oo) {
         if (!high_acc) {
             o << "    { double r = " << xv(oo, P) << ";\n";
-            for (int k = P - 1; k >= 0; --k) o << "      r = __builtin_fma(r, h, " << xv(oo, k) << ");\n";
+            o << "#pragma unroll\n      for (int k = HYX_P - 1; k >= 0; --k) r = __builtin_fma(r, h, fl[HYX_XB + k * HYX_NX + "
+              << vcol(oo) << "]);\n";
         } else {
             o << "    { double r = " << xv(oo, 0) << ", cp = 0.0, hk = 1.0;\n";
-            for (int k = 1; k <= P; ++k)
-                o << "      { hk *= h; const double y = " << xv(oo, k)
-                  << " * hk - cp; const double tt = r + y; cp = (tt - r) - y; r = tt; }\n";
+            o << "#pragma unroll\n      for (int k = 1; k <= HYX_P; ++k) { hk *= h; const double y = fl[HYX_XB + k * HYX_NX + "
+              << vcol(oo) << "] * hk - cp; const double tt = r + y; cp = (tt - r) - y; r = tt; }\n";
         }
-        o << "      xo[" << oo << "] = r; nf |= (tab.ov[" << oo << "] && !isfinite(r)) ? 1 : 0; }\n";
+        o << "      xo[" << oo << "] = r; nf |= (tab.vo[" << oo << "] >= 0 && !isfinite(r)) ? 1 : 0; }\n";
     }
     o << "#pragma unroll\n    for (int off = 1; off < HYX_G; off <<= 1) nf |= __shfl_xor(nf, off);\n";
     o << "    h_out = h;\n    nf_out = nf != 0;\n}\n";

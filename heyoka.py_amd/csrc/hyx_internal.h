@@ -34,16 +34,21 @@ bool choose_compact(const Sys &S, int order);
 // Group form: a template cluster evaluated by each lane of a group of G lanes
 // per instance, with per-lane bindings (see plan_groups in hyx_codegen.cpp).
 struct GroupPlan {
-    int G = 0, ncl = 0, nin = 0, nout = 0, nkc = 1, own = 0, maxt = 1, flat = 0;
+    int G = 0, ncl = 0, nin = 0, nout = 0, nkc = 1, own = 0;
     Sys tmpl;                       // cluster template: inputs are its state variables 0..nin-1
     std::vector<char> perlane;      // template constant held per lane
     std::vector<uint32_t> outs;     // template nodes feeding right-hand sides, by output slot
     std::vector<int> iv;            // [G][nin] state variable of each input
     std::vector<double> kc;         // [G][nkc] constants
-    std::vector<int> ti;            // [G][own][maxt] flat LDS index of each rhs term
-    std::vector<double> ts;         // [G][own][maxt] term signs
-    std::vector<double> rc;         // [G][own] constant rhs parts
-    std::vector<int> ov;            // [G][own] owned variable exists
+    // Owned state variables: lane g, slot o owns vo[g][o] (-1: none). Its right-hand
+    // side is rc + sum of output terms (LDS output index, sign) + sum of state terms
+    // (variable index, sign); per slot, nto[o] / nts[o] terms (padded: output index
+    // G * nout and variable index n read zeros).
+    std::vector<int> vo;            // [G][own]
+    std::vector<double> rc;         // [G][own]
+    std::vector<int> nto, nts;      // [own]
+    std::vector<std::vector<int>> tio, tis;          // [own][G * nto[o]] / [own][G * nts[o]]
+    std::vector<std::vector<double>> tso, tss;       // signs, same shapes
 };
 
 bool plan_groups(const Sys &S, GroupPlan &gp, std::string &why);

@@ -2,46 +2,22 @@
 // instances per wavefront (hand-written CDNA4 kernels, compiled with hipRTC
 // after hyx_kcommon.hip).
 //
-// The generated prelude defines the group plan and
+// The generated prelude defines the group plan, the per-lane table struct
+// hyx_gtab (cluster input / constant bindings, owned variables, right-hand-side
+// terms) with its loader hyx_gtab_load, and
 //   hyx_gstep<WTC>(xo, pv, t0, max_dt, h, nf, g, fl, tab, tc, tcs)
 // which advances one instance by one adaptive step. Lane g of the group
 // evaluates cluster g of the decomposition (one of the isomorphic subgraphs
 // the code generator found, e.g. one body pair of an N-body system) with its
 // coefficient histories in registers, and owns the state variables
-// v = g + o * HYX_G (o < HYX_OWN). Per order, coefficients are exchanged
-// through this instance's LDS block `fl` (two buffers of HYX_FLAT doubles:
-// state coefficients, cluster outputs, a zero slot).
+// tab.vo[o] (o < HYX_OWN). Per order, coefficients are exchanged through this
+// instance's LDS block `fl` of HYX_FLAT doubles: the cluster outputs of the
+// current order, a zero slot, and every order of every state variable
+// (rows of HYX_NX = n + 1, the last column zero), which the state update reads.
 //
 // Every lane of the wave executes every step, also lanes whose group has no
 // instance (their results are not stored): hyx_gstep contains workgroup
 // barriers, and a block is exactly one wavefront.
-
-struct hyx_gtab {
-    int iv[HYX_NIN];                    // state variable behind each cluster input
-    double kc[HYX_NKC];                 // per-lane constants of the cluster
-    int ti[HYX_OWN * HYX_MAXT];         // right-hand-side terms of the owned variables (flat indices)
-    double ts[HYX_OWN * HYX_MAXT];      // their signs
-    double rc[HYX_OWN];                 // constant part of the owned right-hand sides
-    int ov[HYX_OWN];                    // owned variable exists (v < HYX_N)
-};
-
-__device__ __forceinline__ void hyx_gtab_load(hyx_gtab &t, int g)
-{
-#pragma unroll
-    for (int q = 0; q < HYX_NIN; ++q) t.iv[q] = hyx_tab_iv[g * HYX_NIN + q];
-#pragma unroll
-    for (int q = 0; q < HYX_NKC; ++q) t.kc[q] = hyx_tab_kc[g * HYX_NKC + q];
-#pragma unroll
-    for (int q = 0; q < HYX_OWN * HYX_MAXT; ++q) {
-        t.ti[q] = hyx_tab_ti[g * HYX_OWN * HYX_MAXT + q];
-        t.ts[q] = hyx_tab_ts[g * HYX_OWN * HYX_MAXT + q];
-    }
-#pragma unroll
-    for (int q = 0; q < HYX_OWN; ++q) {
-        t.rc[q] = hyx_tab_rc[g * HYX_OWN + q];
-        t.ov[q] = hyx_tab_ov[g * HYX_OWN + q];
-    }
-}
 
 struct hyx_ginst {
     double xo[HYX_OWN];
@@ -55,7 +31,7 @@ __device__ __forceinline__ void hyx_gload(hyx_ginst &s, const hyx_kargs &a, hyx_
 {
 #pragma unroll
     for (int o = 0; o < HYX_OWN; ++o)
-        s.xo[o] = t.ov[o] ? a.state[(hyx_u64)(g + o * HYX_G) * a.ld + i] : 0.0;
+        s.xo[o] = (t.vo[o] >= 0) ? a.state[(hyx_u64)t.vo[o] * a.ld + i] : 0.0;
 #pragma unroll
     for (int q = 0; q < HYX_NPAR; ++q) s.pv[q] = a.pars ? a.pars[(hyx_u64)q * a.ld + i] : 0.0;
     s.thi = a.t_hi[i];
@@ -67,7 +43,7 @@ __device__ __forceinline__ void hyx_gstore_state(const hyx_ginst &s, const hyx_k
 {
 #pragma unroll
     for (int o = 0; o < HYX_OWN; ++o)
-        if (t.ov[o]) a.state[(hyx_u64)(g + o * HYX_G) * a.ld + i] = s.xo[o];
+        if (t.vo[o] >= 0) a.state[(hyx_u64)t.vo[o] * a.ld + i] = s.xo[o];
     if (g == 0) {
         a.t_hi[i] = s.thi;
         a.t_lo[i] = s.tlo;
@@ -141,12 +117,12 @@ __device__ __forceinline__ bool hyx_gprop_iter(hyx_ginst &s, const hyx_kargs &a,
 }
 
 #define HYX_GROUP_SETUP                                                                                               \
-    __shared__ double hyx_lds[HYX_I][2 * HYX_FLAT];                                                                  \
+    __shared__ double hyx_lds[HYX_I][HYX_FLAT];                                                                      \
     const int lane = (int)__lane_id();                                                                               \
     const int grp = lane / HYX_G;                                                                                    \
     const int g = lane % HYX_G;                                                                                      \
     double *fl = &hyx_lds[grp][0];                                                                                   \
-    for (int q = g; q < 2 * HYX_FLAT; q += HYX_G) fl[q] = 0.0;                                                       \
+    for (int q = g; q < HYX_FLAT; q += HYX_G) fl[q] = 0.0;                                                           \
     hyx_gtab tab;                                                                                                    \
     hyx_gtab_load(tab, g);                                                                                           \
     __syncthreads();

@@ -40,8 +40,8 @@ class Workload:
     masses: list | None = None
 
 
-def pendulum(n=4):
-    rng = np.random.default_rng(0)
+def pendulum(n=4, seed=0):
+    rng = np.random.default_rng(seed)
     x0 = rng.uniform(-1, 1, n)
     v0 = rng.uniform(-0.5, 0.5, n)
     return Workload("pendulum", model.pendulum(), np.stack([x0, v0]), None, np.full(n, 100.0),
@@ -66,9 +66,9 @@ def nbody6_base():
     return m, pos, vel
 
 
-def nbody6(n=65536, t_final=1000.0):
+def nbody6(n=65536, t_final=1000.0, seed=2):
     m, pos, vel = nbody6_base()
-    rng = np.random.default_rng(2)
+    rng = np.random.default_rng(seed)
     st = np.zeros((36, n))
     for i in range(6):
         for q in range(3):
@@ -77,8 +77,8 @@ def nbody6(n=65536, t_final=1000.0):
     return Workload("nbody6", model.nbody(6, masses=list(m)), st, None, np.full(n, t_final), 1e-15, masses=list(m))
 
 
-def cr3bp(n=131072, log_t=(1.0, 4.0)):
-    rng = np.random.default_rng(3)
+def cr3bp(n=131072, log_t=(1.0, 4.0), seed=3):
+    rng = np.random.default_rng(seed)
     mu = rng.uniform(0.001, 0.05, n)
     sign = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
     st = np.stack([0.5 - mu, sign * np.sqrt(3.0) / 2, np.zeros(n), np.zeros(n)])
@@ -111,8 +111,8 @@ def kepler_to_cartesian(a, e, inc, Om, om, M, mu=1.0):
     return pos, vel
 
 
-def j2var(n=32768, periods=200.0):
-    rng = np.random.default_rng(4)
+def j2var(n=32768, periods=200.0, seed=4):
+    rng = np.random.default_rng(seed)
     a = rng.uniform(1.05, 1.5, n)
     e = rng.uniform(0.0, 0.1, n)
     inc = rng.uniform(0.0, np.pi, n)

@@ -129,32 +129,27 @@ static int xch_i[64];
 static double __shfl_xor(double v, int off) { xch_d[tl_g] = v; __syncthreads(); double r = xch_d[tl_g ^ off]; __syncthreads(); return r; }
 static int __shfl_xor(int v, int off) { xch_i[tl_g] = v; __syncthreads(); int r = xch_i[tl_g ^ off]; __syncthreads(); return r; }
 #include "prelude.inc"
-struct Tab { int iv[HYX_NIN]; double kc[HYX_NKC]; int ti[HYX_OWN * HYX_MAXT]; double ts[HYX_OWN * HYX_MAXT];
-             double rc[HYX_OWN]; int ov[HYX_OWN]; };
-static double fl[2 * HYX_FLAT];
+static double fl[HYX_FLAT];
 struct Arg { int g; const double *x; const double *pv; double t0, mdt; double *xout; double *h; int *nf; double *tc; };
 static void *run(void *p)
 {
     Arg *a = (Arg *)p;
     const int g = a->g;
     tl_g = g;
-    Tab t;
-    for (int q = 0; q < HYX_NIN; ++q) t.iv[q] = hyx_tab_iv[g * HYX_NIN + q];
-    for (int q = 0; q < HYX_NKC; ++q) t.kc[q] = hyx_tab_kc[g * HYX_NKC + q];
-    for (int q = 0; q < HYX_OWN * HYX_MAXT; ++q) { t.ti[q] = hyx_tab_ti[g * HYX_OWN * HYX_MAXT + q]; t.ts[q] = hyx_tab_ts[g * HYX_OWN * HYX_MAXT + q]; }
-    for (int q = 0; q < HYX_OWN; ++q) { t.rc[q] = hyx_tab_rc[g * HYX_OWN + q]; t.ov[q] = hyx_tab_ov[g * HYX_OWN + q]; }
+    hyx_gtab t;
+    hyx_gtab_load(t, g);
     double xo[HYX_OWN], pv[HYX_NPAR];
-    for (int o = 0; o < HYX_OWN; ++o) xo[o] = t.ov[o] ? a->x[g + o * HYX_G] : 0.0;
+    for (int o = 0; o < HYX_OWN; ++o) xo[o] = (t.vo[o] >= 0) ? a->x[t.vo[o]] : 0.0;
     for (int q = 0; q < HYX_NPAR; ++q) pv[q] = a->pv[q];
     double h; bool nf;
     hyx_gstep<true>(xo, pv, a->t0, a->mdt, h, nf, g, fl, t, a->tc, 1);
-    for (int o = 0; o < HYX_OWN; ++o) if (t.ov[o]) a->xout[g + o * HYX_G] = xo[o];
+    for (int o = 0; o < HYX_OWN; ++o) if (t.vo[o] >= 0) a->xout[t.vo[o]] = xo[o];
     a->h[g] = h; a->nf[g] = nf;
     return nullptr;
 }
 extern "C" int gstep_cpu(const double *x, const double *pv, double t0, double mdt, double *xout, double *h, int *nf, double *tc)
 {
-    for (int q = 0; q < 2 * HYX_FLAT; ++q) fl[q] = 0.0;
+    for (int q = 0; q < HYX_FLAT; ++q) fl[q] = 0.0;
     pthread_barrier_init(&bar, nullptr, HYX_G);
     pthread_t th[HYX_G]; Arg args[HYX_G];
     for (int g = 0; g < HYX_G; ++g) { args[g] = Arg{g, x, pv, t0, mdt, xout, h, nf, tc}; pthread_create(&th[g], nullptr, run, &args[g]); }
