@@ -46,6 +46,50 @@ def _cpu_baseline(dec, order, state, pars, t0, t1, n_sample, threads, lane_batch
     return float(r["nsteps"].astype(np.float64).sum()) / dt, dt
 
 
+def _nbody_energy(st, masses):
+    n = len(masses)
+    pos = st.reshape(n, 6, -1)[:, :3, :]
+    vel = st.reshape(n, 6, -1)[:, 3:, :]
+    e = sum(0.5 * masses[i] * np.sum(vel[i] ** 2, axis=0) for i in range(n))
+    for i in range(n):
+        for j in range(i + 1, n):
+            e = e - masses[i] * masses[j] / np.sqrt(np.sum((pos[j] - pos[i]) ** 2, axis=0))
+    return e
+
+
+def _cr3bp_jacobi(st, mu):
+    x, y, vx, vy = st
+    r1 = np.sqrt((x + mu) ** 2 + y ** 2)
+    r2 = np.sqrt((x - 1 + mu) ** 2 + y ** 2)
+    return x ** 2 + y ** 2 + 2 * (1 - mu) / r1 + 2 * mu / r2 - (vx ** 2 + vy ** 2)
+
+
+def _accuracy_vs_real128(system, dec, order, wl, n_sample, t_final, threads):
+    """GPU error against the real128 oracle on the first instances of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    from heyoka_amd import _lib
+
+    st = np.ascontiguousarray(wl.state[:, :n_sample])
+    pa = None if wl.pars is None else np.ascontiguousarray(wl.pars[:, :n_sample])
+    tf = np.broadcast_to(np.asarray(t_final, dtype=float), (n_sample,)).copy()
+    b = _lib.Batch(system, n_sample)
+    b.write(_lib.FIELD_STATE, st)
+    if pa is not None and system.npar:
+        b.write(_lib.FIELD_PARS, pa)
+    b.propagate_until(tf, None, None, 0, _lib.MODE_PERSISTENT)
+    g = b.read(_lib.FIELD_STATE, st.shape)
+    q = O.propagate_until(dec, 2 * order, st, tf, pars=pa, nthreads=threads, lane_batch=1, f128=True)
+    r = O.propagate_until(dec, order, st, tf, pars=pa, nthreads=threads, lane_batch=1)
+    qs = q["state"] + q["state_lo"]
+    scale = np.maximum(np.abs(qs), 1.0)
+    eg = np.max(np.abs(g - qs) / scale, axis=0)
+    e64 = np.max(np.abs(r["state"] - qs) / scale, axis=0)
+    return {"sample": f"first {n_sample} instances to t={float(np.max(tf))}",
+            "gpu_vs_real128_median": float(np.median(eg)), "gpu_vs_real128_max": float(np.max(eg)),
+            "cpu_f64_vs_real128_median": float(np.median(e64)), "cpu_f64_vs_real128_max": float(np.max(e64))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +105,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -189,6 +234,20 @@ def main():
         },
         "cpu_baseline": None,
     }
+    # Invariant drift of the whole ensemble over the timed horizon.
+    final = batch.read(_lib.FIELD_STATE, wl.state.shape)
+    if args.config == "nbody6":
+        e0, e1 = _nbody_energy(wl.state, wl.masses), _nbody_energy(final, wl.masses)
+        d = np.abs((e1 - e0) / e0)
+        out["drift"] = {"invariant": "energy", "median_rel": float(np.median(d)), "max_rel": float(np.max(d))}
+    elif args.config == "cr3bp":
+        c0, c1 = _cr3bp_jacobi(wl.state, wl.pars[0]), _cr3bp_jacobi(final, wl.pars[0])
+        d = np.abs((c1 - c0) / c0)
+        out["drift"] = {"invariant": "Jacobi constant", "median_rel": float(np.median(d)), "max_rel": float(np.max(d))}
+    if not args.no_accuracy:
+        ns = {"nbody6": 16, "cr3bp": 64, "j2var": 8}[args.config]
+        ta = {"nbody6": 2.0, "cr3bp": 5.0, "j2var": 2.0}[args.config]
+        out["accuracy"] = _accuracy_vs_real128(system, dec, order, wl, ns, ta, args.cpu_threads)
     if args.config == "cr3bp" and not args.no_plain:
         load()
         t = time.perf_counter()

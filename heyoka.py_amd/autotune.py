@@ -1,0 +1,117 @@
+"""Autotuner for the persistent form.
+
+Searches, for one system on the current GPU:
+- waves per CU of the persistent launch (occupancy, i.e. instances in flight),
+- the refill threshold (idle lanes that trigger a work-stealing claim),
+- optionally the code form (unrolled / compact / group), which changes the
+  arithmetic order and therefore results at the ulp level.
+
+Launch parameters never change an instance's arithmetic, so every candidate of
+a given code form gives bit-identical results (tests/test_gpu_autotune.py).
+Each candidate propagates the same sample of instances for a fixed time and is
+scored by instance-steps per second. Results are cached on disk, keyed by the
+op-list, the order and the GPU architecture, like the code objects.
+"""
+
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import time
+
+import numpy as np
+
+from . import _lib
+from .integrator import get_system
+
+
+def _cache_path():
+    d = os.environ.get("HYX_CACHE_DIR") or os.path.join(_lib.PKG_DIR, "code_cache")
+    os.makedirs(d, exist_ok=True)
+    return os.path.join(d, "autotune.json")
+
+
+def _key(dec, order, form):
+    words, consts = _lib.oplist_args(dec)
+    h = hashlib.sha1(words.tobytes() + consts.tobytes()).hexdigest()[:16]
+    return f"{h}-p{order}-{form}-gfx950"
+
+
+def _load():
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(db):
+    p = _cache_path()
+    tmp = p + f".tmp{os.getpid()}"
+    with open(tmp, "w") as f:
+        json.dump(db, f, indent=1)
+    os.replace(tmp, p)
+
+
+def measure(system, state, pars, t_final, waves_per_cu, refill_min, repeats=1):
+    """instance-steps/s of one persistent propagation of the sample (device-resident)."""
+    N = state.shape[1]
+    b = _lib.Batch(system, N)
+    b.set_persistent_config(waves_per_cu, refill_min)
+    best = 0.0
+    final = None
+    for _ in range(repeats):
+        b.write(_lib.FIELD_STATE, state)
+        if pars is not None and system.npar:
+            b.write(_lib.FIELD_PARS, pars)
+        b.write(_lib.FIELD_TIME_HI, np.zeros(N))
+        b.write(_lib.FIELD_TIME_LO, np.zeros(N))
+        b.sync()
+        t = time.perf_counter()
+        r = b.propagate_until(t_final, None, None, 0, _lib.MODE_PERSISTENT)
+        dt = time.perf_counter() - t
+        best = max(best, float(r["nsteps"].astype(np.float64).sum()) / dt)
+        final = b.read(_lib.FIELD_STATE, state.shape)
+    return best, final
+
+
+def autotune(dec, order, state, pars=None, t_final=1.0, forms=(None,), waves=(0, 1, 2, 4), refills=(1, 8, 32),
+             use_cache=True, verbose=False):
+    """Pick the fastest (form, waves_per_cu, refill_min) on a sample; returns (best, table)."""
+    db = _load() if use_cache else {}
+    table = []
+    best = None
+    tf = np.broadcast_to(np.asarray(t_final, dtype=float), (state.shape[1],)).copy()
+    for form in forms:
+        system = get_system(dec, order, compact=form)
+        key = _key(dec, order, system.form)
+        if use_cache and key in db:
+            cand = dict(db[key], form=system.form)
+            table.append(cand)
+            if best is None or cand["rate"] > best["rate"]:
+                best = cand
+            continue
+        local = None
+        for w in waves:
+            for rf in refills:
+                rate, _ = measure(system, state, pars, tf, w, rf)
+                row = {"form": system.form, "waves_per_cu": w, "refill_min": rf, "rate": rate}
+                table.append(row)
+                if verbose:
+                    print(f"[autotune] {row}", flush=True)
+                if local is None or rate > local["rate"]:
+                    local = row
+        db[key] = {k: local[k] for k in ("waves_per_cu", "refill_min", "rate")}
+        if best is None or local["rate"] > best["rate"]:
+            best = local
+    if use_cache:
+        _store(db)
+    return best, table
+
+
+def tuned_config(dec, order, form=None):
+    """The cached (waves_per_cu, refill_min) for a system, or the defaults (0, 1)."""
+    system = get_system(dec, order, compact=form)
+    rec = _load().get(_key(dec, order, system.form))
+    return (rec["waves_per_cu"], rec["refill_min"]) if rec else (0, 1)

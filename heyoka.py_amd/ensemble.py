@@ -73,6 +73,8 @@ def _ensemble_sequential(tp, ta, arg, n_iter, gen, kwargs):
 
 
 def _ensemble_packed(tp, ta, arg, n_iter, gen, kwargs):
+    if n_iter == 0:
+        return []
     if not isinstance(ta, taylor_adaptive_batch):
         raise TypeError("The template integrator must be a taylor_adaptive_batch")
     max_steps = int(kwargs.pop("max_steps", 0))

@@ -542,3 +542,49 @@ def diff(e, x):
         return r
 
     return d(e)
+
+
+def evaluate(e, variables: dict, pars=None, time_value=0.0):
+    """Evaluate an expression with numpy: ``variables`` maps names to arrays,
+    ``pars`` is indexable by parameter index."""
+    import numpy as _np
+
+    memo = {}
+    fns = {"sin": _np.sin, "cos": _np.cos, "exp": _np.exp, "log": _np.log, "sqrt": _np.sqrt,
+           "square": lambda a: a * a, "neg": lambda a: -a}
+
+    def ev(x):
+        k = id(x)
+        if k in memo:
+            return memo[k][1]
+        if x._k == _VAR:
+            r = variables[x._a]
+        elif x._k == _NUM:
+            r = x._a
+        elif x._k == _PAR:
+            r = pars[x._a]
+        elif x._k == _TIME:
+            r = time_value
+        else:
+            name, args = x._a
+            a = [ev(y) for y in args]
+            if name == "add":
+                r = a[0] + a[1]
+            elif name == "sub":
+                r = a[0] - a[1]
+            elif name == "mul":
+                r = a[0] * a[1]
+            elif name == "div":
+                r = a[0] / a[1]
+            elif name == "pow":
+                r = a[0] ** a[1]
+            elif name == "sum":
+                r = a[0]
+                for y in a[1:]:
+                    r = r + y
+            else:
+                r = fns[name](*a)
+        memo[k] = (x, r)
+        return r
+
+    return ev(_ex(e))

@@ -134,7 +134,11 @@ class taylor_adaptive_batch:
         if st.shape[0] == 0:
             st = np.zeros((n, B))
         elif vsys is not None and st.shape[0] == vsys.n_orig_sv:
-            ext = np.repeat(np.array(vsys.initial_derivatives(), dtype=np.float64)[:, None], B, axis=1)
+            t_init = np.zeros(B) if time is None else np.asarray(time, dtype=np.float64)
+            p_init = None if pars is None else np.asarray(pars, dtype=np.float64)
+            ext = vsys.initial_derivatives(st, p_init, t_init)
+            ext = np.stack([np.broadcast_to(np.asarray(v, dtype=np.float64), (B,)) for v in ext]) if ext else \
+                np.zeros((0, B))
             st = np.concatenate([st, ext], axis=0)
         if st.shape[0] != n:
             raise ValueError(
@@ -487,6 +491,50 @@ class taylor_adaptive_batch:
     def get_mindex(self, i):
         self._need_var()
         return self._hyx_vsys.get_mindex(i)
+
+    @property
+    def tstate(self):
+        """The state of the original variables given by the last eval_taylor_map (read-only)."""
+        self._need_var()
+        if "_hyx_tstate" not in self.__dict__:
+            self.__dict__["_hyx_tstate"] = np.zeros((self._hyx_vsys.n_orig_sv, self._hyx_B))
+        return self._ro(self._hyx_tstate)
+
+    def eval_taylor_map(self, inputs):
+        """Evaluate the Taylor map of the variational state at the displacements
+        ``inputs`` ([n_vargs, batch_size]): x_i + sum over multi-indices m of
+        d^m x_i / m! * prod_j dx_j^m_j (``heyoka/expose_batch_integrators.cpp:573-649``)."""
+        self._need_var()
+        vs = self._hyx_vsys
+        if isinstance(inputs, np.ndarray) and inputs.dtype != np.float64:
+            raise TypeError("Invalid dtype detected for the inputs of a Taylor map evaluation: the expected dtype is "
+                            f"'float64', but the dtype of the inputs array is '{inputs.dtype}' instead")
+        a = np.asarray(inputs, dtype=np.float64)
+        nv = len(vs.vargs)
+        if a.ndim != 2:
+            raise ValueError(f"The array of inputs provided for the evaluation of a Taylor map has {a.ndim} "
+                             "dimension(s), but it must have 2 dimensions instead")
+        if a.shape[0] != nv:
+            raise ValueError(f"The array of inputs provided for the evaluation of a Taylor map has {a.shape[0]} "
+                             f"row(s), but it must have {nv} row(s) instead")
+        if a.shape[1] != self._hyx_B:
+            raise ValueError(f"The array of inputs provided for the evaluation of a Taylor map has {a.shape[1]} "
+                             f"column(s), but it must have {self._hyx_B} column(s) instead")
+        from math import factorial
+
+        out = self._hyx_state[: vs.n_orig_sv].copy()
+        for idx in range(vs.n_orig_sv, self.dim):
+            mi = vs.get_mindex(idx)
+            i, m = mi[0], mi[1:]
+            term = self._hyx_state[idx].copy()
+            for j, e in enumerate(m):
+                if e:
+                    term = term * a[j] ** e / factorial(e)
+            out[i] += term
+        if "_hyx_tstate" not in self.__dict__:
+            self.__dict__["_hyx_tstate"] = np.zeros((vs.n_orig_sv, self._hyx_B))
+        self._hyx_tstate[...] = out
+        return self.tstate
 
     # ---- GPU specifics ------------------------------------------------------------------
     @property

@@ -23,7 +23,9 @@ from __future__ import annotations
 import enum
 import itertools
 
-from .expr import expression, diff, par as _par, get_variables, _ex
+import numpy as np
+
+from .expr import expression, diff, par as _par, get_variables, _ex, evaluate, time as _time
 
 
 class var_args(enum.IntFlag):
@@ -51,8 +53,6 @@ class var_ode_sys:
         pairs = [(_ex(l), _ex(r)) for l, r in sys]
         svars = [l for l, _ in pairs]
         if isinstance(args, var_args):
-            if args & var_args.time:
-                raise NotImplementedError("Derivatives with respect to the initial time are not supported")
             vargs = []
             if args & var_args.vars:
                 vargs += svars
@@ -66,11 +66,13 @@ class var_ode_sys:
                     elif e.is_func():
                         stack.extend(e.args)
                 vargs += [_par[i] for i in range(npar)]
+            if args & var_args.time:
+                vargs.append(_time)
         else:
             vargs = [_ex(a) for a in args]
             for a in vargs:
-                if not (a.is_var() or a.is_par()):
-                    raise TypeError("The arguments of a var_ode_sys must be state variables or parameters")
+                if not (a.is_var() or a.is_par() or a.is_time()):
+                    raise TypeError("The arguments of a var_ode_sys must be state variables, parameters or time")
                 if a.is_var() and all(a != v for v in svars):
                     raise ValueError(f"The variable '{a}' is not a state variable of the ODE system")
         if not vargs:
@@ -167,11 +169,23 @@ class var_ode_sys:
     def get_mindex(self, i):
         return list(self._mindex[i])
 
-    def initial_derivatives(self):
-        """Identity initial conditions for the derivative variables (order 1: δ_ij for state
-        arguments, 0 for parameters; higher orders: 0)."""
+    def initial_derivatives(self, x0=None, pars=None, t0=0.0):
+        """Initial values of the derivative variables.
+
+        Order 1: d x_i / d x_j(t0) = delta_ij, d x_i / d p = 0, d x_i / d t0 = -f_i(x0, t0)
+        (the solution shifted in its initial time); higher orders: 0, except that
+        higher derivatives involving t0 are not formulated (NotImplementedError).
+        With x0 given ([n_orig] or [n_orig, B]) the values are arrays over the batch."""
         n, A = self._n_orig, len(self._vargs)
         vals = []
+        has_t = any(a.is_time() for a in self._vargs)
+        fvals = None
+        if has_t:
+            if x0 is None:
+                raise ValueError("The initial state is needed for derivatives with respect to the initial time")
+            x0 = [np.asarray(x0)[i] for i in range(n)]
+            env = {l.name: x0[i] for i, (l, _) in enumerate(self._orig)}
+            fvals = [evaluate(r, env, pars, t0) for _, r in self._orig]
         for idx in range(n, len(self._sys)):
             mi = self._mindex[idx]
             i, m = mi[0], mi[1:]
@@ -181,6 +195,11 @@ class var_ode_sys:
                 va = self._vargs[a]
                 if va.is_var() and va.name == self._orig[i][0].name:
                     v = 1.0
+                elif va.is_time():
+                    v = -fvals[i]
+            elif has_t and any(m[j] for j, va in enumerate(self._vargs) if va.is_time()):
+                raise NotImplementedError("Initial values of higher-order derivatives with respect to the initial "
+                                          "time are not supported")
             vals.append(v)
         return vals
 
