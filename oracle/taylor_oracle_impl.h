@@ -284,7 +284,12 @@ static void FN(step)(const FN(sys) * s, FN(work) * wk, REAL rhofac, int high_acc
     const uint32_t n = s->n_state, P = wk->P, L = wk->L;
     REAL t0[64];
     for (uint32_t l = 0; l < L; ++l) t0[l] = t[l].hi;
+#ifdef ORA_GENERATED_COEFFS
+    /* Compiled-port build (cport.py): the system's own generated recurrences. */
+    ORA_GENERATED_COEFFS(wk->tape, wk->pars, x, t0, wk->L);
+#else
     FN(taylor_coeffs)(s, wk, x, t0);
+#endif
     for (uint32_t l = 0; l < L; ++l) {
         REAL mx = 0, mo = 0, mom1 = 0;
         for (uint32_t i = 0; i < n; ++i) {
