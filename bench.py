@@ -102,7 +102,7 @@ def main():
     ap.add_argument("--duration", type=float, default=3.0, help="serving: seconds of arrivals per rate")
     ap.add_argument("--n", type=int, default=None, help="instances (default: the config's)")
     ap.add_argument("--dt", type=float, default=25.0, help="nbody6: time units per bench step")
-    ap.add_argument("--code-form", default=None, choices=[None, "compact", "unrolled"])
+    ap.add_argument("--code-form", default=None, choices=[None, "compact", "unrolled", "group"])
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
@@ -142,7 +142,7 @@ def main():
     sys_ = wl.sys.sys if hasattr(wl.sys, "n_orig_sv") else wl.sys
     dec = hy.taylor_decompose(sys_)
     order = hy.order_from_tol(wl.tol)
-    compact = {"compact": True, "unrolled": False, None: None}[args.code_form]
+    compact = args.code_form
     t_c = time.perf_counter()
     system = get_system(dec, order, compact=compact)
     t_compile = time.perf_counter() - t_c
@@ -260,7 +260,7 @@ def main():
         out["plain_gpu"] = {"value": plain_value, "unit": "instance-steps/s", "launches": nl,
                             "persistent_over_plain": value / plain_value}
     if not args.no_cpu:
-        ns = args.cpu_sample or (8192 if args.config == "nbody6" else 2048)
+        ns = args.cpu_sample or (32768 if args.config == "nbody6" else 2048)
         if args.config in ("nbody6", "j2var"):
             v, secs = _cpu_baseline(dec, order, wl.state, wl.pars, 0.0, args.dt, ns, args.cpu_threads)
             sample = (f"{ns} instances of the same ensemble, t in [0, {args.dt}], {args.cpu_threads} threads, "

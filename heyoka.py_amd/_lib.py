@@ -23,6 +23,7 @@ MODE_PERSISTENT, MODE_PLAIN, MODE_GRAPH = 0, 1, 2
 FLAG_HIGH_ACCURACY, FLAG_COMPILE_ONLY, FLAG_FORCE_COMPACT, FLAG_FORCE_UNROLLED, FLAG_SOURCE_ONLY = 1, 2, 4, 8, 16
 FLAG_FORCE_GROUP = 32
 FORMS = {0: "unrolled", 1: "compact", 2: "group"}
+FLAG_GRID = 64
 
 
 class prop_res(ct.Structure):
@@ -77,6 +78,7 @@ def lib():
             "hyx_step": (ct.c_int, [vp, P(d), ct.c_int, P(i64), P(d)]),
             "hyx_read_tc": (ct.c_int, [vp, P(d)]),
             "hyx_propagate_until": (ct.c_int, [vp, P(d), P(d), P(d), u64, ct.c_int, P(prop_res)]),
+            "hyx_propagate_grid": (ct.c_int, [vp, P(d), u64, P(d), u64, ct.c_int, P(prop_res), P(d)]),
             "hyx_batch_last_timing": (ct.c_int, [vp, P(d), P(u64)]),
             "hyx_batch_set_persistent_config": (ct.c_int, [vp, u32, u32]),
             "hyx_batch_set_queue": (ct.c_int, [vp, P(u32)]),
@@ -106,6 +108,7 @@ EXPORTED = (
     "hyx_sys_source",
     "hyx_sys_kernel_info", "hyx_sys_destroy", "hyx_batch_create", "hyx_batch_destroy", "hyx_batch_size",
     "hyx_batch_write", "hyx_batch_read", "hyx_batch_device_ptr", "hyx_step", "hyx_read_tc", "hyx_propagate_until",
+    "hyx_propagate_grid",
     "hyx_batch_last_timing", "hyx_batch_set_persistent_config", "hyx_batch_set_queue", "hyx_batch_set_iter_budget",
     "hyx_batch_sync", "hyx_slots_reset", "hyx_slots_write", "hyx_slots_read", "hyx_slots_arm", "hyx_slots_launch",
     "hyx_slots_wait", "hyx_slots_results",
@@ -141,10 +144,11 @@ def _flags(high_accuracy=False, compact=None):
     return f
 
 
-def precompile(dec, order: int, high_accuracy: bool = False, arch: str = "gfx950", compact=None):
-    """Compile the system's kernels into the code-object cache (no GPU needed)."""
+def precompile(dec, order: int, high_accuracy: bool = False, arch: str = "gfx950", compact=None, grid=False):
+    """Compile the system's kernels (grid=True: its propagate_grid kernels) into the code-object
+    cache (no GPU needed)."""
     words, consts = oplist_args(dec)
-    flags = _flags(high_accuracy, compact)
+    flags = _flags(high_accuracy, compact) | (FLAG_GRID if grid else 0)
     check(lib().hyx_sys_precompile(dptr(words, ct.c_uint32), len(words), dptr(consts), len(dec.consts), order, flags,
                                    arch.encode()))
 
@@ -241,6 +245,20 @@ class Batch:
         arr = np.frombuffer(res, dtype=np.dtype([("outcome", "<i8"), ("min_h", "<f8"), ("max_h", "<f8"),
                                                  ("nsteps", "<u8")]), count=self.N)
         return arr.copy()
+
+    def propagate_grid(self, grid, max_delta_t=None, max_steps=0, mode=MODE_PERSISTENT):
+        """grid: [n_pts, N] times. Returns (prop_res records, out [n_pts, n, N])."""
+        g = np.ascontiguousarray(grid, dtype=np.float64)
+        if g.ndim != 2 or g.shape[1] != self.N:
+            raise ValueError(f"the grid must have shape (n_pts, {self.N})")
+        md = None if max_delta_t is None else np.ascontiguousarray(max_delta_t, dtype=np.float64)
+        res = (prop_res * self.N)()
+        out = np.empty((g.shape[0], self.sys.n, self.N), dtype=np.float64)
+        check(self._L.hyx_propagate_grid(self.h, dptr(g), g.shape[0], dptr(md), int(max_steps), int(mode), res,
+                                         dptr(out)))
+        arr = np.frombuffer(res, dtype=np.dtype([("outcome", "<i8"), ("min_h", "<f8"), ("max_h", "<f8"),
+                                                 ("nsteps", "<u8")]), count=self.N)
+        return arr.copy(), out
 
     def last_timing(self):
         ms, n = ct.c_double(), ct.c_uint64()

@@ -54,6 +54,10 @@ struct hyx_kargs {
     unsigned *unfinished;   // persistent form: instances suspended by the budget
     unsigned refill_min;    // persistent form: refill when this many lanes idle
     unsigned pad_;
+    const double *grid;     // propagate_grid: [npts][ld] output times per instance
+    double *gout;           // propagate_grid: [npts][n][ld] dense output at the grid times
+    unsigned *gcur;         // propagate_grid: [N] next grid point of each instance
+    hyx_u64 npts;           // propagate_grid: grid points (0 outside propagate_grid)
 };
 
 // ---- double-length time -----------------------------------------------------------
@@ -125,6 +129,26 @@ extern "C" __global__ void hyx_fill_u32_k(unsigned *p, unsigned v, hyx_u64 n)
 {
     const hyx_u64 i = (hyx_u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// Grid-stride fill of n 64-bit words (the NaN prefill of propagate_grid's output).
+extern "C" __global__ void hyx_fill_u64_k(hyx_u64 *p, hyx_u64 v, hyx_u64 n)
+{
+    const hyx_u64 stride = (hyx_u64)gridDim.x * blockDim.x;
+    for (hyx_u64 i = (hyx_u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// Dense-output time of grid point g for a step of size h that ended at
+// (t_hi, t_lo): the offset from the start of the step, as heyoka's
+// update_d_output forms it ((g - t) + h; heyoka/expose_batch_integrators.cpp:505-541,
+// doc/notebooks/Batch mode overview.ipynb:702).
+// *inside is false when g lies beyond the end of the step in the direction of
+// integration.
+__device__ __forceinline__ double hyx_grid_offset(double g, double t_hi, double t_lo, double h, int tdir, bool &inside)
+{
+    const double d = (g - t_hi) - t_lo;
+    inside = tdir ? (d <= 0.0) : (d >= 0.0);
+    return d + h;
 }
 
 // Arm slots for a propagation: final time, max_delta_t, direction and zeroed

@@ -18,6 +18,11 @@
 // Every lane of the wave executes every step, also lanes whose group has no
 // instance (their results are not stored): hyx_gstep contains workgroup
 // barriers, and a block is exactly one wavefront.
+//
+// Built with HYX_GRID defined (the propagate_grid code object), the propagate
+// kernels also evaluate the step's Taylor polynomials at the grid times inside
+// the step; each lane evaluates its owned state variables from the
+// coefficient rows in LDS, which stay intact until the next hyx_gstep.
 
 struct hyx_ginst {
     double xo[HYX_OWN];
@@ -25,6 +30,7 @@ struct hyx_ginst {
     double thi, tlo, tfhi, tflo, mdt, minh, maxh;
     hyx_u64 iter, nst;
     int tdir;
+    unsigned gc; // propagate_grid: next grid point
 };
 
 __device__ __forceinline__ void hyx_gload(hyx_ginst &s, const hyx_kargs &a, hyx_u64 i, int g, const hyx_gtab &t)
@@ -62,6 +68,9 @@ __device__ __forceinline__ void hyx_gresume(hyx_ginst &s, const hyx_kargs &a, hy
     s.maxh = a.max_h[i];
     s.iter = a.iters[i];
     s.nst = a.nsteps[i];
+#ifdef HYX_GRID
+    s.gc = a.gcur[i];
+#endif
 }
 
 __device__ __forceinline__ void hyx_gsuspend(const hyx_ginst &s, const hyx_kargs &a, hyx_u64 i, int g,
@@ -73,6 +82,9 @@ __device__ __forceinline__ void hyx_gsuspend(const hyx_ginst &s, const hyx_kargs
         a.max_h[i] = s.maxh;
         a.iters[i] = s.iter;
         a.nsteps[i] = s.nst;
+#ifdef HYX_GRID
+        a.gcur[i] = s.gc;
+#endif
         if (done) {
             a.status[i] = 1u;
             a.outcome[i] = res;
@@ -80,12 +92,42 @@ __device__ __forceinline__ void hyx_gsuspend(const hyx_ginst &s, const hyx_kargs
     }
 }
 
-// One propagate_until iteration of the group's instance; every lane of the
-// group computes the same bookkeeping (h and the non-finite flag are reduced
-// across the group inside hyx_gstep).
-__device__ __forceinline__ bool hyx_gprop_iter(hyx_ginst &s, const hyx_kargs &a, int g, double *fl,
-                                               const hyx_gtab &t, hyx_i64 *res)
+#ifdef HYX_GRID
+// Dense output of the group's instance i at the grid points inside the step of
+// size h just taken (time already advanced): every lane of the group walks the
+// same points and writes its owned variables. fin: the step reached the final
+// time, so every remaining point is evaluated.
+__device__ __forceinline__ void hyx_ggrid(hyx_ginst &s, const hyx_kargs &a, hyx_u64 i, const double *fl,
+                                          const hyx_gtab &t, double h, bool fin)
 {
+    unsigned c = s.gc;
+    while (c < a.npts) {
+        bool inside;
+        const double dt = hyx_grid_offset(a.grid[(hyx_u64)c * a.ld + i], s.thi, s.tlo, h, s.tdir, inside);
+        if (!inside && !fin) break;
+#pragma unroll
+        for (int o = 0; o < HYX_OWN; ++o) {
+            if (t.vo[o] < 0) continue;
+            const double *row = fl + HYX_XB + t.vo[o];
+            double r = row[HYX_P * HYX_NX];
+            for (int k = HYX_P - 1; k >= 0; --k) r = __builtin_fma(r, dt, row[k * HYX_NX]);
+            a.gout[((hyx_u64)c * HYX_N + t.vo[o]) * a.ld + i] = r;
+        }
+        ++c;
+    }
+    s.gc = c;
+}
+#endif
+
+// One propagate_until iteration of the group's instance i; every lane of the
+// group computes the same bookkeeping (h and the non-finite flag are reduced
+// across the group inside hyx_gstep). wr: the group holds a live instance
+// (grid output is written only then).
+__device__ __forceinline__ bool hyx_gprop_iter(hyx_ginst &s, const hyx_kargs &a, int g, double *fl,
+                                               const hyx_gtab &t, hyx_u64 i, bool wr, hyx_i64 *res)
+{
+    (void)i;
+    (void)wr;
     const double rem = hyx_dfl_sub_hi(s.tfhi, s.tflo, s.thi, s.tlo);
     const double lim = s.tdir ? ((s.mdt < rem) ? s.mdt : rem) : ((-s.mdt > rem) ? -s.mdt : rem);
     double h;
@@ -105,7 +147,11 @@ __device__ __forceinline__ bool hyx_gprop_iter(hyx_ginst &s, const hyx_kargs &a,
         *res = HYX_ERR_NF_STATE;
         return true;
     }
-    if (oc == HYX_TIME_LIMIT && h == rem) {
+    const bool fin = oc == HYX_TIME_LIMIT && h == rem;
+#ifdef HYX_GRID
+    if (wr) hyx_ggrid(s, a, i, fl, t, h, fin);
+#endif
+    if (fin) {
         *res = HYX_TIME_LIMIT;
         return true;
     }
@@ -180,7 +226,7 @@ extern "C" __global__ void __launch_bounds__(64) hyx_iter_k(hyx_kargs a)
     hyx_ginst s;
     hyx_gresume(s, a, ii, st, g, tab);
     hyx_i64 res = HYX_TIME_LIMIT;
-    const bool done = hyx_gprop_iter(s, a, g, fl, tab, &res);
+    const bool done = hyx_gprop_iter(s, a, g, fl, tab, ii, valid, &res);
     if (valid) hyx_gsuspend(s, a, i, g, tab, done, res);
     const hyx_u64 m = __ballot(valid && !done && g == 0);
     if (lane == 0 && m != 0) atomicAdd(a.active_count, (unsigned)__popcll(m));
@@ -218,6 +264,7 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
     s.maxh = 0.0;
     s.iter = 0;
     s.nst = 0;
+    s.gc = 0;
     hyx_u64 inst = 0, budget = 0;
     bool active = false;
     bool exhausted = false;
@@ -248,7 +295,7 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
         }
         if (!__any(active)) break;
         hyx_i64 res = HYX_TIME_LIMIT;
-        const bool done = hyx_gprop_iter(s, a, g, fl, tab, &res);
+        const bool done = hyx_gprop_iter(s, a, g, fl, tab, inst, active, &res);
         if (active) {
             budget += 1;
             if (done || budget >= a.iter_budget) {

@@ -5,6 +5,17 @@
 //       one adaptive Taylor step of the instance held by the calling lane:
 //       coefficient recurrences, step size from the last two orders, Horner
 //       update of x in place; WTC writes the state's Taylor coefficients.
+//
+// Built with HYX_GRID defined (a second code object, compiled on the first
+// propagate_grid of a system), the propagate kernels also write each step's
+// Taylor coefficients to a.tc and evaluate them at the grid times that fall
+// inside the step (dense output, heyoka's propagate_grid:
+// heyoka/expose_batch_integrators.cpp:315-392).
+#ifdef HYX_GRID
+#define HYX_PROP_WTC true
+#else
+#define HYX_PROP_WTC false
+#endif
 
 // ---- one instance's state held in registers ------------------------------------------
 struct hyx_inst {
@@ -13,6 +24,7 @@ struct hyx_inst {
     double thi, tlo, tfhi, tflo, mdt, minh, maxh;
     hyx_u64 iter, nst;
     int tdir;
+    unsigned gc; // propagate_grid: next grid point
 };
 
 __device__ __forceinline__ void hyx_load(hyx_inst &s, const hyx_kargs &a, hyx_u64 i)
@@ -33,17 +45,45 @@ __device__ __forceinline__ void hyx_store_state(const hyx_inst &s, const hyx_kar
     a.t_lo[i] = s.tlo;
 }
 
-// One iteration of propagate_until for one instance. Returns true when the
-// instance is done; *res then holds its outcome.
-__device__ __forceinline__ bool hyx_prop_iter(hyx_inst &s, const hyx_kargs &a, hyx_u64 lane_tape, hyx_i64 *res)
+#ifdef HYX_GRID
+// Dense output of instance i at the grid points inside the step of size h just
+// taken (the time is already advanced), from the coefficients the step wrote
+// to a.tc. fin: the step reached the final time, so every remaining point
+// (the last grid time) is evaluated.
+__device__ __forceinline__ void hyx_grid_lane(hyx_inst &s, const hyx_kargs &a, hyx_u64 i, double h, bool fin)
 {
+    unsigned c = s.gc;
+    while (c < a.npts) {
+        bool inside;
+        const double dt = hyx_grid_offset(a.grid[(hyx_u64)c * a.ld + i], s.thi, s.tlo, h, s.tdir, inside);
+        if (!inside && !fin) break;
+#pragma unroll 1
+        for (int q = 0; q < HYX_N; ++q) {
+            const double *cq = a.tc + (hyx_u64)q * (HYX_P + 1) * a.ld + i;
+            double r = cq[(hyx_u64)HYX_P * a.ld];
+            for (int k = HYX_P - 1; k >= 0; --k) r = __builtin_fma(r, dt, cq[(hyx_u64)k * a.ld]);
+            a.gout[((hyx_u64)c * HYX_N + q) * a.ld + i] = r;
+        }
+        ++c;
+    }
+    s.gc = c;
+}
+#endif
+
+// One iteration of propagate_until for one instance i (lane_tape: its column
+// of the compact-form tape). Returns true when the instance is done; *res then
+// holds its outcome.
+__device__ __forceinline__ bool hyx_prop_iter(hyx_inst &s, const hyx_kargs &a, hyx_u64 lane_tape, hyx_u64 i,
+                                              hyx_i64 *res)
+{
+    (void)i;
     // Clamp: the remaining time, or max_delta_t if smaller.
     const double rem = hyx_dfl_sub_hi(s.tfhi, s.tflo, s.thi, s.tlo);
     const double lim = s.tdir ? ((s.mdt < rem) ? s.mdt : rem) : ((-s.mdt > rem) ? -s.mdt : rem);
     double h;
     bool nf;
-    hyx_taylor_step<false>(s.x, s.pv, s.thi, lim, h, nf, nullptr, 0, a.tape ? a.tape + lane_tape : nullptr,
-                           a.tape_stride);
+    hyx_taylor_step<HYX_PROP_WTC>(s.x, s.pv, s.thi, lim, h, nf, HYX_PROP_WTC ? a.tc + i : nullptr, a.ld,
+                                  a.tape ? a.tape + lane_tape : nullptr, a.tape_stride);
     hyx_dfl_add(s.thi, s.tlo, h, 0.0);
     nf = nf || !isfinite(s.thi);
     s.iter += 1;
@@ -58,7 +98,11 @@ __device__ __forceinline__ bool hyx_prop_iter(hyx_inst &s, const hyx_kargs &a, h
         *res = HYX_ERR_NF_STATE;
         return true;
     }
-    if (oc == HYX_TIME_LIMIT && h == rem) {
+    const bool fin = oc == HYX_TIME_LIMIT && h == rem;
+#ifdef HYX_GRID
+    hyx_grid_lane(s, a, i, h, fin);
+#endif
+    if (fin) {
         *res = HYX_TIME_LIMIT;
         return true;
     }
@@ -81,6 +125,9 @@ __device__ __forceinline__ void hyx_resume(hyx_inst &s, const hyx_kargs &a, hyx_
     s.maxh = a.max_h[i];
     s.iter = a.iters[i];
     s.nst = a.nsteps[i];
+#ifdef HYX_GRID
+    s.gc = a.gcur[i];
+#endif
 }
 
 // Write an instance back; done instances also record their outcome.
@@ -91,6 +138,9 @@ __device__ __forceinline__ void hyx_suspend(const hyx_inst &s, const hyx_kargs &
     a.max_h[i] = s.maxh;
     a.iters[i] = s.iter;
     a.nsteps[i] = s.nst;
+#ifdef HYX_GRID
+    a.gcur[i] = s.gc;
+#endif
     if (done) {
         a.status[i] = 1u;
         a.outcome[i] = res;
@@ -145,7 +195,7 @@ extern "C" __global__ void __launch_bounds__(64) hyx_iter_k(hyx_kargs a)
             hyx_inst s;
             hyx_resume(s, a, i, st);
             hyx_i64 res = HYX_TIME_LIMIT;
-            const bool done = hyx_prop_iter(s, a, i, &res);
+            const bool done = hyx_prop_iter(s, a, i, i, &res);
             hyx_suspend(s, a, i, done, res);
             still = !done;
         }
@@ -208,7 +258,7 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
         if (!__any(active)) break;
         if (active) {
             hyx_i64 res = HYX_TIME_LIMIT;
-            const bool done = hyx_prop_iter(s, a, gl, &res);
+            const bool done = hyx_prop_iter(s, a, gl, inst, &res);
             budget += 1;
             if (done || budget >= a.iter_budget) {
                 hyx_suspend(s, a, inst, done, res);

@@ -79,6 +79,10 @@ struct KArgs {
     unsigned *unfinished;
     unsigned refill_min;
     unsigned pad_;
+    const double *grid;
+    double *gout;
+    unsigned *gcur;
+    uint64_t npts;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -208,8 +212,12 @@ struct hyx_sys {
     hipModule_t mod = nullptr;
     hipFunction_t k_step = nullptr, k_step_tc = nullptr, k_init = nullptr, k_iter = nullptr, k_persist = nullptr,
                   k_clr32 = nullptr, k_clr64 = nullptr, k_scatter = nullptr, k_gather = nullptr, k_gather32 = nullptr,
-                  k_fill32 = nullptr, k_arm = nullptr;
+                  k_fill32 = nullptr, k_fill64 = nullptr, k_arm = nullptr;
     int vgprs = -1, scratch = -1;
+    // propagate_grid: the same source built with HYX_GRID, loaded on first use.
+    std::mutex gmu;
+    hipModule_t gmod = nullptr;
+    hipFunction_t kg_init = nullptr, kg_iter = nullptr, kg_persist = nullptr;
 };
 
 struct hyx_batch {
@@ -237,6 +245,10 @@ struct hyx_batch {
     unsigned *stage_idx = nullptr;
     size_t stage_idx_n = 0;
     bool slots_running = false;
+    double *grid = nullptr, *gout = nullptr; // propagate_grid: [npts][N] times, [npts][n][N] output
+    unsigned *gcur = nullptr;                // propagate_grid: [N] next grid point
+    size_t grid_cap = 0, gout_cap = 0;       // allocated doubles
+    uint64_t npts = 0;
 };
 
 namespace
@@ -398,7 +410,7 @@ int hyx_sys_precompile(const uint32_t *ops, uint32_t n_words, const double *cons
     int rc = build_sys(ops, n_words, consts, n_consts, order, flags, s);
     if (rc) return rc;
     std::vector<char> code;
-    return get_code(s.src, arch ? arch : "gfx950", code);
+    return get_code((flags & HYX_FLAG_GRID) ? "#define HYX_GRID 1\n" + s.src : s.src, arch ? arch : "gfx950", code);
 }
 
 int hyx_sys_create(const uint32_t *ops, uint32_t n_words, const double *consts, uint32_t n_consts, uint32_t order,
@@ -452,7 +464,8 @@ int hyx_sys_create(const uint32_t *ops, uint32_t n_words, const double *consts, 
     } fns[] = {{&s->k_step, "hyx_step_k"},       {&s->k_step_tc, "hyx_step_tc_k"}, {&s->k_init, "hyx_iter_init_k"},
                {&s->k_iter, "hyx_iter_k"},       {&s->k_persist, "hyx_persist_k"}, {&s->k_clr32, "hyx_clear_u32_k"},
                {&s->k_clr64, "hyx_clear_u64_k"}, {&s->k_scatter, "hyx_scatter_k"}, {&s->k_gather, "hyx_gather_k"},
-               {&s->k_gather32, "hyx_gather32_k"}, {&s->k_fill32, "hyx_fill_u32_k"}, {&s->k_arm, "hyx_slots_arm_k"}};
+               {&s->k_gather32, "hyx_gather32_k"}, {&s->k_fill32, "hyx_fill_u32_k"}, {&s->k_fill64, "hyx_fill_u64_k"},
+               {&s->k_arm, "hyx_slots_arm_k"}};
     for (auto &f : fns) {
         e = hipModuleGetFunction(f.f, s->mod, f.name);
         if (e != hipSuccess) {
@@ -496,6 +509,7 @@ int hyx_sys_kernel_info(const hyx_sys *s, int *vgprs, int *scratch_bytes)
 void hyx_sys_destroy(hyx_sys *s)
 {
     if (!s) return;
+    if (s->gmod) hipModuleUnload(s->gmod);
     if (s->mod) hipModuleUnload(s->mod);
     delete s;
 }
@@ -549,7 +563,8 @@ void hyx_batch_destroy(hyx_batch *b)
     if (b->graph) hipGraphDestroy(b->graph);
     void *ptrs[] = {b->state, b->pars, b->thi,    b->tlo,   b->tfhi,   b->tflo,  b->mdt,
                     b->hout,  b->minh, b->maxh,   b->tc,    b->oc,     b->nsteps, b->iters,
-                    b->work,  b->status, b->active, b->unfinished, b->queue, b->tape, b->stage, b->stage_idx};
+                    b->work,  b->status, b->active, b->unfinished, b->queue, b->tape, b->stage, b->stage_idx,
+                    b->grid,  b->gout, b->gcur};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (b->ev0) hipEventDestroy(b->ev0);
@@ -640,27 +655,63 @@ int hyx_read_tc(hyx_batch *b, double *dst)
     return 0;
 }
 
-int hyx_propagate_until(hyx_batch *b, const double *tf_hi, const double *tf_lo, const double *max_delta_t,
-                        uint64_t max_steps, int mode, hyx_prop_res *res)
+} // extern "C"
+
+namespace
 {
-    if (!b || !tf_hi) return fail(HYX_EINVAL, "null argument");
-    if (mode != HYX_MODE_PERSISTENT && mode != HYX_MODE_PLAIN && mode != HYX_MODE_GRAPH)
-        return fail(HYX_EINVAL, "unknown propagation mode");
+
+// Loads the propagate_grid code object of a system (the same source with
+// HYX_GRID defined) on first use.
+int ensure_grid_module(hyx_sys *s)
+{
+    std::lock_guard<std::mutex> lk(s->gmu);
+    if (s->gmod) return 0;
+    std::vector<char> code;
+    if (int rc = get_code("#define HYX_GRID 1\n" + s->src, device_arch(s->device), code)) return rc;
+    hipModule_t m = nullptr;
+    HIPCHK(hipModuleLoadData(&m, code.data()));
+    struct {
+        hipFunction_t *f;
+        const char *name;
+    } fns[] = {{&s->kg_init, "hyx_iter_init_k"}, {&s->kg_iter, "hyx_iter_k"}, {&s->kg_persist, "hyx_persist_k"}};
+    for (auto &f : fns) {
+        hipError_t e = hipModuleGetFunction(f.f, m, f.name);
+        if (e != hipSuccess) {
+            hipModuleUnload(m);
+            return fail(HYX_EHIP, std::string("hipModuleGetFunction(") + f.name + "): " + hipGetErrorString(e));
+        }
+    }
+    s->gmod = m;
+    return 0;
+}
+
+// The propagate loop shared by propagate_until and propagate_grid: final
+// times, max_delta_t (and the grid buffers) are already on the device.
+int run_propagation(hyx_batch *b, uint64_t max_steps, int mode, hyx_prop_res *res, bool grid_run)
+{
     const uint64_t N = b->N;
-    if (int rc = upload(b, b->tfhi, tf_hi, N, 0.0)) return rc;
-    if (int rc = upload(b, b->tflo, tf_lo, N, 0.0)) return rc;
-    if (int rc = upload(b, b->mdt, max_delta_t, N, std::numeric_limits<double>::infinity())) return rc;
+    hyx_sys *s = b->sys;
+    hipFunction_t k_init = grid_run ? s->kg_init : s->k_init;
+    hipFunction_t k_iter = grid_run ? s->kg_iter : s->k_iter;
+    hipFunction_t k_persist = grid_run ? s->kg_persist : s->k_persist;
+    if (grid_run && mode == HYX_MODE_GRAPH) mode = HYX_MODE_PLAIN; // no captured graph for the grid code object
     uint64_t pgrid = 0;
     if (mode == HYX_MODE_PERSISTENT) {
         int occ = 0;
-        HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, b->sys->k_persist, 64, 0));
+        HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_persist, 64, 0));
         const uint64_t per_cu = b->waves_per_cu ? b->waves_per_cu : (uint64_t)std::max(occ, 1);
         pgrid = std::max<uint64_t>(1, std::min<uint64_t>(per_cu * (uint64_t)b->n_cu, blocks_of(b)));
     }
     if (int rc = ensure_tape(b, pgrid * 64)) return rc;
     KArgs a = make_args(b);
     a.max_steps = max_steps;
-    HIPCHK(launch(b->sys->k_init, blocks_for(N), b->stream, &a));
+    if (grid_run) {
+        a.grid = b->grid;
+        a.gout = b->gout;
+        a.gcur = b->gcur;
+        a.npts = b->npts;
+    }
+    HIPCHK(launch(k_init, blocks_for(N), b->stream, &a));
     uint64_t launches = 0;
     HIPCHK(hipEventRecord(b->ev0, b->stream));
     if (mode == HYX_MODE_PERSISTENT) {
@@ -668,7 +719,7 @@ int hyx_propagate_until(hyx_batch *b, const double *tf_hi, const double *tf_lo, 
         for (;;) {
             HIPCHK(clear64(b, b->work, 1));
             HIPCHK(clear32(b, b->unfinished, 1));
-            HIPCHK(launch(b->sys->k_persist, grid, b->stream, &a));
+            HIPCHK(launch(k_persist, grid, b->stream, &a));
             ++launches;
             unsigned unf = 0;
             HIPCHK(hipMemcpyAsync(&unf, b->unfinished, 4, hipMemcpyDeviceToHost, b->stream));
@@ -684,7 +735,7 @@ int hyx_propagate_until(hyx_batch *b, const double *tf_hi, const double *tf_lo, 
                 KArgs ai = a;
                 ai.active_count = b->active + i;
                 ai.max_steps = 0; // the graph is replayed only when max_steps == 0
-                e = launch(b->sys->k_iter, grid, b->stream, &ai);
+                e = launch(k_iter, grid, b->stream, &ai);
             }
             hipGraph_t g = nullptr;
             hipError_t e2 = hipStreamEndCapture(b->stream, &g);
@@ -701,7 +752,7 @@ int hyx_propagate_until(hyx_batch *b, const double *tf_hi, const double *tf_lo, 
                 for (int i = 0; i < kPlainGroup; ++i) {
                     KArgs ai = a;
                     ai.active_count = b->active + i;
-                    HIPCHK(launch(b->sys->k_iter, grid, b->stream, &ai));
+                    HIPCHK(launch(k_iter, grid, b->stream, &ai));
                 }
             }
             launches += kPlainGroup;
@@ -728,6 +779,95 @@ int hyx_propagate_until(hyx_batch *b, const double *tf_hi, const double *tf_lo, 
         HIPCHK(hipStreamSynchronize(b->stream));
         for (uint64_t i = 0; i < N; ++i) res[i] = hyx_prop_res{oc[i], mn[i], mx[i], ns[i]};
     }
+    return 0;
+}
+
+// Grows a device buffer to hold at least `count` elements of `elem` bytes.
+template <class T>
+int reserve(T **p, size_t *cap, size_t count)
+{
+    if (count <= *cap && *p) return 0;
+    if (*p) HIPCHK(hipFree(*p));
+    *p = nullptr;
+    *cap = 0;
+    HIPCHK(hipMalloc((void **)p, count * sizeof(T)));
+    *cap = count;
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int hyx_propagate_until(hyx_batch *b, const double *tf_hi, const double *tf_lo, const double *max_delta_t,
+                        uint64_t max_steps, int mode, hyx_prop_res *res)
+{
+    if (!b || !tf_hi) return fail(HYX_EINVAL, "null argument");
+    if (mode != HYX_MODE_PERSISTENT && mode != HYX_MODE_PLAIN && mode != HYX_MODE_GRAPH)
+        return fail(HYX_EINVAL, "unknown propagation mode");
+    const uint64_t N = b->N;
+    if (int rc = upload(b, b->tfhi, tf_hi, N, 0.0)) return rc;
+    if (int rc = upload(b, b->tflo, tf_lo, N, 0.0)) return rc;
+    if (int rc = upload(b, b->mdt, max_delta_t, N, std::numeric_limits<double>::infinity())) return rc;
+    return run_propagation(b, max_steps, mode, res, false);
+}
+
+int hyx_propagate_grid(hyx_batch *b, const double *grid, uint64_t n_pts, const double *max_delta_t,
+                       uint64_t max_steps, int mode, hyx_prop_res *res, double *out)
+{
+    if (!b || !grid || !out) return fail(HYX_EINVAL, "null argument");
+    if (n_pts == 0) return fail(HYX_EINVAL, "the time grid of propagate_grid() must not be empty");
+    if (mode != HYX_MODE_PERSISTENT && mode != HYX_MODE_PLAIN && mode != HYX_MODE_GRAPH)
+        return fail(HYX_EINVAL, "unknown propagation mode");
+    const uint64_t N = b->N, n = b->sys->S.n, P = b->sys->order;
+    // Grid checks, per instance: finite, starting at the current time, strictly monotonic.
+    std::vector<double> thi(N);
+    HIPCHK(hipMemcpyAsync(thi.data(), b->thi, N * 8, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (uint64_t i = 0; i < n_pts * N; ++i)
+        if (!std::isfinite(grid[i])) return fail(HYX_EINVAL, "a non-finite time was passed to propagate_grid()");
+    for (uint64_t i = 0; i < N; ++i) {
+        if (grid[i] != thi[i])
+            return fail(HYX_EINVAL, "the first point of the time grid of instance " + std::to_string(i) +
+                                        " does not match its current time in propagate_grid()");
+        if (n_pts < 2) continue;
+        const bool fwd = grid[(n_pts - 1) * N + i] > grid[i];
+        for (uint64_t j = 1; j < n_pts; ++j) {
+            const double a = grid[(j - 1) * N + i], c = grid[j * N + i];
+            if (fwd ? !(c > a) : !(c < a))
+                return fail(HYX_EINVAL, "a non-monotonic time grid was passed to propagate_grid() (instance " +
+                                            std::to_string(i) + ")");
+        }
+    }
+    if (int rc = ensure_grid_module(b->sys)) return rc;
+    if (b->sys->form != HYX_FORM_GROUP && !b->tc) HIPCHK(hipMalloc((void **)&b->tc, (size_t)n * (P + 1) * N * 8));
+    if (int rc = reserve(&b->grid, &b->grid_cap, n_pts * N)) return rc;
+    if (int rc = reserve(&b->gout, &b->gout_cap, n_pts * n * N)) return rc;
+    if (!b->gcur) HIPCHK(hipMalloc((void **)&b->gcur, N * 4));
+    HIPCHK(hipMemcpyAsync(b->grid, grid, n_pts * N * 8, hipMemcpyHostToDevice, b->stream));
+    {
+        // Unreached points stay NaN (a propagation that ends early, as in heyoka's batch propagate_grid).
+        uint64_t *p = (uint64_t *)b->gout;
+        uint64_t v = 0x7ff8000000000000ull, cnt = n_pts * n * N;
+        void *params[] = {&p, &v, &cnt};
+        const unsigned blocks = (unsigned)std::min<uint64_t>((cnt + 255) / 256, 8192);
+        HIPCHK(hipModuleLaunchKernel(b->sys->k_fill64, blocks, 1, 1, 256, 1, 1, 0, b->stream, params, nullptr));
+        unsigned *q = b->gcur;
+        unsigned one = 1;
+        uint64_t nn = N;
+        void *params2[] = {&q, &one, &nn};
+        HIPCHK(hipModuleLaunchKernel(b->sys->k_fill32, (unsigned)((N + 63) / 64), 1, 1, 64, 1, 1, 0, b->stream, params2,
+                                     nullptr));
+    }
+    // The first grid point is the current time: its output is the current state.
+    HIPCHK(hipMemcpyAsync(b->gout, b->state, n * N * 8, hipMemcpyDeviceToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->tfhi, b->grid + (n_pts - 1) * N, N * 8, hipMemcpyDeviceToDevice, b->stream));
+    if (int rc = upload(b, b->tflo, nullptr, N, 0.0)) return rc;
+    if (int rc = upload(b, b->mdt, max_delta_t, N, std::numeric_limits<double>::infinity())) return rc;
+    b->npts = n_pts;
+    if (int rc = run_propagation(b, max_steps, mode, res, true)) return rc;
+    HIPCHK(hipMemcpyAsync(out, b->gout, n_pts * n * N * 8, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
     return 0;
 }
 

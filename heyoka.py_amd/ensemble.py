@@ -12,6 +12,10 @@ integrators. Instances are independent and their arithmetic does not depend on
 where they sit in the batch, so every member ends bit-for-bit where its own
 ``propagate_until`` would have taken it (tests/test_gpu_ensemble.py checks this).
 Members with a step callback are propagated one at a time instead.
+``ensemble_propagate_grid[_batch]`` splats the 1-D grid over every instance
+(``_ensemble_impl.py:13-19``) and returns ``(ta_i, None, out_i)`` with
+``out_i`` of shape (n_pts, n, batch_size), cut from one device-side
+propagate_grid over the packed batch.
 """
 
 from __future__ import annotations
@@ -52,13 +56,16 @@ def _ensemble_generic(tp, ta, arg, n_iter, gen, **kwargs):
         kwargs.pop("chunksize", None)
     if "chunksize" in kwargs:
         raise TypeError("propagate_until() got an unexpected keyword argument 'chunksize'")
-    if tp == "grid":
-        raise NotImplementedError("Ensemble propagate_grid() is not supported yet by the GPU batch integrator")
     if kwargs.get("c_output"):
         raise NotImplementedError("Continuous output is not supported by the GPU batch integrator")
     if "callback" in kwargs and kwargs["callback"] is not None:
         return _ensemble_sequential(tp, ta, arg, n_iter, gen, kwargs)
     return _ensemble_packed(tp, ta, arg, n_iter, gen, kwargs)
+
+
+def _splat_grid(grid, ta):
+    """The ensemble's 1-D grid as the (n_pts, batch_size) grid of one member (heyoka/_ensemble_impl.py:13-19)."""
+    return np.repeat(np.asarray(grid, dtype=np.float64), ta.batch_size).reshape((-1, ta.batch_size))
 
 
 def _ensemble_sequential(tp, ta, arg, n_iter, gen, kwargs):
@@ -67,7 +74,12 @@ def _ensemble_sequential(tp, ta, arg, n_iter, gen, kwargs):
         kw = copy(kwargs)
         kw["callback"] = deepcopy(kwargs["callback"])
         local = gen(deepcopy(ta), i)
-        ret = local.propagate_until(arg, **kw) if tp == "until" else local.propagate_for(arg, **kw)
+        if tp == "until":
+            ret = local.propagate_until(arg, **kw)
+        elif tp == "for":
+            ret = local.propagate_for(arg, **kw)
+        else:
+            ret = local.propagate_grid(_splat_grid(arg, local), **kw)
         out.append((local,) + tuple(ret))
     return out
 
@@ -101,15 +113,31 @@ def _ensemble_packed(tp, ta, arg, n_iter, gen, kwargs):
     tlo = np.concatenate([m._hyx_tlo for m in members])
     if tp == "until":
         tf_hi, tf_lo = np.full(N, float(arg)), np.zeros(N)
-    else:
+    elif tp == "for":
         tf_hi, tf_lo = dfl_add(thi, tlo, np.full(N, float(arg)), np.zeros(N))
+    else:
+        grid = np.repeat(np.asarray(arg, dtype=np.float64), N).reshape((-1, N))
+        if grid.shape[0] == 0:
+            raise ValueError("Cannot invoke propagate_grid() in an adaptive Taylor integrator in batch mode if the "
+                             "time grid is empty")
+        if not np.all(np.isfinite(grid)):
+            raise ValueError("A non-finite time value was passed to propagate_grid() in an adaptive Taylor "
+                             "integrator")
+        if np.any(grid[0] != thi):
+            raise ValueError("When invoking propagate_grid(), the first element of the time grid must match the "
+                             "current time coordinate of the integrator")
+        if grid.shape[0] > 1:
+            d = np.diff(grid[:, 0])
+            if not (np.all(d > 0) or np.all(d < 0)):
+                raise ValueError("A non-monotonic time grid was passed to propagate_grid()")
+        tf_hi = grid[-1].copy()
     if mdt is None:
         md = np.full(N, np.inf)
     else:
         md = np.full(N, float(mdt))
         if np.isnan(md).any() or (md <= 0).any():
             raise ValueError("A non-positive or nan max_delta_t was passed to an ensemble propagation")
-    if not np.all(np.isfinite(tf_hi)):
+    if tp != "grid" and not np.all(np.isfinite(tf_hi)):
         raise ValueError("A non-finite time was passed to an ensemble propagation")
     batch = _lib.Batch(ta._hyx_system, N)
     batch.write(_lib.FIELD_STATE, st)
@@ -117,7 +145,10 @@ def _ensemble_packed(tp, ta, arg, n_iter, gen, kwargs):
         batch.write(_lib.FIELD_PARS, pa)
     batch.write(_lib.FIELD_TIME_HI, thi)
     batch.write(_lib.FIELD_TIME_LO, tlo)
-    r = batch.propagate_until(tf_hi, tf_lo, md, max_steps, _MODES[ta._hyx_flags[3]])
+    if tp == "grid":
+        r, gout = batch.propagate_grid(grid, md, max_steps, _MODES[ta._hyx_flags[3]])
+    else:
+        r = batch.propagate_until(tf_hi, tf_lo, md, max_steps, _MODES[ta._hyx_flags[3]])
     st = batch.read(_lib.FIELD_STATE, (n, N))
     thi = batch.read(_lib.FIELD_TIME_HI, (N,))
     tlo = batch.read(_lib.FIELD_TIME_LO, (N,))
@@ -129,7 +160,7 @@ def _ensemble_packed(tp, ta, arg, n_iter, gen, kwargs):
         m._hyx_tlo[...] = tlo[sl]
         m._hyx_prop_res = [(_outcome(o), float(a), float(b), int(c))
                            for o, a, b, c in zip(r["outcome"][sl], r["min_h"][sl], r["max_h"][sl], r["nsteps"][sl])]
-        out.append((m, None, None))
+        out.append((m, None, gout[:, :, sl].copy()) if tp == "grid" else (m, None, None))
     return out
 
 

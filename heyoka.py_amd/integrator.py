@@ -254,9 +254,7 @@ class taylor_adaptive_batch:
                 f"The object of type '{type(callback)}' cannot be used as a step callback because it is not callable")
         return callback
 
-    def _prop_impl(self, tf_hi, tf_lo, max_steps, max_delta_t, callback, write_tc, c_output):
-        if c_output:
-            raise NotImplementedError("Continuous output is not supported by the GPU batch integrator")
+    def _prop_args(self, max_steps, max_delta_t):
         if not isinstance(max_steps, (int, np.integer)) or max_steps < 0:
             raise ValueError("max_steps must be a non-negative integer")
         B = self._hyx_B
@@ -270,6 +268,12 @@ class taylor_adaptive_batch:
         if (mdt <= 0).any():
             raise ValueError("A non-positive max_delta_t was passed to the propagate_until() function of an adaptive "
                              "Taylor integrator")
+        return mdt
+
+    def _prop_impl(self, tf_hi, tf_lo, max_steps, max_delta_t, callback, write_tc, c_output):
+        if c_output:
+            raise NotImplementedError("Continuous output is not supported by the GPU batch integrator")
+        mdt = self._prop_args(max_steps, max_delta_t)
         if not np.all(np.isfinite(tf_hi)) or not np.all(np.isfinite(self._hyx_thi)):
             raise ValueError("A non-finite time was passed to the propagate_until() function of an adaptive Taylor "
                              "integrator")
@@ -284,9 +288,10 @@ class taylor_adaptive_batch:
             return None, None
         return self._prop_stepwise(tf_hi, tf_lo, int(max_steps), mdt, cb, write_tc)
 
-    def _prop_stepwise(self, tf_hi, tf_lo, max_steps, mdt, cb, write_tc):
+    def _prop_stepwise(self, tf_hi, tf_lo, max_steps, mdt, cb, write_tc, after_step=None):
         # Host-driven loop, one device step per iteration, so that a step
         # callback can observe and modify the integrator between steps.
+        # after_step(h, live, fin, tdir) runs before the callback (dense output of propagate_grid).
         B = self._hyx_B
         rem0 = dfl_add(tf_hi, tf_lo, -self._hyx_thi, -self._hyx_tlo)[0]
         tdir = rem0 >= 0
@@ -314,6 +319,8 @@ class taylor_adaptive_batch:
             done |= nf
             fin = act & ~nf & (oc == int(taylor_outcome.time_limit)) & (h == rem)
             done |= fin
+            if after_step is not None:
+                after_step(h, act & ~nf, fin, tdir)
             if max_steps and it >= max_steps:
                 res_oc[act & ~done] = int(taylor_outcome.step_limit)
                 done[:] = True
@@ -345,6 +352,13 @@ class taylor_adaptive_batch:
         return self._prop_impl(hi, lo, max_steps, max_delta_t, callback, write_tc, c_output)
 
     def propagate_grid(self, grid, max_steps=0, max_delta_t=None, callback=None):
+        """Propagate to the last grid time of each instance, returning (callback | None, out[n_pts, n, B]).
+
+        heyoka/expose_batch_integrators.cpp:315-392. Per instance the grid must be finite, strictly
+        monotonic and start at the current time (doc/notebooks/The adaptive integrator.ipynb, cell 26).
+        The steps are those of propagate_until to the last grid time; the state at each grid time comes
+        from the Taylor polynomials of the step containing it (dense output). Points not reached
+        (step limit, non-finite state, a callback stop) are NaN (Batch mode overview.ipynb, cell 23)."""
         g = np.asarray(grid)
         if g.ndim != 2:
             raise ValueError(
@@ -354,7 +368,53 @@ class taylor_adaptive_batch:
             raise ValueError(
                 "Invalid grid passed to the propagate_grid() method of a batch integrator: the shape must be "
                 f"(n, {self._hyx_B}) but the number of columns is {g.shape[1]} instead")
-        raise NotImplementedError("propagate_grid() is not supported yet by the GPU batch integrator")
+        g = np.ascontiguousarray(g.astype(np.float64, casting="safe"))
+        npts = g.shape[0]
+        if npts == 0:
+            raise ValueError("Cannot invoke propagate_grid() in an adaptive Taylor integrator in batch mode if the "
+                             "time grid is empty")
+        if not np.all(np.isfinite(g)):
+            raise ValueError("A non-finite time value was passed to propagate_grid() in an adaptive Taylor integrator")
+        if np.any(g[0] != self._hyx_thi):
+            raise ValueError("When invoking propagate_grid(), the first element of the time grid must match the "
+                             "current time coordinate of the integrator")
+        if npts > 1:
+            d = np.diff(g, axis=0)
+            fwd = g[-1] > g[0]
+            if not np.all(np.where(fwd, d > 0, d < 0)):
+                raise ValueError("A non-monotonic time grid was passed to propagate_grid()")
+        mdt = self._prop_args(max_steps, max_delta_t)
+        cb = self._check_cb(callback)
+        if cb is None:
+            self._push()
+            r, out = self._hyx_batch.propagate_grid(g, mdt, int(max_steps), _MODES[self._hyx_flags[3]])
+            self._pull()
+            self._hyx_prop_res = [(_outcome(o), float(a), float(b), int(c))
+                                  for o, a, b, c in zip(r["outcome"], r["min_h"], r["max_h"], r["nsteps"])]
+            return None, out
+        # With a step callback: host-driven steps with write_tc, dense output on the host.
+        n, B = self._hyx_state.shape[0], self._hyx_B
+        out = np.full((npts, n, B), np.nan)
+        out[0] = self._hyx_state
+        cur = np.ones(B, dtype=np.int64)
+
+        def after_step(h, live, fin, tdir):
+            tc = self._hyx_tc
+            for b in np.nonzero(live)[0]:
+                while cur[b] < npts:
+                    d = (g[cur[b], b] - self._hyx_thi[b]) - self._hyx_tlo[b]
+                    inside = d <= 0 if tdir[b] else d >= 0
+                    if not inside and not fin[b]:
+                        break
+                    dt = d + h[b]
+                    acc = tc[:, -1, b].copy()
+                    for k in range(tc.shape[1] - 2, -1, -1):
+                        acc = acc * dt + tc[:, k, b]
+                    out[cur[b], :, b] = acc
+                    cur[b] += 1
+
+        _, cb = self._prop_stepwise(g[-1].copy(), np.zeros(B), int(max_steps), mdt, cb, True, after_step)
+        return cb, out
 
     # ---- dense output from the last step's Taylor coefficients ----------------------------
     def update_d_output(self, t, rel_time=False):

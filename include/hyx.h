@@ -59,6 +59,7 @@ typedef struct {
 #define HYX_FLAG_FORCE_UNROLLED 8u /* fully unrolled, histories in registers */
 #define HYX_FLAG_SOURCE_ONLY 16u   /* generate the HIP source only (hyx_sys_source); no compile, no device */
 #define HYX_FLAG_FORCE_GROUP 32u   /* group form: G lanes per instance over isomorphic clusters */
+#define HYX_FLAG_GRID 64u          /* hyx_sys_precompile: the propagate_grid code object */
 
 /* Code forms (hyx_sys_code_form). */
 #define HYX_FORM_UNROLLED 0 /* one instance per lane, fully unrolled, histories in registers */
@@ -128,6 +129,17 @@ int hyx_read_tc(hyx_batch *b, double *dst);
  * (no limit); max_steps: 0 = unlimited; res: [N] host output, may be NULL. */
 int hyx_propagate_until(hyx_batch *b, const double *tf_hi, const double *tf_lo, const double *max_delta_t,
                         uint64_t max_steps, int mode, hyx_prop_res *res);
+
+/* propagate_grid (heyoka/expose_batch_integrators.cpp:315-392): propagates every
+ * instance to its last grid time and writes the state at each grid time by dense
+ * output of the step that contains it. grid: [n_pts][N] host times; per instance
+ * finite, strictly monotonic, the first equal to the current time. out:
+ * [n_pts][n][N] host output (the reference's (n_pts, n, B) array); points an
+ * instance does not reach (step limit, non-finite state) are NaN. Steps are those
+ * of propagate_until to the last grid time. HYX_MODE_GRAPH runs as HYX_MODE_PLAIN.
+ * The first call on a system compiles its grid code object (HYX_GRID). */
+int hyx_propagate_grid(hyx_batch *b, const double *grid, uint64_t n_pts, const double *max_delta_t,
+                       uint64_t max_steps, int mode, hyx_prop_res *res, double *out);
 
 /* Kernel time (ms, HIP events on the batch's stream) and launch count of the
  * last hyx_propagate_until / hyx_step call. */

@@ -47,6 +47,10 @@ def lib():
             f.restype = ct.c_int
             f.argtypes = [P(u32), P(d), u32, ct.c_int, u32, u32, P(d), P(d), P(d), P(d), P(d), P(d), P(d), P(d),
                           u64, P(i64), P(d), P(d), P(u64), ct.c_int]
+            f = getattr(L, f"ora_{sfx}_propagate_grid")
+            f.restype = ct.c_int
+            f.argtypes = [P(u32), P(d), u32, ct.c_int, u32, u32, P(d), P(d), P(d), P(d), P(d), u64, P(d), u64,
+                          P(i64), P(d), P(d), P(u64), P(d), ct.c_int]
             f = getattr(L, f"ora_{sfx}_rhofac_value")
             f.restype = d
             f.argtypes = [u32]
@@ -117,6 +121,35 @@ def propagate_until(dec, order, state, tf, pars=None, t_hi=None, t_lo=None, tf_l
     if rc != 0:
         raise RuntimeError(f"oracle propagate_until failed with code {rc}")
     return dict(state=state, state_lo=lo, t_hi=t_hi, t_lo=t_lo, outcome=oc, min_h=mn, max_h=mx, nsteps=ns)
+
+
+def propagate_grid(dec, order, state, grid, pars=None, t_hi=None, t_lo=None, max_delta_t=np.inf, max_steps=0,
+                   high_accuracy=False, f128=False, lane_batch=8, nthreads=1):
+    """propagate_grid for every lane: grid [n_pts, B] (first row = current time). Returns the
+    propagate_until dict plus out [n_pts, n, B] (NaN where a lane did not reach the point)."""
+    words = np.ascontiguousarray(dec.to_words())
+    consts = dec.consts_array()
+    state = _f64(state)
+    n, B = state.shape
+    grid = _f64(grid)
+    assert grid.ndim == 2 and grid.shape[1] == B and grid.shape[0] >= 1
+    pars = _f64(pars if pars is not None else np.zeros((max(dec.n_par, 1), B)))
+    t_hi = _f64(grid[0] if t_hi is None else t_hi, (B,))
+    t_lo = _f64(0.0 if t_lo is None else t_lo, (B,))
+    mdt = _f64(max_delta_t, (B,))
+    oc = np.zeros(B, dtype=np.int64)
+    mn = np.zeros(B)
+    mx = np.zeros(B)
+    ns = np.zeros(B, dtype=np.uint64)
+    out = np.full((grid.shape[0], n, B), np.nan)
+    f = getattr(lib(), "ora_f128_propagate_grid" if f128 else "ora_f64_propagate_grid")
+    rc = f(_p(words, ct.c_uint32), _p(consts, ct.c_double), order, int(high_accuracy), B, lane_batch,
+           _p(state, ct.c_double), _p(pars, ct.c_double), _p(t_hi, ct.c_double), _p(t_lo, ct.c_double),
+           _p(grid, ct.c_double), grid.shape[0], _p(mdt, ct.c_double), int(max_steps), _p(oc, ct.c_int64),
+           _p(mn, ct.c_double), _p(mx, ct.c_double), _p(ns, ct.c_uint64), _p(out, ct.c_double), int(nthreads))
+    if rc != 0:
+        raise RuntimeError(f"oracle propagate_grid failed with code {rc}")
+    return dict(state=state, t_hi=t_hi, t_lo=t_lo, outcome=oc, min_h=mn, max_h=mx, nsteps=ns, out=out)
 
 
 def rhofac(order, f128=False):

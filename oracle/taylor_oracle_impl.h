@@ -427,9 +427,35 @@ typedef struct {
     int64_t *oc;
     double *min_h, *max_h;
     uint64_t *nsteps;
+    const double *grid; /* propagate_grid: [npts][B] times (NULL for propagate_until) */
+    uint64_t npts;
+    double *gout; /* propagate_grid: [npts][n][B] */
     volatile uint32_t next_chunk;
     int err;
 } FN(prop_job);
+
+/* Dense output of lane l (instance b) at the grid points inside the step of
+   size h that ended at time t: the step's Taylor polynomials evaluated at the
+   offset (g - t) + h from the start of the step, as heyoka's update_d_output
+   forms it (heyoka/expose_batch_integrators.cpp:505-541). fin: the step reached
+   the final time, so every remaining point is evaluated. */
+static void FN(grid_eval)(FN(prop_job) * jb, FN(work) * wk, uint32_t l, uint32_t b, uint64_t *gc, FN(dfl) t, REAL h,
+                          int tdir, int fin)
+{
+    const uint32_t n = jb->s->n_state, P = wk->P, B = jb->B;
+    while (*gc < jb->npts) {
+        REAL d = ((REAL)jb->grid[*gc * B + b] - t.hi) - t.lo;
+        int inside = tdir ? (d <= 0) : (d >= 0);
+        if (!inside && !fin) break;
+        REAL dt = d + h;
+        for (uint32_t i = 0; i < n; ++i) {
+            REAL r = TP(wk, i, P, l);
+            for (int k = (int)P - 1; k >= 0; --k) r = r * dt + TP(wk, i, k, l);
+            jb->gout[(*gc * n + i) * B + b] = (double)r;
+        }
+        ++*gc;
+    }
+}
 
 /* propagate_until for the lanes of one chunk, run in lock-step like one batch
    integrator of the reference: lanes that are done take zero-length steps. */
@@ -442,7 +468,7 @@ static void FN(prop_chunk)(FN(prop_job) * jb, FN(work) * wk, uint32_t b0, uint32
     FN(dfl) t[64], tf[64];
     REAL mdt[64], h[64], min_h[64], max_h[64], lim[64], remv[64];
     int64_t oc[64], res_oc[64];
-    uint64_t nst[64];
+    uint64_t nst[64], gc[64];
     int done[64], tdir[64];
     for (uint32_t l = 0; l < L; ++l) {
         uint32_t b = b0 + (l < nl ? l : 0);
@@ -458,6 +484,9 @@ static void FN(prop_chunk)(FN(prop_job) * jb, FN(work) * wk, uint32_t b0, uint32
         nst[l] = 0;
         done[l] = (l >= nl);
         res_oc[l] = ORA_TIME_LIMIT;
+        gc[l] = 1;
+        if (jb->grid && l < nl)
+            for (uint32_t i = 0; i < n; ++i) jb->gout[(size_t)i * B + b] = jb->state[(size_t)i * B + b];
     }
     uint64_t iter = 0;
     for (;;) {
@@ -482,6 +511,9 @@ static void FN(prop_chunk)(FN(prop_job) * jb, FN(work) * wk, uint32_t b0, uint32
                 min_h[l] = (ah < min_h[l]) ? ah : min_h[l];
                 max_h[l] = (ah > max_h[l]) ? ah : max_h[l];
             }
+            if (jb->grid && oc[l] != ORA_ERR_NF_STATE)
+                FN(grid_eval)(jb, wk, l, b0 + l, &gc[l], t[l], h[l], tdir[l],
+                              oc[l] == ORA_TIME_LIMIT && h[l] == remv[l]);
             if (oc[l] == ORA_ERR_NF_STATE) {
                 res_oc[l] = ORA_ERR_NF_STATE;
                 done[l] = 1;
@@ -529,6 +561,20 @@ static void *FN(prop_worker)(void *arg)
     return NULL;
 }
 
+static void FN(prop_run)(FN(prop_job) * jb, int nthreads)
+{
+    jb->next_chunk = 0;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    if (nthreads == 1) {
+        FN(prop_worker)(jb);
+    } else {
+        pthread_t th[256];
+        for (int i = 0; i < nthreads; ++i) pthread_create(&th[i], NULL, FN(prop_worker), jb);
+        for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
+    }
+}
+
 /* propagate_until for B lanes, processed in lock-step chunks of lane_batch lanes
    (1..64) by nthreads threads. tf = (tf_hi, tf_lo) is the final time per lane,
    max_delta_t[b] > 0 (+inf for none). */
@@ -564,16 +610,53 @@ int FN(propagate_until)(const uint32_t *words, const double *consts, uint32_t or
     jb.min_h = min_h;
     jb.max_h = max_h;
     jb.nsteps = nsteps;
-    jb.next_chunk = 0;
-    if (nthreads < 1) nthreads = 1;
-    if (nthreads > 256) nthreads = 256;
-    if (nthreads == 1) {
-        FN(prop_worker)(&jb);
-    } else {
-        pthread_t th[256];
-        for (int i = 0; i < nthreads; ++i) pthread_create(&th[i], NULL, FN(prop_worker), &jb);
-        for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
+    FN(prop_run)(&jb, nthreads);
+    free(s.rec);
+    return 0;
+}
+
+/* propagate_grid (heyoka/expose_batch_integrators.cpp:315-392): propagate_until
+   to the last grid time of each lane, writing the state at every grid time
+   ([npts][B], first row = current time) into gout [npts][n][B] by dense output
+   of the step containing it. gout must be prefilled (NaN) by the caller: points
+   a lane does not reach keep that value. */
+int FN(propagate_grid)(const uint32_t *words, const double *consts, uint32_t order, int high_acc, uint32_t B,
+                       uint32_t lane_batch, double *state, const double *pars, double *t_hi, double *t_lo,
+                       const double *grid, uint64_t npts, const double *max_delta_t, uint64_t max_steps, int64_t *oc,
+                       double *min_h, double *max_h, uint64_t *nsteps, double *gout, int nthreads)
+{
+    if (npts == 0) return -3;
+    FN(sys) s;
+    if (FN(parse)(&s, words, consts) != 0) return -1;
+    if (order < 2 || lane_batch == 0 || lane_batch > 64) {
+        free(s.rec);
+        return -2;
     }
+    double *tf_lo = (double *)calloc(B ? B : 1, sizeof(double));
+    FN(prop_job) jb;
+    memset(&jb, 0, sizeof(jb));
+    jb.s = &s;
+    jb.order = order;
+    jb.B = B;
+    jb.L = lane_batch;
+    jb.high_acc = high_acc;
+    jb.state = state;
+    jb.pars = pars;
+    jb.t_hi = t_hi;
+    jb.t_lo = t_lo;
+    jb.tf_hi = grid + (size_t)(npts - 1) * B;
+    jb.tf_lo = tf_lo;
+    jb.max_delta_t = max_delta_t;
+    jb.max_steps = max_steps;
+    jb.oc = oc;
+    jb.min_h = min_h;
+    jb.max_h = max_h;
+    jb.nsteps = nsteps;
+    jb.grid = grid;
+    jb.npts = npts;
+    jb.gout = gout;
+    FN(prop_run)(&jb, nthreads);
+    free(tf_lo);
     free(s.rec);
     return 0;
 }

@@ -200,3 +200,61 @@ def test_variational_order2_goldens(goldens):
     o2 = sv[6:12].reshape(2, 3)
     approx = approx + o2 @ np.array([dx[0] ** 2 / 2, dx[0] * dx[1], dx[1] ** 2 / 2])
     np.testing.assert_allclose(approx, goldens["var2_displaced"]["nonvar_state"], rtol=0, atol=5e-9)
+
+
+def test_pendulum_propagate_grid(pend, goldens):
+    # ta.propagate_grid([0.0, 0.1, ..., 1.0]) from the notebook's initial state:
+    # 5 steps, the last clamped to the final grid time, and the dense output.
+    g = goldens["pend_grid"]
+    grid = np.array(g["grid"])[:, None]
+    r = O.propagate_grid(pend, 20, np.array([[0.05], [0.025]]), grid, lane_batch=1)
+    assert r["outcome"][0] == O.TIME_LIMIT and int(r["nsteps"][0]) == g["nsteps"]
+    assert ulp_diff(r["min_h"][0], g["min_h"]) <= 4 and ulp_diff(r["max_h"][0], g["max_h"]) <= 4
+    np.testing.assert_allclose(r["out"][:, :, 0], g["out"], rtol=0, atol=5e-9)
+    assert r["t_hi"][0] == 1.0
+
+
+def test_fdp_batch_propagate_grid(fdp, goldens):
+    # Batch mode overview: grid linspace(0, 100, 1000) on the 4 lanes from the
+    # initial conditions; the reference prints propagate_res afterwards.
+    s = goldens["fdp_setup"]
+    grid = np.repeat(np.linspace(0, 100, 1000), 4).reshape(1000, -1)
+    r = O.propagate_grid(fdp, 20, s["state"], grid, pars=s["pars"], lane_batch=4)
+    for b, (oc, mn, mx, ns) in enumerate(goldens["fdp_grid_res"]["res"]):
+        assert r["outcome"][b] == oc and int(r["nsteps"][b]) == ns
+        # ~400 steps of a forced, weakly chaotic system: round-off differences in the
+        # state grow to ~1e-13 in h (the step counts agree exactly).
+        np.testing.assert_allclose([r["min_h"][b], r["max_h"][b]], [mn, mx], rtol=5e-13)
+    assert np.all(np.isfinite(r["out"])) and np.all(r["t_hi"] == 100.0)
+
+
+def test_grid_dense_output_matches_propagate_until(pend):
+    # The dense output at an interior grid time agrees with propagating to that
+    # time directly (different steps, same trajectory) to near round-off, and a
+    # one-point grid returns the current state with no step.
+    x0 = np.array([[0.05, -0.3, 0.7], [0.025, 0.4, -0.2]])
+    grid = np.repeat(np.linspace(0.0, 3.0, 31), 3).reshape(31, 3)
+    r = O.propagate_grid(pend, 20, x0, grid, lane_batch=3)
+    for j in (1, 7, 18, 30):
+        p = O.propagate_until(pend, 20, x0, grid[j], lane_batch=3)
+        np.testing.assert_allclose(r["out"][j], p["state"], rtol=0, atol=2e-14)
+    np.testing.assert_array_equal(r["out"][-1], r["state"])
+    one = O.propagate_grid(pend, 20, x0, grid[:1], lane_batch=3)
+    np.testing.assert_array_equal(one["out"][0], x0)
+    assert np.all(one["nsteps"] == 0) and np.all(one["outcome"] == O.TIME_LIMIT)
+
+
+def test_grid_backward_and_step_limit(pend):
+    # Backward grids walk the same rule in the other direction; a step limit
+    # leaves the unreached points NaN (batch propagate_grid fills them so).
+    x0 = np.array([[0.05, 0.3], [0.025, -0.1]])
+    grid = np.repeat(np.linspace(0.0, -2.0, 9), 2).reshape(9, 2)
+    r = O.propagate_grid(pend, 20, x0, grid, lane_batch=2)
+    p = O.propagate_until(pend, 20, x0, grid[4], lane_batch=2)
+    np.testing.assert_allclose(r["out"][4], p["state"], rtol=0, atol=2e-14)
+    grid = np.repeat(np.linspace(0.0, 20.0, 41), 2).reshape(41, 2)
+    r = O.propagate_grid(pend, 20, x0, grid, max_steps=3, lane_batch=2)
+    assert np.all(r["outcome"] == O.STEP_LIMIT) and np.all(r["nsteps"] == 3)
+    reached = np.isfinite(r["out"][:, 0, 0])
+    k = int(reached.sum())
+    assert 1 < k < 41 and reached[:k].all() and not reached[k:].any()
