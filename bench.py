@@ -109,6 +109,8 @@ def main():
     ap.add_argument("--n", type=int, default=None, help="instances (default: the config's)")
     ap.add_argument("--dt", type=float, default=25.0, help="nbody6: time units per bench step")
     ap.add_argument("--code-form", default=None, choices=[None, "compact", "unrolled", "group"])
+    ap.add_argument("--split-chains", action="store_true",
+                    help="two accumulators per long convolution sum instead of one (A/B of the code variant)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
@@ -150,7 +152,8 @@ def main():
     order = hy.order_from_tol(wl.tol)
     compact = args.code_form
     t_c = time.perf_counter()
-    system = get_system(dec, order, compact=compact)
+    system = (_lib.System(dec, order, compact=compact, split_chains=True) if args.split_chains
+              else get_system(dec, order, compact=compact))
     t_compile = time.perf_counter() - t_c
     N = wl.state.shape[1]
     flops = dec.algorithmic_flops(order)["total"]
@@ -226,6 +229,7 @@ def main():
             "instances": N,
             "order": order,
             "code_form": system.form,
+            "sum_chains": "split" if args.split_chains else "serial",
             "form": "persistent",
             "algorithmic_flops_per_instance_step": flops,
             "hiprtc_compile_s": round(t_compile, 3),

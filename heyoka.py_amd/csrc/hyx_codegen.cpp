@@ -174,10 +174,25 @@ struct Gen {
     std::set<std::pair<uint32_t, int>> dvars; // generated j * a^[j] values
     std::vector<char> perlane;                // constants held per lane (kc_prefix[]) instead of literals
     std::string kc_prefix = "kc";
+    // State coefficients in LDS: each lane keeps the rows x^[k] of its instance in
+    // xs[(k * n + i) * 64] (a per-block __shared__ array, the lane's column), and reads
+    // the orders below the one being generated back from there, instead of holding
+    // n * (p + 1) doubles in registers until the state update.
+    bool lds_state = false;
+    int cur_k = 0;
 
     Gen(const Sys &s, int p, bool ha) : S(s), P(p), high_acc(ha), perlane(s.nconsts, 0) {}
 
     static std::string cv(uint32_t node, int k) { return "c" + std::to_string(node) + "_" + std::to_string(k); }
+
+    std::string xs(uint32_t i, int k) const { return "xs[" + std::to_string(((uint32_t)k * S.n + i) * 64u) + "]"; }
+
+    // Coefficient k of node idx: a register, or for a state variable in LDS an older row.
+    std::string nv(uint32_t idx, int k) const
+    {
+        if (lds_state && idx < S.n && k != cur_k) return xs(idx, k);
+        return cv(idx, k);
+    }
 
     bool zero(uint32_t w, int k) const
     {
@@ -198,7 +213,7 @@ struct Gen {
     std::string val(uint32_t w, int k) const
     {
         const uint32_t kind = HYX_OPERAND_KIND(w), idx = HYX_OPERAND_INDEX(w);
-        if (kind == HYX_KIND_NODE) return cv(idx, k);
+        if (kind == HYX_KIND_NODE) return nv(idx, k);
         if (k > 0) return "0.0";
         if (kind == HYX_KIND_CONST)
             return perlane[idx] ? kc_prefix + "[" + std::to_string(idx) + "]" : lit(S.consts[idx]);
@@ -236,11 +251,26 @@ struct Gen {
         o << "    const double " << cv(node, k) << " = 0.0;\n";
     }
 
-    // acc = sum of products (terms), as an fma chain; "" when empty.
-    static std::string fma_chain(const std::vector<std::pair<std::string, std::string>> &t, const std::string &nm,
-                                 std::ostringstream &o)
+    // acc = sum of products (terms), as an fma chain; "" when empty. With split_min > 0,
+    // chains of at least split_min terms run as two interleaved accumulators (even and
+    // odd terms) added at the end: more independent FP64 work in flight per wave, at the
+    // price of a different summation order.
+    int split_min = 0;
+    std::string fma_chain(const std::vector<std::pair<std::string, std::string>> &t, const std::string &nm,
+                          std::ostringstream &o) const
     {
         if (t.empty()) return "";
+        if (split_min > 0 && (int)t.size() >= split_min) {
+            const std::string a = nm + "a", b = nm + "b";
+            o << "    double " << a << " = " << t[0].first << " * " << t[0].second << ";\n";
+            o << "    double " << b << " = " << t[1].first << " * " << t[1].second << ";\n";
+            for (size_t q = 2; q < t.size(); ++q) {
+                const std::string &acc = (q % 2 == 0) ? a : b;
+                o << "    " << acc << " = __builtin_fma(" << t[q].first << ", " << t[q].second << ", " << acc << ");\n";
+            }
+            o << "    const double " << nm << " = " << a << " + " << b << ";\n";
+            return nm;
+        }
         o << "    double " << nm << " = " << t[0].first << " * " << t[0].second << ";\n";
         for (size_t q = 1; q < t.size(); ++q)
             o << "    " << nm << " = __builtin_fma(" << t[q].first << ", " << t[q].second << ", " << nm << ");\n";
@@ -464,38 +494,63 @@ struct Gen {
         o << "#define HYX_NPAR " << std::max<uint32_t>(S.npar, 1) << "\n";
         o << "#define HYX_P " << P << "\n";
         o << "typedef unsigned long long hyx_u64;\n";
+        if (lds_state) {
+            o << "#define HYX_LDS_STATE 1\n";
+            o << "#ifndef HYX_CPU_EMU\n";
+            o << "__shared__ double hyx_xs[" << (uint64_t)(P + 1) * n * 64 << "];\n";
+            o << "__device__ __forceinline__ double *hyx_xs_lane() { return hyx_xs + threadIdx.x; }\n";
+            o << "#endif\n";
+        }
         o << "template <bool WTC>\n"
              "__device__ __forceinline__ void hyx_taylor_step(double (&x)[HYX_N], const double (&pv)[HYX_NPAR], "
              "double t0, double max_dt, double &h_out, bool &nf_out, double *__restrict__ tc, hyx_u64 tcs, "
              "double *__restrict__ tape, hyx_u64 tps)\n{\n";
         o << "    (void)pv; (void)t0; (void)tape; (void)tps; (void)tc; (void)tcs;\n";
+        if (lds_state) o << "    double *const xs = hyx_xs_lane();\n";
+        // The rows of order k go to LDS as soon as they exist; the empty asm at the
+        // end of each order keeps the compiler from forwarding them in registers.
+        auto store_rows = [&](int k) {
+            if (!lds_state) return;
+            for (uint32_t i = 0; i < n; ++i) o << "    " << xs(i, k) << " = " << cv(i, k) << ";\n";
+            o << "    __asm__ volatile(\"\" ::: \"memory\");\n";
+        };
         // order 0
+        cur_k = 0;
         for (uint32_t i = 0; i < n; ++i) o << "    const double " << cv(i, 0) << " = x[" << i << "];\n";
+        store_rows(0);
         for (uint32_t j = n; j < S.nnodes; ++j) order0(j);
         for (int k = 1; k <= P; ++k) {
             o << "    // order " << k << "\n";
+            cur_k = k;
             for (uint32_t i = 0; i < n; ++i) {
                 const uint32_t r = S.rhs[i];
                 if (zero(r, k - 1))
                     set_zero(i, k);
-                else
-                    o << "    const double " << cv(i, k) << " = " << val(r, k - 1) << " * " << lit(1.0 / k) << ";\n";
+                else {
+                    // The right-hand side's order k-1 is a node of the previous order.
+                    cur_k = k - 1;
+                    const std::string rv = val(r, k - 1);
+                    cur_k = k;
+                    o << "    const double " << cv(i, k) << " = " << rv << " * " << lit(1.0 / k) << ";\n";
+                }
             }
+            store_rows(k);
             if (k < P)
                 for (uint32_t j = n; j < S.nnodes; ++j) orderk(j, k);
         }
+        cur_k = P;
         // Taylor coefficients of the state, when asked for.
         o << "    if constexpr (WTC) {\n";
         for (uint32_t i = 0; i < n; ++i)
             for (int k = 0; k <= P; ++k)
-                o << "        tc[(hyx_u64)" << (i * (P + 1) + k) << " * tcs] = " << cv(i, k) << ";\n";
+                o << "        tc[(hyx_u64)" << (i * (P + 1) + k) << " * tcs] = " << nv(i, k) << ";\n";
         o << "    }\n";
         // Step size: infinity norms of the state and of the last two orders.
         o << "    double mx = 0.0, mo = 0.0, mom1 = 0.0, ab;\n";
         for (uint32_t i = 0; i < n; ++i) {
-            o << "    ab = fabs(" << cv(i, 0) << "); mx = (ab > mx) ? ab : mx;\n";
-            o << "    ab = fabs(" << cv(i, P) << "); mo = (ab > mo) ? ab : mo;\n";
-            o << "    ab = fabs(" << cv(i, P - 1) << "); mom1 = (ab > mom1) ? ab : mom1;\n";
+            o << "    ab = fabs(" << nv(i, 0) << "); mx = (ab > mx) ? ab : mx;\n";
+            o << "    ab = fabs(" << nv(i, P) << "); mo = (ab > mo) ? ab : mo;\n";
+            o << "    ab = fabs(" << nv(i, P - 1) << "); mom1 = (ab > mom1) ? ab : mom1;\n";
         }
         o << "    const double num = (mx <= 1.0) ? 1.0 : mx;\n";
         o << "    const double rho_o = pow(num / mo, " << lit(1.0 / P) << ");\n";
@@ -510,14 +565,14 @@ struct Gen {
         o << "    bool nf = false;\n";
         for (uint32_t i = 0; i < n; ++i) {
             if (!high_acc) {
-                o << "    { double r = " << cv(i, P) << ";\n";
-                for (int k = P - 1; k >= 0; --k) o << "      r = __builtin_fma(r, h, " << cv(i, k) << ");\n";
+                o << "    { double r = " << nv(i, P) << ";\n";
+                for (int k = P - 1; k >= 0; --k) o << "      r = __builtin_fma(r, h, " << nv(i, k) << ");\n";
                 o << "      x[" << i << "] = r; nf = nf || !isfinite(r); }\n";
             } else {
                 // Kahan-compensated sum of x^[k] h^k, k = 0..p.
-                o << "    { double sm = " << cv(i, 0) << ", cp = 0.0, hk = 1.0;\n";
+                o << "    { double sm = " << nv(i, 0) << ", cp = 0.0, hk = 1.0;\n";
                 for (int k = 1; k <= P; ++k)
-                    o << "      { hk *= h; const double y = " << cv(i, k)
+                    o << "      { hk *= h; const double y = " << nv(i, k)
                       << " * hk - cp; const double tt = sm + y; cp = (tt - sm) - y; sm = tt; }\n";
                 o << "      x[" << i << "] = sm; nf = nf || !isfinite(sm); }\n";
             }
@@ -1078,7 +1133,7 @@ bool plan_groups(const Sys &S, GroupPlan &gp, std::string &why)
     return true;
 }
 
-std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, bool high_acc)
+std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, bool high_acc, int split_min)
 {
     const int G = gp.G, own = gp.own, nin = gp.nin, nout = gp.nout;
     const int n = (int)S.n, nx = n + 1;
@@ -1144,6 +1199,7 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
     h << "}\n";
 
     Gen gen(gp.tmpl, P, high_acc);
+    gen.split_min = split_min;
     gen.perlane = gp.perlane;
     gen.kc_prefix = "tab.kc";
     auto &o = gen.o;
@@ -1277,7 +1333,7 @@ bool choose_compact(const Sys &S, int order)
     return (long)history_nodes(S) * (order + 1) > 320;
 }
 
-std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compact)
+std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compact, int split_min)
 {
     if (order < 2) throw std::invalid_argument("the Taylor order must be at least 2");
     if (compact) {
@@ -1285,6 +1341,13 @@ std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compac
         return g.run();
     }
     Gen g(S, order, high_acc);
+    g.split_min = split_min;
+    // State rows go to LDS when that brings the remaining histories within about 200
+    // VGPRs (two waves per SIMD) and the rows of a 64-lane block fit in 64 KB. When the
+    // node histories alone exceed that, the extra LDS traffic buys no occupancy.
+    const long hist = history_nodes(S), nst = S.n;
+    g.lds_state = hist * (order + 1) > 160 && (hist - nst) * (order + 1) <= 96
+                  && (long)(order + 1) * S.n * 64 * 8 <= 65536;
     return g.run();
 }
 

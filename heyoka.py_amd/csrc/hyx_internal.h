@@ -25,7 +25,8 @@ int parse_oplist(const uint32_t *w, uint32_t n_words, const double *consts, uint
                  std::string &err);
 
 // HIP source of hyx_taylor_step<WTC>() for S at the given order, unrolled or compact.
-std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compact);
+// split_min: fma chains of at least this many terms use two accumulators (0: never).
+std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compact, int split_min = 0);
 // Number of nodes whose coefficient history the recurrences read.
 int history_nodes(const Sys &S);
 // Default choice of code form: compact when the histories exceed the register file.
@@ -52,7 +53,7 @@ struct GroupPlan {
 };
 
 bool plan_groups(const Sys &S, GroupPlan &gp, std::string &why);
-std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int order, bool high_acc);
+std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int order, bool high_acc, int split_min = 0);
 
 // Exact hexadecimal double literal.
 std::string lit(double v);

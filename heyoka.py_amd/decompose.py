@@ -258,6 +258,19 @@ class _Lowerer:
         else:
             out.append(e)
 
+    def _scalar(self, e: expression) -> bool:
+        """True when e is a number, a parameter or a function of those only (a degree-0 series)."""
+        key = ("scalar", e)
+        if key not in self.ex_memo:
+            if e.is_var() or e.is_time():
+                r = False
+            elif e.is_func():
+                r = all(self._scalar(a) for a in e.args)
+            else:
+                r = True
+            self.ex_memo[key] = r
+        return self.ex_memo[key]
+
     def lower(self, e: expression) -> int:
         key = e
         if key in self.ex_memo:
@@ -294,7 +307,19 @@ class _Lowerer:
         if name == "neg":
             return self.node(OP_NEG, [self.lower(args[0])])
         if name == "mul":
-            return self.node(OP_MUL, [self.lower(args[0]), self.lower(args[1])])
+            a, b = args
+            # Scalar factors outermost: (s * c) * b -> s * (c * b) when s holds no variable
+            # and no time. The series product then runs on c and b, a multiplication by s
+            # costs one flop per order, and no coefficient history of (s * c) is kept.
+            if not self._scalar(a) and not self._scalar(b):
+                for x, y in ((a, b), (b, a)):
+                    if x.is_func() and x.name == "mul":
+                        p, q = x.args
+                        if self._scalar(p) and not self._scalar(q):
+                            return self.lower(expression._func("mul", p, expression._func("mul", q, y)))
+                        if self._scalar(q) and not self._scalar(p):
+                            return self.lower(expression._func("mul", q, expression._func("mul", p, y)))
+            return self.node(OP_MUL, [self.lower(a), self.lower(b)])
         if name == "div":
             return self.node(OP_DIV, [self.lower(args[0]), self.lower(args[1])])
         if name == "square":

@@ -26,7 +26,11 @@ HARNESS = r"""
 #define __device__
 #define __forceinline__ inline
 #define __constant__ static const
+#define HYX_CPU_EMU 1
 using std::sqrt; using std::pow; using std::exp; using std::log; using std::fabs; using std::isfinite;
+// LDS state rows of the lane form: lane 0 of a 64-lane column layout.
+static double hyx_xs_emu[64 * 64 * 64];
+static inline double *hyx_xs_lane() { return hyx_xs_emu; }
 
 #include "prelude.inc"
 extern "C" int step_cpu(const double *x_in, const double *pv_in, double t0, double max_dt, double *x_out,
