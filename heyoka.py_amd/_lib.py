@@ -25,6 +25,7 @@ FLAG_FORCE_GROUP = 32
 FORMS = {0: "unrolled", 1: "compact", 2: "group"}
 FLAG_GRID = 64
 FLAG_SPLIT_CHAINS = 128
+FLAG_NO_PIPELINE = 256
 
 
 class prop_res(ct.Structure):
@@ -162,11 +163,13 @@ class System:
     """Owner of a hyx_sys handle."""
 
     def __init__(self, dec, order: int, high_accuracy: bool = False, compile_only: bool = False, compact=None,
-                 source_only: bool = False, split_chains: bool = False):
+                 source_only: bool = False, split_chains: bool = False,
+                 pipeline: bool = True):
         L = lib()
         words, consts = oplist_args(dec)
         flags = (_flags(high_accuracy, compact) | (FLAG_COMPILE_ONLY if compile_only else 0)
-                 | (FLAG_SOURCE_ONLY if source_only else 0) | (FLAG_SPLIT_CHAINS if split_chains else 0))
+                 | (FLAG_SOURCE_ONLY if source_only else 0) | (FLAG_SPLIT_CHAINS if split_chains else 0)
+                 | (0 if pipeline else FLAG_NO_PIPELINE))
         h = ct.c_void_p()
         check(L.hyx_sys_create(dptr(words, ct.c_uint32), len(words), dptr(consts), len(dec.consts), order, flags,
                                ct.byref(h)))

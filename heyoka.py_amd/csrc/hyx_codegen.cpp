@@ -1133,7 +1133,8 @@ bool plan_groups(const Sys &S, GroupPlan &gp, std::string &why)
     return true;
 }
 
-std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, bool high_acc, int split_min)
+std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, bool high_acc, int split_min,
+                                   bool pipeline)
 {
     const int G = gp.G, own = gp.own, nin = gp.nin, nout = gp.nout;
     const int n = (int)S.n, nx = n + 1;
@@ -1146,6 +1147,16 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
       << "\n#define HYX_OWN " << own << "\n#define HYX_NX " << nx << "\n#define HYX_XB " << xb << "\n#define HYX_FLAT "
       << flat << "\n";
     h << "typedef unsigned long long hyx_u64;\n";
+    h << "#ifndef HYX_CPU_EMU\n"
+         "// A block is one wavefront, and the LDS operations of a wavefront complete in issue\n"
+         "// order: the per-order exchanges only need the compiler to keep memory accesses on\n"
+         "// their side of the fence (no s_waitcnt, no s_barrier).\n"
+         "__device__ __forceinline__ void hyx_wsync()\n{\n"
+         "    __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"wavefront\");\n"
+         "    __builtin_amdgcn_wave_barrier();\n"
+         "    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, \"wavefront\");\n}\n"
+         "#define HYX_EMU_SYNC() hyx_wsync()\n"
+         "#endif\n";
     auto itab = [&](const std::string &nm, const std::vector<int> &v) {
         h << "__constant__ int " << nm << "[" << std::max<size_t>(v.size(), 1) << "] = {";
         for (size_t q = 0; q < v.size(); ++q) h << (q ? "," : "") << v[q];
@@ -1158,8 +1169,65 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
         if (v.empty()) h << "0.0";
         h << "};\n";
     };
+    // Pipelined schedule applies when every cluster input is a "lagged" state variable:
+    // its right-hand side has no cluster-output term. Lane tables then give, per input,
+    // the state terms (variable, scale) of that right-hand side, in the owner's order.
+    std::vector<std::pair<int, int>> owner(n, {-1, -1});
+    for (int g = 0; g < G; ++g)
+        for (int oo = 0; oo < own; ++oo) {
+            const int v = gp.vo[(size_t)g * own + oo];
+            if (v >= 0 && v < n) owner[v] = {g, oo};
+        }
+    bool lagged = pipeline && nin > 0 && nout > 0;
+    int nls = 0;
+    for (int g = 0; g < G && lagged; ++g)
+        for (int q = 0; q < nin && lagged; ++q) {
+            const int v = gp.iv[(size_t)g * nin + q];
+            if (v < 0 || v >= n || owner[v].first < 0) {
+                lagged = false;
+                break;
+            }
+            const int g2 = owner[v].first, oo = owner[v].second;
+            for (int t = 0; t < gp.nto[oo]; ++t)
+                if (gp.tio[oo][(size_t)g2 * gp.nto[oo] + t] != G * nout) lagged = false;
+            nls = std::max(nls, gp.nts[oo]);
+        }
+    if (nls == 0) lagged = false;
+    std::vector<int> lis;
+    std::vector<double> lss, lrc;
+    std::vector<std::string> lss_sign;
+    bool lrc_nonzero = false, lss_table = false;
+    if (lagged) {
+        lis.assign((size_t)G * nin * nls, n);
+        lss.assign((size_t)G * nin * nls, 1.0);
+        lrc.assign((size_t)G * nin, 0.0);
+        for (int g = 0; g < G; ++g)
+            for (int q = 0; q < nin; ++q) {
+                const int v = gp.iv[(size_t)g * nin + q];
+                const int g2 = owner[v].first, oo = owner[v].second;
+                for (int t = 0; t < gp.nts[oo]; ++t) {
+                    lis[((size_t)g * nin + q) * nls + t] = gp.tis[oo][(size_t)g2 * gp.nts[oo] + t];
+                    lss[((size_t)g * nin + q) * nls + t] = gp.tss[oo][(size_t)g2 * gp.nts[oo] + t];
+                }
+                lrc[(size_t)g * nin + q] = gp.rc[(size_t)g2 * own + oo];
+                lrc_nonzero = lrc_nonzero || lrc[(size_t)g * nin + q] != 0.0;
+            }
+        for (int qt = 0; qt < nin * nls; ++qt) {
+            bool same = true;
+            for (int g = 1; g < G; ++g) same = same && lss[(size_t)g * nin * nls + qt] == lss[qt];
+            const double v = lss[qt];
+            lss_sign.push_back(!same ? std::string() : v == 1.0 ? std::string("+") : v == -1.0 ? std::string("-") : lit(v));
+            lss_table = lss_table || !same;
+        }
+    }
+    h << "#define HYX_PIPELINED " << (lagged ? 1 : 0) << "\n";
     itab("hyx_tab_iv", gp.iv);
     dtab("hyx_tab_kc", gp.kc);
+    if (lagged) {
+        itab("hyx_tab_lis", lis);
+        if (lss_table) dtab("hyx_tab_lss", lss);
+        if (lrc_nonzero) dtab("hyx_tab_lrc", lrc);
+    }
     itab("hyx_tab_vo", gp.vo);
     dtab("hyx_tab_rc", gp.rc);
     // Per-slot term tables; a sign uniform over the lanes becomes a literal.
@@ -1180,6 +1248,11 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
     }
     // The per-lane table struct and its loader.
     h << "struct hyx_gtab {\n    int iv[HYX_NIN];\n    double kc[HYX_NKC];\n    int vo[HYX_OWN];\n    double rc[HYX_OWN];\n";
+    if (lagged) {
+        h << "    int lis[" << nin * nls << "];\n";
+        if (lss_table) h << "    double lss[" << nin * nls << "];\n";
+        if (lrc_nonzero) h << "    double lrc[HYX_NIN];\n";
+    }
     for (int o = 0; o < own; ++o)
         h << "    int tio" << o << "[" << std::max(gp.nto[o], 1) << "];\n    int tis" << o << "["
           << std::max(gp.nts[o], 1) << "];\n    double tso" << o << "[" << std::max(gp.nto[o], 1) << "];\n    double tss"
@@ -1188,6 +1261,11 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
     h << "    for (int q = 0; q < HYX_NIN; ++q) t.iv[q] = hyx_tab_iv[g * HYX_NIN + q];\n";
     h << "    for (int q = 0; q < HYX_NKC; ++q) t.kc[q] = hyx_tab_kc[g * HYX_NKC + q];\n";
     h << "    for (int q = 0; q < HYX_OWN; ++q) { t.vo[q] = hyx_tab_vo[g * HYX_OWN + q]; t.rc[q] = hyx_tab_rc[g * HYX_OWN + q]; }\n";
+    if (lagged) {
+        h << "    for (int q = 0; q < " << nin * nls << "; ++q) t.lis[q] = hyx_tab_lis[g * " << nin * nls << " + q];\n";
+        if (lss_table) h << "    for (int q = 0; q < " << nin * nls << "; ++q) t.lss[q] = hyx_tab_lss[g * " << nin * nls << " + q];\n";
+        if (lrc_nonzero) h << "    for (int q = 0; q < HYX_NIN; ++q) t.lrc[q] = hyx_tab_lrc[g * HYX_NIN + q];\n";
+    }
     for (int o = 0; o < own; ++o) {
         if (gp.nto[o])
             h << "    for (int q = 0; q < " << gp.nto[o] << "; ++q) { t.tio" << o << "[q] = hyx_tab_tio" << o << "[g * "
@@ -1214,17 +1292,30 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
     // Column of each owned variable in the x rows (column n reads zero for empty slots).
     for (int oo = 0; oo < own; ++oo)
         o << "    const int " << vcol(oo) << " = (tab.vo[" << oo << "] >= 0) ? tab.vo[" << oo << "] : HYX_N;\n";
-    o << "    __syncthreads();\n";
+    o << "    hyx_wsync();\n";
     for (int oo = 0; oo < own; ++oo) {
         o << "    const double " << xv(oo, 0) << " = xo[" << oo << "];\n";
         // Empty slots hold +0.0 and write it into the zero column: no branch needed.
         o << "    " << X(0, vcol(oo)) << " = " << xv(oo, 0) << ";\n";
     }
-    o << "    __syncthreads();\n";
-    for (int k = 0; k < P; ++k) {
-        o << "    // order " << k << "\n";
-        for (int q = 0; q < nin; ++q)
-            o << "    const double " << Gen::cv((uint32_t)q, k) << " = " << X(k, "tab.iv[" + std::to_string(q) + "]") << ";\n";
+    o << "    hyx_wsync();\n";
+    // Owner sum of slot oo for x^[k+1], from the term values t_o (outputs) and t_s (state).
+    auto owner_expr = [&](int oo, int k, const std::vector<std::string> &t_o, const std::vector<std::string> &t_s) {
+        std::string r = (k == 0) ? "tab.rc[" + std::to_string(oo) + "]" : "";
+        auto add = [&](const std::string &sg, const std::string &val) {
+            if (sg == "+")
+                r = r.empty() ? val : "(" + r + " + " + val + ")";
+            else if (sg == "-")
+                r = r.empty() ? "(-" + val + ")" : "(" + r + " - " + val + ")";
+            else
+                r = r.empty() ? "(" + sg + " * " + val + ")" : "__builtin_fma(" + sg + ", " + val + ", " + r + ")";
+        };
+        for (int t = 0; t < gp.nto[oo]; ++t) add(sgn_o[oo][t], t_o[t]);
+        for (int t = 0; t < gp.nts[oo]; ++t) add(sgn_s[oo][t], t_s[t]);
+        if (r.empty()) r = "0.0";
+        return r + " * " + lit(1.0 / (k + 1));
+    };
+    auto cluster_order = [&](int k) {
         for (uint32_t j = gp.tmpl.n; j < gp.tmpl.nnodes; ++j) {
             if (k == 0)
                 gen.order0(j);
@@ -1235,26 +1326,82 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
             const uint32_t w = HYX_OPERAND(HYX_KIND_NODE, gp.outs[q]);
             o << "    fl[g * " << nout << " + " << q << "] = " << (gen.zero(w, k) ? "0.0" : gen.val(w, k)) << ";\n";
         }
-        o << "    __syncthreads();\n";
-        for (int oo = 0; oo < own; ++oo) {
-            std::string r = (k == 0) ? "tab.rc[" + std::to_string(oo) + "]" : "";
-            auto add = [&](const std::string &sg, const std::string &val) {
-                if (sg == "+")
-                    r = r.empty() ? val : "(" + r + " + " + val + ")";
-                else if (sg == "-")
-                    r = r.empty() ? "(-" + val + ")" : "(" + r + " - " + val + ")";
-                else
-                    r = r.empty() ? "(" + sg + " * " + val + ")" : "__builtin_fma(" + sg + ", " + val + ", " + r + ")";
-            };
-            for (int t = 0; t < gp.nto[oo]; ++t)
-                add(sgn_o[oo][t], "fl[tab.tio" + std::to_string(oo) + "[" + std::to_string(t) + "]]");
-            for (int t = 0; t < gp.nts[oo]; ++t)
-                add(sgn_s[oo][t], X(k, "tab.tis" + std::to_string(oo) + "[" + std::to_string(t) + "]"));
-            if (r.empty()) r = "0.0";
-            o << "    const double " << xv(oo, k + 1) << " = " << r << " * " << lit(1.0 / (k + 1)) << ";\n";
-            o << "    " << X(k + 1, vcol(oo)) << " = " << xv(oo, k + 1) << ";\n";
+    };
+    if (lagged) {
+        // Pipelined schedule, one exchange per order: every cluster input is a state
+        // variable whose right-hand side holds state terms only (N-body positions,
+        // x' = v), so a lane forms its order-k inputs itself from the order k-1 rows
+        // instead of waiting for the owners' order-k rows. In the phase of order k the
+        // owners' loads (outputs of order k-1) and the cluster-input loads are issued
+        // together, the owners write x^[k], then the clusters compute order k and
+        // overwrite the outputs: the owners read them first, and one wavefront runs
+        // the whole group, whose LDS accesses complete in issue order.
+        o << "    // order 0\n";
+        for (int q = 0; q < nin; ++q)
+            o << "    const double " << Gen::cv((uint32_t)q, 0) << " = " << X(0, "tab.iv[" + std::to_string(q) + "]") << ";\n";
+        cluster_order(0);
+        o << "    hyx_wsync();\n";
+        for (int k = 1; k <= P; ++k) {
+            o << "    // owner loads of order " << k - 1 << (k < P ? ", cluster order " + std::to_string(k) : "") << "\n";
+            std::vector<std::vector<std::string>> to(own), ts(own);
+            for (int oo = 0; oo < own; ++oo) {
+                for (int t = 0; t < gp.nto[oo]; ++t) {
+                    const std::string nm = "ot" + std::to_string(oo) + "_" + std::to_string(t) + "_" + std::to_string(k);
+                    o << "    const double " << nm << " = fl[tab.tio" << oo << "[" << t << "]];\n";
+                    to[oo].push_back(nm);
+                }
+                for (int t = 0; t < gp.nts[oo]; ++t) {
+                    const std::string nm = "os" + std::to_string(oo) + "_" + std::to_string(t) + "_" + std::to_string(k);
+                    o << "    const double " << nm << " = " << X(k - 1, "tab.tis" + std::to_string(oo) + "[" + std::to_string(t) + "]")
+                      << ";\n";
+                    ts[oo].push_back(nm);
+                }
+            }
+            if (k < P)
+                for (int q = 0; q < nin; ++q) {
+                    std::string r = (k == 1 && lrc_nonzero) ? "tab.lrc[" + std::to_string(q) + "]" : "";
+                    for (int t = 0; t < nls; ++t) {
+                        const std::string val = X(k - 1, "tab.lis[" + std::to_string(q * nls + t) + "]");
+                        const std::string &sg = lss_sign[(size_t)q * nls + t];
+                        const std::string sgv = sg.empty() ? "tab.lss[" + std::to_string(q * nls + t) + "]" : sg;
+                        if (sgv == "+")
+                            r = r.empty() ? val : "(" + r + " + " + val + ")";
+                        else if (sgv == "-")
+                            r = r.empty() ? "(-" + val + ")" : "(" + r + " - " + val + ")";
+                        else
+                            r = r.empty() ? "(" + sgv + " * " + val + ")" : "__builtin_fma(" + sgv + ", " + val + ", " + r + ")";
+                    }
+                    if (r.empty()) r = "0.0";
+                    o << "    const double " << Gen::cv((uint32_t)q, k) << " = " << r << " * " << lit(1.0 / k) << ";\n";
+                }
+            // The owners finish order k before the cluster work, so no sum stays live across it.
+            for (int oo = 0; oo < own; ++oo) {
+                o << "    const double " << xv(oo, k) << " = " << owner_expr(oo, k - 1, to[oo], ts[oo]) << ";\n";
+                o << "    " << X(k, vcol(oo)) << " = " << xv(oo, k) << ";\n";
+            }
+            if (k < P) {
+                o << "    HYX_EMU_SYNC();\n";
+                cluster_order(k);
+            }
+            o << "    hyx_wsync();\n";
         }
-        o << "    __syncthreads();\n";
+    } else {
+        for (int k = 0; k < P; ++k) {
+            o << "    // order " << k << "\n";
+            for (int q = 0; q < nin; ++q)
+                o << "    const double " << Gen::cv((uint32_t)q, k) << " = " << X(k, "tab.iv[" + std::to_string(q) + "]") << ";\n";
+            cluster_order(k);
+            o << "    hyx_wsync();\n";
+            for (int oo = 0; oo < own; ++oo) {
+                std::vector<std::string> t_o, t_s;
+                for (int t = 0; t < gp.nto[oo]; ++t) t_o.push_back("fl[tab.tio" + std::to_string(oo) + "[" + std::to_string(t) + "]]");
+                for (int t = 0; t < gp.nts[oo]; ++t)
+                    t_s.push_back(X(k, "tab.tis" + std::to_string(oo) + "[" + std::to_string(t) + "]"));
+                o << "    const double " << xv(oo, k + 1) << " = " << owner_expr(oo, k, t_o, t_s) << ";\n";
+                o << "    " << X(k + 1, vcol(oo)) << " = " << xv(oo, k + 1) << ";\n";
+            }
+            o << "    hyx_wsync();\n";
+        }
     }
     o << "    if constexpr (WTC) if (tc) {\n";
     for (int oo = 0; oo < own; ++oo) {

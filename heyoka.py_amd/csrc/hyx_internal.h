@@ -53,7 +53,9 @@ struct GroupPlan {
 };
 
 bool plan_groups(const Sys &S, GroupPlan &gp, std::string &why);
-std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int order, bool high_acc, int split_min = 0);
+// pipeline: one LDS exchange per order when every cluster input is a lagged state variable.
+std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int order, bool high_acc, int split_min = 0,
+                                   bool pipeline = true);
 
 // Exact hexadecimal double literal.
 std::string lit(double v);

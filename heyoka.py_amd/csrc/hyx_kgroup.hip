@@ -16,8 +16,9 @@
 // (rows of HYX_NX = n + 1, the last column zero), which the state update reads.
 //
 // Every lane of the wave executes every step, also lanes whose group has no
-// instance (their results are not stored): hyx_gstep contains workgroup
-// barriers, and a block is exactly one wavefront.
+// instance (their results are not stored): hyx_gstep exchanges data between the
+// lanes of the wavefront through LDS at every order, and a block is exactly one
+// wavefront.
 //
 // Built with HYX_GRID defined (the propagate_grid code object), the propagate
 // kernels also evaluate the step's Taylor polynomials at the grid times inside
@@ -171,7 +172,7 @@ __device__ __forceinline__ bool hyx_gprop_iter(hyx_ginst &s, const hyx_kargs &a,
     for (int q = g; q < HYX_FLAT; q += HYX_G) fl[q] = 0.0;                                                           \
     hyx_gtab tab;                                                                                                    \
     hyx_gtab_load(tab, g);                                                                                           \
-    __syncthreads();
+    hyx_wsync();
 
 // ---- step(): one adaptive step per instance ----------------------------------------------
 template <bool WTC>

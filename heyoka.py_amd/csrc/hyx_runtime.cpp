@@ -285,7 +285,7 @@ int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint3
         const int split = (flags & HYX_FLAG_SPLIT_CHAINS) ? 8 : 0;
         if (s.form == HYX_FORM_GROUP) {
             s.inst_per_block = 64u / (uint32_t)gp.G;
-            s.src = hyx::generate_group_prelude(s.S, gp, (int)order, ha, split) + "\n" + k_hyx_kcommon + "\n" + k_hyx_kgroup;
+            s.src = hyx::generate_group_prelude(s.S, gp, (int)order, ha, split, !(flags & HYX_FLAG_NO_PIPELINE)) + "\n" + k_hyx_kcommon + "\n" + k_hyx_kgroup;
         } else {
             s.inst_per_block = 64;
             s.src = hyx::generate_prelude(s.S, (int)order, ha, s.compact, split) + "\n" + k_hyx_kcommon + "\n" + k_hyx_klane;

@@ -128,6 +128,12 @@ using std::sqrt; using std::pow; using std::exp; using std::log; using std::fabs
 static pthread_barrier_t bar;
 static thread_local int tl_g;
 static void __syncthreads() { pthread_barrier_wait(&bar); }
+#define HYX_CPU_EMU 1
+// The lanes of a wavefront run in lock step on the GPU; threads need a barrier at
+// every exchange point, including the one between the owners' loads and the
+// clusters' stores of the pipelined schedule.
+static void hyx_wsync() { pthread_barrier_wait(&bar); }
+#define HYX_EMU_SYNC() hyx_wsync()
 static double xch_d[64];
 static int xch_i[64];
 static double __shfl_xor(double v, int off) { xch_d[tl_g] = v; __syncthreads(); double r = xch_d[tl_g ^ off]; __syncthreads(); return r; }
@@ -164,8 +170,9 @@ extern "C" int gstep_cpu(const double *x, const double *pv, double t0, double md
 """
 
 
+@pytest.mark.parametrize("pipeline", [True, False])
 @pytest.mark.parametrize("name", ["nbody6", "nbody3"])
-def test_group_step_matches_oracle(name, tmp_path):
+def test_group_step_matches_oracle(name, pipeline, tmp_path):
     # The group form emulated on the CPU: one thread per lane of a group, a
     # barrier for __syncthreads and a shared array for __shfl_xor.
     if name == "nbody6":
@@ -176,9 +183,10 @@ def test_group_step_matches_oracle(name, tmp_path):
         st = np.array([0, 0, 0, 0, 0, 0, 1, 0, 0.1, 0, 1.1, 0, -2, 0.5, 0, 0.2, -0.6, 0.05], dtype=float)[:, None]
     dec = hy.taylor_decompose(sys_)
     p = 19
-    s = _lib.System(dec, p, compact="group", source_only=True)
+    s = _lib.System(dec, p, compact="group", source_only=True, pipeline=pipeline)
     assert s.form == "group"
     src = s.source()
+    assert ("#define HYX_PIPELINED 1" in src) == pipeline
     prelude = src.split("// Hand-written CDNA4 kernels")[0]
     d = tmp_path / name
     d.mkdir()
