@@ -92,6 +92,7 @@ def lib():
             "hyx_slots_arm": (ct.c_int, [vp, P(u32), u32, P(d), P(d), P(d)]),
             "hyx_slots_launch": (ct.c_int, [vp, u64]),
             "hyx_slots_wait": (ct.c_int, [vp, P(u32)]),
+            "hyx_slots_poll": (ct.c_int, [vp, P(ct.c_int)]),
             "hyx_slots_results": (ct.c_int, [vp, P(u32), u32, P(u32), P(prop_res)]),
             "hyx_last_error": (ct.c_char_p, []),
             "hyx_device_count": (ct.c_int, []),
@@ -113,7 +114,7 @@ EXPORTED = (
     "hyx_propagate_grid",
     "hyx_batch_last_timing", "hyx_batch_set_persistent_config", "hyx_batch_set_queue", "hyx_batch_set_iter_budget",
     "hyx_batch_sync", "hyx_slots_reset", "hyx_slots_write", "hyx_slots_read", "hyx_slots_arm", "hyx_slots_launch",
-    "hyx_slots_wait", "hyx_slots_results",
+    "hyx_slots_wait", "hyx_slots_poll", "hyx_slots_results",
     "hyx_last_error", "hyx_device_count",
 )
 
@@ -315,6 +316,12 @@ class Batch:
         n = ct.c_uint32()
         check(self._L.hyx_slots_wait(self.h, ct.byref(n)))
         return n.value
+
+    def slots_poll(self) -> bool:
+        """True when the running epoch (if any) has finished; never blocks."""
+        f = ct.c_int()
+        check(self._L.hyx_slots_poll(self.h, ct.byref(f)))
+        return bool(f.value)
 
     def slots_results(self, slots):
         sl = np.ascontiguousarray(slots, dtype=np.uint32)

@@ -1071,10 +1071,12 @@ bool plan_groups(const Sys &S, GroupPlan &gp, std::string &why)
         for (size_t q = 0; q < B[c].kconsts.size(); ++q) gp.kc[(size_t)c * gp.nkc + q] = S.consts[B[c].kconsts[q]];
         for (size_t q = B[c].kconsts.size(); q < gp.tmpl.nconsts; ++q) gp.kc[(size_t)c * gp.nkc + q] = gp.tmpl.consts[q];
     }
-    // Outputs of lane c live at LDS index c * nout + q; index G * nout reads zero.
+    // Output q of lane c lives at LDS index q * G + c (the lanes of a group write one
+    // output row with consecutive 8-byte words: no bank conflicts); index G * nout
+    // reads zero.
     std::vector<int> outslot(NN, -1);
     for (int c = 0; c < ncl; ++c)
-        for (size_t q = 0; q < B[c].outs.size(); ++q) outslot[B[c].outs[q]] = c * gp.nout + (int)q;
+        for (size_t q = 0; q < B[c].outs.size(); ++q) outslot[B[c].outs[q]] = (int)q * G + c;
     // Ownership: variables sorted by (output terms, state terms), dealt to lanes slot
     // by slot, so that each slot's term counts are as uniform as possible.
     std::vector<uint32_t> order(n);
@@ -1324,7 +1326,7 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
         }
         for (int q = 0; q < nout; ++q) {
             const uint32_t w = HYX_OPERAND(HYX_KIND_NODE, gp.outs[q]);
-            o << "    fl[g * " << nout << " + " << q << "] = " << (gen.zero(w, k) ? "0.0" : gen.val(w, k)) << ";\n";
+            o << "    fl[" << q * G << " + g] = " << (gen.zero(w, k) ? "0.0" : gen.val(w, k)) << ";\n";
         }
     };
     if (lagged) {

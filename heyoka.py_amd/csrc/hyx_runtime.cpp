@@ -1077,6 +1077,25 @@ int hyx_slots_wait(hyx_batch *b, uint32_t *n_active)
     return 0;
 }
 
+int hyx_slots_poll(hyx_batch *b, int *finished)
+{
+    if (!b || !finished) return fail(HYX_EINVAL, "null argument");
+    if (!b->slots_running) {
+        *finished = 1;
+        return 0;
+    }
+    const hipError_t e = hipEventQuery(b->ev1);
+    if (e == hipSuccess) {
+        *finished = 1;
+        return 0;
+    }
+    if (e == hipErrorNotReady) {
+        *finished = 0;
+        return 0;
+    }
+    return fail(HYX_EHIP, std::string("hyx_slots_poll: ") + hipGetErrorString(e));
+}
+
 int hyx_slots_results(hyx_batch *b, const uint32_t *slots, uint32_t count, uint32_t *done, hyx_prop_res *res)
 {
     if (!b) return fail(HYX_EINVAL, "null batch");

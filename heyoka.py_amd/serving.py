@@ -9,11 +9,12 @@ a loop of bounded epochs:
    system's pool (creating the pool if the HBM budget allows): scatter their
    state, parameters and time into the slots and arm them with their final
    times;
-2. start one epoch on every pool with armed slots (asynchronous launches on
-   separate streams, so pools of different systems run concurrently), each
-   slot advancing by at most ``epoch_iters`` steps;
-3. wait, harvest the requests whose slots are all done (gather their results,
-   free the slots) and record their latency.
+2. start an epoch on every pool with armed slots and no epoch in flight
+   (asynchronous launches on separate streams, so pools of different systems
+   run concurrently), each slot advancing by at most ``epoch_iters`` steps;
+3. poll the pools without blocking; harvest each pool as soon as its own epoch
+   ends: gather the results of requests whose slots are all done, free the
+   slots and record their latency.
 
 A request that arrives while others are running is admitted at the next epoch
 boundary, as soon as slots are free; running slots stay on the device between
@@ -74,6 +75,8 @@ class _Pool:
         self.free = deque(range(capacity))
         self.running: list[Request] = []
         self.npar = system.npar
+        self.in_epoch = False  # an epoch of this pool is on the GPU
+        self.epoch_start = 0.0
 
     def bytes(self):
         n, npar = self.dec.n_state, max(self.npar, 1)
@@ -154,8 +157,10 @@ class Server:
                 keep.append(req)
                 continue
             pool = self._pool(req.key)
-            if pool is None or len(pool.free) < req.n:
-                blocked.add(req.key)  # keep first-come-first-served order within a system
+            if pool is None or pool.in_epoch or len(pool.free) < req.n:
+                # Admission waits for the pool's epoch boundary (its slots are written and
+                # armed between epochs), keeping first-come-first-served order within a system.
+                blocked.add(req.key)
                 keep.append(req)
                 continue
             slots = np.array([pool.free.popleft() for _ in range(req.n)], dtype=np.uint32)
@@ -172,39 +177,49 @@ class Server:
         with self._lock:
             self.waiting.extendleft(reversed(keep))
 
+    def _harvest(self, p):
+        still = []
+        for req in p.running:
+            done, res = p.batch.slots_results(req.slots)
+            if not done.all():
+                still.append(req)
+                continue
+            req.result = res
+            req.result_state = p.batch.slots_read(_lib.FIELD_STATE, req.slots, p.dec.n_state)
+            req.result_time = p.batch.slots_read(_lib.FIELD_TIME_HI, req.slots, 1)[0]
+            req.finished = _time.perf_counter()
+            self.instance_steps += int(res["nsteps"].astype(np.int64).sum())
+            p.free.extend(int(s) for s in req.slots)
+            self.done.append(req)
+            req.done_event.set()
+        p.running = still
+
     def step(self) -> int:
-        """One scheduling round: admit, run one epoch on every busy pool, harvest.
-        Returns the number of requests still waiting or running."""
+        """One scheduling round: admit, start an epoch on every busy pool that is idle,
+        then harvest every pool whose epoch has ended. Pools run independently: a pool
+        whose epochs are long (a large system) does not hold back the others' harvests
+        and admissions. Returns the number of requests still waiting or running."""
         now = _time.perf_counter()
         self._admit(now)
-        busy = [p for p in self.pools.values() if isinstance(p, _Pool) and p.running]
-        t = _time.perf_counter()
-        for p in busy:
-            p.batch.slots_launch(self.epoch_iters)
-        for p in busy:
+        pools = [p for p in self.pools.values() if isinstance(p, _Pool)]
+        for p in pools:
+            if p.running and not p.in_epoch:
+                p.batch.slots_launch(self.epoch_iters)
+                p.in_epoch = True
+                p.epoch_start = _time.perf_counter()
+        inflight = [p for p in pools if p.in_epoch]
+        ended = [p for p in inflight if p.batch.slots_poll()]
+        if inflight and not ended:
+            _time.sleep(0.0002)
+        for p in ended:
             p.batch.slots_wait()
-        if busy:
-            self.busy_time += _time.perf_counter() - t
+            p.in_epoch = False
+            self.busy_time += _time.perf_counter() - p.epoch_start
             self.epochs += 1
-        for p in busy:
-            still = []
-            for req in p.running:
-                done, res = p.batch.slots_results(req.slots)
-                if not done.all():
-                    still.append(req)
-                    continue
-                req.result = res
-                req.result_state = p.batch.slots_read(_lib.FIELD_STATE, req.slots, p.dec.n_state)
-                req.result_time = p.batch.slots_read(_lib.FIELD_TIME_HI, req.slots, 1)[0]
-                req.finished = _time.perf_counter()
-                self.instance_steps += int(res["nsteps"].astype(np.int64).sum())
-                p.free.extend(int(s) for s in req.slots)
-                self.done.append(req)
-                req.done_event.set()
-            p.running = still
+            self._harvest(p)
         with self._lock:
             nwait = len(self.waiting)
-        return nwait + sum(len(p.running) for p in self.pools.values() if isinstance(p, _Pool))
+        return nwait + sum(len(p.running) for p in pools)
 
     def run_until_idle(self, max_rounds: int = 10 ** 9):
         for _ in range(max_rounds):
@@ -212,7 +227,7 @@ class Server:
                 return
             with self._lock:
                 only_future = self.waiting and all(r.arrival > _time.perf_counter() for r in self.waiting)
-            busy = any(isinstance(p, _Pool) and p.running for p in self.pools.values())
+            busy = any(isinstance(p, _Pool) and (p.running or p.in_epoch) for p in self.pools.values())
             if only_future and not busy:
                 _time.sleep(max(0.0, min(r.arrival for r in self.waiting) - _time.perf_counter()))
 

@@ -175,6 +175,9 @@ int hyx_slots_arm(hyx_batch *b, const uint32_t *slots, uint32_t count, const dou
 int hyx_slots_launch(hyx_batch *b, uint64_t iter_budget);
 /* Wait for the epoch; n_active = armed slots still unfinished. */
 int hyx_slots_wait(hyx_batch *b, uint32_t *n_active);
+/* Non-blocking: *finished = 1 when the epoch's kernels are complete (or none runs),
+ * so a server can harvest each batch as soon as its own epoch ends. */
+int hyx_slots_poll(hyx_batch *b, int *finished);
 /* done[c] = 1 once slot c finished (then res[c] holds its propagate result). */
 int hyx_slots_results(hyx_batch *b, const uint32_t *slots, uint32_t count, uint32_t *done, hyx_prop_res *res);
 
