@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE (x1.000) counter runs calibrated on a 1 GiB device copy
 # (profiles/hyx_persist_k_nbody6_group.md). A launch loads and stores every instance once and
 # keeps the Taylor coefficients on chip, so the traffic scales with the instances, not the steps.
-MEASURED_TRAFFIC_B_PER_INSTANCE = {("nbody6", "group"): (31686 * 2.0 + 33163 * 1.0) * 1024 / 65536}
+MEASURED_TRAFFIC_B_PER_INSTANCE = {("nbody6", "group"): (31690 * 2.0 + 32891 * 1.0) * 1024 / 65536}
 
 
 def _cpu_baseline(dec, order, state, pars, t0, t1, n_sample, threads):
