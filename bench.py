@@ -113,6 +113,8 @@ def main():
                     help="two accumulators per long convolution sum instead of one (A/B of the code variant)")
     ap.add_argument("--no-pipeline", action="store_true",
                     help="group form: two LDS exchanges per order instead of one (A/B of the code variant)")
+    ap.add_argument("--sub", type=int, default=2, choices=[1, 2, 4, 8],
+                    help="compact form: lanes per instance (A/B of the code variant)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
@@ -154,8 +156,9 @@ def main():
     order = hy.order_from_tol(wl.tol)
     compact = args.code_form
     t_c = time.perf_counter()
-    system = (_lib.System(dec, order, compact=compact, split_chains=args.split_chains, pipeline=not args.no_pipeline)
-              if (args.split_chains or args.no_pipeline) else get_system(dec, order, compact=compact))
+    system = (_lib.System(dec, order, compact=compact, split_chains=args.split_chains, pipeline=not args.no_pipeline,
+                          sub=args.sub)
+              if (args.split_chains or args.no_pipeline or args.sub != 2) else get_system(dec, order, compact=compact))
     t_compile = time.perf_counter() - t_c
     N = wl.state.shape[1]
     flops = dec.algorithmic_flops(order)["total"]
@@ -233,6 +236,7 @@ def main():
             "code_form": system.form,
             "sum_chains": "split" if args.split_chains else "serial",
             "group_exchanges_per_order": ((2 if args.no_pipeline else 1) if system.form == "group" else None),
+            "lanes_per_instance": {"group": None, "compact": args.sub, "unrolled": 1}[system.form],
             "form": "persistent",
             "algorithmic_flops_per_instance_step": flops,
             "hiprtc_compile_s": round(t_compile, 3),

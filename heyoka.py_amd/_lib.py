@@ -26,6 +26,7 @@ FORMS = {0: "unrolled", 1: "compact", 2: "group"}
 FLAG_GRID = 64
 FLAG_SPLIT_CHAINS = 128
 FLAG_NO_PIPELINE = 256
+FLAG_SUB = {1: 512, 2: 0, 4: 1024, 8: 2048}  # compact form: lanes per instance
 
 
 class prop_res(ct.Structure):
@@ -165,12 +166,12 @@ class System:
 
     def __init__(self, dec, order: int, high_accuracy: bool = False, compile_only: bool = False, compact=None,
                  source_only: bool = False, split_chains: bool = False,
-                 pipeline: bool = True):
+                 pipeline: bool = True, sub: int = 2):
         L = lib()
         words, consts = oplist_args(dec)
         flags = (_flags(high_accuracy, compact) | (FLAG_COMPILE_ONLY if compile_only else 0)
                  | (FLAG_SOURCE_ONLY if source_only else 0) | (FLAG_SPLIT_CHAINS if split_chains else 0)
-                 | (0 if pipeline else FLAG_NO_PIPELINE))
+                 | (0 if pipeline else FLAG_NO_PIPELINE) | FLAG_SUB[int(sub)])
         h = ct.c_void_p()
         check(L.hyx_sys_create(dptr(words, ct.c_uint32), len(words), dptr(consts), len(dec.consts), order, flags,
                                ct.byref(h)))

@@ -595,7 +595,18 @@ struct CompactGen {
     const bool high_acc;
     std::ostringstream o;
 
+    // Lanes per instance: each sums every sub-th term of a convolution, and a butterfly
+    // of __shfl_xor adds the partial sums (every lane gets the same value, because the
+    // two operands of each addition only swap between partner lanes).
+    int sub = 1;
+
     CompactGen(const Sys &s, int p, bool ha) : S(s), P(p), high_acc(ha) {}
+
+    // "for (int q = lo + hyx_sub; cond; q += HYX_SUB)" and its reduction.
+    static std::string loop(int lo, const std::string &cond)
+    {
+        return "for (int q = " + std::to_string(lo) + " + hyx_sub; " + cond + "; q += HYX_SUB)";
+    }
 
     static std::string T(uint32_t node, const std::string &k)
     {
@@ -708,8 +719,8 @@ struct CompactGen {
             else if (s1)
                 o << "        " << c << " = " << vk(A[0], "k") << " * " << v0(A[1]) << ";\n";
             else
-                o << "        { double acc = 0.0; for (int q = 0; q <= k; ++q) acc = __builtin_fma(" << T(idx(A[0]), "q")
-                  << ", " << T(idx(A[1]), "k - q") << ", acc); " << c << " = acc; }\n";
+                o << "        { double acc = 0.0; " << loop(0, "q <= k") << " acc = __builtin_fma(" << T(idx(A[0]), "q")
+                  << ", " << T(idx(A[1]), "k - q") << ", acc); " << c << " = hyx_sub_sum(acc); }\n";
             break;
         }
         case HYX_OP_DIV: {
@@ -717,21 +728,24 @@ struct CompactGen {
             if (scalar(A[1]))
                 o << "        " << c << " = " << a << " * rb" << j << ";\n";
             else
-                o << "        { double acc = 0.0; for (int q = 1; q <= k; ++q) acc = __builtin_fma(" << T(idx(A[1]), "q")
-                  << ", " << T(j, "k - q") << ", acc); " << c << " = (" << a << " - acc) * rb" << j << "; }\n";
+                o << "        { double acc = 0.0; " << loop(1, "q <= k") << " acc = __builtin_fma(" << T(idx(A[1]), "q")
+                  << ", " << T(j, "k - q") << ", acc); acc = hyx_sub_sum(acc); " << c << " = (" << a << " - acc) * rb" << j
+                  << "; }\n";
             break;
         }
         case HYX_OP_SQUARE: {
             const uint32_t a = idx(A[0]);
-            o << "        { double acc = 0.0; for (int q = 0; 2 * q < k; ++q) acc = __builtin_fma(" << T(a, "q") << ", "
-              << T(a, "k - q") << ", acc); acc = acc + acc; if ((k & 1) == 0) { const double m = " << T(a, "k >> 1")
+            o << "        { double acc = 0.0; " << loop(0, "2 * q < k") << " acc = __builtin_fma(" << T(a, "q") << ", "
+              << T(a, "k - q") << ", acc); acc = hyx_sub_sum(acc); acc = acc + acc; if ((k & 1) == 0) { const double m = "
+              << T(a, "k >> 1")
               << "; acc = __builtin_fma(m, m, acc); } " << c << " = acc; }\n";
             break;
         }
         case HYX_OP_SQRT: {
             const std::string a = scalar(A[0]) ? "0.0" : vk(A[0], "k");
-            o << "        { double acc = 0.0; for (int q = 1; 2 * q < k; ++q) acc = __builtin_fma(" << T(j, "q") << ", "
-              << T(j, "k - q") << ", acc); acc = acc + acc; if ((k & 1) == 0) { const double m = " << T(j, "k >> 1")
+            o << "        { double acc = 0.0; " << loop(1, "2 * q < k") << " acc = __builtin_fma(" << T(j, "q") << ", "
+              << T(j, "k - q") << ", acc); acc = hyx_sub_sum(acc); acc = acc + acc; if ((k & 1) == 0) { const double m = "
+              << T(j, "k >> 1")
               << "; acc = __builtin_fma(m, m, acc); } " << c << " = (" << a << " - acc) * rs" << j << "; }\n";
             break;
         }
@@ -739,29 +753,29 @@ struct CompactGen {
             const double al = S.consts[S.aux[j]];
             const uint32_t a = idx(A[0]);
             o << "        { double acc = 0.0; const double ka = (double)k * " << lit(al)
-              << "; for (int q = 0; q < k; ++q) acc = __builtin_fma(__builtin_fma(-(double)q, " << lit(al + 1.0)
-              << ", ka) * " << T(a, "k - q") << ", " << T(j, "q") << ", acc); " << c << " = acc * (ra" << j
+              << "; " << loop(0, "q < k") << " acc = __builtin_fma(__builtin_fma(-(double)q, " << lit(al + 1.0)
+              << ", ka) * " << T(a, "k - q") << ", " << T(j, "q") << ", acc); " << c << " = hyx_sub_sum(acc) * (ra" << j
               << " * hyx_inv[k]); }\n";
             break;
         }
         case HYX_OP_EXP: {
             const uint32_t a = idx(A[0]);
-            o << "        { double acc = 0.0; for (int q = 1; q <= k; ++q) acc = __builtin_fma((double)q * " << T(a, "q")
-              << ", " << T(j, "k - q") << ", acc); " << c << " = acc * hyx_inv[k]; }\n";
+            o << "        { double acc = 0.0; " << loop(1, "q <= k") << " acc = __builtin_fma((double)q * " << T(a, "q")
+              << ", " << T(j, "k - q") << ", acc); " << c << " = hyx_sub_sum(acc) * hyx_inv[k]; }\n";
             break;
         }
         case HYX_OP_LOG: {
             const uint32_t a = idx(A[0]);
-            o << "        { double acc = 0.0; for (int q = 1; q < k; ++q) acc = __builtin_fma((double)q * " << T(j, "q")
-              << ", " << T(a, "k - q") << ", acc); " << c << " = (" << T(a, "k") << " - acc * hyx_inv[k]) * ra" << j
-              << "; }\n";
+            o << "        { double acc = 0.0; " << loop(1, "q < k") << " acc = __builtin_fma((double)q * " << T(j, "q")
+              << ", " << T(a, "k - q") << ", acc); acc = hyx_sub_sum(acc); " << c << " = (" << T(a, "k")
+              << " - acc * hyx_inv[k]) * ra" << j << "; }\n";
             break;
         }
         case HYX_OP_SIN:
         case HYX_OP_COS: {
             const uint32_t a = idx(A[0]), p = S.aux[j];
-            o << "        { double acc = 0.0; for (int q = 1; q <= k; ++q) acc = __builtin_fma((double)q * " << T(a, "q")
-              << ", " << T(p, "k - q") << ", acc); " << c << " = acc * " << (S.op[j] == HYX_OP_SIN ? "" : "-")
+            o << "        { double acc = 0.0; " << loop(1, "q <= k") << " acc = __builtin_fma((double)q * " << T(a, "q")
+              << ", " << T(p, "k - q") << ", acc); " << c << " = hyx_sub_sum(acc) * " << (S.op[j] == HYX_OP_SIN ? "" : "-")
               << "hyx_inv[k]; }\n";
             break;
         }
@@ -777,6 +791,14 @@ struct CompactGen {
         o << "#define HYX_NPAR " << std::max<uint32_t>(S.npar, 1) << "\n";
         o << "#define HYX_P " << P << "\n";
         o << "#define HYX_COMPACT 1\n";
+        o << "#define HYX_SUB " << sub << "\n";
+        if (sub > 1) {
+            o << "__device__ __forceinline__ double hyx_sub_sum(double v)\n{\n"
+                 "#pragma unroll\n    for (int off = 1; off < HYX_SUB; off <<= 1) v += __shfl_xor(v, off);\n"
+                 "    return v;\n}\n";
+        } else {
+            o << "#define hyx_sub_sum(v) (v)\n";
+        }
         o << "#define HYX_TAPE_ROWS " << (S.nnodes * (uint64_t)(P + 1)) << "ull\n";
         o << "typedef unsigned long long hyx_u64;\n";
         o << "__constant__ double hyx_inv[" << (P + 1) << "] = {0.0";
@@ -788,6 +810,10 @@ struct CompactGen {
              "double t0, double max_dt, double &h_out, bool &nf_out, double *__restrict__ tc, hyx_u64 tcs, "
              "double *__restrict__ tape, hyx_u64 tps)\n{\n";
         o << "    (void)pv; (void)t0; (void)tc; (void)tcs;\n";
+        if (sub > 1)
+            o << "    const int hyx_sub = (int)(__lane_id() & (HYX_SUB - 1));\n";
+        else
+            o << "    const int hyx_sub = 0;\n";
         for (uint32_t i = 0; i < n; ++i) o << "    " << T(i, "0") << " = x[" << i << "];\n";
         for (uint32_t j = n; j < S.nnodes; ++j) order0(j);
         o << "    for (int k = 1; k <= HYX_P; ++k) {\n";
@@ -1482,11 +1508,12 @@ bool choose_compact(const Sys &S, int order)
     return (long)history_nodes(S) * (order + 1) > 320;
 }
 
-std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compact, int split_min)
+std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compact, int split_min, int sub)
 {
     if (order < 2) throw std::invalid_argument("the Taylor order must be at least 2");
     if (compact) {
         CompactGen g(S, order, high_acc);
+        g.sub = sub;
         return g.run();
     }
     Gen g(S, order, high_acc);

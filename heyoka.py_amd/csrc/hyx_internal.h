@@ -26,7 +26,8 @@ int parse_oplist(const uint32_t *w, uint32_t n_words, const double *consts, uint
 
 // HIP source of hyx_taylor_step<WTC>() for S at the given order, unrolled or compact.
 // split_min: fma chains of at least this many terms use two accumulators (0: never).
-std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compact, int split_min = 0);
+// sub: lanes per instance of the compact form (1, 2, 4 or 8).
+std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compact, int split_min = 0, int sub = 1);
 // Number of nodes whose coefficient history the recurrences read.
 int history_nodes(const Sys &S);
 // Default choice of code form: compact when the histories exceed the register file.

@@ -16,6 +16,14 @@
 #else
 #define HYX_PROP_WTC false
 #endif
+//
+// The compact prelude may set HYX_SUB > 1: HYX_SUB adjacent lanes then hold the
+// same instance (identical registers, identical stores), each summing every
+// HYX_SUB-th convolution term inside hyx_taylor_step; claims, counts and the tape
+// column are per instance.
+#ifndef HYX_SUB
+#define HYX_SUB 1
+#endif
 
 // ---- one instance's state held in registers ------------------------------------------
 struct hyx_inst {
@@ -151,7 +159,7 @@ __device__ __forceinline__ void hyx_suspend(const hyx_inst &s, const hyx_kargs &
 template <bool WTC>
 __device__ __forceinline__ void hyx_step_body(const hyx_kargs &a)
 {
-    const hyx_u64 i = (hyx_u64)blockIdx.x * blockDim.x + threadIdx.x;
+    const hyx_u64 i = ((hyx_u64)blockIdx.x * blockDim.x + threadIdx.x) / HYX_SUB;
     if (i >= a.N) return;
     hyx_inst s;
     hyx_load(s, a, i);
@@ -187,7 +195,8 @@ extern "C" __global__ void __launch_bounds__(64) hyx_iter_init_k(hyx_kargs a)
 
 extern "C" __global__ void __launch_bounds__(64) hyx_iter_k(hyx_kargs a)
 {
-    const hyx_u64 i = (hyx_u64)blockIdx.x * blockDim.x + threadIdx.x;
+    const hyx_u64 i = ((hyx_u64)blockIdx.x * blockDim.x + threadIdx.x) / HYX_SUB;
+    const bool lead = (threadIdx.x % HYX_SUB) == 0;
     bool still = false;
     if (i < a.N) {
         const unsigned st = a.status[i];
@@ -200,7 +209,7 @@ extern "C" __global__ void __launch_bounds__(64) hyx_iter_k(hyx_kargs a)
             still = !done;
         }
     }
-    const hyx_u64 m = __ballot(still);
+    const hyx_u64 m = __ballot(still && lead);
     if (threadIdx.x == 0 && m != 0) atomicAdd(a.active_count, (unsigned)__popcll(m));
 }
 
@@ -228,22 +237,26 @@ extern "C" __global__ void hyx_clear_u64_k(hyx_u64 *p, unsigned n)
 extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
 {
     const int lane = (int)__lane_id();
-    const hyx_u64 gl = (hyx_u64)blockIdx.x * 64 + lane;
+    const int sub = lane % HYX_SUB;
+    const int lead = lane - sub; // the lane that claims for this instance slot
+    const hyx_u64 gslot = ((hyx_u64)blockIdx.x * 64 + lane) / HYX_SUB; // tape column of the slot
     hyx_inst s;
     hyx_u64 inst = 0, budget = 0;
     bool active = false;
     bool exhausted = false;
     for (;;) {
-        const hyx_u64 idle = __ballot(!active);
+        const hyx_u64 idle = __ballot(!active && sub == 0);
         const unsigned nidle = (unsigned)__popcll(idle);
-        if (!exhausted && nidle != 0 && (nidle >= a.refill_min || nidle == 64u)) {
+        if (!exhausted && nidle != 0 && (nidle * HYX_SUB >= a.refill_min || nidle == 64u / HYX_SUB)) {
             const int leader = __ffsll((long long)idle) - 1;
             hyx_u64 base = 0;
             if (lane == leader) base = atomicAdd(a.work_counter, (hyx_u64)nidle);
             base = __shfl(base, leader);
             if (base + nidle >= a.N) exhausted = true;
             if (!active) {
-                const hyx_u64 pos = base + (hyx_u64)__popcll(idle & __lanemask_lt());
+                // Rank of this slot among the idle slots: idle leaders below its leader lane.
+                const hyx_u64 below = (lead == 0) ? 0ull : ((~0ull) >> (64 - lead));
+                const hyx_u64 pos = base + (hyx_u64)__popcll(idle & below);
                 if (pos < a.N) {
                     inst = a.queue ? (hyx_u64)a.queue[pos] : pos;
                     const unsigned st = a.status[inst];
@@ -258,11 +271,11 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
         if (!__any(active)) break;
         if (active) {
             hyx_i64 res = HYX_TIME_LIMIT;
-            const bool done = hyx_prop_iter(s, a, gl, inst, &res);
+            const bool done = hyx_prop_iter(s, a, gslot, inst, &res);
             budget += 1;
             if (done || budget >= a.iter_budget) {
                 hyx_suspend(s, a, inst, done, res);
-                if (!done) atomicAdd(a.unfinished, 1u);
+                if (!done && sub == 0) atomicAdd(a.unfinished, 1u);
                 active = false;
             }
         }

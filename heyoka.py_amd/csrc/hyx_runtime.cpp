@@ -206,6 +206,7 @@ struct hyx_sys {
     bool compact = false;
     int form = 0;                 // HYX_FORM_UNROLLED / COMPACT / GROUP
     uint32_t inst_per_block = 64; // instances handled by one 64-lane block
+    int sub = 1;                  // lanes per instance (lane forms)
     std::string src;
     std::vector<char> code;
     int device = -1;
@@ -287,8 +288,16 @@ int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint3
             s.inst_per_block = 64u / (uint32_t)gp.G;
             s.src = hyx::generate_group_prelude(s.S, gp, (int)order, ha, split, !(flags & HYX_FLAG_NO_PIPELINE)) + "\n" + k_hyx_kcommon + "\n" + k_hyx_kgroup;
         } else {
-            s.inst_per_block = 64;
-            s.src = hyx::generate_prelude(s.S, (int)order, ha, s.compact, split) + "\n" + k_hyx_kcommon + "\n" + k_hyx_klane;
+            // Compact form: lanes per instance (default 2: measured best on the J2 +
+            // variational ensemble, 1.5x over one lane), each summing every sub-th
+            // convolution term; the unrolled form keeps one lane per instance.
+            int sub = 1;
+            if (s.compact)
+                sub = (flags & HYX_FLAG_SUB1) ? 1 : (flags & HYX_FLAG_SUB4) ? 4 : (flags & HYX_FLAG_SUB8) ? 8 : 2;
+            s.inst_per_block = 64u / (uint32_t)sub;
+            s.sub = sub;
+            s.src = hyx::generate_prelude(s.S, (int)order, ha, s.compact, split, sub) + "\n" + k_hyx_kcommon + "\n" +
+                    k_hyx_klane;
         }
     } catch (const std::exception &e) {
         return fail(HYX_EINVAL, e.what());
@@ -703,7 +712,7 @@ int run_propagation(hyx_batch *b, uint64_t max_steps, int mode, hyx_prop_res *re
         const uint64_t per_cu = b->waves_per_cu ? b->waves_per_cu : (uint64_t)std::max(occ, 1);
         pgrid = std::max<uint64_t>(1, std::min<uint64_t>(per_cu * (uint64_t)b->n_cu, blocks_of(b)));
     }
-    if (int rc = ensure_tape(b, pgrid * 64)) return rc;
+    if (int rc = ensure_tape(b, pgrid * b->sys->inst_per_block)) return rc;
     KArgs a = make_args(b);
     a.max_steps = max_steps;
     if (grid_run) {
@@ -1045,7 +1054,7 @@ int hyx_slots_launch(hyx_batch *b, uint64_t iter_budget)
     HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, b->sys->k_persist, 64, 0));
     const uint64_t per_cu = b->waves_per_cu ? b->waves_per_cu : (uint64_t)std::max(occ, 1);
     const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>(per_cu * (uint64_t)b->n_cu, blocks_of(b)));
-    if (int rc = ensure_tape(b, grid * 64)) return rc;
+    if (int rc = ensure_tape(b, grid * b->sys->inst_per_block)) return rc;
     KArgs a = make_args(b);
     a.iter_budget = iter_budget;
     HIPCHK(clear64(b, b->work, 1));

@@ -62,6 +62,10 @@ typedef struct {
 #define HYX_FLAG_GRID 64u          /* hyx_sys_precompile: the propagate_grid code object */
 #define HYX_FLAG_SPLIT_CHAINS 128u /* two accumulators per convolution sum of >= 8 terms (default: one) */
 #define HYX_FLAG_NO_PIPELINE 256u  /* group form: two LDS exchanges per order even where one suffices */
+/* Compact form: lanes per instance (default 2); each lane sums every n-th convolution term. */
+#define HYX_FLAG_SUB1 512u
+#define HYX_FLAG_SUB4 1024u
+#define HYX_FLAG_SUB8 2048u
 
 /* Code forms (hyx_sys_code_form). */
 #define HYX_FORM_UNROLLED 0 /* one instance per lane, fully unrolled, histories in registers */

@@ -88,12 +88,13 @@ def _case(name):
 @pytest.mark.parametrize("compact", [False, True])
 @pytest.mark.parametrize("name", list(CASES))
 def test_generated_step_matches_oracle(name, compact, tmp_path):
+    # One lane per instance here; the compact form's sub-lane split is emulated below.
     if name in ("nbody6", "j2var") and not compact:
         pytest.skip("large systems use the compact form; their unrolled step takes minutes to compile")
     sys_, st, pars, t0, tol = _case(name)
     dec = hy.taylor_decompose(sys_)
     p = hy.order_from_tol(tol)
-    s = _lib.System(dec, p, compact=compact, source_only=True)
+    s = _lib.System(dec, p, compact=compact, source_only=True, sub=1)
     L = _build(s.source(), tmp_path, f"{name}_{int(compact)}")
     n = dec.n_state
     xin = np.ascontiguousarray(st[:, 0], dtype=np.float64)
@@ -215,3 +216,88 @@ def test_group_step_matches_oracle(name, pipeline, tmp_path):
     assert np.max(np.abs(tcg - tco) / scale) < 1e-11
     assert abs(h[0] - r["h"][0]) <= 1e-13 * abs(r["h"][0])
     np.testing.assert_allclose(xout, r["state"][:, 0], rtol=1e-13, atol=1e-15)
+
+
+SUB_HARNESS = r"""
+#include <cmath>
+#include <cstdint>
+#include <math.h>
+#include <pthread.h>
+#define __device__
+#define __forceinline__ inline
+#define __constant__ static const
+using std::sqrt; using std::pow; using std::exp; using std::log; using std::fabs; using std::isfinite;
+static pthread_barrier_t bar;
+static thread_local int tl_lane;
+static double xch_d[64];
+static unsigned __lane_id() { return (unsigned)tl_lane; }
+static double __shfl_xor(double v, int off) { xch_d[tl_lane] = v; pthread_barrier_wait(&bar); double r = xch_d[tl_lane ^ off]; pthread_barrier_wait(&bar); return r; }
+#include "prelude.inc"
+struct Arg { int lane; const double *x; const double *pv; double t0, mdt; double *xout; double *h; int *nf; double *tc; double *tape; };
+static void *run(void *p)
+{
+    Arg *a = (Arg *)p;
+    tl_lane = a->lane;
+    double x[HYX_N], pv[HYX_NPAR];
+    for (int i = 0; i < HYX_N; ++i) x[i] = a->x[i];
+    for (int i = 0; i < HYX_NPAR; ++i) pv[i] = a->pv[i];
+    double h; bool nf;
+    hyx_taylor_step<true>(x, pv, a->t0, a->mdt, h, nf, a->tc + (size_t)a->lane * HYX_N * (HYX_P + 1), 1, a->tape, 1);
+    for (int i = 0; i < HYX_N; ++i) a->xout[a->lane * HYX_N + i] = x[i];
+    a->h[a->lane] = h; a->nf[a->lane] = nf;
+    return nullptr;
+}
+extern "C" int sub_step_cpu(const double *x, const double *pv, double t0, double mdt, double *xout, double *h, int *nf,
+                            double *tc, double *tape)
+{
+    pthread_barrier_init(&bar, nullptr, HYX_SUB);
+    pthread_t th[HYX_SUB]; Arg args[HYX_SUB];
+    for (int l = 0; l < HYX_SUB; ++l) { args[l] = Arg{l, x, pv, t0, mdt, xout, h, nf, tc, tape}; pthread_create(&th[l], nullptr, run, &args[l]); }
+    for (int l = 0; l < HYX_SUB; ++l) pthread_join(th[l], nullptr);
+    pthread_barrier_destroy(&bar);
+    return HYX_SUB;
+}
+"""
+
+
+@pytest.mark.parametrize("sub", [2, 4, 8])
+@pytest.mark.parametrize("name", ["cr3bp", "j2var"])
+def test_compact_sublanes_match_oracle(name, sub, tmp_path):
+    # The compact form with `sub` lanes per instance, one thread per lane: every
+    # lane must end with the same (bit-identical) step, equal to the oracle's.
+    sys_, st, pars, t0, tol = _case(name)
+    dec = hy.taylor_decompose(sys_)
+    p = hy.order_from_tol(tol)
+    src = _lib.System(dec, p, compact=True, source_only=True, sub=sub).source()
+    assert f"#define HYX_SUB {sub}" in src
+    prelude = src.split("// Hand-written CDNA4 kernels")[0]
+    d = tmp_path / f"{name}_{sub}"
+    d.mkdir()
+    (d / "prelude.inc").write_text(prelude)
+    (d / "s.cpp").write_text(SUB_HARNESS)
+    so = d / "s.so"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-ffp-contract=off", "-shared", "-fPIC", "-pthread", "-o", str(so),
+                    str(d / "s.cpp")], check=True, capture_output=True)
+    L = ct.CDLL(str(so))
+    P = ct.POINTER(ct.c_double)
+    L.sub_step_cpu.argtypes = [P, P, ct.c_double, ct.c_double, P, P, ct.POINTER(ct.c_int), P, P]
+    n = dec.n_state
+    x = np.ascontiguousarray(st[:, 0], dtype=np.float64)
+    pv = np.ascontiguousarray(pars[:, 0] if pars is not None else np.zeros(1), dtype=np.float64)
+    xout = np.zeros(sub * n)
+    h = np.zeros(sub)
+    nf = np.zeros(sub, dtype=np.int32)
+    tc = np.zeros(sub * n * (p + 1))
+    tape = np.zeros(dec.n_nodes * (p + 1))
+    ptr = lambda a: a.ctypes.data_as(ct.POINTER(ct.c_double))
+    L.sub_step_cpu(ptr(x), ptr(pv), t0, np.inf, ptr(xout), ptr(h), nf.ctypes.data_as(ct.POINTER(ct.c_int)), ptr(tc),
+                   ptr(tape))
+    xo = xout.reshape(sub, n)
+    assert np.all(xo == xo[0]) and np.all(h == h[0]) and not nf.any()
+    r = O.step_batch(dec, p, st, pars=pars, t_hi=[t0], want_tc=True)
+    assert abs(h[0] - r["h"][0]) <= 1e-13 * abs(r["h"][0])
+    np.testing.assert_allclose(xo[0], r["state"][:, 0], rtol=1e-13, atol=1e-15)
+    tcg = tc.reshape(sub, n, p + 1)[0]
+    tco = r["tc"][:, :, 0]
+    scale = np.maximum(np.abs(tco), np.max(np.abs(tco), axis=1, keepdims=True) * 1e-3) + 1e-300
+    assert np.max(np.abs(tcg - tco) / scale) < 1e-11
