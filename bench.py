@@ -284,10 +284,12 @@ def main():
         out["plain_gpu"] = {"value": plain_value, "unit": "instance-steps/s", "launches": nl,
                             "persistent_over_plain": value / plain_value}
     if not args.no_cpu:
-        ns = args.cpu_sample or (32768 if args.config == "nbody6" else 2048)
+        ns = args.cpu_sample or {"nbody6": 32768, "j2var": 32768, "cr3bp": 16384}[args.config]
         if args.config in ("nbody6", "j2var"):
-            v, secs = _cpu_baseline(dec, order, wl.state, wl.pars, 0.0, args.dt, ns, args.cpu_threads)
-            sample = (f"{ns} instances of the same ensemble, t in [0, {args.dt}], {args.cpu_threads} threads, "
+            # J2V takes few steps per time unit: its sample spans ten slices for ~5-10 s of CPU work.
+            t_cpu = args.dt * (10 if args.config == "j2var" else 1)
+            v, secs = _cpu_baseline(dec, order, wl.state, wl.pars, 0.0, t_cpu, ns, args.cpu_threads)
+            sample = (f"{ns} instances of the same ensemble, t in [0, {t_cpu}], {args.cpu_threads} threads, "
                       "compiled port of the recurrences")
         else:
             v, secs = _cpu_baseline(dec, order, wl.state, wl.pars, 0.0, wl.t_final, ns, args.cpu_threads)

@@ -258,3 +258,28 @@ def test_grid_backward_and_step_limit(pend):
     reached = np.isfinite(r["out"][:, 0, 0])
     k = int(reached.sum())
     assert 1 < k < 41 and reached[:k].all() and not reached[k:].any()
+
+
+def test_high_accuracy_update_is_compensated(fdp, goldens):
+    # high_accuracy replaces Horner by the compensated sum of x^[k] h^k (Customising the
+    # adaptive integrator.ipynb, cell 12). With the step's own coefficients and h, the
+    # result must be within 2 ulp of the sum taken in extended precision (h^k formed by
+    # repeated multiplication, as the update does).
+    s = goldens["fdp_setup"]
+    rng = np.random.default_rng(11)
+    st = np.vstack([rng.uniform(-1, 1, 64), rng.uniform(-2, 2, 64)])
+    pa = rng.uniform(0.05, 0.2, (1, 64))
+    r = O.step_batch(fdp, 20, st, pars=pa, high_accuracy=True, want_tc=True)
+    tc = r["tc"].astype(np.longdouble)
+    h = r["h"].astype(np.longdouble)
+    exact = np.zeros((2, 64), dtype=np.longdouble)
+    hk = np.ones(64, dtype=np.longdouble)
+    for k in range(21):
+        exact += tc[:, k, :] * hk
+        hk = hk * h
+    ref = exact.astype(np.float64)
+    u = [[ulp_diff(a, b) for a, b in zip(ra, rb)] for ra, rb in zip(r["state"], ref)]
+    assert max(max(x) for x in u) <= 2
+    p = O.step_batch(fdp, 20, st, pars=pa, want_tc=True)
+    assert np.array_equal(p["h"], r["h"])  # same coefficients and step, different summation
+    assert np.max(np.abs(p["state"] - r["state"])) < 1e-14
