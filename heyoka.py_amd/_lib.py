@@ -33,6 +33,11 @@ class prop_res(ct.Structure):
     _fields_ = [("outcome", ct.c_int64), ("min_h", ct.c_double), ("max_h", ct.c_double), ("nsteps", ct.c_uint64)]
 
 
+class tune_res(ct.Structure):
+    _fields_ = [("waves_per_cu", ct.c_uint32), ("refill_min", ct.c_uint32), ("rate", ct.c_double),
+                ("from_cache", ct.c_int)]
+
+
 class HyxError(RuntimeError):
     pass
 
@@ -94,6 +99,7 @@ def lib():
             "hyx_slots_launch": (ct.c_int, [vp, u64]),
             "hyx_slots_wait": (ct.c_int, [vp, P(u32)]),
             "hyx_slots_poll": (ct.c_int, [vp, P(ct.c_int)]),
+            "hyx_autotune": (ct.c_int, [vp, P(d), ct.c_char_p, P(tune_res)]),
             "hyx_slots_results": (ct.c_int, [vp, P(u32), u32, P(u32), P(prop_res)]),
             "hyx_last_error": (ct.c_char_p, []),
             "hyx_device_count": (ct.c_int, []),
@@ -115,7 +121,7 @@ EXPORTED = (
     "hyx_propagate_grid",
     "hyx_batch_last_timing", "hyx_batch_set_persistent_config", "hyx_batch_set_queue", "hyx_batch_set_iter_budget",
     "hyx_batch_sync", "hyx_slots_reset", "hyx_slots_write", "hyx_slots_read", "hyx_slots_arm", "hyx_slots_launch",
-    "hyx_slots_wait", "hyx_slots_poll", "hyx_slots_results",
+    "hyx_slots_wait", "hyx_slots_poll", "hyx_slots_results", "hyx_autotune",
     "hyx_last_error", "hyx_device_count",
 )
 
@@ -265,6 +271,16 @@ class Batch:
         arr = np.frombuffer(res, dtype=np.dtype([("outcome", "<i8"), ("min_h", "<f8"), ("max_h", "<f8"),
                                                  ("nsteps", "<u8")]), count=self.N)
         return arr.copy(), out
+
+    def autotune(self, tf_hi, cache_path=None):
+        """Native autotuner: the fastest persistent (waves_per_cu, refill_min) for the batch's
+        current contents propagated to tf_hi; the batch keeps it. Returns a dict."""
+        th = np.ascontiguousarray(np.broadcast_to(np.asarray(tf_hi, dtype=np.float64), (self.N,)))
+        r = tune_res()
+        check(self._L.hyx_autotune(self.h, dptr(th), None if cache_path is None else str(cache_path).encode(),
+                                   ct.byref(r)))
+        return {"waves_per_cu": r.waves_per_cu, "refill_min": r.refill_min, "rate": r.rate,
+                "from_cache": bool(r.from_cache)}
 
     def last_timing(self):
         ms, n = ct.c_double(), ct.c_uint64()

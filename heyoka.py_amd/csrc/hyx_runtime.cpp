@@ -881,6 +881,70 @@ int hyx_propagate_grid(hyx_batch *b, const double *grid, uint64_t n_pts, const d
     return 0;
 }
 
+int hyx_autotune(hyx_batch *b, const double *tf_hi, const char *cache_path, hyx_tune_res *best)
+{
+    if (!b || !tf_hi) return fail(HYX_EINVAL, "null argument");
+    const uint64_t N = b->N, n = b->sys->S.n, np = std::max<uint32_t>(b->sys->S.npar, 1);
+    const std::string key = std::to_string(fnv1a(b->sys->src)) + "-" + device_arch(b->device) + "-" +
+                            std::to_string(N);
+    // A cached result for this code object, device and batch size.
+    if (cache_path) {
+        std::ifstream in(cache_path);
+        std::string k;
+        unsigned w, r;
+        double rate;
+        while (in >> k >> w >> r >> rate)
+            if (k == key) {
+                b->waves_per_cu = w;
+                b->refill_min = r;
+                if (best) *best = hyx_tune_res{w, r, rate, 1};
+                return 0;
+            }
+    }
+    // Snapshot of the batch's contents: every candidate starts from it, and it is restored at the end.
+    std::vector<double> snap((size_t)(n + np + 2) * N);
+    HIPCHK(hipMemcpyAsync(snap.data(), b->state, n * N * 8, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipMemcpyAsync(snap.data() + n * N, b->pars, np * N * 8, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipMemcpyAsync(snap.data() + (n + np) * N, b->thi, N * 8, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipMemcpyAsync(snap.data() + (n + np + 1) * N, b->tlo, N * 8, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    auto restore = [&]() -> int {
+        HIPCHK(hipMemcpyAsync(b->state, snap.data(), n * N * 8, hipMemcpyHostToDevice, b->stream));
+        HIPCHK(hipMemcpyAsync(b->pars, snap.data() + n * N, np * N * 8, hipMemcpyHostToDevice, b->stream));
+        HIPCHK(hipMemcpyAsync(b->thi, snap.data() + (n + np) * N, N * 8, hipMemcpyHostToDevice, b->stream));
+        HIPCHK(hipMemcpyAsync(b->tlo, snap.data() + (n + np + 1) * N, N * 8, hipMemcpyHostToDevice, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        return 0;
+    };
+    const uint32_t waves[] = {0, 1, 2, 4, 8};
+    const uint32_t refills[] = {1, 8, 32};
+    hyx_tune_res top{0, 1, -1.0, 0};
+    std::vector<hyx_prop_res> res(N);
+    for (uint32_t w : waves)
+        for (uint32_t r : refills) {
+            if (int rc = restore()) return rc;
+            b->waves_per_cu = w;
+            b->refill_min = r;
+            if (int rc = upload(b, b->tfhi, tf_hi, N, 0.0)) return rc;
+            if (int rc = upload(b, b->tflo, nullptr, N, 0.0)) return rc;
+            if (int rc = upload(b, b->mdt, nullptr, N, std::numeric_limits<double>::infinity())) return rc;
+            if (int rc = run_propagation(b, 0, HYX_MODE_PERSISTENT, res.data(), false)) return rc;
+            double steps = 0.0;
+            for (auto &x : res) steps += (double)x.nsteps;
+            const double rate = b->last_ms > 0.0 ? steps / (b->last_ms * 1e-3) : 0.0;
+            if (rate > top.rate) top = hyx_tune_res{w, r, rate, 0};
+        }
+    if (int rc = restore()) return rc;
+    b->waves_per_cu = top.waves_per_cu;
+    b->refill_min = top.refill_min;
+    if (cache_path) {
+        std::ofstream out(cache_path, std::ios::app);
+        out << key << " " << top.waves_per_cu << " " << top.refill_min << " " << top.rate << "\n";
+    }
+    if (best) *best = top;
+    return 0;
+}
+
 int hyx_batch_last_timing(const hyx_batch *b, double *kernel_ms, uint64_t *launches)
 {
     if (!b) return fail(HYX_EINVAL, "null batch");

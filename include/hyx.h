@@ -157,6 +157,21 @@ int hyx_batch_set_persistent_config(hyx_batch *b, uint32_t waves_per_cu, uint32_
  * written back and the kernel relaunched (default 65536). Bounds the duration of
  * every launch; results do not depend on it. */
 int hyx_batch_set_iter_budget(hyx_batch *b, uint64_t iters);
+/* Autotuner (no counterpart in the reference, whose LLVM JIT has no launch shape): times the persistent propagation of the batch's current contents
+ * (state, pars, time as written) to tf_hi [N] for every candidate (waves per CU in
+ * {occupancy, 1, 2, 4, 8} x refill threshold in {1, 8, 32}), restoring the contents
+ * before each run and at the end; keeps the fastest as the batch's persistent config.
+ * With cache_path, a result cached for the same code object, device and batch size is
+ * reused (from_cache = 1), and a new one is appended. Launch parameters never change
+ * results, so tuning is invisible to the arithmetic. */
+typedef struct {
+    uint32_t waves_per_cu;
+    uint32_t refill_min;
+    double rate; /* instance-steps/s of the winning run (kernel time) */
+    int from_cache;
+} hyx_tune_res;
+int hyx_autotune(hyx_batch *b, const double *tf_hi, const char *cache_path, hyx_tune_res *best);
+
 /* Persistent-form instance order (a permutation of 0..N-1), or NULL for identity. */
 int hyx_batch_set_queue(hyx_batch *b, const uint32_t *order);
 int hyx_batch_sync(hyx_batch *b);

@@ -34,3 +34,36 @@ def test_launch_configs_bitwise_identical(case, tmp_path, monkeypatch):
     best, table = AT.autotune(dec, 19, st, pars, tf, waves=(0, 2), refills=(1, 16))
     assert best["rate"] > 0 and len(table) == 4
     assert AT.tuned_config(dec, 19) == (best["waves_per_cu"], best["refill_min"])
+
+
+def test_native_autotune_restores_contents_and_caches(tmp_path):
+    # hyx_autotune times every candidate from the batch's current contents, restores them,
+    # keeps the winner, and reuses a cached result; the propagation after tuning is bitwise
+    # the untuned one.
+    print("[gpu] native autotune", flush=True)
+    from heyoka_amd import _lib
+
+    wl = W.cr3bp(2000, log_t=(0.5, 1.2))
+    dec = hy.taylor_decompose(wl.sys)
+    system = get_system(dec, 19)
+
+    def fresh():
+        b = _lib.Batch(system, wl.state.shape[1])
+        b.write(_lib.FIELD_STATE, wl.state)
+        b.write(_lib.FIELD_PARS, wl.pars)
+        b.write(_lib.FIELD_TIME_HI, np.zeros(wl.state.shape[1]))
+        b.write(_lib.FIELD_TIME_LO, np.zeros(wl.state.shape[1]))
+        return b
+
+    ref = fresh()
+    ref.propagate_until(wl.t_final)
+    ref_state = ref.read(_lib.FIELD_STATE, wl.state.shape)
+    b = fresh()
+    cache = tmp_path / "tune.txt"
+    r1 = b.autotune(wl.t_final, cache)
+    assert r1["rate"] > 0 and not r1["from_cache"]
+    np.testing.assert_array_equal(b.read(_lib.FIELD_STATE, wl.state.shape), wl.state)  # contents restored
+    b.propagate_until(wl.t_final)
+    np.testing.assert_array_equal(b.read(_lib.FIELD_STATE, wl.state.shape), ref_state)
+    r2 = fresh().autotune(wl.t_final, cache)
+    assert r2["from_cache"] and (r2["waves_per_cu"], r2["refill_min"]) == (r1["waves_per_cu"], r1["refill_min"])
