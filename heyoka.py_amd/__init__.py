@@ -13,6 +13,7 @@ from .expr import expression, make_vars, par, time, diff, subs  # noqa: F401
 from .decompose import taylor_decompose, order_from_tol  # noqa: F401
 from . import model  # noqa: F401
 from .var_ode_sys import var_ode_sys, var_args  # noqa: F401
+from .cfunc import cfunc  # noqa: F401
 from .integrator import taylor_adaptive_batch as _tab, taylor_outcome  # noqa: F401
 from .ensemble import (  # noqa: F401
     ensemble_propagate_until,

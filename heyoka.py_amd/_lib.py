@@ -100,6 +100,11 @@ def lib():
             "hyx_slots_wait": (ct.c_int, [vp, P(u32)]),
             "hyx_slots_poll": (ct.c_int, [vp, P(ct.c_int)]),
             "hyx_autotune": (ct.c_int, [vp, P(d), ct.c_char_p, P(tune_res)]),
+            "hyx_cfunc_create": (ct.c_int, [P(u32), u32, P(d), u32, u32, u32, u32, P(vp)]),
+            "hyx_cfunc_source": (ct.c_size_t, [vp, ct.c_char_p, ct.c_size_t]),
+            "hyx_cfunc_eval": (ct.c_int, [vp, P(d), P(d), u32, P(d), u64, P(d)]),
+            "hyx_cfunc_eval_batch": (ct.c_int, [vp, vp, P(d)]),
+            "hyx_cfunc_destroy": (None, [vp]),
             "hyx_slots_results": (ct.c_int, [vp, P(u32), u32, P(u32), P(prop_res)]),
             "hyx_last_error": (ct.c_char_p, []),
             "hyx_device_count": (ct.c_int, []),
@@ -122,6 +127,7 @@ EXPORTED = (
     "hyx_batch_last_timing", "hyx_batch_set_persistent_config", "hyx_batch_set_queue", "hyx_batch_set_iter_budget",
     "hyx_batch_sync", "hyx_slots_reset", "hyx_slots_write", "hyx_slots_read", "hyx_slots_arm", "hyx_slots_launch",
     "hyx_slots_wait", "hyx_slots_poll", "hyx_slots_results", "hyx_autotune",
+    "hyx_cfunc_create", "hyx_cfunc_source", "hyx_cfunc_eval", "hyx_cfunc_eval_batch", "hyx_cfunc_destroy",
     "hyx_last_error", "hyx_device_count",
 )
 

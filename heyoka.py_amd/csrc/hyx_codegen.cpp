@@ -1521,6 +1521,22 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
     return h.str() + o.str();
 }
 
+std::string generate_cfunc_prelude(const Sys &S, uint32_t n_out)
+{
+    if (n_out > S.n) throw std::invalid_argument("a compiled function has at most as many outputs as op-list rows");
+    Gen g(S, 2, false);
+    std::ostringstream &o = g.o;
+    o << "#define HYX_N " << S.n << "\n#define HYX_NPAR " << std::max<uint32_t>(S.npar, 1) << "\n#define HYX_NOUT "
+      << n_out << "\ntypedef unsigned long long hyx_u64;\n";
+    o << "__device__ __forceinline__ void hyx_cfunc_eval(const double (&x)[HYX_N], const double (&pv)[HYX_NPAR], "
+         "double t0, double *__restrict__ out, hyx_u64 os)\n{\n    (void)pv; (void)t0;\n";
+    for (uint32_t i = 0; i < S.n; ++i) o << "    const double " << Gen::cv(i, 0) << " = x[" << i << "];\n";
+    for (uint32_t j = S.n; j < S.nnodes; ++j) g.order0(j);
+    for (uint32_t q = 0; q < n_out; ++q) o << "    out[(hyx_u64)" << q << " * os] = " << g.val(S.rhs[q], 0) << ";\n";
+    o << "}\n";
+    return o.str();
+}
+
 int history_nodes(const Sys &S)
 {
     // Nodes whose coefficient history later orders read: state variables, the

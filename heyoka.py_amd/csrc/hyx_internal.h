@@ -58,6 +58,9 @@ bool plan_groups(const Sys &S, GroupPlan &gp, std::string &why);
 std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int order, bool high_acc, int split_min = 0,
                                    bool pipeline = true);
 
+// Compiled function: the order-0 values of the first n_out right-hand sides (hyx_cfunc_eval).
+std::string generate_cfunc_prelude(const Sys &S, uint32_t n_out);
+
 // Exact hexadecimal double literal.
 std::string lit(double v);
 

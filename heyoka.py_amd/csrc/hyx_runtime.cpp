@@ -31,7 +31,7 @@
 #include "../../include/hyx.h"
 #include "hyx_internal.h"
 
-#include "build/hyx_kernels_embed.inc" // k_hyx_kcommon, k_hyx_klane, k_hyx_kgroup: the kernel templates
+#include "build/hyx_kernels_embed.inc" // k_hyx_kcommon, k_hyx_klane, k_hyx_kgroup, k_hyx_kcfunc: kernel templates
 
 namespace
 {
@@ -403,6 +403,147 @@ int upload(hyx_batch *b, double *dst, const double *src, uint64_t count, double 
 } // namespace
 
 extern "C" {
+
+// ---- compiled functions ------------------------------------------------------------------
+
+struct hyx_cfunc {
+    hyx::Sys S;
+    uint32_t n_in = 0, n_out = 0;
+    std::string src;
+    int device = -1;
+    hipModule_t mod = nullptr;
+    hipFunction_t k = nullptr;
+    hipStream_t stream = nullptr;
+    double *dbuf = nullptr; // device staging for host-array evaluations
+    size_t dcap = 0;        // doubles
+};
+
+int hyx_cfunc_create(const uint32_t *ops, uint32_t n_words, const double *consts, uint32_t n_consts, uint32_t n_in,
+                     uint32_t n_out, uint32_t flags, hyx_cfunc **out)
+{
+    if (!out || !ops) return fail(HYX_EINVAL, "null argument");
+    *out = nullptr;
+    auto *c = new hyx_cfunc;
+    auto bad = [&](int code, const std::string &m) {
+        hyx_cfunc_destroy(c);
+        return fail(code, m);
+    };
+    std::string err;
+    if (hyx::parse_oplist(ops, n_words, consts, n_consts, c->S, err) != 0) return bad(HYX_EINVAL, err);
+    if (n_in > c->S.n || n_out > c->S.n || n_out == 0)
+        return bad(HYX_EINVAL, "invalid input / output counts for the op-list of a compiled function");
+    c->n_in = n_in;
+    c->n_out = n_out;
+    try {
+        c->src = hyx::generate_cfunc_prelude(c->S, n_out) + "\n" + k_hyx_kcfunc;
+    } catch (const std::exception &e) {
+        return bad(HYX_EINVAL, e.what());
+    }
+    if (flags & HYX_FLAG_SOURCE_ONLY) {
+        *out = c;
+        return 0;
+    }
+    std::vector<char> code;
+    if (flags & HYX_FLAG_COMPILE_ONLY) {
+        if (int rc = get_code(c->src, "gfx950", code)) return bad(rc, hyx_last_error());
+        *out = c;
+        return 0;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return bad(HYX_ENODEV, "no HIP device is available; compiled functions run on an MI355X (gfx950)");
+    if (hipGetDevice(&c->device) != hipSuccess) return bad(HYX_EHIP, "hipGetDevice failed");
+    if (int rc = get_code(c->src, device_arch(c->device), code)) return bad(rc, hyx_last_error());
+    if (hipModuleLoadData(&c->mod, code.data()) != hipSuccess) return bad(HYX_EHIP, "hipModuleLoadData failed");
+    if (hipModuleGetFunction(&c->k, c->mod, "hyx_cfunc_k") != hipSuccess)
+        return bad(HYX_EHIP, "hipModuleGetFunction(hyx_cfunc_k) failed");
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bad(HYX_EHIP, "hipStreamCreate failed");
+    *out = c;
+    return 0;
+}
+
+size_t hyx_cfunc_source(const hyx_cfunc *c, char *buf, size_t cap)
+{
+    if (!c) return 0;
+    if (buf && cap) {
+        const size_t n = std::min(cap - 1, c->src.size());
+        std::memcpy(buf, c->src.data(), n);
+        buf[n] = '\0';
+    }
+    return c->src.size();
+}
+
+static int cfunc_launch(hyx_cfunc *c, hipStream_t st, const double *in, uint64_t ld_in, const double *pars,
+                        uint64_t ld_par, uint32_t npar, const double *time, double *outp, uint64_t ld_out, uint64_t n)
+{
+    if (n == 0) return 0;
+    void *params[] = {&in, &ld_in, &pars, &ld_par, &npar, &time, &outp, &ld_out, &n};
+    HIPCHK(hipModuleLaunchKernel(c->k, (unsigned)((n + 63) / 64), 1, 1, 64, 1, 1, 0, st, params, nullptr));
+    return 0;
+}
+
+int hyx_cfunc_eval(hyx_cfunc *c, const double *in, const double *pars, uint32_t npar, const double *time,
+                   uint64_t n_points, double *out)
+{
+    if (!c || !in || !out) return fail(HYX_EINVAL, "null argument");
+    if (!c->mod) return fail(HYX_EINVAL, "the compiled function was created without a device");
+    if (npar > c->S.npar && c->S.npar > 0) npar = c->S.npar;
+    const uint64_t N = n_points, nin = c->S.n, nout = c->n_out;
+    const size_t need = (size_t)(nin + npar + 1 + nout) * N;
+    if (need > c->dcap) {
+        if (c->dbuf) HIPCHK(hipFree(c->dbuf));
+        c->dbuf = nullptr;
+        c->dcap = 0;
+        HIPCHK(hipMalloc((void **)&c->dbuf, std::max<size_t>(need, 1) * 8));
+        c->dcap = need;
+    }
+    double *din = c->dbuf, *dpar = din + nin * N, *dt = dpar + (size_t)npar * N, *dout = dt + N;
+    // Inputs beyond the function's variables (padding rows of the op-list) are zero.
+    HIPCHK(hipMemsetAsync(din, 0, nin * N * 8, c->stream));
+    HIPCHK(hipMemcpyAsync(din, in, (size_t)c->n_in * N * 8, hipMemcpyHostToDevice, c->stream));
+    if (npar && pars) HIPCHK(hipMemcpyAsync(dpar, pars, (size_t)npar * N * 8, hipMemcpyHostToDevice, c->stream));
+    if (time) HIPCHK(hipMemcpyAsync(dt, time, N * 8, hipMemcpyHostToDevice, c->stream));
+    if (int rc = cfunc_launch(c, c->stream, din, N, (npar && pars) ? dpar : nullptr, N, npar, time ? dt : nullptr, dout,
+                              N, N))
+        return rc;
+    HIPCHK(hipMemcpyAsync(out, dout, nout * N * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int hyx_cfunc_eval_batch(hyx_cfunc *c, hyx_batch *b, double *out)
+{
+    if (!c || !b || !out) return fail(HYX_EINVAL, "null argument");
+    if (!c->mod) return fail(HYX_EINVAL, "the compiled function was created without a device");
+    if (c->n_in != b->sys->S.n || c->S.n != b->sys->S.n)
+        return fail(HYX_EINVAL, "the compiled function's variables must be the batch's state variables");
+    const uint64_t N = b->N, nout = c->n_out;
+    const uint32_t npar = std::min<uint32_t>(c->S.npar, b->sys->S.npar);
+    if (c->dcap < nout * N) {
+        if (c->dbuf) HIPCHK(hipFree(c->dbuf));
+        c->dbuf = nullptr;
+        c->dcap = 0;
+        HIPCHK(hipMalloc((void **)&c->dbuf, std::max<size_t>(nout * N, 1) * 8));
+        c->dcap = nout * N;
+    }
+    // On the batch's stream: ordered after its propagations, reading its device-resident state.
+    if (int rc = cfunc_launch(c, b->stream, b->state, N, npar ? b->pars : nullptr, N, npar, b->thi, c->dbuf, N, N))
+        return rc;
+    HIPCHK(hipMemcpyAsync(out, c->dbuf, nout * N * 8, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+void hyx_cfunc_destroy(hyx_cfunc *c)
+{
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->dbuf) hipFree(c->dbuf);
+    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->mod) hipModuleUnload(c->mod);
+    delete c;
+}
 
 const char *hyx_last_error(void) { return g_err.c_str(); }
 

@@ -200,6 +200,25 @@ int hyx_slots_poll(hyx_batch *b, int *finished);
 /* done[c] = 1 once slot c finished (then res[c] holds its propagate result). */
 int hyx_slots_results(hyx_batch *b, const uint32_t *slots, uint32_t count, uint32_t *done, hyx_prop_res *res);
 
+/* Compiled functions: heyoka's cfunc (heyoka/cfunc.cpp:90-121, the call operator
+ * :121-440) evaluated on the GPU, one lane per point. The op-list holds the function's
+ * variables as its rows 0..n_in-1 (any further rows are zero-valued padding) and the
+ * outputs as the right-hand sides of rows 0..n_out-1; numbers, par[] and time as in a
+ * system. */
+typedef struct hyx_cfunc hyx_cfunc;
+int hyx_cfunc_create(const uint32_t *ops, uint32_t n_words, const double *consts, uint32_t n_consts, uint32_t n_in,
+                     uint32_t n_out, uint32_t flags /* HYX_FLAG_COMPILE_ONLY / SOURCE_ONLY */, hyx_cfunc **out);
+size_t hyx_cfunc_source(const hyx_cfunc *c, char *buf, size_t cap);
+/* Host arrays: in [n_in][n_points], pars [npar][n_points] or NULL, time [n_points] or
+ * NULL (0), out [n_out][n_points]. */
+int hyx_cfunc_eval(hyx_cfunc *c, const double *in, const double *pars, uint32_t npar, const double *time,
+                   uint64_t n_points, double *out);
+/* Device diagnostics: evaluate at every instance of a batch from its device-resident
+ * state, parameters and time (the function's variables must be the batch's state
+ * variables, in order), ordered after the batch's pending work; out [n_out][N] host. */
+int hyx_cfunc_eval_batch(hyx_cfunc *c, hyx_batch *b, double *out);
+void hyx_cfunc_destroy(hyx_cfunc *c);
+
 const char *hyx_last_error(void);
 /* Number of HIP devices visible (0 without a GPU). */
 int hyx_device_count(void);
