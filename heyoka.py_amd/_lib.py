@@ -100,6 +100,9 @@ def lib():
             "hyx_slots_wait": (ct.c_int, [vp, P(u32)]),
             "hyx_slots_poll": (ct.c_int, [vp, P(ct.c_int)]),
             "hyx_autotune": (ct.c_int, [vp, P(d), ct.c_char_p, P(tune_res)]),
+            "hyx_slots_snapshot_words": (u64, [vp]),
+            "hyx_slots_snapshot": (ct.c_int, [vp, P(u32), u32, P(u64)]),
+            "hyx_slots_restore": (ct.c_int, [vp, P(u32), u32, P(u64)]),
             "hyx_cfunc_create": (ct.c_int, [P(u32), u32, P(d), u32, u32, u32, u32, P(vp)]),
             "hyx_cfunc_source": (ct.c_size_t, [vp, ct.c_char_p, ct.c_size_t]),
             "hyx_cfunc_eval": (ct.c_int, [vp, P(d), P(d), u32, P(d), u64, P(d)]),
@@ -127,6 +130,7 @@ EXPORTED = (
     "hyx_batch_last_timing", "hyx_batch_set_persistent_config", "hyx_batch_set_queue", "hyx_batch_set_iter_budget",
     "hyx_batch_sync", "hyx_slots_reset", "hyx_slots_write", "hyx_slots_read", "hyx_slots_arm", "hyx_slots_launch",
     "hyx_slots_wait", "hyx_slots_poll", "hyx_slots_results", "hyx_autotune",
+    "hyx_slots_snapshot_words", "hyx_slots_snapshot", "hyx_slots_restore",
     "hyx_cfunc_create", "hyx_cfunc_source", "hyx_cfunc_eval", "hyx_cfunc_eval_batch", "hyx_cfunc_destroy",
     "hyx_last_error", "hyx_device_count",
 )
@@ -340,6 +344,19 @@ class Batch:
         check(self._L.hyx_slots_wait(self.h, ct.byref(n)))
         return n.value
 
+    def slots_snapshot(self, slots) -> np.ndarray:
+        """Every word the propagation of these slots depends on (host copy, uint64)."""
+        sl = np.ascontiguousarray(slots, dtype=np.uint32)
+        w = int(self._L.hyx_slots_snapshot_words(self.h))
+        out = np.empty(w * len(sl), dtype=np.uint64)
+        check(self._L.hyx_slots_snapshot(self.h, dptr(sl, ct.c_uint32), len(sl), dptr(out, ct.c_uint64)))
+        return out
+
+    def slots_restore(self, slots, blob):
+        sl = np.ascontiguousarray(slots, dtype=np.uint32)
+        b = np.ascontiguousarray(blob, dtype=np.uint64)
+        check(self._L.hyx_slots_restore(self.h, dptr(sl, ct.c_uint32), len(sl), dptr(b, ct.c_uint64)))
+
     def slots_poll(self) -> bool:
         """True when the running epoch (if any) has finished; never blocks."""
         f = ct.c_int()
@@ -355,8 +372,11 @@ class Batch:
                                                  ("nsteps", "<u8")]), count=len(sl)).copy()
         return done.astype(bool), arr
 
-    def __del__(self):
+    def close(self):
+        """Free the batch's device memory now (also done when the object is collected)."""
         h = getattr(self, "h", None)
         if h is not None and h.value:
             self._L.hyx_batch_destroy(h)
             self.h = None
+
+    __del__ = close

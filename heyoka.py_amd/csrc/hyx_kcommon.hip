@@ -125,6 +125,12 @@ extern "C" __global__ void hyx_gather32_k(const unsigned *srcm, unsigned *dst, c
     if (c < count) dst[c] = srcm[idx[c]];
 }
 
+extern "C" __global__ void hyx_scatter32_k(unsigned *dst, const unsigned *src, const unsigned *idx, unsigned count)
+{
+    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < count) dst[idx[c]] = src[c];
+}
+
 extern "C" __global__ void hyx_fill_u32_k(unsigned *p, unsigned v, hyx_u64 n)
 {
     const hyx_u64 i = (hyx_u64)blockIdx.x * blockDim.x + threadIdx.x;

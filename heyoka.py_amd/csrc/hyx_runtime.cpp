@@ -213,6 +213,7 @@ struct hyx_sys {
     hipModule_t mod = nullptr;
     hipFunction_t k_step = nullptr, k_step_tc = nullptr, k_init = nullptr, k_iter = nullptr, k_persist = nullptr,
                   k_clr32 = nullptr, k_clr64 = nullptr, k_scatter = nullptr, k_gather = nullptr, k_gather32 = nullptr,
+                  k_scatter32 = nullptr,
                   k_fill32 = nullptr, k_fill64 = nullptr, k_arm = nullptr;
     int vgprs = -1, scratch = -1;
     // propagate_grid: the same source built with HYX_GRID, loaded on first use.
@@ -615,7 +616,7 @@ int hyx_sys_create(const uint32_t *ops, uint32_t n_words, const double *consts, 
     } fns[] = {{&s->k_step, "hyx_step_k"},       {&s->k_step_tc, "hyx_step_tc_k"}, {&s->k_init, "hyx_iter_init_k"},
                {&s->k_iter, "hyx_iter_k"},       {&s->k_persist, "hyx_persist_k"}, {&s->k_clr32, "hyx_clear_u32_k"},
                {&s->k_clr64, "hyx_clear_u64_k"}, {&s->k_scatter, "hyx_scatter_k"}, {&s->k_gather, "hyx_gather_k"},
-               {&s->k_gather32, "hyx_gather32_k"}, {&s->k_fill32, "hyx_fill_u32_k"}, {&s->k_fill64, "hyx_fill_u64_k"},
+               {&s->k_gather32, "hyx_gather32_k"}, {&s->k_scatter32, "hyx_scatter32_k"}, {&s->k_fill32, "hyx_fill_u32_k"}, {&s->k_fill64, "hyx_fill_u64_k"},
                {&s->k_arm, "hyx_slots_arm_k"}};
     for (auto &f : fns) {
         e = hipModuleGetFunction(f.f, s->mod, f.name);
@@ -1225,6 +1226,78 @@ int hyx_slots_read(hyx_batch *b, int field, const uint32_t *slots, uint32_t coun
     const unsigned rows = (unsigned)(total / b->N);
     if (rows == 0) return 0;
     return gather64(b, p, rows, slots, count, dst);
+}
+
+// Per-slot columns of a snapshot, in blob order: state, pars, then the propagation
+// bookkeeping (time, final time, max_delta_t, min_h, max_h, nsteps, iterations,
+// outcome), then the 32-bit status widened to one word.
+static std::vector<std::pair<void *, unsigned>> snapshot_cols(hyx_batch *b)
+{
+    return {{b->state, b->sys->S.n}, {b->pars, b->sys->S.npar}, {b->thi, 1},  {b->tlo, 1},
+            {b->tfhi, 1},            {b->tflo, 1},              {b->mdt, 1},  {b->minh, 1},
+            {b->maxh, 1},            {b->nsteps, 1},            {b->iters, 1}, {b->oc, 1}};
+}
+
+uint64_t hyx_slots_snapshot_words(const hyx_batch *b)
+{
+    return b ? (uint64_t)b->sys->S.n + b->sys->S.npar + 11 : 0;
+}
+
+int hyx_slots_snapshot(hyx_batch *b, const uint32_t *slots, uint32_t count, uint64_t *dst)
+{
+    if (!b || (!dst && count)) return fail(HYX_EINVAL, "null argument");
+    if (int rc = check_slots(b, slots, count)) return rc;
+    if (count == 0) return 0;
+    uint64_t *o = dst;
+    for (auto &c : snapshot_cols(b)) {
+        if (c.second == 0) continue;
+        if (int rc = gather64(b, c.first, c.second, slots, count, o)) return rc;
+        o += (size_t)c.second * count;
+    }
+    // status: 32-bit column, gathered then widened.
+    if (int rc = stage_reserve(b, (size_t)count * 8, count)) return rc;
+    HIPCHK(hipMemcpyAsync(b->stage_idx, slots, (size_t)count * 4, hipMemcpyHostToDevice, b->stream));
+    unsigned *st = b->status, *sg = (unsigned *)b->stage, *idx = b->stage_idx;
+    void *params[] = {&st, &sg, &idx, &count};
+    HIPCHK(launch_raw(b, b->sys->k_gather32, count, params));
+    std::vector<unsigned> s32(count);
+    HIPCHK(hipMemcpyAsync(s32.data(), b->stage, (size_t)count * 4, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (uint32_t c = 0; c < count; ++c) o[c] = s32[c];
+    return 0;
+}
+
+int hyx_slots_restore(hyx_batch *b, const uint32_t *slots, uint32_t count, const uint64_t *src)
+{
+    if (!b || (!src && count)) return fail(HYX_EINVAL, "null argument");
+    if (int rc = check_slots(b, slots, count)) return rc;
+    if (count == 0) return 0;
+    const uint64_t *in = src;
+    const uint64_t words = hyx_slots_snapshot_words(b);
+    if (int rc = stage_reserve(b, (size_t)words * count * 8, count)) return rc;
+    HIPCHK(hipMemcpyAsync(b->stage_idx, slots, (size_t)count * 4, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->stage, src, (size_t)words * count * 8, hipMemcpyHostToDevice, b->stream));
+    uint64_t off = 0;
+    for (auto &c : snapshot_cols(b)) {
+        if (c.second == 0) continue;
+        void *dstp = c.first;
+        uint64_t ld = b->N;
+        unsigned rows = c.second;
+        void *srcp = (char *)b->stage + off * 8;
+        void *params[] = {&dstp, &ld, &rows, &srcp, &b->stage_idx, &count};
+        HIPCHK(launch_raw(b, b->sys->k_scatter, (uint64_t)rows * count, params));
+        off += (uint64_t)rows * count;
+    }
+    std::vector<unsigned> s32(count);
+    for (uint32_t c = 0; c < count; ++c) s32[c] = (unsigned)in[off + c];
+    // The 32-bit status goes through a second staging area after the 64-bit columns.
+    unsigned *sg = (unsigned *)((char *)b->stage + off * 8);
+    HIPCHK(hipMemcpyAsync(sg, s32.data(), (size_t)count * 4, hipMemcpyHostToDevice, b->stream));
+    unsigned *st = b->status, *idx = b->stage_idx;
+    void *params[] = {&st, &sg, &idx, &count};
+    HIPCHK(launch_raw(b, b->sys->k_scatter32, count, params));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
 }
 
 int hyx_slots_arm(hyx_batch *b, const uint32_t *slots, uint32_t count, const double *tf_hi, const double *tf_lo,

@@ -18,7 +18,11 @@ a loop of bounded epochs:
 
 A request that arrives while others are running is admitted at the next epoch
 boundary, as soon as slots are free; running slots stay on the device between
-epochs. A slot's arithmetic does not depend on the slot, the pool's other
+epochs. When a new system's pool does not fit the HBM budget, the pool holding the
+youngest work is parked: its running requests are snapshotted to host memory
+(every word their propagation depends on, hyx_slots_snapshot) and its device memory
+is freed; parked requests are restored (hyx_slots_restore, into any free slots) as
+soon as their pool fits again, before new admissions, and continue bit for bit. A slot's arithmetic does not depend on the slot, the pool's other
 occupants or the epoch boundaries, so a result is bit-for-bit the same alone
 and in a mix (tests/test_gpu_serving.py).
 """
@@ -86,10 +90,14 @@ class _Pool:
 class Server:
     """Continuous-batching server over one GPU."""
 
-    def __init__(self, capacity: int = 16384, epoch_iters: int = 512, hbm_budget_bytes: int = 32 << 30):
+    def __init__(self, capacity: int = 16384, epoch_iters: int = 512, hbm_budget_bytes: int = 32 << 30,
+                 evict: bool = True):
         self.capacity = int(capacity)
         self.epoch_iters = int(epoch_iters)
         self.hbm_budget = int(hbm_budget_bytes)
+        self.evict = bool(evict)
+        self.parked: dict = {}  # system key -> [(request, slot snapshot)]
+        self.evictions = 0
         self.pools: dict = {}
         self.waiting: deque = deque()
         self.done: list[Request] = []
@@ -132,19 +140,81 @@ class Server:
         return req
 
     # ---- scheduling ---------------------------------------------------------------------
-    def _pool(self, key):
+    def _used(self):
+        return sum(q.bytes() for q in self.pools.values() if isinstance(q, _Pool))
+
+    def _pool(self, key, evict=False):
+        """The pool of a system, created if the HBM budget allows; with evict=True another
+        pool may be parked (its running requests snapshotted to host memory and its device
+        memory freed) to make room."""
         p = self.pools[key]
         if isinstance(p, _Pool):
             return p
         system, dec = p
         cand = _Pool.__new__(_Pool)
         cand.capacity, cand.dec, cand.npar = self.capacity, dec, system.npar
-        used = sum(q.bytes() for q in self.pools.values() if isinstance(q, _Pool))
-        if used + cand.bytes() > self.hbm_budget:
-            return None
+        if self._used() + cand.bytes() > self.hbm_budget:
+            if not evict:
+                return None
+            # Free idle pools first, then park the pool holding the youngest work (its
+            # earliest request arrived last), so the longest-waiting requests keep their slots.
+            victims = [(0 if not q.running else 1, -min(r.arrival for r in q.running) if q.running else 0.0, k)
+                       for k, q in self.pools.items() if isinstance(q, _Pool) and k != key and not q.in_epoch]
+            victims.sort()
+            for busy, _, k in victims:
+                if self._used() + cand.bytes() <= self.hbm_budget:
+                    break
+                if busy:
+                    self._park(k)
+                else:
+                    q = self.pools[k]
+                    self.pools[k] = (q.system, q.dec)
+                    q.batch.close()
+            if self._used() + cand.bytes() > self.hbm_budget:
+                return None
         pool = _Pool(system, dec, self.capacity)
         self.pools[key] = pool
         return pool
+
+    def _park(self, key):
+        """Snapshot every running request of a pool (hyx_slots_snapshot) and free the pool."""
+        pool = self.pools[key]
+        for req in pool.running:
+            self.parked.setdefault(key, []).append((req, pool.batch.slots_snapshot(req.slots)))
+            req.slots = None
+        self.pools[key] = (pool.system, pool.dec)
+        pool.batch.close()
+        self.evictions += 1
+
+    def _unpark(self):
+        """Restore parked requests into their system's pool when it can be (re)created."""
+        for key in list(self.parked):
+            pool = self._pool(key)
+            if pool is None or pool.in_epoch:
+                continue
+            items = self.parked[key]
+            keep = []
+            for req, blob in items:
+                if len(pool.free) < req.n:
+                    keep.append((req, blob))
+                    continue
+                slots = np.array([pool.free.popleft() for _ in range(req.n)], dtype=np.uint32)
+                pool.batch.slots_restore(slots, blob)
+                req.slots = slots
+                pool.running.append(req)
+            if keep:
+                self.parked[key] = keep
+            else:
+                del self.parked[key]
+
+    def _release_idle(self):
+        """Free pools with nothing running while parked work waits for HBM."""
+        if not self.parked:
+            return
+        for k, q in list(self.pools.items()):
+            if isinstance(q, _Pool) and not q.running and not q.in_epoch and k not in self.parked:
+                self.pools[k] = (q.system, q.dec)
+                q.batch.close()
 
     def _admit(self, now):
         with self._lock:
@@ -156,7 +226,7 @@ class Server:
             if req.arrival > now or req.key in blocked:
                 keep.append(req)
                 continue
-            pool = self._pool(req.key)
+            pool = self._pool(req.key, evict=self.evict and not self.parked)
             if pool is None or pool.in_epoch or len(pool.free) < req.n:
                 # Admission waits for the pool's epoch boundary (its slots are written and
                 # armed between epochs), keeping first-come-first-served order within a system.
@@ -200,6 +270,8 @@ class Server:
         whose epochs are long (a large system) does not hold back the others' harvests
         and admissions. Returns the number of requests still waiting or running."""
         now = _time.perf_counter()
+        self._release_idle()
+        self._unpark()
         self._admit(now)
         pools = [p for p in self.pools.values() if isinstance(p, _Pool)]
         for p in pools:
@@ -219,7 +291,8 @@ class Server:
             self._harvest(p)
         with self._lock:
             nwait = len(self.waiting)
-        return nwait + sum(len(p.running) for p in pools)
+        nparked = sum(len(v) for v in self.parked.values())
+        return nwait + nparked + sum(len(p.running) for p in self.pools.values() if isinstance(p, _Pool))
 
     def run_until_idle(self, max_rounds: int = 10 ** 9):
         for _ in range(max_rounds):

@@ -197,6 +197,15 @@ int hyx_slots_wait(hyx_batch *b, uint32_t *n_active);
 /* Non-blocking: *finished = 1 when the epoch's kernels are complete (or none runs),
  * so a server can harvest each batch as soon as its own epoch ends. */
 int hyx_slots_poll(hyx_batch *b, int *finished);
+/* Snapshot / restore of slots (no counterpart in the reference: serving eviction under
+ * HBM pressure; the continuation contract is that of the reference's pickling,
+ * heyoka/pickle_wrappers.hpp:28-74): every per-instance word a propagation depends on
+ * (state, pars, time, final time, max_delta_t, min_h / max_h, nsteps, iterations,
+ * outcome, status), hyx_slots_snapshot_words() words per slot, column-major per field.
+ * Restoring into any free slots of a batch of the same system continues bit for bit. */
+uint64_t hyx_slots_snapshot_words(const hyx_batch *b);
+int hyx_slots_snapshot(hyx_batch *b, const uint32_t *slots, uint32_t count, uint64_t *dst);
+int hyx_slots_restore(hyx_batch *b, const uint32_t *slots, uint32_t count, const uint64_t *src);
 /* done[c] = 1 once slot c finished (then res[c] holds its propagate result). */
 int hyx_slots_results(hyx_batch *b, const uint32_t *slots, uint32_t count, uint32_t *done, hyx_prop_res *res);
 

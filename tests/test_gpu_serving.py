@@ -79,3 +79,34 @@ def test_capacity_pressure_and_validation():
     assert all(h.finished is not None for h in hs)
     with pytest.raises(ValueError):
         srv.submit(hy.model.pendulum(), np.zeros((2, 25)), 1.0)
+
+
+def test_eviction_under_hbm_pressure_continues_bitwise():
+    # An HBM budget that holds one pool at a time: admitting a request of a second system
+    # parks the first pool mid-flight (slot snapshots to host, device memory freed); it is
+    # restored into fresh slots once the second pool drains, and every result is still bit
+    # for bit the request's propagate_until.
+    print("[gpu] serving: eviction and restore", flush=True)
+    reqs = _requests()
+    cr_req, pend_req, nb_req = reqs[1], reqs[0], reqs[2]
+    cap = 32
+    one_pool = cap * 8 * (4 + 1 + 16)  # the largest of the pools involved (CR3BP: 4 states, 1 par)
+    big = cap * 8 * (18 + 1 + 16)
+    srv = Server(capacity=cap, epoch_iters=8, hbm_budget_bytes=max(one_pool, big) + 1)
+    h1 = srv.submit(cr_req["sys"], cr_req["state"], cr_req["t_final"], pars=cr_req["pars"], tol=1e-15)
+    for _ in range(4):
+        srv.step()
+    while any(p.in_epoch for p in srv.pools.values() if hasattr(p, "in_epoch")):
+        srv.step()
+    assert h1.finished is None  # still running when the other systems arrive
+    h2 = srv.submit(pend_req["sys"], pend_req["state"], pend_req["t_final"], tol=1e-15)
+    h3 = srv.submit(nb_req["sys"], nb_req["state"], nb_req["t_final"], tol=1e-15)
+    srv.run_until_idle()
+    assert srv.evictions >= 1
+    for r, h in ((cr_req, h1), (pend_req, h2), (nb_req, h3)):
+        assert h.finished is not None
+        ta = hy.taylor_adaptive_batch(r["sys"], r["state"], pars=r["pars"], tol=1e-15)
+        ta.propagate_until(r["t_final"])
+        assert np.array_equal(ta.state, h.result_state)
+        assert [int(p[3]) for p in ta.propagate_res] == h.result["nsteps"].tolist()
+        assert [int(p[0]) for p in ta.propagate_res] == h.result["outcome"].tolist()
