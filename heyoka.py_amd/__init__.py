@@ -2,7 +2,8 @@
 
 Drop-in names for heyoka.py's batch path (heyoka/__init__.py:78-86, 251-336):
 ``taylor_adaptive_batch``, ``ensemble_propagate_until/for[_batch]``, the
-expression system (``make_vars``, ``par``, ``time``, ``sin``, ...),
+expression system (``make_vars``, ``par``, ``time``, ``sin``, ...), the batch
+events (``t_event_batch``, ``nt_event_batch``, ``event_direction``),
 ``var_ode_sys`` / ``var_args``, ``taylor_outcome`` and ``model``. The
 integration runs in hand-written HIP kernels for gfx950 behind the C-ABI of
 include/hyx.h (libhyx.so, loaded through ``_lib``).
@@ -15,6 +16,7 @@ from . import model  # noqa: F401
 from .var_ode_sys import var_ode_sys, var_args  # noqa: F401
 from .cfunc import cfunc  # noqa: F401
 from .integrator import taylor_adaptive_batch as _tab, taylor_outcome  # noqa: F401
+from .events import event_direction, t_event_batch, nt_event_batch  # noqa: F401
 from .ensemble import (  # noqa: F401
     ensemble_propagate_until,
     ensemble_propagate_for,

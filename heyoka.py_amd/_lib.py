@@ -38,6 +38,12 @@ class tune_res(ct.Structure):
                 ("from_cache", ct.c_int)]
 
 
+class events_desc(ct.Structure):
+    _fields_ = [("ops", ct.POINTER(ct.c_uint32)), ("n_words", ct.c_uint32), ("consts", ct.POINTER(ct.c_double)),
+                ("n_consts", ct.c_uint32), ("n_te", ct.c_uint32), ("n_nt", ct.c_uint32),
+                ("direction", ct.POINTER(ct.c_int32)), ("cooldown", ct.POINTER(ct.c_double)), ("tol", ct.c_double)]
+
+
 class HyxError(RuntimeError):
     pass
 
@@ -109,6 +115,11 @@ def lib():
             "hyx_cfunc_eval_batch": (ct.c_int, [vp, vp, P(d)]),
             "hyx_cfunc_destroy": (None, [vp]),
             "hyx_slots_results": (ct.c_int, [vp, P(u32), u32, P(u32), P(prop_res)]),
+            "hyx_sys_create_events": (ct.c_int, [P(u32), u32, P(d), u32, u32, u32, P(events_desc), P(vp)]),
+            "hyx_sys_n_events": (ct.c_int, [vp, P(u32), P(u32)]),
+            "hyx_events_read": (ct.c_int, [vp, P(ct.c_int32), P(ct.c_int32), P(u32), P(d), P(ct.c_int32), P(u32)]),
+            "hyx_cooldowns_read": (ct.c_int, [vp, P(d)]),
+            "hyx_cooldowns_write": (ct.c_int, [vp, P(d)]),
             "hyx_last_error": (ct.c_char_p, []),
             "hyx_device_count": (ct.c_int, []),
         }
@@ -132,6 +143,7 @@ EXPORTED = (
     "hyx_slots_wait", "hyx_slots_poll", "hyx_slots_results", "hyx_autotune",
     "hyx_slots_snapshot_words", "hyx_slots_snapshot", "hyx_slots_restore",
     "hyx_cfunc_create", "hyx_cfunc_source", "hyx_cfunc_eval", "hyx_cfunc_eval_batch", "hyx_cfunc_destroy",
+    "hyx_sys_create_events", "hyx_sys_n_events", "hyx_events_read", "hyx_cooldowns_read", "hyx_cooldowns_write",
     "hyx_last_error", "hyx_device_count",
 )
 
@@ -177,20 +189,45 @@ def device_count() -> int:
     return int(lib().hyx_device_count())
 
 
+class EventSpec:
+    """Events of a system: the event op-list (rows = the system's state variables, then
+    padding; right-hand sides = event functions, terminal ones first), directions,
+    terminal cooldowns (< 0: automatic) and the integrator tolerance."""
+
+    def __init__(self, dec, n_te: int, n_nt: int, directions, cooldowns, tol: float):
+        self.dec, self.n_te, self.n_nt, self.tol = dec, int(n_te), int(n_nt), float(tol)
+        self.directions = np.ascontiguousarray(directions, dtype=np.int32)
+        self.cooldowns = np.ascontiguousarray(cooldowns if len(cooldowns) else [0.0], dtype=np.float64)
+
+    def key(self):
+        w, c = oplist_args(self.dec)
+        return (w.tobytes(), c.tobytes(), self.n_te, self.n_nt, self.directions.tobytes(), self.cooldowns.tobytes(),
+                self.tol)
+
+
 class System:
     """Owner of a hyx_sys handle."""
 
     def __init__(self, dec, order: int, high_accuracy: bool = False, compile_only: bool = False, compact=None,
                  source_only: bool = False, split_chains: bool = False,
-                 pipeline: bool = True, sub: int = 2):
+                 pipeline: bool = True, sub: int = 2, events: EventSpec | None = None):
         L = lib()
         words, consts = oplist_args(dec)
         flags = (_flags(high_accuracy, compact) | (FLAG_COMPILE_ONLY if compile_only else 0)
                  | (FLAG_SOURCE_ONLY if source_only else 0) | (FLAG_SPLIT_CHAINS if split_chains else 0)
                  | (0 if pipeline else FLAG_NO_PIPELINE) | FLAG_SUB[int(sub)])
         h = ct.c_void_p()
-        check(L.hyx_sys_create(dptr(words, ct.c_uint32), len(words), dptr(consts), len(dec.consts), order, flags,
-                               ct.byref(h)))
+        if events is None:
+            check(L.hyx_sys_create(dptr(words, ct.c_uint32), len(words), dptr(consts), len(dec.consts), order, flags,
+                                   ct.byref(h)))
+        else:
+            ew, ec = oplist_args(events.dec)
+            self._ev_keep = (ew, ec, events.directions, events.cooldowns)
+            ed = events_desc(dptr(ew, ct.c_uint32), len(ew), dptr(ec), len(events.dec.consts), events.n_te,
+                             events.n_nt, dptr(events.directions, ct.c_int32), dptr(events.cooldowns), events.tol)
+            check(L.hyx_sys_create_events(dptr(words, ct.c_uint32), len(words), dptr(consts), len(dec.consts), order,
+                                          flags, ct.byref(ed), ct.byref(h)))
+        self.events = events
         self.h = h
         self.order = order
         self.n = dec.n_state
@@ -249,6 +286,32 @@ class Batch:
         md = None if max_dt is None else np.ascontiguousarray(max_dt, dtype=np.float64)
         check(self._L.hyx_step(self.h, dptr(md), int(write_tc), dptr(oc, ct.c_int64), dptr(h)))
         return oc, h
+
+    def events_read(self):
+        """The last step's events: (te [N] (-1: none), te_dsgn [N], nt_count [N],
+        nt_tau [cap, N], nt_code [cap, N])."""
+        te = np.empty(self.N, dtype=np.int32)
+        ds = np.empty(self.N, dtype=np.int32)
+        cnt = np.empty(self.N, dtype=np.uint32)
+        cap = ct.c_uint32()
+        check(self._L.hyx_events_read(self.h, None, None, None, None, None, ct.byref(cap)))
+        tau = np.empty((cap.value, self.N), dtype=np.float64)
+        code = np.empty((cap.value, self.N), dtype=np.int32)
+        check(self._L.hyx_events_read(self.h, dptr(te, ct.c_int32), dptr(ds, ct.c_int32), dptr(cnt, ct.c_uint32),
+                                      dptr(tau), dptr(code, ct.c_int32), ct.byref(cap)))
+        return te, ds, cnt, tau, code
+
+    def cooldowns(self) -> np.ndarray:
+        """[n_te, 3, N]: trigger time hi, lo and length (< 0: no cooldown)."""
+        out = np.empty((self.sys.events.n_te, 3, self.N), dtype=np.float64)
+        if out.size:
+            check(self._L.hyx_cooldowns_read(self.h, dptr(out)))
+        return out
+
+    def set_cooldowns(self, cd):
+        a = np.ascontiguousarray(cd, dtype=np.float64)
+        if a.size:
+            check(self._L.hyx_cooldowns_write(self.h, dptr(a)))
 
     def read_tc(self) -> np.ndarray:
         out = np.empty((self.sys.n, self.sys.order + 1, self.N), dtype=np.float64)

@@ -35,6 +35,12 @@ namespace hyx
 
 static const int INF_DEG = 1 << 20;
 
+// Systems with events: the event functions' coefficients of the last two orders
+// join the state's in the step-size norms (heyoka includes its event functions in
+// the Taylor jet whose last two orders set the step size; the step counts and
+// step sizes of doc/notebooks/Event detection.ipynb, cell 21, follow from it).
+static const char *EV_NORMS_HOOK = "#ifdef HYX_EVENTS\n    if constexpr (WTC) hyx_ev_norms(tc, tcs, pv, t0, mo, mom1);\n#endif\n";
+
 void compute_degrees(Sys &S)
 {
     // Highest order with a possibly non-zero coefficient; state variables are full series.
@@ -597,6 +603,7 @@ struct Gen {
             o << "    ab = fabs(" << nv(i, P) << "); mo = (ab > mo) ? ab : mo;\n";
             o << "    ab = fabs(" << nv(i, P - 1) << "); mom1 = (ab > mom1) ? ab : mom1;\n";
         }
+        o << EV_NORMS_HOOK;
         o << "    const double num = (mx <= 1.0) ? 1.0 : mx;\n";
         o << "    const double rho_o = pow(num / mo, " << lit(1.0 / P) << ");\n";
         o << "    const double rho_om1 = pow(num / mom1, " << lit(1.0 / (P - 1)) << ");\n";
@@ -882,6 +889,7 @@ struct CompactGen {
              "        ab = fabs(T(i, HYX_P)); mo = (ab > mo) ? ab : mo;\n"
              "        ab = fabs(T(i, HYX_P - 1)); mom1 = (ab > mom1) ? ab : mom1;\n"
              "    }\n";
+        o << EV_NORMS_HOOK;
         o << "    const double num = (mx <= 1.0) ? 1.0 : mx;\n";
         o << "    const double rho_o = pow(num / mo, " << lit(1.0 / P) << ");\n";
         o << "    const double rho_om1 = pow(num / mom1, " << lit(1.0 / (P - 1)) << ");\n";
@@ -1535,6 +1543,82 @@ std::string generate_cfunc_prelude(const Sys &S, uint32_t n_out)
     for (uint32_t q = 0; q < n_out; ++q) o << "    out[(hyx_u64)" << q << " * os] = " << g.val(S.rhs[q], 0) << ";\n";
     o << "}\n";
     return o.str();
+}
+
+std::string generate_events_prelude(const Sys &E, uint32_t n_sys, int P, const EventSpec &ev, bool high_acc)
+{
+    const uint32_t nev = (uint32_t)ev.dir.size();
+    if (nev == 0 || ev.n_te > nev) throw std::invalid_argument("invalid event specification");
+    if (nev > E.n) throw std::invalid_argument("the event op-list has fewer rows than events");
+    if (E.n < n_sys) throw std::invalid_argument("the event op-list has fewer rows than the system has state variables");
+    CompactGen g(E, P, high_acc);
+    std::ostringstream &o = g.o;
+    const uint32_t nrows = E.n, nodes = E.nnodes - nrows;
+    o << "#define HYX_NEV " << nev << "\n#define HYX_NTE " << ev.n_te << "\n";
+    o << "#define HYX_EV_ROWS " << (uint64_t)nodes * (P + 1) << "ull\n";
+    o << "#define HYX_TOL " << lit(ev.tol) << "\n";
+    if (high_acc) o << "#define HYX_HIGH_ACC 1\n";
+    o << "#ifndef HYX_SUB\n#define HYX_SUB 1\n#endif\n";
+    o << "#ifndef HYX_COMPACT\n#define hyx_sub_sum(v) (v)\n";
+    o << "__constant__ double hyx_inv[" << (P + 1) << "] = {0.0";
+    for (int k = 1; k <= P; ++k) o << ", " << lit(1.0 / k);
+    o << "};\n#endif\n";
+    o << "__constant__ int hyx_ev_dir[" << nev << "] = {";
+    for (uint32_t e = 0; e < nev; ++e) o << (e ? ", " : "") << ev.dir[e];
+    o << "};\n";
+    o << "__constant__ double hyx_ev_cd[" << std::max<uint32_t>(ev.n_te, 1) << "] = {";
+    for (uint32_t e = 0; e < std::max<uint32_t>(ev.n_te, 1); ++e)
+        o << (e ? ", " : "") << lit(e < ev.n_te ? ev.cooldown[e] : -1.0);
+    o << "};\n";
+    // Event-function jet: the event op-list's nodes at orders 0..P from the state
+    // coefficients of the step (rows 0..n-1 of tc); node rows live in et.
+    o << "#define T(node, kk) (((node) < " << n_sys << "u) ? tc[((hyx_u64)(node) * (HYX_P + 1) + (hyx_u64)(kk)) * ld] : "
+      << "et[((hyx_u64)((node) - " << nrows << "u) * (HYX_P + 1) + (hyx_u64)(kk)) * ld])\n";
+    o << "__device__ __forceinline__ void hyx_event_jet(double *__restrict__ tc, double *__restrict__ et, "
+         "hyx_u64 ld, const double *__restrict__ pv, double t0)\n{\n";
+    o << "    (void)pv; (void)t0; (void)tc; (void)et; (void)ld;\n";
+    o << "    const int hyx_sub = (HYX_SUB > 1) ? (int)(__lane_id() & (HYX_SUB - 1)) : 0;\n    (void)hyx_sub;\n";
+    for (uint32_t j = nrows; j < E.nnodes; ++j) g.order0(j);
+    if (nodes > 0) {
+        o << "    for (int k = 1; k <= HYX_P; ++k) {\n";
+        for (uint32_t j = nrows; j < E.nnodes; ++j) g.orderk(j);
+        o << "    }\n";
+    }
+    o << "}\n";
+    // Coefficient k of event e: a node row, a state row, or a number / parameter (order 0 only).
+    o << "__device__ __forceinline__ double hyx_ev_coeff(int e, int k, const double *__restrict__ tc, "
+         "const double *__restrict__ et, hyx_u64 ld, const double *__restrict__ pv)\n{\n";
+    o << "    (void)tc; (void)et; (void)ld; (void)pv;\n    switch (e) {\n";
+    for (uint32_t e = 0; e < nev; ++e) {
+        const uint32_t w = E.rhs[e], kind = HYX_OPERAND_KIND(w), i = HYX_OPERAND_INDEX(w);
+        o << "    case " << e << ": return ";
+        if (kind == HYX_KIND_NODE) {
+            if (i >= n_sys && i < nrows) throw std::invalid_argument("an event function refers to a padding row");
+            o << "T(" << i << "u, k)";
+        } else if (kind == HYX_KIND_CONST)
+            o << "(k == 0) ? " << lit(E.consts[i]) << " : 0.0";
+        else
+            o << "(k == 0) ? pv[" << i << "] : 0.0";
+        o << ";\n";
+    }
+    o << "    }\n    return 0.0;\n}\n#undef T\n";
+    // The event-node rows follow the state rows in tc.
+    o << "__device__ __forceinline__ double *hyx_ev_rows(double *tc, hyx_u64 ld) { return tc + (hyx_u64)HYX_N * "
+         "(HYX_P + 1) * ld; }\n";
+    o << "__device__ __forceinline__ void hyx_ev_norms(double *tc, hyx_u64 ld, const double *pv, double t0, double &mo, "
+         "double &mom1)\n{\n    double *et = hyx_ev_rows(tc, ld);\n    hyx_event_jet(tc, et, ld, pv, t0);\n"
+         "    for (int e = 0; e < HYX_NEV; ++e) {\n"
+         "        const double a = fabs(hyx_ev_coeff(e, HYX_P, tc, et, ld, pv));\n"
+         "        const double b = fabs(hyx_ev_coeff(e, HYX_P - 1, tc, et, ld, pv));\n"
+         "        mo = (a > mo) ? a : mo;\n        mom1 = (b > mom1) ? b : mom1;\n    }\n}\n";
+    return o.str();
+}
+
+std::string events_forward_decl()
+{
+    return "#define HYX_EVENTS 1\ntypedef unsigned long long hyx_u64;\n"
+           "__device__ __forceinline__ void hyx_ev_norms(double *tc, hyx_u64 ld, const double *pv, double t0, "
+           "double &mo, double &mom1);\n";
 }
 
 int history_nodes(const Sys &S)

@@ -61,6 +61,23 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int order,
 // Compiled function: the order-0 values of the first n_out right-hand sides (hyx_cfunc_eval).
 std::string generate_cfunc_prelude(const Sys &S, uint32_t n_out);
 
+// Event detection: directions (-1, 0, 1) of every event (terminal ones first),
+// the terminal events' cooldowns (< 0: automatic) and the integrator tolerance.
+struct EventSpec {
+    uint32_t n_te = 0;
+    std::vector<int> dir;
+    std::vector<double> cooldown;
+    double tol = 0.0;
+};
+
+// Event prelude: hyx_event_jet() (the event op-list E's nodes at orders 0..P from the
+// step's state coefficients) and hyx_ev_coeff(); E's rows 0..n_sys-1 are the state
+// variables and its right-hand sides 0..nev-1 the event functions.
+std::string generate_events_prelude(const Sys &E, uint32_t n_sys, int P, const EventSpec &ev, bool high_acc);
+// Placed before the step prelude of a system with events: HYX_EVENTS and the declaration
+// of hyx_ev_norms(), which the step calls to fold the event functions into its norms.
+std::string events_forward_decl();
+
 // Exact hexadecimal double literal.
 std::string lit(double v);
 

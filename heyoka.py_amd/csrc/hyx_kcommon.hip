@@ -58,6 +58,15 @@ struct hyx_kargs {
     double *gout;           // propagate_grid: [npts][n][ld] dense output at the grid times
     unsigned *gcur;         // propagate_grid: [N] next grid point of each instance
     hyx_u64 npts;           // propagate_grid: grid points (0 outside propagate_grid)
+    double *et;             // events: [event nodes * (P+1)][ld] event-function coefficients
+    double *evcd;           // events: [n_te][3][ld] cooldowns: trigger time hi, lo, length (< 0: none)
+    int *te;                // events: [N] terminal event of the last step (-1: none)
+    int *te_dsgn;           // events: [N] its direction (-1, 0, 1)
+    unsigned *ntc;          // events: [N] non-terminal events found in the last step
+    double *nt_tau;         // events: [nt_cap][ld] their offsets from the start of the step
+    int *nt_code;           // events: [nt_cap][ld] 4 * (non-terminal index) + d_sgn + 1
+    unsigned nt_cap;        // events: records kept per instance (0: none)
+    unsigned pad2_;
 };
 
 // ---- double-length time -----------------------------------------------------------

@@ -1,0 +1,306 @@
+// Event detection inside the Taylor step (lane forms), compiled after
+// hyx_kcommon.hip when a system has events. The generated event prelude defines
+//   HYX_NEV, HYX_NTE            events, terminal ones first
+//   hyx_ev_dir[], hyx_ev_cd[]   direction of every event, cooldown of every terminal event
+//   HYX_TOL                     the integrator's tolerance
+//   hyx_event_jet(tc, et, ...)  the event functions' Taylor coefficients at orders 0..P
+//   hyx_ev_coeff(e, k, ...)     coefficient k of event e
+//   hyx_ev_norms(...)           called by the step: the event jet, folded into the
+//                               step-size norms of the last two orders
+//
+// Semantics follow heyoka's events (heyoka/taylor_expose_events.cpp:195-300; the
+// batch tests heyoka/test.py:693-1000 and heyoka/_test_batch_integrator.py:555-600;
+// doc/notebooks/Event detection.ipynb):
+//   - the roots of an event's Taylor polynomial g(tau) in the step, tau from 0
+//     (included) towards h (excluded), are the event's triggers; d_sgn is the
+//     sign of dg/dt there, and an event with a direction keeps only the roots of
+//     that sign;
+//   - the earliest terminal trigger truncates the step: the state is evaluated
+//     at its tau, and the event enters a cooldown (its own value, or
+//     10 * tol * max|x| / |dg/dt| when negative), during which its roots are
+//     ignored; cooldowns expire once the time has moved that far away;
+//   - non-terminal triggers before the end of the (truncated) step are recorded
+//     for the host, which runs their callbacks in chronological order.
+//
+// Root isolation (per lane, only when a fast exclusion test fails): the
+// polynomial is rescaled to s in [0, 1] (tau = s h) and dyadic subintervals are
+// visited left to right; on each, Descartes' rule of signs counts the sign
+// variations of (1 + x)^P r(1 / (1 + x)), r the polynomial moved onto the
+// subinterval. 0 variations: no root; 1: exactly one, refined by the Illinois
+// variant of regula falsi on g(tau); more: bisect. The traversal keeps only the
+// current level and index, so no stack and no per-lane scratch are needed.
+
+#define HYX_EV_MAX_LVL ((44 < 960 / HYX_P) ? 44 : (960 / HYX_P))
+#define HYX_EV_MAX_VISITS 4096
+
+// Coefficients of q(x + a) from those of q (repeated synthetic division).
+__device__ __forceinline__ void hyx_ev_shift(double (&q)[HYX_P + 1], double a)
+{
+#pragma unroll
+    for (int i = 0; i < HYX_P; ++i)
+#pragma unroll
+        for (int k = HYX_P - 1; k >= i; --k) q[k] = __builtin_fma(a, q[k + 1], q[k]);
+}
+
+__device__ __forceinline__ double hyx_ev_eval(const double (&g)[HYX_P + 1], double x)
+{
+    double r = g[HYX_P];
+#pragma unroll
+    for (int k = HYX_P - 1; k >= 0; --k) r = __builtin_fma(r, x, g[k]);
+    return r;
+}
+
+// dg/dtau.
+__device__ __forceinline__ double hyx_ev_deval(const double (&g)[HYX_P + 1], double x)
+{
+    double r = (double)HYX_P * g[HYX_P];
+#pragma unroll
+    for (int k = HYX_P - 1; k >= 1; --k) r = __builtin_fma(r, x, (double)k * g[k]);
+    return r;
+}
+
+__device__ __forceinline__ int hyx_ev_sgn(double v) { return (v > 0.0) - (v < 0.0); }
+
+// Sign variations of (1 + x)^P r(1 / (1 + x)), r(u) = c(a + w u): bounds the
+// number of roots of c in (a, a + w), with the same parity. *ca = c(a).
+__device__ __forceinline__ int hyx_ev_variations(const double (&c)[HYX_P + 1], double a, double w, double &ca)
+{
+    double r[HYX_P + 1];
+#pragma unroll
+    for (int k = 0; k <= HYX_P; ++k) r[k] = c[k];
+    if (a != 0.0) hyx_ev_shift(r, a);
+    ca = r[0];
+    double q[HYX_P + 1];
+    double wk = 1.0;
+#pragma unroll
+    for (int k = 0; k <= HYX_P; ++k) {
+        q[HYX_P - k] = r[k] * wk;
+        wk *= w;
+    }
+    hyx_ev_shift(q, 1.0);
+    int v = 0, last = 0;
+#pragma unroll
+    for (int k = 0; k <= HYX_P; ++k) {
+        const int s = hyx_ev_sgn(q[k]);
+        if (s != 0) {
+            v += (last != 0 && s != last) ? 1 : 0;
+            last = s;
+        }
+    }
+    return v;
+}
+
+// The root of g between x0 and x1 (f0 = g(x0), f1 = g(x1) of opposite signs,
+// neither zero): Illinois iteration, bisection when the secant point leaves the
+// bracket; stops at adjacent doubles or an exact zero.
+__device__ __forceinline__ double hyx_ev_refine(const double (&g)[HYX_P + 1], double x0, double x1, double f0,
+                                                double f1)
+{
+    int side = 0;
+    for (int it = 0; it < 160; ++it) {
+        const double m = 0.5 * (x0 + x1);
+        if (m == x0 || m == x1) break;
+        double x = x0 - f0 * (x1 - x0) / (f1 - f0);
+        const double lo = fmin(x0, x1), hi = fmax(x0, x1);
+        if (!(x > lo && x < hi)) x = m;
+        const double fx = hyx_ev_eval(g, x);
+        if (fx == 0.0) return x;
+        if ((fx > 0.0) == (f1 > 0.0)) {
+            x1 = x;
+            f1 = fx;
+            if (side == 1) f0 *= 0.5;
+            side = 1;
+        } else {
+            x0 = x;
+            f0 = fx;
+            if (side == -1) f1 *= 0.5;
+            side = -1;
+        }
+    }
+    return (fabs(f0) <= fabs(f1)) ? x0 : x1;
+}
+
+// Roots of g(tau) for tau from 0 (included) towards h (excluded), in
+// chronological order: on_root(tau, d_sgn) returns true to stop the scan.
+template <class F>
+__device__ __forceinline__ void hyx_ev_scan(const double (&g)[HYX_P + 1], double h, F &&on_root)
+{
+    double c[HYX_P + 1];
+    double hk = 1.0, mag = 0.0, lo = 0.0, hi = 0.0;
+#pragma unroll
+    for (int k = 0; k <= HYX_P; ++k) {
+        c[k] = g[k] * hk;
+        hk *= h;
+        mag += fabs(c[k]);
+        if (k > 0) {
+            lo += fmin(c[k], 0.0);
+            hi += fmax(c[k], 0.0);
+        }
+    }
+    if (!(mag > 0.0) || !isfinite(mag)) return;
+    // Fast exclusion: q(s) lies in c0 + [lo, hi] over s in [0, 1] (with a rounding margin).
+    const double slack = (4.0 * (HYX_P + 1)) * 1.1102230246251565e-16 * mag;
+    if (c[0] + lo > slack || c[0] + hi < -slack) return;
+    double last = __builtin_nan("");
+    if (c[0] == 0.0) {
+        last = 0.0;
+        if (on_root(0.0, hyx_ev_sgn(g[1]))) return;
+    }
+    int lvl = 0;
+    hyx_u64 idx = 0;
+    bool fresh = false; // arrived at this interval from its left neighbour
+    for (int visit = 0; visit < HYX_EV_MAX_VISITS; ++visit) {
+        const double w = __builtin_ldexp(1.0, -lvl), a = (double)idx * w;
+        double ca;
+        const int v = hyx_ev_variations(c, a, w, ca);
+        if (fresh && ca == 0.0) {
+            const double tau = a * h;
+            if (tau != last) {
+                last = tau;
+                if (on_root(tau, hyx_ev_sgn(hyx_ev_deval(g, tau)))) return;
+            }
+        }
+        bool split = v > 1;
+        if (v == 1) {
+            // One root inside: refine when the end values bracket it; an exact zero at an
+            // end (reported as the left end of an interval) or a rounding-level sign
+            // disagreement is resolved by bisecting further.
+            const double ta = a * h, tb = (a + w) * h;
+            const double fa = hyx_ev_eval(g, ta), fb = hyx_ev_eval(g, tb);
+            if (fa != 0.0 && fb != 0.0 && (fa > 0.0) != (fb > 0.0)) {
+                const double tau = hyx_ev_refine(g, ta, tb, fa, fb);
+                if (tau != last && fabs(tau) < fabs(h)) {
+                    last = tau;
+                    if (on_root(tau, hyx_ev_sgn(hyx_ev_deval(g, tau)))) return;
+                }
+            } else {
+                split = true;
+            }
+        }
+        const bool descend = split && lvl < HYX_EV_MAX_LVL;
+        if (descend) {
+            ++lvl;
+            idx <<= 1;
+            fresh = false;
+            continue;
+        }
+        while (lvl > 0 && (idx & 1ull)) {
+            idx >>= 1;
+            --lvl;
+        }
+        if (lvl == 0) return;
+        ++idx;
+        fresh = true;
+    }
+}
+
+// Event processing after hyx_taylor_step<true> took a step of size h from
+// (thi, tlo): tc holds the step's state coefficients (column i, stride ld) and x
+// the state advanced by h. Returns the step actually taken: h, or the offset of
+// the earliest terminal trigger (x is then re-evaluated there and nf updated).
+// te / te_dsgn: that event (-1: none) and its direction. The sub-lanes of an
+// instance hold identical values and write identical records.
+__device__ __forceinline__ double hyx_ev_step(const hyx_kargs &a, hyx_u64 i, const double *pv, double thi,
+                                              double tlo, double h, double (&x)[HYX_N], bool &nf, int &te,
+                                              int &te_dsgn)
+{
+    // The step already formed the event functions' coefficients (hyx_ev_norms).
+    double *tc = a.tc + i;
+    const double *et = hyx_ev_rows(tc, a.ld);
+    const hyx_u64 ld = a.ld;
+    te = -1;
+    te_dsgn = 0;
+    unsigned ntc = 0;
+    double hend = h;
+    double g[HYX_P + 1];
+    if (h != 0.0) {
+        double best = fabs(h);
+        for (int e = 0; e < HYX_NTE; ++e) {
+#pragma unroll
+            for (int k = 0; k <= HYX_P; ++k) g[k] = hyx_ev_coeff(e, k, tc, et, ld, pv);
+            const double cdl = a.evcd[((hyx_u64)e * 3 + 2) * ld + i];
+            const double dt0 = (cdl >= 0.0) ? hyx_dfl_sub_hi(thi, tlo, a.evcd[((hyx_u64)e * 3) * ld + i],
+                                                             a.evcd[((hyx_u64)e * 3 + 1) * ld + i])
+                                            : 0.0;
+            const int dir = hyx_ev_dir[e];
+            hyx_ev_scan(g, h, [&](double tau, int ds) {
+                if (fabs(tau) >= best) return true;
+                if (dir != 0 && ds != dir) return false;
+                if (cdl >= 0.0 && fabs(dt0 + tau) < cdl) return false;
+                best = fabs(tau);
+                te = e;
+                te_dsgn = ds;
+                hend = tau;
+                return true;
+            });
+        }
+        for (int e = HYX_NTE; e < HYX_NEV; ++e) {
+#pragma unroll
+            for (int k = 0; k <= HYX_P; ++k) g[k] = hyx_ev_coeff(e, k, tc, et, ld, pv);
+            const int dir = hyx_ev_dir[e];
+            hyx_ev_scan(g, h, [&](double tau, int ds) {
+                if (fabs(tau) >= best) return true;
+                if (dir != 0 && ds != dir) return false;
+                if (ntc < a.nt_cap) {
+                    a.nt_tau[(hyx_u64)ntc * ld + i] = tau;
+                    a.nt_code[(hyx_u64)ntc * ld + i] = 4 * (e - HYX_NTE) + ds + 1;
+                }
+                ++ntc;
+                return false;
+            });
+        }
+    }
+    double cd = 0.0;
+    if (te >= 0) {
+        // The state at the trigger, from the step's coefficients.
+        bool bad = false;
+        double mx = 0.0;
+#pragma unroll 1
+        for (int q = 0; q < HYX_N; ++q) {
+            const double *cq = tc + (hyx_u64)q * (HYX_P + 1) * ld;
+            const double a0 = fabs(cq[0]);
+            mx = (a0 > mx) ? a0 : mx;
+#ifdef HYX_HIGH_ACC
+            double r = cq[0], cp = 0.0, hk = 1.0;
+            for (int k = 1; k <= HYX_P; ++k) {
+                hk *= hend;
+                const double y = cq[(hyx_u64)k * ld] * hk - cp;
+                const double tt = r + y;
+                cp = (tt - r) - y;
+                r = tt;
+            }
+#else
+            double r = cq[(hyx_u64)HYX_P * ld];
+            for (int k = HYX_P - 1; k >= 0; --k) r = __builtin_fma(r, hend, cq[(hyx_u64)k * ld]);
+#endif
+            x[q] = r;
+            bad = bad || !isfinite(r);
+        }
+        nf = bad;
+#pragma unroll
+        for (int k = 0; k <= HYX_P; ++k) g[k] = hyx_ev_coeff(te, k, tc, et, ld, pv);
+        cd = hyx_ev_cd[te];
+        if (cd < 0.0) {
+            cd = 10.0 * (HYX_TOL * mx) / fabs(hyx_ev_deval(g, hend));
+            if (!isfinite(cd)) cd = 0.0;
+        }
+    }
+    // Cooldowns: expire those the step moved past, then start the triggered one's.
+    double nhi = thi, nlo = tlo;
+    hyx_dfl_add(nhi, nlo, hend, 0.0);
+    for (int e = 0; e < HYX_NTE; ++e) {
+        const hyx_u64 o = (hyx_u64)e * 3 * ld + i;
+        const double cdl = a.evcd[o + 2 * ld];
+        if (cdl >= 0.0 && !(fabs(hyx_dfl_sub_hi(nhi, nlo, a.evcd[o], a.evcd[o + ld])) < cdl)) a.evcd[o + 2 * ld] = -1.0;
+    }
+    if (te >= 0) {
+        const hyx_u64 o = (hyx_u64)te * 3 * ld + i;
+        a.evcd[o] = nhi;
+        a.evcd[o + ld] = nlo;
+        a.evcd[o + 2 * ld] = cd;
+    }
+    a.te[i] = te;
+    a.te_dsgn[i] = te_dsgn;
+    a.ntc[i] = ntc;
+    return hend;
+}

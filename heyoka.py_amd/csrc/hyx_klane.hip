@@ -11,7 +11,11 @@
 // Taylor coefficients to a.tc and evaluate them at the grid times that fall
 // inside the step (dense output, heyoka's propagate_grid:
 // heyoka/expose_batch_integrators.cpp:315-392).
-#ifdef HYX_GRID
+//
+// With events (HYX_EVENTS, hyx_kevent.hip), every step writes its coefficients to
+// a.tc as well; event processing may shorten the step to a terminal trigger, and
+// a propagation ends at a terminal event with outcome -(index + 1).
+#if defined(HYX_GRID) || defined(HYX_EVENTS)
 #define HYX_PROP_WTC true
 #else
 #define HYX_PROP_WTC false
@@ -92,11 +96,16 @@ __device__ __forceinline__ bool hyx_prop_iter(hyx_inst &s, const hyx_kargs &a, h
     bool nf;
     hyx_taylor_step<HYX_PROP_WTC>(s.x, s.pv, s.thi, lim, h, nf, HYX_PROP_WTC ? a.tc + i : nullptr, a.ld,
                                   a.tape ? a.tape + lane_tape : nullptr, a.tape_stride);
+    int te = -1;
+#ifdef HYX_EVENTS
+    int te_dsgn;
+    h = hyx_ev_step(a, i, s.pv, s.thi, s.tlo, h, s.x, nf, te, te_dsgn);
+#endif
     hyx_dfl_add(s.thi, s.tlo, h, 0.0);
     nf = nf || !isfinite(s.thi);
     s.iter += 1;
     s.nst += (h != 0.0) ? 1 : 0;
-    const hyx_i64 oc = nf ? HYX_ERR_NF_STATE : ((h == lim) ? HYX_TIME_LIMIT : HYX_SUCCESS);
+    const hyx_i64 oc = nf ? HYX_ERR_NF_STATE : ((te < 0 && h == lim) ? HYX_TIME_LIMIT : HYX_SUCCESS);
     if (oc == HYX_SUCCESS) {
         const double ah = fabs(h);
         s.minh = (ah < s.minh) ? ah : s.minh;
@@ -110,6 +119,10 @@ __device__ __forceinline__ bool hyx_prop_iter(hyx_inst &s, const hyx_kargs &a, h
 #ifdef HYX_GRID
     hyx_grid_lane(s, a, i, h, fin);
 #endif
+    if (te >= 0) {
+        *res = -(hyx_i64)te - 1;
+        return true;
+    }
     if (fin) {
         *res = HYX_TIME_LIMIT;
         return true;
@@ -166,13 +179,24 @@ __device__ __forceinline__ void hyx_step_body(const hyx_kargs &a)
     const double mdt = a.max_dt[i];
     double h;
     bool nf;
+#ifdef HYX_EVENTS
+    // Events need the coefficients of every step.
+    hyx_taylor_step<true>(s.x, s.pv, s.thi, mdt, h, nf, a.tc + i, a.ld, a.tape ? a.tape + i : nullptr, a.tape_stride);
+    int te, te_dsgn;
+    h = hyx_ev_step(a, i, s.pv, s.thi, s.tlo, h, s.x, nf, te, te_dsgn);
+#else
     hyx_taylor_step<WTC>(s.x, s.pv, s.thi, mdt, h, nf, WTC ? a.tc + i : nullptr, a.ld, a.tape ? a.tape + i : nullptr,
                          a.tape_stride);
+    const int te = -1;
+#endif
     hyx_dfl_add(s.thi, s.tlo, h, 0.0);
     nf = nf || !isfinite(s.thi);
     hyx_store_state(s, a, i);
     a.h_out[i] = h;
-    a.outcome[i] = nf ? HYX_ERR_NF_STATE : ((h == mdt) ? HYX_TIME_LIMIT : HYX_SUCCESS);
+    // A terminal event reports -(index + 1); the host turns it into the index when
+    // the event's callback lets the integration continue.
+    a.outcome[i] = nf ? HYX_ERR_NF_STATE
+                      : (te >= 0 ? -(hyx_i64)te - 1 : ((h == mdt) ? HYX_TIME_LIMIT : HYX_SUCCESS));
 }
 
 extern "C" __global__ void __launch_bounds__(64) hyx_step_k(hyx_kargs a) { hyx_step_body<false>(a); }

@@ -31,7 +31,7 @@
 #include "../../include/hyx.h"
 #include "hyx_internal.h"
 
-#include "build/hyx_kernels_embed.inc" // k_hyx_kcommon, k_hyx_klane, k_hyx_kgroup, k_hyx_kcfunc: kernel templates
+#include "build/hyx_kernels_embed.inc" // k_hyx_kcommon, k_hyx_klane, k_hyx_kgroup, k_hyx_kcfunc, k_hyx_kevent: kernel templates
 
 namespace
 {
@@ -83,6 +83,15 @@ struct KArgs {
     double *gout;
     unsigned *gcur;
     uint64_t npts;
+    double *et;
+    double *evcd;
+    int *te;
+    int *te_dsgn;
+    unsigned *ntc;
+    double *nt_tau;
+    int *nt_code;
+    unsigned nt_cap;
+    unsigned pad2_;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -220,6 +229,12 @@ struct hyx_sys {
     std::mutex gmu;
     hipModule_t gmod = nullptr;
     hipFunction_t kg_init = nullptr, kg_iter = nullptr, kg_persist = nullptr;
+    // Events: the event op-list (rows = state variables, right-hand sides = event
+    // functions, terminal ones first) and its specification.
+    bool has_events = false;
+    hyx::Sys E;
+    hyx::EventSpec ev;
+    uint64_t ev_rows = 0; // event-node coefficient rows per instance
 };
 
 struct hyx_batch {
@@ -251,6 +266,13 @@ struct hyx_batch {
     unsigned *gcur = nullptr;                // propagate_grid: [N] next grid point
     size_t grid_cap = 0, gout_cap = 0;       // allocated doubles
     uint64_t npts = 0;
+    // Events (systems with events only): et = the event-node coefficient rows, which
+    // follow the state rows inside tc; [n_te][3][N] cooldowns, the last step's terminal event and its direction,
+    // and its non-terminal records ([nt_cap][N] offsets and codes).
+    double *et = nullptr, *evcd = nullptr, *nt_tau = nullptr;
+    int *te = nullptr, *te_dsgn = nullptr, *nt_code = nullptr;
+    unsigned *ntc = nullptr;
+    unsigned nt_cap = 0;
 };
 
 namespace
@@ -259,7 +281,7 @@ namespace
 const int kPlainGroup = 16; // plain form: launches between host checks
 
 int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint32_t n_consts, uint32_t order,
-              uint32_t flags, hyx_sys &s)
+              uint32_t flags, hyx_sys &s, const hyx_events *ev = nullptr)
 {
     if (!ops) return fail(HYX_EINVAL, "null op-list");
     std::string err;
@@ -267,6 +289,32 @@ int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint3
     if (order < 2 || order > 200) return fail(HYX_EINVAL, "the Taylor order must be in [2, 200]");
     s.order = order;
     s.flags = flags;
+    if (ev) {
+        const uint32_t nev = ev->n_te + ev->n_nt;
+        if (nev == 0) return fail(HYX_EINVAL, "an event specification needs at least one event");
+        if (!ev->ops || (nev && !ev->direction) || (ev->n_te && !ev->cooldown))
+            return fail(HYX_EINVAL, "null pointer in the event specification");
+        if (hyx::parse_oplist(ev->ops, ev->n_words, ev->consts, ev->n_consts, s.E, err) != 0)
+            return fail(HYX_EINVAL, "event op-list: " + err);
+        if (s.E.n < s.S.n || s.E.n < nev)
+            return fail(HYX_EINVAL, "the event op-list must have a row per state variable and per event");
+        if (s.E.npar > s.S.npar)
+            return fail(HYX_EINVAL, "the event functions use more parameters than the system");
+        s.ev.n_te = ev->n_te;
+        s.ev.tol = ev->tol;
+        for (uint32_t e = 0; e < nev; ++e) {
+            if (ev->direction[e] < -1 || ev->direction[e] > 1) return fail(HYX_EINVAL, "invalid event direction");
+            s.ev.dir.push_back(ev->direction[e]);
+        }
+        for (uint32_t e = 0; e < ev->n_te; ++e) {
+            if (!(ev->cooldown[e] == ev->cooldown[e])) return fail(HYX_EINVAL, "a NaN event cooldown was given");
+            s.ev.cooldown.push_back(ev->cooldown[e]);
+        }
+        if (!(ev->tol > 0.0)) return fail(HYX_EINVAL, "the event specification needs the tolerance");
+        if (flags & HYX_FLAG_FORCE_GROUP) return fail(HYX_EINVAL, "the group form does not support events");
+        s.has_events = true;
+        s.ev_rows = (uint64_t)(s.E.nnodes - s.E.n) * (order + 1);
+    }
     try {
         const bool ha = (flags & HYX_FLAG_HIGH_ACCURACY) != 0;
         hyx::GroupPlan gp;
@@ -279,7 +327,7 @@ int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint3
         else if (flags & HYX_FLAG_FORCE_GROUP) {
             if (!hyx::plan_groups(s.S, gp, why)) return fail(HYX_EINVAL, "the group form does not apply: " + why);
             s.form = HYX_FORM_GROUP;
-        } else if (lane_compact && hyx::plan_groups(s.S, gp, why))
+        } else if (lane_compact && !s.has_events && hyx::plan_groups(s.S, gp, why))
             s.form = HYX_FORM_GROUP;
         else
             s.form = lane_compact ? HYX_FORM_COMPACT : HYX_FORM_UNROLLED;
@@ -297,8 +345,14 @@ int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint3
                 sub = (flags & HYX_FLAG_SUB1) ? 1 : (flags & HYX_FLAG_SUB4) ? 4 : (flags & HYX_FLAG_SUB8) ? 8 : 2;
             s.inst_per_block = 64u / (uint32_t)sub;
             s.sub = sub;
-            s.src = hyx::generate_prelude(s.S, (int)order, ha, s.compact, split, sub) + "\n" + k_hyx_kcommon + "\n" +
-                    k_hyx_klane;
+            s.src = (s.has_events ? hyx::events_forward_decl() : std::string()) +
+                    hyx::generate_prelude(s.S, (int)order, ha, s.compact, split, sub) + "\n";
+            if (s.has_events)
+                s.src += hyx::generate_events_prelude(s.E, s.S.n, (int)order, s.ev, ha) + "\n" + k_hyx_kcommon + "\n" +
+                         k_hyx_kevent + "\n";
+            else
+                s.src += std::string(k_hyx_kcommon) + "\n";
+            s.src += k_hyx_klane;
         }
     } catch (const std::exception &e) {
         return fail(HYX_EINVAL, e.what());
@@ -336,6 +390,14 @@ KArgs make_args(hyx_batch *b)
     a.iter_budget = b->iter_budget;
     a.unfinished = b->unfinished;
     a.refill_min = b->refill_min;
+    a.et = b->et;
+    a.evcd = b->evcd;
+    a.te = b->te;
+    a.te_dsgn = b->te_dsgn;
+    a.ntc = b->ntc;
+    a.nt_tau = b->nt_tau;
+    a.nt_code = b->nt_code;
+    a.nt_cap = 0; // records are kept by hyx_step only
     return a;
 }
 
@@ -565,13 +627,13 @@ int hyx_sys_precompile(const uint32_t *ops, uint32_t n_words, const double *cons
     return get_code((flags & HYX_FLAG_GRID) ? "#define HYX_GRID 1\n" + s.src : s.src, arch ? arch : "gfx950", code);
 }
 
-int hyx_sys_create(const uint32_t *ops, uint32_t n_words, const double *consts, uint32_t n_consts, uint32_t order,
-                   uint32_t flags, hyx_sys **out)
+static int create_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint32_t n_consts, uint32_t order,
+                      uint32_t flags, const hyx_events *ev, hyx_sys **out)
 {
     if (!out) return fail(HYX_EINVAL, "null output pointer");
     *out = nullptr;
     auto *s = new hyx_sys;
-    int rc = build_sys(ops, n_words, consts, n_consts, order, flags, *s);
+    int rc = build_sys(ops, n_words, consts, n_consts, order, flags, *s, ev);
     if (rc) {
         delete s;
         return rc;
@@ -630,6 +692,28 @@ int hyx_sys_create(const uint32_t *ops, uint32_t n_words, const double *consts, 
     if (hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_NUM_REGS, s->k_persist) == hipSuccess) s->vgprs = v;
     if (hipFuncGetAttribute(&sc, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, s->k_persist) == hipSuccess) s->scratch = sc;
     *out = s;
+    return 0;
+}
+
+int hyx_sys_create(const uint32_t *ops, uint32_t n_words, const double *consts, uint32_t n_consts, uint32_t order,
+                   uint32_t flags, hyx_sys **out)
+{
+    return create_sys(ops, n_words, consts, n_consts, order, flags, nullptr, out);
+}
+
+int hyx_sys_create_events(const uint32_t *ops, uint32_t n_words, const double *consts, uint32_t n_consts,
+                          uint32_t order, uint32_t flags, const hyx_events *ev, hyx_sys **out)
+{
+    if (!ev) return fail(HYX_EINVAL, "null event specification");
+    return create_sys(ops, n_words, consts, n_consts, order, flags, ev, out);
+}
+
+int hyx_sys_n_events(const hyx_sys *s, uint32_t *n_te, uint32_t *n_nt)
+{
+    if (!s) return fail(HYX_EINVAL, "null system");
+    const uint32_t nev = (uint32_t)s->ev.dir.size();
+    if (n_te) *n_te = s->has_events ? s->ev.n_te : 0;
+    if (n_nt) *n_nt = s->has_events ? nev - s->ev.n_te : 0;
     return 0;
 }
 
@@ -702,6 +786,35 @@ int hyx_batch_create(hyx_sys *s, uint64_t n_inst, hyx_batch **out)
         if (hipMalloc(al.p, al.bytes) != hipSuccess) return cleanup(HYX_ENOMEM, "hipMalloc failed (device memory)");
         if (hipMemsetAsync(*al.p, 0, al.bytes, b->stream) != hipSuccess) return cleanup(HYX_EHIP, "hipMemset failed");
     }
+    if (s->has_events) {
+        // Every step of a system with events writes its coefficients (tc) and the
+        // event functions' (et); non-terminal records: 16 per instance and step.
+        const uint64_t P1 = s->order + 1, nte = std::max<uint32_t>(s->ev.n_te, 1);
+        b->nt_cap = 16;
+        struct {
+            void **p;
+            size_t bytes;
+        } ev_allocs[] = {
+            {(void **)&b->tc, (n * P1 + s->ev_rows) * N * 8},
+            {(void **)&b->evcd, 3 * nte * N * 8},
+            {(void **)&b->te, N * 4},
+            {(void **)&b->te_dsgn, N * 4},
+            {(void **)&b->ntc, N * 4},
+            {(void **)&b->nt_tau, (uint64_t)b->nt_cap * N * 8},
+            {(void **)&b->nt_code, (uint64_t)b->nt_cap * N * 4},
+        };
+        for (auto &al : ev_allocs) {
+            if (hipMalloc(al.p, al.bytes) != hipSuccess) return cleanup(HYX_ENOMEM, "hipMalloc failed (device memory)");
+            if (hipMemsetAsync(*al.p, 0, al.bytes, b->stream) != hipSuccess)
+                return cleanup(HYX_EHIP, "hipMemset failed");
+        }
+        b->et = b->tc + n * P1 * N; // event-node rows follow the state rows
+        // No cooldown running: every field -1 (only the length is read as a flag).
+        const std::vector<double> m1(3 * nte * N, -1.0);
+        if (hipMemcpyAsync(b->evcd, m1.data(), m1.size() * 8, hipMemcpyHostToDevice, b->stream) != hipSuccess)
+            return cleanup(HYX_EHIP, "hipMemcpy failed");
+        if (hipStreamSynchronize(b->stream) != hipSuccess) return cleanup(HYX_EHIP, "hipStreamSynchronize failed");
+    }
     if (hipStreamSynchronize(b->stream) != hipSuccess) return cleanup(HYX_EHIP, "hipStreamSynchronize failed");
     *out = b;
     return 0;
@@ -716,7 +829,7 @@ void hyx_batch_destroy(hyx_batch *b)
     void *ptrs[] = {b->state, b->pars, b->thi,    b->tlo,   b->tfhi,   b->tflo,  b->mdt,
                     b->hout,  b->minh, b->maxh,   b->tc,    b->oc,     b->nsteps, b->iters,
                     b->work,  b->status, b->active, b->unfinished, b->queue, b->tape, b->stage, b->stage_idx,
-                    b->grid,  b->gout, b->gcur};
+                    b->grid,  b->gout, b->gcur, b->evcd, b->te, b->te_dsgn, b->ntc, b->nt_tau, b->nt_code};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (b->ev0) hipEventDestroy(b->ev0);
@@ -784,6 +897,7 @@ int hyx_step(hyx_batch *b, const double *max_dt, int write_tc, int64_t *outcome,
     if (int rc = upload(b, b->mdt, max_dt, N, std::numeric_limits<double>::infinity())) return rc;
     if (int rc = ensure_tape(b, N)) return rc;
     KArgs a = make_args(b);
+    a.nt_cap = b->nt_cap;
     HIPCHK(hipEventRecord(b->ev0, b->stream));
     HIPCHK(launch(write_tc ? b->sys->k_step_tc : b->sys->k_step, blocks_of(b), b->stream, &a));
     HIPCHK(hipEventRecord(b->ev1, b->stream));
@@ -794,6 +908,45 @@ int hyx_step(hyx_batch *b, const double *max_dt, int write_tc, int64_t *outcome,
     hipEventElapsedTime(&ms, b->ev0, b->ev1);
     b->last_ms = ms;
     b->last_launches = 1;
+    return 0;
+}
+
+int hyx_events_read(hyx_batch *b, int32_t *te, int32_t *te_dsgn, uint32_t *nt_count, double *nt_tau,
+                    int32_t *nt_code, uint32_t *cap)
+{
+    if (!b) return fail(HYX_EINVAL, "null batch");
+    if (!b->sys->has_events) return fail(HYX_EINVAL, "the system has no events");
+    const uint64_t N = b->N;
+    if (cap) *cap = b->nt_cap;
+    if (te) HIPCHK(hipMemcpyAsync(te, b->te, N * 4, hipMemcpyDeviceToHost, b->stream));
+    if (te_dsgn) HIPCHK(hipMemcpyAsync(te_dsgn, b->te_dsgn, N * 4, hipMemcpyDeviceToHost, b->stream));
+    if (nt_count) HIPCHK(hipMemcpyAsync(nt_count, b->ntc, N * 4, hipMemcpyDeviceToHost, b->stream));
+    if (nt_tau) HIPCHK(hipMemcpyAsync(nt_tau, b->nt_tau, (uint64_t)b->nt_cap * N * 8, hipMemcpyDeviceToHost, b->stream));
+    if (nt_code)
+        HIPCHK(hipMemcpyAsync(nt_code, b->nt_code, (uint64_t)b->nt_cap * N * 4, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int hyx_cooldowns_read(hyx_batch *b, double *dst)
+{
+    if (!b || !dst) return fail(HYX_EINVAL, "null argument");
+    if (!b->sys->has_events) return fail(HYX_EINVAL, "the system has no events");
+    const uint64_t cnt = 3ull * b->sys->ev.n_te * b->N;
+    if (cnt == 0) return 0;
+    HIPCHK(hipMemcpyAsync(dst, b->evcd, cnt * 8, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int hyx_cooldowns_write(hyx_batch *b, const double *src)
+{
+    if (!b || !src) return fail(HYX_EINVAL, "null argument");
+    if (!b->sys->has_events) return fail(HYX_EINVAL, "the system has no events");
+    const uint64_t cnt = 3ull * b->sys->ev.n_te * b->N;
+    if (cnt == 0) return 0;
+    HIPCHK(hipMemcpyAsync(b->evcd, src, cnt * 8, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
     return 0;
 }
 

@@ -25,8 +25,13 @@ import numpy as np
 
 from . import _lib
 from .decompose import taylor_decompose, order_from_tol
-from .expr import _ex
+from .events import nt_event_batch, t_event_batch, _event_ref
+from .expr import _ex, expression, get_variables
 from .var_ode_sys import var_ode_sys as _var_ode_sys
+
+
+def _var(name):
+    return expression(name)
 
 
 class taylor_outcome(enum.IntEnum):
@@ -48,6 +53,26 @@ def _outcome(v):
 
 _MODES = {"persistent": _lib.MODE_PERSISTENT, "plain": _lib.MODE_PLAIN, "graph": _lib.MODE_GRAPH}
 
+_PAD = "__hyx_event_pad_"
+
+
+def _event_spec(dec, tes, ntes, tol):
+    """The events' op-list: a row per state variable (then padding rows), the event
+    functions as right-hand sides, terminal events first."""
+    exprs = [e.expression for e in tes] + [e.expression for e in ntes]
+    names = list(dec.state_names)
+    missing = sorted(set(get_variables(exprs)) - set(names))
+    if missing:
+        raise ValueError(f"The event equations contain variables that are not state variables of the system: "
+                         f"{missing}")
+    n, ne = len(names), len(exprs)
+    rows = [_var(v) for v in names]
+    rows += [_var(f"{_PAD}{i}") for i in range(max(n, ne) - n)]
+    rhs = exprs + [expression(0.0)] * (len(rows) - ne)
+    edec = taylor_decompose(list(zip(rows, rhs)))
+    return _lib.EventSpec(edec, len(tes), len(ntes), [int(e.direction) for e in tes + ntes],
+                          [float(e.cooldown) for e in tes], tol)
+
 # Compiled systems keyed by (op-list, constants, order, flags): the analogue of
 # heyoka's in-memory JIT cache (heyoka/core.cpp:148-149). Copies of an
 # integrator share one compiled system.
@@ -55,13 +80,22 @@ _sys_cache: dict = {}
 _sys_cache_lock = threading.Lock()
 
 
-def get_system(dec, order, high_accuracy=False, compact=None):
+def _call_event(cb, *args):
+    """Run an event callback; its failures surface as RuntimeError, as in the reference."""
+    try:
+        return cb(*args)
+    except Exception as e:  # noqa: BLE001
+        raise RuntimeError(f"An exception was raised by an event callback: {type(e).__name__}: {e}") from e
+
+
+def get_system(dec, order, high_accuracy=False, compact=None, events=None):
     words, consts = _lib.oplist_args(dec)
-    key = (words.tobytes(), consts.tobytes(), int(order), bool(high_accuracy), compact)
+    key = (words.tobytes(), consts.tobytes(), int(order), bool(high_accuracy), compact,
+           None if events is None else events.key())
     with _sys_cache_lock:
         s = _sys_cache.get(key)
         if s is None:
-            s = _lib.System(dec, order, high_accuracy=high_accuracy, compact=compact)
+            s = _lib.System(dec, order, high_accuracy=high_accuracy, compact=compact, events=events)
             _sys_cache[key] = s
         return s
 
@@ -97,8 +131,6 @@ class taylor_adaptive_batch:
 
     def __init__(self, sys, state, time=None, pars=None, tol=0.0, high_accuracy=False, compact_mode=False,
                  t_events=None, nt_events=None, parallel_mode=False, **kwargs):
-        if t_events or nt_events:
-            raise NotImplementedError("Event detection is not supported by the GPU batch integrator")
         gpu_mode = kwargs.pop("gpu_mode", "persistent")
         code_form = kwargs.pop("code_form", None)  # None (automatic), "compact" or "unrolled"
         # LLVM-specific options of the reference are accepted and ignored.
@@ -109,11 +141,24 @@ class taylor_adaptive_batch:
             raise TypeError(f"Unexpected keyword argument(s): {sorted(kwargs)}")
         if gpu_mode not in _MODES:
             raise ValueError(f"gpu_mode must be one of {sorted(_MODES)}")
-        self._hyx_init(sys, state, time, pars, tol, high_accuracy, compact_mode, parallel_mode, gpu_mode, code_form)
+        tes = list(t_events) if t_events is not None else []
+        ntes = list(nt_events) if nt_events is not None else []
+        for e in tes:
+            if not isinstance(e, t_event_batch):
+                raise TypeError(f"The list of terminal events can contain only t_event_batch objects, not {type(e)}")
+        for e in ntes:
+            if not isinstance(e, nt_event_batch):
+                raise TypeError(f"The list of non-terminal events can contain only nt_event_batch objects, not "
+                                f"{type(e)}")
+        # The integrator keeps its own copies of the events (callbacks deep-copied).
+        tes = [_copy.copy(e) for e in tes]
+        ntes = [_copy.copy(e) for e in ntes]
+        self._hyx_init(sys, state, time, pars, tol, high_accuracy, compact_mode, parallel_mode, gpu_mode, code_form,
+                       events=(tes, ntes))
 
     # ------------------------------------------------------------------------------------
     def _hyx_init(self, sys, state, time, pars, tol, high_accuracy, compact_mode, parallel_mode, gpu_mode,
-                  code_form, restore=None):
+                  code_form, restore=None, events=([], [])):
         d = self.__dict__
         vsys = sys if isinstance(sys, _var_ode_sys) else None
         pairs = vsys.sys if vsys is not None else [(_ex(a), _ex(b)) for a, b in sys]
@@ -173,7 +218,20 @@ class taylor_adaptive_batch:
             tol = float(np.finfo(np.float64).eps)
         order = order_from_tol(tol)
         compact = {"compact": True, "unrolled": False, "group": "group", None: None}[code_form]
-        system = get_system(dec, order, high_accuracy=bool(high_accuracy), compact=compact)
+        tes, ntes = events
+        spec = None
+        if tes or ntes:
+            if compact == "group":
+                raise ValueError("The group code form does not support events: use the unrolled or compact form")
+            spec = _event_spec(dec, tes, ntes, tol)
+            if spec.dec.n_par > dec.n_par:
+                dec.n_par = spec.dec.n_par
+                if pa.shape[0] < dec.n_par:
+                    pa = np.concatenate([pa, np.zeros((dec.n_par - pa.shape[0], B))])
+                npar = max(npar, dec.n_par)
+        system = get_system(dec, order, high_accuracy=bool(high_accuracy), compact=compact, events=spec)
+        d["_hyx_tes"] = tes
+        d["_hyx_ntes"] = ntes
         d["_hyx_pairs"] = pairs
         d["_hyx_vsys"] = vsys
         d["_hyx_dec"] = dec
@@ -225,13 +283,56 @@ class taylor_adaptive_batch:
     # ---- stepping -----------------------------------------------------------------------
     def _step_impl(self, mdt, write_tc):
         self._push()
+        if self.with_events:
+            t0 = (self._hyx_thi.copy(), self._hyx_tlo.copy())
         oc, h = self._hyx_batch.step(mdt, write_tc)
         self._pull()
         self._hyx_last_h[...] = h
-        self._hyx_step_res = [(_outcome(o), float(x)) for o, x in zip(oc, h)]
-        if write_tc:
+        if self.with_events:
+            oc = self._run_events(oc, t0, write_tc)
+        elif write_tc:
             self._hyx_tc[...] = self._hyx_batch.read_tc()
+        self._hyx_step_res = [(_outcome(o), float(x)) for o, x in zip(oc, h)]
         return oc, h
+
+    def _run_events(self, oc, t0, write_tc):
+        """Callbacks of the events the last device step found, as heyoka's batch step runs
+        them: per batch element, the non-terminal ones in chronological order (state and
+        time at the end of the step), then the terminal one; a terminal event whose
+        callback returns True reports its index, otherwise -(index + 1)."""
+        te, ds, cnt, tau, code = self._hyx_batch.events_read()
+        cap = tau.shape[0]
+        over = np.nonzero(cnt > cap)[0]
+        if over.size:
+            raise RuntimeError(f"Batch element {int(over[0])} triggered {int(cnt[over[0]])} non-terminal events in one "
+                               f"step; at most {cap} are recorded per step: lower max_delta_t")
+        busy = (te >= 0) | (cnt > 0)
+        if write_tc or busy.any():
+            # Callbacks may call update_d_output(): the coefficients of this step.
+            self._hyx_tc[...] = self._hyx_batch.read_tc()
+        oc = oc.copy()
+        for b in np.nonzero(cnt > 0)[0]:
+            k = int(cnt[b])
+            taus = tau[:k, b]
+            recs = sorted(range(k), key=lambda r: (abs(taus[r]), int(code[r, b]) >> 2))
+            for r in recs:
+                c = int(code[r, b])
+                ev = self._hyx_ntes[c >> 2]
+                t = dfl_add(t0[0][b], t0[1][b], taus[r], 0.0)[0]
+                _call_event(ev.callback, self, float(t), (c & 3) - 1, int(b))
+        for b in np.nonzero(te >= 0)[0]:
+            i = int(te[b])
+            cb = self._hyx_tes[i].callback
+            go = False
+            if cb is not None:
+                r = _call_event(cb, self, int(ds[b]), int(b))
+                if not isinstance(r, (bool, np.bool_, int, float, np.integer, np.floating)):
+                    raise RuntimeError("The call operator of a terminal event callback returned an object of type "
+                                       f"{type(r)} which cannot be converted to bool: the error happened in the "
+                                       "construction of the return value of an event callback")
+                go = bool(r)
+            oc[b] = i if go else -(i + 1)
+        return oc
 
     def step(self, max_delta_t=None, write_tc=False):
         """One adaptive step per lane; ``max_delta_t`` (per lane) clamps |h| and sets its sign."""
@@ -278,7 +379,7 @@ class taylor_adaptive_batch:
             raise ValueError("A non-finite time was passed to the propagate_until() function of an adaptive Taylor "
                              "integrator")
         cb = self._check_cb(callback)
-        if cb is None and not write_tc:
+        if cb is None and not write_tc and self._device_events_ok():
             self._push()
             r = self._hyx_batch.propagate_until(tf_hi, tf_lo, mdt, int(max_steps),
                                                 _MODES[self._hyx_flags[3]])
@@ -287,6 +388,11 @@ class taylor_adaptive_batch:
                                   for o, a, b, c in zip(r["outcome"], r["min_h"], r["max_h"], r["nsteps"])]
             return None, None
         return self._prop_stepwise(tf_hi, tf_lo, int(max_steps), mdt, cb, write_tc)
+
+    def _device_events_ok(self):
+        """Propagations run entirely on the device unless an event needs the host: every
+        non-terminal event and every terminal event with a callback does."""
+        return not self._hyx_ntes and all(e.callback is None for e in self._hyx_tes)
 
     def _prop_stepwise(self, tf_hi, tf_lo, max_steps, mdt, cb, write_tc, after_step=None):
         # Host-driven loop, one device step per iteration, so that a step
@@ -311,12 +417,17 @@ class taylor_adaptive_batch:
             it += 1
             act = ~done
             nst += (act & (h != 0)).astype(np.int64)
-            succ = act & (oc == int(taylor_outcome.success))
+            # Steps that ended normally or at a terminal event (codes above success) count
+            # towards min_h / max_h; a stopping terminal event (-(index + 1)) ends the lane.
+            succ = act & (oc >= int(taylor_outcome.success))
             min_h = np.where(succ, np.minimum(min_h, np.abs(h)), min_h)
             max_h = np.where(succ, np.maximum(max_h, np.abs(h)), max_h)
             nf = act & (oc == int(taylor_outcome.err_nf_state))
             res_oc[nf] = int(taylor_outcome.err_nf_state)
             done |= nf
+            ste = act & (oc > int(taylor_outcome.success)) & (oc < 0)
+            res_oc[ste] = oc[ste]
+            done |= ste
             fin = act & ~nf & (oc == int(taylor_outcome.time_limit)) & (h == rem)
             done |= fin
             if after_step is not None:
@@ -385,7 +496,7 @@ class taylor_adaptive_batch:
                 raise ValueError("A non-monotonic time grid was passed to propagate_grid()")
         mdt = self._prop_args(max_steps, max_delta_t)
         cb = self._check_cb(callback)
-        if cb is None:
+        if cb is None and self._device_events_ok():
             self._push()
             r, out = self._hyx_batch.propagate_grid(g, mdt, int(max_steps), _MODES[self._hyx_flags[3]])
             self._pull()
@@ -521,7 +632,39 @@ class taylor_adaptive_batch:
 
     @property
     def with_events(self):
-        return False
+        return bool(self._hyx_tes or self._hyx_ntes)
+
+    @property
+    def t_events(self):
+        return [_event_ref(e, self) for e in self._hyx_tes]
+
+    @property
+    def nt_events(self):
+        return [_event_ref(e, self) for e in self._hyx_ntes]
+
+    @property
+    def te_cooldowns(self):
+        """Per batch element, per terminal event: None, or (trigger time, cooldown length)."""
+        B = self._hyx_B
+        if not self._hyx_tes:
+            return [[] for _ in range(B)]
+        cd = self._hyx_batch.cooldowns()
+        return [[(float(cd[e, 0, b] + cd[e, 1, b]), float(cd[e, 2, b])) if cd[e, 2, b] >= 0 else None
+                 for e in range(len(self._hyx_tes))] for b in range(B)]
+
+    def reset_cooldowns(self, i=None):
+        """Clear the cooldowns of every batch element, or of batch element i."""
+        if not self._hyx_tes:
+            return
+        cd = self._hyx_batch.cooldowns()
+        if i is None:
+            cd[:, 2, :] = -1.0
+        else:
+            if not 0 <= int(i) < self._hyx_B:
+                raise ValueError(f"Cannot reset the cooldowns at batch index {i}: the batch size for this integrator "
+                                 f"is only {self._hyx_B}")
+            cd[:, 2, int(i)] = -1.0
+        self._hyx_batch.set_cooldowns(cd)
 
     # ---- variational --------------------------------------------------------------------
     @property
@@ -612,13 +755,18 @@ class taylor_adaptive_batch:
     def _user_attrs(self):
         return {k: v for k, v in self.__dict__.items() if not k.startswith("_hyx_")}
 
+    def _events_copy(self):
+        return ([_copy.copy(e) for e in self._hyx_tes], [_copy.copy(e) for e in self._hyx_ntes])
+
     def _clone(self, user):
         new = taylor_adaptive_batch.__new__(taylor_adaptive_batch)
         f = self._hyx_flags
         restore = {k: _copy.deepcopy(self.__dict__[k]) for k in self._STATE_KEYS}
         sysarg = self._hyx_vsys if self._hyx_vsys is not None else self._hyx_pairs
         new._hyx_init(sysarg, self._hyx_state, self._hyx_thi, self._hyx_pars, self._hyx_tol, f[0], f[1], f[2], f[3],
-                      f[4], restore=restore)
+                      f[4], restore=restore, events=self._events_copy())
+        if self._hyx_tes:
+            new._hyx_batch.set_cooldowns(self._hyx_batch.cooldowns())
         new.__dict__.update(user)
         return new
 
@@ -633,13 +781,17 @@ class taylor_adaptive_batch:
         st["_hyx_ctor"] = (self._hyx_vsys if self._hyx_vsys is not None else self._hyx_pairs, self._hyx_tol,
                            self._hyx_flags)
         st["_hyx_user"] = self._user_attrs()
+        st["_hyx_events"] = (self._hyx_tes, self._hyx_ntes)
+        st["_hyx_cooldowns"] = self._hyx_batch.cooldowns() if self._hyx_tes else None
         return st
 
     def __setstate__(self, st):
         sysarg, tol, f = st["_hyx_ctor"]
         restore = {k: st[k] for k in self._STATE_KEYS}
         self._hyx_init(sysarg, restore["_hyx_state"], restore["_hyx_thi"], restore["_hyx_pars"], tol, f[0], f[1], f[2],
-                       f[3], f[4], restore=restore)
+                       f[3], f[4], restore=restore, events=st.get("_hyx_events", ([], [])))
+        if st.get("_hyx_cooldowns") is not None:
+            self._hyx_batch.set_cooldowns(st["_hyx_cooldowns"])
         self.__dict__.update(st["_hyx_user"])
 
     def __repr__(self):

@@ -147,6 +147,43 @@ int hyx_propagate_until(hyx_batch *b, const double *tf_hi, const double *tf_lo, 
 int hyx_propagate_grid(hyx_batch *b, const double *grid, uint64_t n_pts, const double *max_delta_t,
                        uint64_t max_steps, int mode, hyx_prop_res *res, double *out);
 
+/* Event detection: the t_events / nt_events of taylor_adaptive_batch
+ * (heyoka/expose_batch_integrators.cpp:166-208, events bound at
+ * heyoka/taylor_expose_events.cpp:195-300). The event op-list has a row per state
+ * variable of the system, in the same order (any further rows are zero-valued
+ * padding), and the event functions as the right-hand sides of its first
+ * n_te + n_nt rows, terminal events first. direction: [n_te + n_nt] values -1
+ * (negative), 0 (any), 1 (positive); cooldown: [n_te], negative = deduced from tol
+ * and the event's time derivative at the trigger. Systems with events use the lane
+ * code forms (unrolled or compact); every step detects the events of every instance:
+ * a terminal trigger shortens the step to it (outcome -(index + 1)), and propagations
+ * end there; non-terminal triggers are recorded for hyx_events_read. */
+typedef struct {
+    const uint32_t *ops;
+    uint32_t n_words;
+    const double *consts;
+    uint32_t n_consts;
+    uint32_t n_te, n_nt;
+    const int32_t *direction;
+    const double *cooldown;
+    double tol;
+} hyx_events;
+int hyx_sys_create_events(const uint32_t *ops, uint32_t n_words, const double *consts, uint32_t n_consts,
+                          uint32_t order, uint32_t flags, const hyx_events *ev, hyx_sys **out);
+/* with_events / len(t_events) / len(nt_events) (expose_batch_integrators.cpp:651-656). */
+int hyx_sys_n_events(const hyx_sys *s, uint32_t *n_te, uint32_t *n_nt);
+/* Events of the last hyx_step (what heyoka's batch step passes to the callbacks):
+ * te [N] terminal event index or -1, te_dsgn [N] its direction, nt_count [N]
+ * non-terminal triggers, nt_tau / nt_code [cap][N] the first min(count, cap) of them:
+ * offset from the start of the step, and 4 * (non-terminal index) + d_sgn + 1.
+ * A count above cap means records were lost. Any pointer may be NULL. */
+int hyx_events_read(hyx_batch *b, int32_t *te, int32_t *te_dsgn, uint32_t *nt_count, double *nt_tau,
+                    int32_t *nt_code, uint32_t *cap);
+/* te_cooldowns / reset_cooldowns (expose_batch_integrators.cpp:652-654): [n_te][3][N]
+ * trigger time (hi, lo) and cooldown length; a negative length = no cooldown. */
+int hyx_cooldowns_read(hyx_batch *b, double *dst);
+int hyx_cooldowns_write(hyx_batch *b, const double *src);
+
 /* Kernel time (ms, HIP events on the batch's stream) and launch count of the
  * last hyx_propagate_until / hyx_step call. */
 int hyx_batch_last_timing(const hyx_batch *b, double *kernel_ms, uint64_t *launches);

@@ -163,6 +163,28 @@ def main():
     fl = _floats(txt)
     g["var2_displaced"] = {"src": f"doc/notebooks/{vo}:650-651", "dx": 1e-4, "dv": -2e-4, "nonvar_state": fl[0:2], "digits": 8}
 
+    # Event detection (pendulum x' = v, v' = -9.8 sin(x), x0 = -0.05, v0 = 0).
+    ed = "Event detection.ipynb"
+    txt = _out_text(_cell_by_source(ed, "print(zero_vel_times)", 0))
+    g["ev_zero_vel_times"] = {"src": f"doc/notebooks/{ed}:{_line_of(ed, '[0.0, 1.003701787940065')}",
+                              "times": _floats(txt), "event": "nt v = 0, direction any", "ic": [-0.05, 0.0],
+                              "t_end": 5.0}
+    txt = _out_text(_cell_by_source(ed, "print(zero_vel_times)", 1))
+    g["ev_positive_times"] = {"src": f"doc/notebooks/{ed}:{_line_of(ed, '[0.0, 2.00740357588013, 4.01480715176026]')}",
+                              "times": _floats(txt), "event": "nt v = 0, direction positive", "ic": [-0.05, 0.0],
+                              "t_end": 5.0}
+    txt = _out_text(_cell_by_source(ed, "ta.propagate_until(5.0)"))
+    trig = re.findall(r"Event (\d) triggering at t=([-+0-9.eE]+)", txt)
+    tail = txt[txt.index("(<taylor_outcome"):]
+    fl = _floats(tail.split(">", 1)[1])
+    g["ev_two_events"] = {"src": f"doc/notebooks/{ed}:{_line_of(ed, 'Event 0 triggering at t=0.0')}",
+                          "events": [[int(i), float(t)] for i, t in trig], "ev0": "v", "ev1": "v * v - 1e-12",
+                          "ic": [-0.05, 0.0], "t_end": 5.0, "outcome": "time_limit", "min_h": fl[0], "max_h": fl[1],
+                          "nsteps": int(fl[2])}
+    txt = _out_text(_cell_by_source(ed, "x_hist = ta.propagate_grid(t_grid)[5][:, 0]", 0))
+    g["ev_x_at_zero_vel"] = {"src": f"doc/notebooks/{ed}:{_line_of(ed, 'Value of x when v is zero: -0.05')}",
+                             "x": [float(m) for m in re.findall(r"Value of x when v is zero: ([-+0-9.eE]+)", txt)]}
+
     with open(OUT, "w") as f:
         json.dump(g, f, indent=1)
     print(f"wrote {OUT} ({len(g)} goldens)")
