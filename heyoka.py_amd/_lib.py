@@ -41,7 +41,8 @@ class tune_res(ct.Structure):
 class events_desc(ct.Structure):
     _fields_ = [("ops", ct.POINTER(ct.c_uint32)), ("n_words", ct.c_uint32), ("consts", ct.POINTER(ct.c_double)),
                 ("n_consts", ct.c_uint32), ("n_te", ct.c_uint32), ("n_nt", ct.c_uint32),
-                ("direction", ct.POINTER(ct.c_int32)), ("cooldown", ct.POINTER(ct.c_double)), ("tol", ct.c_double)]
+                ("direction", ct.POINTER(ct.c_int32)), ("cooldown", ct.POINTER(ct.c_double)),
+                ("stopping", ct.POINTER(ct.c_int32)), ("tol", ct.c_double)]
 
 
 class HyxError(RuntimeError):
@@ -117,7 +118,12 @@ def lib():
             "hyx_slots_results": (ct.c_int, [vp, P(u32), u32, P(u32), P(prop_res)]),
             "hyx_sys_create_events": (ct.c_int, [P(u32), u32, P(d), u32, u32, u32, P(events_desc), P(vp)]),
             "hyx_sys_n_events": (ct.c_int, [vp, P(u32), P(u32)]),
-            "hyx_events_read": (ct.c_int, [vp, P(ct.c_int32), P(ct.c_int32), P(u32), P(d), P(ct.c_int32), P(u32)]),
+            "hyx_events_read": (ct.c_int, [vp, P(i64), P(u32), P(d), P(i64), P(u32)]),
+            "hyx_batch_set_event_pause": (ct.c_int, [vp, ct.c_int]),
+            "hyx_events_parked": (ct.c_int, [vp, P(u32), P(u32)]),
+            "hyx_events_gather": (ct.c_int, [vp, P(u32), u32, P(i64), P(u32), P(d), P(i64), P(d), P(d)]),
+            "hyx_events_resume": (ct.c_int, [vp, P(u32), u32, P(i64)]),
+            "hyx_propagate_resume": (ct.c_int, [vp, u64, ct.c_int, P(prop_res), P(d)]),
             "hyx_cooldowns_read": (ct.c_int, [vp, P(d)]),
             "hyx_cooldowns_write": (ct.c_int, [vp, P(d)]),
             "hyx_last_error": (ct.c_char_p, []),
@@ -144,6 +150,8 @@ EXPORTED = (
     "hyx_slots_snapshot_words", "hyx_slots_snapshot", "hyx_slots_restore",
     "hyx_cfunc_create", "hyx_cfunc_source", "hyx_cfunc_eval", "hyx_cfunc_eval_batch", "hyx_cfunc_destroy",
     "hyx_sys_create_events", "hyx_sys_n_events", "hyx_events_read", "hyx_cooldowns_read", "hyx_cooldowns_write",
+    "hyx_batch_set_event_pause", "hyx_events_parked", "hyx_events_gather", "hyx_events_resume",
+    "hyx_propagate_resume",
     "hyx_last_error", "hyx_device_count",
 )
 
@@ -194,15 +202,17 @@ class EventSpec:
     padding; right-hand sides = event functions, terminal ones first), directions,
     terminal cooldowns (< 0: automatic) and the integrator tolerance."""
 
-    def __init__(self, dec, n_te: int, n_nt: int, directions, cooldowns, tol: float):
+    def __init__(self, dec, n_te: int, n_nt: int, directions, cooldowns, tol: float, stopping=None):
         self.dec, self.n_te, self.n_nt, self.tol = dec, int(n_te), int(n_nt), float(tol)
         self.directions = np.ascontiguousarray(directions, dtype=np.int32)
         self.cooldowns = np.ascontiguousarray(cooldowns if len(cooldowns) else [0.0], dtype=np.float64)
+        st = [1] * self.n_te if stopping is None else [int(bool(x)) for x in stopping]
+        self.stopping = np.ascontiguousarray(st if st else [1], dtype=np.int32)
 
     def key(self):
         w, c = oplist_args(self.dec)
         return (w.tobytes(), c.tobytes(), self.n_te, self.n_nt, self.directions.tobytes(), self.cooldowns.tobytes(),
-                self.tol)
+                self.stopping.tobytes(), self.tol)
 
 
 class System:
@@ -222,9 +232,10 @@ class System:
                                    ct.byref(h)))
         else:
             ew, ec = oplist_args(events.dec)
-            self._ev_keep = (ew, ec, events.directions, events.cooldowns)
+            self._ev_keep = (ew, ec, events.directions, events.cooldowns, events.stopping)
             ed = events_desc(dptr(ew, ct.c_uint32), len(ew), dptr(ec), len(events.dec.consts), events.n_te,
-                             events.n_nt, dptr(events.directions, ct.c_int32), dptr(events.cooldowns), events.tol)
+                             events.n_nt, dptr(events.directions, ct.c_int32), dptr(events.cooldowns),
+                             dptr(events.stopping, ct.c_int32), events.tol)
             check(L.hyx_sys_create_events(dptr(words, ct.c_uint32), len(words), dptr(consts), len(dec.consts), order,
                                           flags, ct.byref(ed), ct.byref(h)))
         self.events = events
@@ -287,19 +298,58 @@ class Batch:
         check(self._L.hyx_step(self.h, dptr(md), int(write_tc), dptr(oc, ct.c_int64), dptr(h)))
         return oc, h
 
-    def events_read(self):
-        """The last step's events: (te [N] (-1: none), te_dsgn [N], nt_count [N],
-        nt_tau [cap, N], nt_code [cap, N])."""
-        te = np.empty(self.N, dtype=np.int32)
-        ds = np.empty(self.N, dtype=np.int32)
-        cnt = np.empty(self.N, dtype=np.uint32)
+    def _cap(self):
         cap = ct.c_uint32()
-        check(self._L.hyx_events_read(self.h, None, None, None, None, None, ct.byref(cap)))
-        tau = np.empty((cap.value, self.N), dtype=np.float64)
-        code = np.empty((cap.value, self.N), dtype=np.int32)
-        check(self._L.hyx_events_read(self.h, dptr(te, ct.c_int32), dptr(ds, ct.c_int32), dptr(cnt, ct.c_uint32),
-                                      dptr(tau), dptr(code, ct.c_int32), ct.byref(cap)))
-        return te, ds, cnt, tau, code
+        check(self._L.hyx_events_read(self.h, None, None, None, None, ct.byref(cap)))
+        return cap.value
+
+    def events_read(self):
+        """The last step's events: (te [N] (4 * index + d_sgn + 1, or -1), nt_count [N],
+        nt_time [cap, N], nt_code [cap, N])."""
+        cap = self._cap()
+        te = np.empty(self.N, dtype=np.int64)
+        cnt = np.empty(self.N, dtype=np.uint32)
+        tm = np.empty((cap, self.N), dtype=np.float64)
+        code = np.empty((cap, self.N), dtype=np.int64)
+        check(self._L.hyx_events_read(self.h, dptr(te, ct.c_int64), dptr(cnt, ct.c_uint32), dptr(tm),
+                                      dptr(code, ct.c_int64), None))
+        return te, cnt, tm, code
+
+    def set_event_pause(self, on: bool):
+        check(self._L.hyx_batch_set_event_pause(self.h, int(bool(on))))
+
+    def events_parked(self) -> np.ndarray:
+        lanes = np.empty(self.N, dtype=np.uint32)
+        n = ct.c_uint32()
+        check(self._L.hyx_events_parked(self.h, dptr(lanes, ct.c_uint32), ct.byref(n)))
+        return lanes[: n.value].copy()
+
+    def events_gather(self, lanes, with_tc=True):
+        """For parked instances: (te, nt_count, nt_time [cap, k], nt_code [cap, k], h, tc [n, p+1, k])."""
+        sl = np.ascontiguousarray(lanes, dtype=np.uint32)
+        k = len(sl)
+        cap = self._cap()
+        te = np.empty(k, dtype=np.int64)
+        cnt = np.empty(k, dtype=np.uint32)
+        tm = np.empty((cap, k), dtype=np.float64)
+        code = np.empty((cap, k), dtype=np.int64)
+        h = np.empty(k, dtype=np.float64)
+        tc = np.empty((self.sys.n, self.sys.order + 1, k), dtype=np.float64) if with_tc else None
+        check(self._L.hyx_events_gather(self.h, dptr(sl, ct.c_uint32), k, dptr(te, ct.c_int64), dptr(cnt, ct.c_uint32),
+                                        dptr(tm), dptr(code, ct.c_int64), dptr(h), dptr(tc)))
+        return te, cnt, tm, code, h, tc
+
+    def events_resume(self, lanes, stop):
+        sl = np.ascontiguousarray(lanes, dtype=np.uint32)
+        st = np.ascontiguousarray(stop, dtype=np.int64)
+        check(self._L.hyx_events_resume(self.h, dptr(sl, ct.c_uint32), len(sl), dptr(st, ct.c_int64)))
+
+    def propagate_resume(self, max_steps=0, mode=MODE_PERSISTENT, grid_out=None):
+        res = (prop_res * self.N)()
+        check(self._L.hyx_propagate_resume(self.h, int(max_steps), int(mode), res, dptr(grid_out)))
+        arr = np.frombuffer(res, dtype=np.dtype([("outcome", "<i8"), ("min_h", "<f8"), ("max_h", "<f8"),
+                                                 ("nsteps", "<u8")]), count=self.N)
+        return arr.copy()
 
     def cooldowns(self) -> np.ndarray:
         """[n_te, 3, N]: trigger time hi, lo and length (< 0: no cooldown)."""

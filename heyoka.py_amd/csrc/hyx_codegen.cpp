@@ -1570,6 +1570,10 @@ std::string generate_events_prelude(const Sys &E, uint32_t n_sys, int P, const E
     for (uint32_t e = 0; e < std::max<uint32_t>(ev.n_te, 1); ++e)
         o << (e ? ", " : "") << lit(e < ev.n_te ? ev.cooldown[e] : -1.0);
     o << "};\n";
+    o << "__constant__ int hyx_ev_stop[" << std::max<uint32_t>(ev.n_te, 1) << "] = {";
+    for (uint32_t e = 0; e < std::max<uint32_t>(ev.n_te, 1); ++e)
+        o << (e ? ", " : "") << ((e < ev.n_te && e < ev.stopping.size()) ? ev.stopping[e] : 1);
+    o << "};\n";
     // Event-function jet: the event op-list's nodes at orders 0..P from the state
     // coefficients of the step (rows 0..n-1 of tc); node rows live in et.
     o << "#define T(node, kk) (((node) < " << n_sys << "u) ? tc[((hyx_u64)(node) * (HYX_P + 1) + (hyx_u64)(kk)) * ld] : "

@@ -67,6 +67,7 @@ struct EventSpec {
     uint32_t n_te = 0;
     std::vector<int> dir;
     std::vector<double> cooldown;
+    std::vector<int> stopping; // terminal events without a callback: always stop the instance
     double tol = 0.0;
 };
 

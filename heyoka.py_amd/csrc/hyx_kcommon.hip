@@ -24,6 +24,7 @@ typedef long long hyx_i64;
 #define HYX_STEP_LIMIT (-4294967298LL)
 #define HYX_TIME_LIMIT (-4294967299LL)
 #define HYX_ERR_NF_STATE (-4294967300LL)
+#define HYX_PARKED (1LL << 40) // internal: the instance parks for the host (never an outcome)
 
 // Kernel argument block. The host fills the identical struct (hyx_runtime.cpp).
 struct hyx_kargs {
@@ -60,14 +61,19 @@ struct hyx_kargs {
     hyx_u64 npts;           // propagate_grid: grid points (0 outside propagate_grid)
     double *et;             // events: [event nodes * (P+1)][ld] event-function coefficients
     double *evcd;           // events: [n_te][3][ld] cooldowns: trigger time hi, lo, length (< 0: none)
-    int *te;                // events: [N] terminal event of the last step (-1: none)
-    int *te_dsgn;           // events: [N] its direction (-1, 0, 1)
+    hyx_i64 *te;            // events: [N] last step's terminal event: 4 * index + d_sgn + 1, or -1
     unsigned *ntc;          // events: [N] non-terminal events found in the last step
-    double *nt_tau;         // events: [nt_cap][ld] their offsets from the start of the step
-    int *nt_code;           // events: [nt_cap][ld] 4 * (non-terminal index) + d_sgn + 1
+    double *nt_time;        // events: [nt_cap][ld] their times
+    hyx_i64 *nt_code;       // events: [nt_cap][ld] 4 * (non-terminal index) + d_sgn + 1
     unsigned nt_cap;        // events: records kept per instance (0: none)
-    unsigned pad2_;
+    unsigned ev_pause;      // events: propagations park a lane after a step that needs the host
 };
+
+// Propagation status of an instance: 0 / 2 running forward / backward, 1 done,
+// 4 / 6 parked (forward / backward) after a step whose events need the host.
+#define HYX_ST_DONE 1u
+#define HYX_ST_PARKED 4u
+__device__ __forceinline__ bool hyx_runnable(unsigned st) { return (st & ~2u) == 0u; }
 
 // ---- double-length time -----------------------------------------------------------
 // Knuth's error-free sum (TAOCP vol. 2, 4.2.2).

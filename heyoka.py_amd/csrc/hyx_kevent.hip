@@ -22,6 +22,11 @@
 //   - non-terminal triggers before the end of the (truncated) step are recorded
 //     for the host, which runs their callbacks in chronological order.
 //
+// In propagations with a.ev_pause set, a lane whose step needs the host (a
+// non-terminal trigger, or a terminal one whose event has a callback) is parked
+// at the end of that step (status HYX_ST_PARKED); the host runs the callbacks and
+// resumes it. Without a.ev_pause every terminal trigger stops the lane.
+//
 // Root isolation (per lane, only when a fast exclusion test fails): the
 // polynomial is rescaled to s in [0, 1] (tau = s h) and dyadic subintervals are
 // visited left to right; on each, Descartes' rule of signs counts the sign
@@ -198,11 +203,13 @@ __device__ __forceinline__ void hyx_ev_scan(const double (&g)[HYX_P + 1], double
 // (thi, tlo): tc holds the step's state coefficients (column i, stride ld) and x
 // the state advanced by h. Returns the step actually taken: h, or the offset of
 // the earliest terminal trigger (x is then re-evaluated there and nf updated).
-// te / te_dsgn: that event (-1: none) and its direction. The sub-lanes of an
-// instance hold identical values and write identical records.
+// te / te_dsgn: that event (-1: none) and its direction; ntc_out: the non-terminal
+// triggers found, whose times and codes go to a.nt_time / a.nt_code (the first
+// a.nt_cap of them). The sub-lanes of an instance hold identical values and write
+// identical records.
 __device__ __forceinline__ double hyx_ev_step(const hyx_kargs &a, hyx_u64 i, const double *pv, double thi,
                                               double tlo, double h, double (&x)[HYX_N], bool &nf, int &te,
-                                              int &te_dsgn)
+                                              int &te_dsgn, unsigned &ntc_out)
 {
     // The step already formed the event functions' coefficients (hyx_ev_norms).
     double *tc = a.tc + i;
@@ -242,7 +249,9 @@ __device__ __forceinline__ double hyx_ev_step(const hyx_kargs &a, hyx_u64 i, con
                 if (fabs(tau) >= best) return true;
                 if (dir != 0 && ds != dir) return false;
                 if (ntc < a.nt_cap) {
-                    a.nt_tau[(hyx_u64)ntc * ld + i] = tau;
+                    double th = thi, tl = tlo;
+                    hyx_dfl_add(th, tl, tau, 0.0);
+                    a.nt_time[(hyx_u64)ntc * ld + i] = th;
                     a.nt_code[(hyx_u64)ntc * ld + i] = 4 * (e - HYX_NTE) + ds + 1;
                 }
                 ++ntc;
@@ -299,8 +308,8 @@ __device__ __forceinline__ double hyx_ev_step(const hyx_kargs &a, hyx_u64 i, con
         a.evcd[o + ld] = nlo;
         a.evcd[o + 2 * ld] = cd;
     }
-    a.te[i] = te;
-    a.te_dsgn[i] = te_dsgn;
+    a.te[i] = (te >= 0) ? (hyx_i64)(4 * te + te_dsgn + 1) : -1;
     a.ntc[i] = ntc;
+    ntc_out = ntc;
     return hend;
 }

@@ -35,6 +35,7 @@ struct hyx_inst {
     double pv[HYX_NPAR];
     double thi, tlo, tfhi, tflo, mdt, minh, maxh;
     hyx_u64 iter, nst;
+    double hlast; // the last step's size (kept for the host when the instance parks)
     int tdir;
     unsigned gc; // propagate_grid: next grid point
 };
@@ -97,11 +98,15 @@ __device__ __forceinline__ bool hyx_prop_iter(hyx_inst &s, const hyx_kargs &a, h
     hyx_taylor_step<HYX_PROP_WTC>(s.x, s.pv, s.thi, lim, h, nf, HYX_PROP_WTC ? a.tc + i : nullptr, a.ld,
                                   a.tape ? a.tape + lane_tape : nullptr, a.tape_stride);
     int te = -1;
+    bool park = false;
 #ifdef HYX_EVENTS
     int te_dsgn;
-    h = hyx_ev_step(a, i, s.pv, s.thi, s.tlo, h, s.x, nf, te, te_dsgn);
+    unsigned ntc;
+    h = hyx_ev_step(a, i, s.pv, s.thi, s.tlo, h, s.x, nf, te, te_dsgn, ntc);
+    park = a.ev_pause && ((te >= 0 && !hyx_ev_stop[te]) || ntc > 0u);
 #endif
     hyx_dfl_add(s.thi, s.tlo, h, 0.0);
+    s.hlast = h;
     nf = nf || !isfinite(s.thi);
     s.iter += 1;
     s.nst += (h != 0.0) ? 1 : 0;
@@ -119,6 +124,11 @@ __device__ __forceinline__ bool hyx_prop_iter(hyx_inst &s, const hyx_kargs &a, h
 #ifdef HYX_GRID
     hyx_grid_lane(s, a, i, h, fin);
 #endif
+    if (park) {
+        // The host runs this step's callbacks, then resumes (or stops) the instance.
+        *res = HYX_PARKED;
+        return true;
+    }
     if (te >= 0) {
         *res = -(hyx_i64)te - 1;
         return true;
@@ -162,8 +172,11 @@ __device__ __forceinline__ void hyx_suspend(const hyx_inst &s, const hyx_kargs &
 #ifdef HYX_GRID
     a.gcur[i] = s.gc;
 #endif
-    if (done) {
-        a.status[i] = 1u;
+    if (done && res == HYX_PARKED) {
+        a.status[i] = HYX_ST_PARKED | (s.tdir ? 0u : 2u);
+        a.h_out[i] = s.hlast;
+    } else if (done) {
+        a.status[i] = HYX_ST_DONE;
         a.outcome[i] = res;
     }
 }
@@ -183,7 +196,8 @@ __device__ __forceinline__ void hyx_step_body(const hyx_kargs &a)
     // Events need the coefficients of every step.
     hyx_taylor_step<true>(s.x, s.pv, s.thi, mdt, h, nf, a.tc + i, a.ld, a.tape ? a.tape + i : nullptr, a.tape_stride);
     int te, te_dsgn;
-    h = hyx_ev_step(a, i, s.pv, s.thi, s.tlo, h, s.x, nf, te, te_dsgn);
+    unsigned ntc;
+    h = hyx_ev_step(a, i, s.pv, s.thi, s.tlo, h, s.x, nf, te, te_dsgn, ntc);
 #else
     hyx_taylor_step<WTC>(s.x, s.pv, s.thi, mdt, h, nf, WTC ? a.tc + i : nullptr, a.ld, a.tape ? a.tape + i : nullptr,
                          a.tape_stride);
@@ -224,7 +238,7 @@ extern "C" __global__ void __launch_bounds__(64) hyx_iter_k(hyx_kargs a)
     bool still = false;
     if (i < a.N) {
         const unsigned st = a.status[i];
-        if (st != 1u) {
+        if (hyx_runnable(st)) {
             hyx_inst s;
             hyx_resume(s, a, i, st);
             hyx_i64 res = HYX_TIME_LIMIT;
@@ -284,7 +298,7 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
                 if (pos < a.N) {
                     inst = a.queue ? (hyx_u64)a.queue[pos] : pos;
                     const unsigned st = a.status[inst];
-                    if (st != 1u) {
+                    if (hyx_runnable(st)) {
                         hyx_resume(s, a, inst, st);
                         budget = 0;
                         active = true;

@@ -85,13 +85,12 @@ struct KArgs {
     uint64_t npts;
     double *et;
     double *evcd;
-    int *te;
-    int *te_dsgn;
+    int64_t *te;
     unsigned *ntc;
-    double *nt_tau;
-    int *nt_code;
+    double *nt_time;
+    int64_t *nt_code;
     unsigned nt_cap;
-    unsigned pad2_;
+    unsigned ev_pause;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -269,10 +268,12 @@ struct hyx_batch {
     // Events (systems with events only): et = the event-node coefficient rows, which
     // follow the state rows inside tc; [n_te][3][N] cooldowns, the last step's terminal event and its direction,
     // and its non-terminal records ([nt_cap][N] offsets and codes).
-    double *et = nullptr, *evcd = nullptr, *nt_tau = nullptr;
-    int *te = nullptr, *te_dsgn = nullptr, *nt_code = nullptr;
+    double *et = nullptr, *evcd = nullptr, *nt_time = nullptr;
+    int64_t *te = nullptr, *nt_code = nullptr;
     unsigned *ntc = nullptr;
     unsigned nt_cap = 0;
+    bool ev_pause = false;       // propagations park instances whose events need the host
+    bool last_grid = false;      // the last propagation was a propagate_grid (for resumes)
 };
 
 namespace
@@ -309,6 +310,7 @@ int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint3
         for (uint32_t e = 0; e < ev->n_te; ++e) {
             if (!(ev->cooldown[e] == ev->cooldown[e])) return fail(HYX_EINVAL, "a NaN event cooldown was given");
             s.ev.cooldown.push_back(ev->cooldown[e]);
+            s.ev.stopping.push_back(ev->stopping ? (ev->stopping[e] != 0) : 1);
         }
         if (!(ev->tol > 0.0)) return fail(HYX_EINVAL, "the event specification needs the tolerance");
         if (flags & HYX_FLAG_FORCE_GROUP) return fail(HYX_EINVAL, "the group form does not support events");
@@ -393,11 +395,12 @@ KArgs make_args(hyx_batch *b)
     a.et = b->et;
     a.evcd = b->evcd;
     a.te = b->te;
-    a.te_dsgn = b->te_dsgn;
     a.ntc = b->ntc;
-    a.nt_tau = b->nt_tau;
+    a.nt_time = b->nt_time;
     a.nt_code = b->nt_code;
-    a.nt_cap = 0; // records are kept by hyx_step only
+    // Records are kept by hyx_step and by propagations that park.
+    a.nt_cap = b->ev_pause ? b->nt_cap : 0;
+    a.ev_pause = b->ev_pause ? 1u : 0u;
     return a;
 }
 
@@ -797,11 +800,10 @@ int hyx_batch_create(hyx_sys *s, uint64_t n_inst, hyx_batch **out)
         } ev_allocs[] = {
             {(void **)&b->tc, (n * P1 + s->ev_rows) * N * 8},
             {(void **)&b->evcd, 3 * nte * N * 8},
-            {(void **)&b->te, N * 4},
-            {(void **)&b->te_dsgn, N * 4},
+            {(void **)&b->te, N * 8},
             {(void **)&b->ntc, N * 4},
-            {(void **)&b->nt_tau, (uint64_t)b->nt_cap * N * 8},
-            {(void **)&b->nt_code, (uint64_t)b->nt_cap * N * 4},
+            {(void **)&b->nt_time, (uint64_t)b->nt_cap * N * 8},
+            {(void **)&b->nt_code, (uint64_t)b->nt_cap * N * 8},
         };
         for (auto &al : ev_allocs) {
             if (hipMalloc(al.p, al.bytes) != hipSuccess) return cleanup(HYX_ENOMEM, "hipMalloc failed (device memory)");
@@ -829,7 +831,7 @@ void hyx_batch_destroy(hyx_batch *b)
     void *ptrs[] = {b->state, b->pars, b->thi,    b->tlo,   b->tfhi,   b->tflo,  b->mdt,
                     b->hout,  b->minh, b->maxh,   b->tc,    b->oc,     b->nsteps, b->iters,
                     b->work,  b->status, b->active, b->unfinished, b->queue, b->tape, b->stage, b->stage_idx,
-                    b->grid,  b->gout, b->gcur, b->evcd, b->te, b->te_dsgn, b->ntc, b->nt_tau, b->nt_code};
+                    b->grid,  b->gout, b->gcur, b->evcd, b->te, b->ntc, b->nt_time, b->nt_code};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (b->ev0) hipEventDestroy(b->ev0);
@@ -898,6 +900,7 @@ int hyx_step(hyx_batch *b, const double *max_dt, int write_tc, int64_t *outcome,
     if (int rc = ensure_tape(b, N)) return rc;
     KArgs a = make_args(b);
     a.nt_cap = b->nt_cap;
+    a.ev_pause = 0;
     HIPCHK(hipEventRecord(b->ev0, b->stream));
     HIPCHK(launch(write_tc ? b->sys->k_step_tc : b->sys->k_step, blocks_of(b), b->stream, &a));
     HIPCHK(hipEventRecord(b->ev1, b->stream));
@@ -911,19 +914,18 @@ int hyx_step(hyx_batch *b, const double *max_dt, int write_tc, int64_t *outcome,
     return 0;
 }
 
-int hyx_events_read(hyx_batch *b, int32_t *te, int32_t *te_dsgn, uint32_t *nt_count, double *nt_tau,
-                    int32_t *nt_code, uint32_t *cap)
+int hyx_events_read(hyx_batch *b, int64_t *te, uint32_t *nt_count, double *nt_time, int64_t *nt_code, uint32_t *cap)
 {
     if (!b) return fail(HYX_EINVAL, "null batch");
     if (!b->sys->has_events) return fail(HYX_EINVAL, "the system has no events");
     const uint64_t N = b->N;
     if (cap) *cap = b->nt_cap;
-    if (te) HIPCHK(hipMemcpyAsync(te, b->te, N * 4, hipMemcpyDeviceToHost, b->stream));
-    if (te_dsgn) HIPCHK(hipMemcpyAsync(te_dsgn, b->te_dsgn, N * 4, hipMemcpyDeviceToHost, b->stream));
+    if (te) HIPCHK(hipMemcpyAsync(te, b->te, N * 8, hipMemcpyDeviceToHost, b->stream));
     if (nt_count) HIPCHK(hipMemcpyAsync(nt_count, b->ntc, N * 4, hipMemcpyDeviceToHost, b->stream));
-    if (nt_tau) HIPCHK(hipMemcpyAsync(nt_tau, b->nt_tau, (uint64_t)b->nt_cap * N * 8, hipMemcpyDeviceToHost, b->stream));
+    if (nt_time)
+        HIPCHK(hipMemcpyAsync(nt_time, b->nt_time, (uint64_t)b->nt_cap * N * 8, hipMemcpyDeviceToHost, b->stream));
     if (nt_code)
-        HIPCHK(hipMemcpyAsync(nt_code, b->nt_code, (uint64_t)b->nt_cap * N * 4, hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipMemcpyAsync(nt_code, b->nt_code, (uint64_t)b->nt_cap * N * 8, hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     return 0;
 }
@@ -992,7 +994,7 @@ int ensure_grid_module(hyx_sys *s)
 
 // The propagate loop shared by propagate_until and propagate_grid: final
 // times, max_delta_t (and the grid buffers) are already on the device.
-int run_propagation(hyx_batch *b, uint64_t max_steps, int mode, hyx_prop_res *res, bool grid_run)
+int run_propagation(hyx_batch *b, uint64_t max_steps, int mode, hyx_prop_res *res, bool grid_run, bool init = true)
 {
     const uint64_t N = b->N;
     hyx_sys *s = b->sys;
@@ -1000,6 +1002,7 @@ int run_propagation(hyx_batch *b, uint64_t max_steps, int mode, hyx_prop_res *re
     hipFunction_t k_iter = grid_run ? s->kg_iter : s->k_iter;
     hipFunction_t k_persist = grid_run ? s->kg_persist : s->k_persist;
     if (grid_run && mode == HYX_MODE_GRAPH) mode = HYX_MODE_PLAIN; // no captured graph for the grid code object
+    if (b->ev_pause && mode == HYX_MODE_GRAPH) mode = HYX_MODE_PLAIN; // the graph holds arguments without parking
     uint64_t pgrid = 0;
     if (mode == HYX_MODE_PERSISTENT) {
         int occ = 0;
@@ -1016,7 +1019,8 @@ int run_propagation(hyx_batch *b, uint64_t max_steps, int mode, hyx_prop_res *re
         a.gcur = b->gcur;
         a.npts = b->npts;
     }
-    HIPCHK(launch(k_init, blocks_for(N), b->stream, &a));
+    if (init) HIPCHK(launch(k_init, blocks_for(N), b->stream, &a));
+    b->last_grid = grid_run;
     uint64_t launches = 0;
     HIPCHK(hipEventRecord(b->ev0, b->stream));
     if (mode == HYX_MODE_PERSISTENT) {
@@ -1573,6 +1577,126 @@ int hyx_batch_sync(hyx_batch *b)
 {
     if (!b) return fail(HYX_EINVAL, "null batch");
     HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+// ---- events: parking, gathering what the callbacks see, resuming --------------------------
+
+int hyx_batch_set_event_pause(hyx_batch *b, int on)
+{
+    if (!b) return fail(HYX_EINVAL, "null batch");
+    if (on && !b->sys->has_events) return fail(HYX_EINVAL, "the system has no events");
+    b->ev_pause = on != 0;
+    return 0;
+}
+
+static int read_status(hyx_batch *b, std::vector<unsigned> &st)
+{
+    st.resize(b->N);
+    HIPCHK(hipMemcpyAsync(st.data(), b->status, b->N * 4, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int hyx_events_parked(hyx_batch *b, uint32_t *lanes, uint32_t *count)
+{
+    if (!b || !count) return fail(HYX_EINVAL, "null argument");
+    std::vector<unsigned> st;
+    if (int rc = read_status(b, st)) return rc;
+    uint32_t c = 0;
+    for (uint64_t i = 0; i < b->N; ++i)
+        if (st[i] & 4u) {
+            if (lanes) lanes[c] = (uint32_t)i;
+            ++c;
+        }
+    *count = c;
+    return 0;
+}
+
+int hyx_events_gather(hyx_batch *b, const uint32_t *lanes, uint32_t count, int64_t *te, uint32_t *nt_count,
+                      double *nt_time, int64_t *nt_code, double *h, double *tc)
+{
+    if (!b) return fail(HYX_EINVAL, "null batch");
+    if (!b->sys->has_events) return fail(HYX_EINVAL, "the system has no events");
+    if (int rc = check_slots(b, lanes, count)) return rc;
+    if (count == 0) return 0;
+    if (te) {
+        if (int rc = gather64(b, b->te, 1, lanes, count, te)) return rc;
+    }
+    if (nt_count) {
+        if (int rc = stage_reserve(b, (size_t)count * 8, count)) return rc;
+        HIPCHK(hipMemcpyAsync(b->stage_idx, lanes, (size_t)count * 4, hipMemcpyHostToDevice, b->stream));
+        unsigned *src = b->ntc, *dst = (unsigned *)b->stage, *idx = b->stage_idx;
+        void *params[] = {&src, &dst, &idx, &count};
+        HIPCHK(launch_raw(b, b->sys->k_gather32, count, params));
+        HIPCHK(hipMemcpyAsync(nt_count, b->stage, (size_t)count * 4, hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+    }
+    if (nt_time) {
+        if (int rc = gather64(b, b->nt_time, b->nt_cap, lanes, count, nt_time)) return rc;
+    }
+    if (nt_code) {
+        if (int rc = gather64(b, b->nt_code, b->nt_cap, lanes, count, nt_code)) return rc;
+    }
+    if (h) {
+        if (int rc = gather64(b, b->hout, 1, lanes, count, h)) return rc;
+    }
+    if (tc) {
+        const unsigned rows = b->sys->S.n * (b->sys->order + 1);
+        if (int rc = gather64(b, b->tc, rows, lanes, count, tc)) return rc;
+    }
+    return 0;
+}
+
+int hyx_events_resume(hyx_batch *b, const uint32_t *lanes, uint32_t count, const int64_t *stop)
+{
+    if (!b) return fail(HYX_EINVAL, "null batch");
+    if (int rc = check_slots(b, lanes, count)) return rc;
+    if (count == 0) return 0;
+    std::vector<unsigned> st;
+    if (int rc = read_status(b, st)) return rc;
+    std::vector<uint32_t> sl;
+    std::vector<int64_t> oc;
+    for (uint32_t c = 0; c < count; ++c) {
+        const uint32_t i = lanes[c];
+        if (!(st[i] & 4u)) return fail(HYX_EINVAL, "instance " + std::to_string(i) + " is not parked");
+        if (stop && stop[c] != 0) {
+            st[i] = 1u;
+            sl.push_back(i);
+            oc.push_back(stop[c]);
+        } else {
+            st[i] &= 3u;
+        }
+    }
+    HIPCHK(hipMemcpyAsync(b->status, st.data(), b->N * 4, hipMemcpyHostToDevice, b->stream));
+    if (!sl.empty()) {
+        const uint32_t n = (uint32_t)sl.size();
+        if (int rc = stage_reserve(b, (size_t)n * 8, n)) return rc;
+        HIPCHK(hipMemcpyAsync(b->stage_idx, sl.data(), (size_t)n * 4, hipMemcpyHostToDevice, b->stream));
+        HIPCHK(hipMemcpyAsync(b->stage, oc.data(), (size_t)n * 8, hipMemcpyHostToDevice, b->stream));
+        uint64_t ld = b->N;
+        unsigned rows = 1;
+        void *dst = b->oc;
+        void *params[] = {&dst, &ld, &rows, &b->stage, &b->stage_idx, (void *)&n};
+        HIPCHK(launch_raw(b, b->sys->k_scatter, n, params));
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int hyx_propagate_resume(hyx_batch *b, uint64_t max_steps, int mode, hyx_prop_res *res, double *grid_out)
+{
+    if (!b) return fail(HYX_EINVAL, "null batch");
+    if (mode != HYX_MODE_PERSISTENT && mode != HYX_MODE_PLAIN && mode != HYX_MODE_GRAPH)
+        return fail(HYX_EINVAL, "unknown propagation mode");
+    if (grid_out && !b->last_grid) return fail(HYX_EINVAL, "the propagation being resumed is not a propagate_grid");
+    if (b->slots_running) return fail(HYX_EINVAL, "a slot epoch is running: call hyx_slots_wait first");
+    const bool grid = b->last_grid;
+    if (int rc = run_propagation(b, max_steps, mode, res, grid, false)) return rc;
+    if (grid && grid_out) {
+        HIPCHK(hipMemcpyAsync(grid_out, b->gout, b->npts * b->sys->S.n * b->N * 8, hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+    }
     return 0;
 }
 

@@ -71,7 +71,7 @@ def _event_spec(dec, tes, ntes, tol):
     rhs = exprs + [expression(0.0)] * (len(rows) - ne)
     edec = taylor_decompose(list(zip(rows, rhs)))
     return _lib.EventSpec(edec, len(tes), len(ntes), [int(e.direction) for e in tes + ntes],
-                          [float(e.cooldown) for e in tes], tol)
+                          [float(e.cooldown) for e in tes], tol, stopping=[e.callback is None for e in tes])
 
 # Compiled systems keyed by (op-list, constants, order, flags): the analogue of
 # heyoka's in-memory JIT cache (heyoka/core.cpp:148-149). Copies of an
@@ -283,56 +283,100 @@ class taylor_adaptive_batch:
     # ---- stepping -----------------------------------------------------------------------
     def _step_impl(self, mdt, write_tc):
         self._push()
-        if self.with_events:
-            t0 = (self._hyx_thi.copy(), self._hyx_tlo.copy())
         oc, h = self._hyx_batch.step(mdt, write_tc)
         self._pull()
         self._hyx_last_h[...] = h
         if self.with_events:
-            oc = self._run_events(oc, t0, write_tc)
+            te, cnt, tm, code = self._hyx_batch.events_read()
+            busy = (te >= 0) | (cnt > 0)
+            if write_tc or busy.any():
+                # Callbacks may call update_d_output(): the coefficients of this step.
+                self._hyx_tc[...] = self._hyx_batch.read_tc()
+            lanes = np.nonzero(busy)[0]
+            oc = oc.copy()
+            for b, r in zip(lanes, self._dispatch_events(lanes, te[lanes], cnt[lanes], tm[:, lanes], code[:, lanes],
+                                                         h[lanes])):
+                if r is not None:
+                    oc[b] = r[0] if r[1] else -(r[0] + 1)
         elif write_tc:
             self._hyx_tc[...] = self._hyx_batch.read_tc()
         self._hyx_step_res = [(_outcome(o), float(x)) for o, x in zip(oc, h)]
         return oc, h
 
-    def _run_events(self, oc, t0, write_tc):
-        """Callbacks of the events the last device step found, as heyoka's batch step runs
-        them: per batch element, the non-terminal ones in chronological order (state and
-        time at the end of the step), then the terminal one; a terminal event whose
-        callback returns True reports its index, otherwise -(index + 1)."""
-        te, ds, cnt, tau, code = self._hyx_batch.events_read()
-        cap = tau.shape[0]
-        over = np.nonzero(cnt > cap)[0]
-        if over.size:
-            raise RuntimeError(f"Batch element {int(over[0])} triggered {int(cnt[over[0]])} non-terminal events in one "
-                               f"step; at most {cap} are recorded per step: lower max_delta_t")
-        busy = (te >= 0) | (cnt > 0)
-        if write_tc or busy.any():
-            # Callbacks may call update_d_output(): the coefficients of this step.
-            self._hyx_tc[...] = self._hyx_batch.read_tc()
-        oc = oc.copy()
-        for b in np.nonzero(cnt > 0)[0]:
-            k = int(cnt[b])
-            taus = tau[:k, b]
-            recs = sorted(range(k), key=lambda r: (abs(taus[r]), int(code[r, b]) >> 2))
+    def _dispatch_events(self, lanes, te, cnt, tm, code, h):
+        """Run the callbacks of one step's events for the given batch elements, as heyoka's
+        batch step does: per element, the non-terminal ones in chronological order (the
+        state and time are those at the end of the step), then the terminal one. Returns,
+        per element, None or (terminal index, whether its callback lets the integration go on)."""
+        cap = tm.shape[0]
+        out = []
+        for j, b in enumerate(lanes):
+            k = int(cnt[j])
+            if k > cap:
+                raise RuntimeError(f"Batch element {int(b)} triggered {k} non-terminal events in one step; at most "
+                                   f"{cap} are recorded per step: lower max_delta_t")
+            sg = 1.0 if h[j] >= 0 else -1.0
+            recs = sorted(range(k), key=lambda r: (sg * tm[r, j], int(code[r, j]) >> 2))
             for r in recs:
-                c = int(code[r, b])
-                ev = self._hyx_ntes[c >> 2]
-                t = dfl_add(t0[0][b], t0[1][b], taus[r], 0.0)[0]
-                _call_event(ev.callback, self, float(t), (c & 3) - 1, int(b))
-        for b in np.nonzero(te >= 0)[0]:
-            i = int(te[b])
+                c = int(code[r, j])
+                _call_event(self._hyx_ntes[c >> 2].callback, self, float(tm[r, j]), (c & 3) - 1, int(b))
+            if te[j] < 0:
+                out.append(None)
+                continue
+            i, d = int(te[j]) >> 2, (int(te[j]) & 3) - 1
             cb = self._hyx_tes[i].callback
             go = False
             if cb is not None:
-                r = _call_event(cb, self, int(ds[b]), int(b))
+                r = _call_event(cb, self, d, int(b))
                 if not isinstance(r, (bool, np.bool_, int, float, np.integer, np.floating)):
                     raise RuntimeError("The call operator of a terminal event callback returned an object of type "
                                        f"{type(r)} which cannot be converted to bool: the error happened in the "
                                        "construction of the return value of an event callback")
                 go = bool(r)
-            oc[b] = i if go else -(i + 1)
-        return oc
+            out.append((i, go))
+        return out
+
+    def _prop_parked(self, first, max_steps, grid_out=None):
+        """Device-resident propagation with events that need Python: every instance runs in
+        the propagate kernels and parks at the end of each step with triggers; only parked
+        instances come to the host, where their callbacks run (each sees its own batch
+        element's state, time and step coefficients), then they are resumed or stopped.
+        first() starts the propagation (with parking on) and returns its records. Returns the
+        number of resumes."""
+        b = self._hyx_batch
+        mode = _MODES[self._hyx_flags[3]]
+        n = self._hyx_state.shape[0]
+        b.set_event_pause(True)
+        rounds = 0
+        try:
+            r = first()
+            while True:
+                lanes = b.events_parked()
+                if lanes.size == 0:
+                    break
+                te, cnt, tm, code, h, tc = b.events_gather(lanes)
+                self._hyx_state[:, lanes] = b.slots_read(_lib.FIELD_STATE, lanes, n)
+                self._hyx_thi[lanes] = b.slots_read(_lib.FIELD_TIME_HI, lanes, 1)[0]
+                self._hyx_tlo[lanes] = b.slots_read(_lib.FIELD_TIME_LO, lanes, 1)[0]
+                self._hyx_last_h[lanes] = h
+                self._hyx_tc[:, :, lanes] = tc
+                res = self._dispatch_events(lanes, te, cnt, tm, code, h)
+                stop = np.array([0 if (x is None or x[1]) else -(x[0] + 1) for x in res], dtype=np.int64)
+                # Callbacks may have changed their element's state, parameters or time.
+                b.slots_write(_lib.FIELD_STATE, lanes, self._hyx_state[:, lanes])
+                if self._hyx_system.npar:
+                    b.slots_write(_lib.FIELD_PARS, lanes, self._hyx_pars[: self._hyx_system.npar][:, lanes])
+                b.slots_write(_lib.FIELD_TIME_HI, lanes, self._hyx_thi[lanes][None, :])
+                b.slots_write(_lib.FIELD_TIME_LO, lanes, self._hyx_tlo[lanes][None, :])
+                b.events_resume(lanes, stop)
+                r = b.propagate_resume(max_steps, mode, grid_out)
+                rounds += 1
+        finally:
+            b.set_event_pause(False)
+        self._pull()
+        self._hyx_prop_res = [(_outcome(o), float(a), float(c), int(d))
+                              for o, a, c, d in zip(r["outcome"], r["min_h"], r["max_h"], r["nsteps"])]
+        return rounds
 
     def step(self, max_delta_t=None, write_tc=False):
         """One adaptive step per lane; ``max_delta_t`` (per lane) clamps |h| and sets its sign."""
@@ -379,7 +423,12 @@ class taylor_adaptive_batch:
             raise ValueError("A non-finite time was passed to the propagate_until() function of an adaptive Taylor "
                              "integrator")
         cb = self._check_cb(callback)
-        if cb is None and not write_tc and self._device_events_ok():
+        if cb is None and not write_tc and not self._device_events_ok():
+            self._push()
+            self._prop_parked(lambda: self._hyx_batch.propagate_until(tf_hi, tf_lo, mdt, int(max_steps),
+                                                                      _MODES[self._hyx_flags[3]]), int(max_steps))
+            return None, None
+        if cb is None and not write_tc:
             self._push()
             r = self._hyx_batch.propagate_until(tf_hi, tf_lo, mdt, int(max_steps),
                                                 _MODES[self._hyx_flags[3]])
@@ -496,7 +545,18 @@ class taylor_adaptive_batch:
                 raise ValueError("A non-monotonic time grid was passed to propagate_grid()")
         mdt = self._prop_args(max_steps, max_delta_t)
         cb = self._check_cb(callback)
-        if cb is None and self._device_events_ok():
+        if cb is None and not self._device_events_ok():
+            self._push()
+            box = {}
+
+            def first():
+                r, box["out"] = self._hyx_batch.propagate_grid(g, mdt, int(max_steps), _MODES[self._hyx_flags[3]])
+                return r
+
+            out = np.empty((npts, self._hyx_state.shape[0], self._hyx_B))
+            rounds = self._prop_parked(first, int(max_steps), grid_out=out)
+            return None, (box["out"] if rounds == 0 else out)
+        if cb is None:
             self._push()
             r, out = self._hyx_batch.propagate_grid(g, mdt, int(max_steps), _MODES[self._hyx_flags[3]])
             self._pull()

@@ -166,6 +166,7 @@ typedef struct {
     uint32_t n_te, n_nt;
     const int32_t *direction;
     const double *cooldown;
+    const int32_t *stopping; /* [n_te]: 1 = no callback, the event always stops; NULL = all 1 */
     double tol;
 } hyx_events;
 int hyx_sys_create_events(const uint32_t *ops, uint32_t n_words, const double *consts, uint32_t n_consts,
@@ -173,12 +174,29 @@ int hyx_sys_create_events(const uint32_t *ops, uint32_t n_words, const double *c
 /* with_events / len(t_events) / len(nt_events) (expose_batch_integrators.cpp:651-656). */
 int hyx_sys_n_events(const hyx_sys *s, uint32_t *n_te, uint32_t *n_nt);
 /* Events of the last hyx_step (what heyoka's batch step passes to the callbacks):
- * te [N] terminal event index or -1, te_dsgn [N] its direction, nt_count [N]
- * non-terminal triggers, nt_tau / nt_code [cap][N] the first min(count, cap) of them:
- * offset from the start of the step, and 4 * (non-terminal index) + d_sgn + 1.
- * A count above cap means records were lost. Any pointer may be NULL. */
-int hyx_events_read(hyx_batch *b, int32_t *te, int32_t *te_dsgn, uint32_t *nt_count, double *nt_tau,
-                    int32_t *nt_code, uint32_t *cap);
+ * te [N] 4 * (terminal index) + d_sgn + 1, or -1; nt_count [N] non-terminal
+ * triggers; nt_time / nt_code [cap][N] the first min(count, cap) of them: trigger
+ * time and 4 * (non-terminal index) + d_sgn + 1. A count above cap means records
+ * were lost. Any pointer may be NULL. */
+int hyx_events_read(hyx_batch *b, int64_t *te, uint32_t *nt_count, double *nt_time, int64_t *nt_code,
+                    uint32_t *cap);
+/* Parking (no counterpart in the reference, whose propagate loop calls the callbacks
+ * between lockstep steps): with pause on, hyx_propagate_until / _grid park an instance
+ * at the end of each step whose events need the host (a non-terminal trigger, or a
+ * terminal one of an event with a callback) and return when every instance is done
+ * or parked. hyx_events_parked lists the parked instances (lanes: [N] capacity);
+ * hyx_events_gather fetches, per listed instance, what that step's callbacks see: te,
+ * nt_count, nt_time / nt_code [cap][count], the step size h [count] and, if tc is not
+ * NULL, the step's Taylor coefficients [n][p+1][count]; hyx_events_resume sets each
+ * listed instance running again (stop[c] == 0) or done with outcome stop[c];
+ * hyx_propagate_resume continues the propagation (grid_out: the output of a resumed
+ * propagate_grid, else NULL). */
+int hyx_batch_set_event_pause(hyx_batch *b, int on);
+int hyx_events_parked(hyx_batch *b, uint32_t *lanes, uint32_t *count);
+int hyx_events_gather(hyx_batch *b, const uint32_t *lanes, uint32_t count, int64_t *te, uint32_t *nt_count,
+                      double *nt_time, int64_t *nt_code, double *h, double *tc);
+int hyx_events_resume(hyx_batch *b, const uint32_t *lanes, uint32_t count, const int64_t *stop);
+int hyx_propagate_resume(hyx_batch *b, uint64_t max_steps, int mode, hyx_prop_res *res, double *grid_out);
 /* te_cooldowns / reset_cooldowns (expose_batch_integrators.cpp:652-654): [n_te][3][N]
  * trigger time (hi, lo) and cooldown length; a negative length = no cooldown. */
 int hyx_cooldowns_read(hyx_batch *b, double *dst);

@@ -305,3 +305,77 @@ def test_backward_and_cooldowns():
         assert len(ts) >= 3 and np.min(np.diff(ts)) > 0.5
     cd = ta.te_cooldowns
     assert len(cd) == 2 and len(cd[0]) == 1
+
+
+@pytest.mark.parametrize("mode", ["persistent", "plain"])
+def test_parked_path_matches_host_stepped(mode):
+    # Events whose callbacks need Python run in the propagate kernels, parking an
+    # instance at the end of each step with triggers; the host-stepped path (forced by a
+    # step callback) gives the same triggers, states and step counts bit for bit.
+    print(f"[gpu] parked vs host-stepped mode={mode}", flush=True)
+    rng = np.random.default_rng(11)
+    B = 40
+    ic = np.stack([rng.uniform(-1.0, 1.0, B), rng.uniform(-1.5, 1.5, B)])
+    logs = ([], [])
+
+    def mk(log):
+        def nt(ta, t, d, b):
+            log.append(("nt", b, t, d, ta.time[b]))
+
+        def tcb(ta, d, b):
+            log.append(("te", b, ta.time[b], d))
+            ta.pars[0, b] += 1.0
+            return ta.pars[0, b] < 3.0
+
+        return ([hy.t_event_batch(x - 0.2, callback=tcb, direction=hy.event_direction.positive)],
+                [hy.nt_event_batch(v, nt), hy.nt_event_batch(x * v - 0.05, nt)])
+
+    sysp = [(x, v), (v, -9.8 * hy.sin(x) - 0.01 * hy.par[0] * v)]
+    runs = []
+    for k, log in enumerate(logs):
+        te, nt = mk(log)
+        ta = hy.taylor_adaptive_batch(sysp, ic, pars=np.zeros((1, B)), t_events=te, nt_events=nt, gpu_mode=mode)
+        if k == 0:
+            ta.propagate_until(np.full(B, 8.0))
+        else:
+            ta.propagate_until(np.full(B, 8.0), callback=lambda t: True)
+        runs.append(ta)
+    a, b = runs
+    np.testing.assert_array_equal(a.state, b.state)
+    np.testing.assert_array_equal(a.pars, b.pars)
+    assert a.propagate_res == b.propagate_res
+    for lane in range(B):
+        la = [e for e in logs[0] if e[1] == lane]
+        lb = [e for e in logs[1] if e[1] == lane]
+        assert la == lb, lane
+    ocs = [int(r[0]) for r in a.propagate_res]
+    assert -1 in ocs and int(to.time_limit) in ocs
+
+
+def test_parked_grid_and_large_batch():
+    print("[gpu] parked grid / large batch", flush=True)
+    B = 4096
+    rng = np.random.default_rng(3)
+    ic = np.stack([rng.uniform(-1.0, 1.0, B), rng.uniform(-1.0, 1.0, B)])
+    counts = np.zeros(B, dtype=int)
+    last = np.full(B, -np.inf)
+
+    def cb(ta, t, d, b):
+        assert t > last[b] and d == 1
+        last[b] = t
+        counts[b] += 1
+
+    ta = hy.taylor_adaptive_batch(PEND, ic, nt_events=[hy.nt_event_batch(v, cb, direction=hy.event_direction.positive)])
+    grid = np.repeat(np.linspace(0.0, 10.0, 21), B).reshape(21, B)
+    _, out = ta.propagate_grid(grid)
+    assert np.isfinite(out).all() and all(r[0] == to.time_limit for r in ta.propagate_res)
+    # Every lane crosses v = 0 upwards once per period (about 2 s at these amplitudes).
+    assert counts.min() >= 4 and counts.max() <= 6
+    plain = hy.taylor_adaptive_batch(PEND, ic)
+    _, o2 = plain.propagate_grid(grid)
+    sub = slice(0, 64)
+    ref = OE.propagate(hy, PEND, ic[:, sub], 10.0, nt_events=[(v, 1)])
+    assert [len(e) for e in ref["nt"]] == list(counts[sub])
+    np.testing.assert_allclose(ta.state[:, sub], ref["state"], rtol=0, atol=1e-13)
+    # The events change the step sizes (their norms join the step's), not the solution.
+    np.testing.assert_allclose(out, o2, rtol=0, atol=1e-12)
