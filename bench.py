@@ -103,7 +103,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="nbody6", choices=["nbody6", "cr3bp", "j2var", "serving"])
+    ap.add_argument("--config", default="nbody6", choices=["nbody6", "cr3bp", "j2var", "serving", "events"])
     ap.add_argument("--rates", default="1,4", help="serving: Poisson arrival rates (requests/s)")
     ap.add_argument("--duration", type=float, default=4.0, help="serving: seconds of arrivals per rate")
     ap.add_argument("--n", type=int, default=None, help="instances (default: the config's)")
@@ -145,6 +145,8 @@ def main():
 
     if args.config == "serving":
         return serving_bench(args)
+    if args.config == "events":
+        return events_bench(args)
     if args.config == "nbody6":
         wl = W.nbody6(args.n or 65536)
     elif args.config == "j2var":
@@ -297,6 +299,72 @@ def main():
                       "compiled port of the recurrences")
         out["cpu_baseline"] = {"value": v, "unit": "instance-steps/s", "cores": args.cpu_threads, "kind": "port",
                                "sample": sample + f" ({secs:.1f} s)"}
+    print(json.dumps(out), flush=True)
+
+
+def events_bench(args):
+    """Event detection at ensemble scale: the CR3BP ensemble (65,536 instances, t in
+    [0, 20], tol 1e-15) propagated (A) without events, (B) with a terminal escape event
+    (r^2 = 4, no callback: stops on the device), (C) with B plus a non-terminal section
+    event x = 1/2 - mu (direction positive) whose Python callback logs every crossing:
+    instances park after a step with crossings and resume after their callbacks.
+    Reports instance-steps/s of each, the crossings per second and the kernel share."""
+    import heyoka_amd as hy
+    from heyoka_amd import workloads as W
+
+    N = args.n or 65536
+    wl = W.cr3bp(N)
+    T = 20.0
+    x, y, vx, vy = hy.make_vars("x", "y", "vx", "vy")
+    results = {}
+    for label in ("A_no_events", "B_terminal_device", "C_terminal_and_section_parked"):
+        crossings = [0]
+
+        def cb(ta, t, d_sgn, bidx):
+            crossings[0] += 1
+
+        kw = {}
+        if label != "A_no_events":
+            kw["t_events"] = [hy.t_event_batch(x * x + y * y - 4.0, direction=hy.event_direction.positive)]
+        if label.startswith("C"):
+            kw["nt_events"] = [hy.nt_event_batch(x - 0.5 + hy.par[0], cb, direction=hy.event_direction.positive)]
+        ta = hy.taylor_adaptive_batch(wl.sys, wl.state, pars=wl.pars, tol=1e-15, code_form="unrolled", **kw)
+        ta.propagate_until(np.full(N, 1.0))  # warm-up (compiles, first launch)
+        ta.set_time(np.zeros(N))
+        ta.state[:] = wl.state
+        ta.reset_cooldowns()
+        crossings[0] = 0
+        t0 = time.perf_counter()
+        ta.propagate_until(np.full(N, T))
+        wall = time.perf_counter() - t0
+        ns = int(sum(r[3] for r in ta.propagate_res))
+        ms, launches = ta.last_kernel_timing()
+        stopped = int(sum(1 for r in ta.propagate_res if int(r[0]) == -1))
+        results[label] = {"instance_steps_per_s": ns / wall, "instance_steps": ns, "wall_s": wall,
+                          "escapes_stopped": stopped, "section_crossings": crossings[0],
+                          "crossings_per_s": crossings[0] / wall if crossings[0] else None}
+        print(f"[bench] events {label}: {ns / wall:.3e} instance-steps/s, {crossings[0]} crossings, "
+              f"{stopped} stopped, {wall:.2f} s", file=sys.stderr, flush=True)
+    base = results["A_no_events"]["instance_steps_per_s"]
+    out = {
+        "metric": METRIC,
+        "value": results["C_terminal_and_section_parked"]["instance_steps_per_s"],
+        "unit": "instance-steps/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": results["C_terminal_and_section_parked"]["wall_s"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"cr3bp ensemble with events: {N} instances, t in [0, {T}], tol 1e-15, unrolled "
+                               "form, persistent; terminal escape event r^2 = 4 and a Python-logged section "
+                               "x = 1/2 - mu (direction positive)"},
+        "events": results,
+        "relative_to_no_events": {k: v["instance_steps_per_s"] / base for k, v in results.items()},
+    }
     print(json.dumps(out), flush=True)
 
 

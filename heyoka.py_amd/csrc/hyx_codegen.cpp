@@ -39,7 +39,6 @@ static const int INF_DEG = 1 << 20;
 // join the state's in the step-size norms (heyoka includes its event functions in
 // the Taylor jet whose last two orders set the step size; the step counts and
 // step sizes of doc/notebooks/Event detection.ipynb, cell 21, follow from it).
-static const char *EV_NORMS_HOOK = "#ifdef HYX_EVENTS\n    if constexpr (WTC) hyx_ev_norms(tc, tcs, pv, t0, mo, mom1);\n#endif\n";
 
 void compute_degrees(Sys &S)
 {
@@ -193,6 +192,12 @@ struct Gen {
     // above the two-wave threshold either way), so the group form keeps the other form.
     bool literal_pow = true;
 
+    // Events (unrolled form): the event op-list, whose rows 0..n-1 are this system's state
+    // variables; its nodes are generated by a second Gen (base = this) in a nested scope.
+    const Sys *E = nullptr;
+    uint32_t nev = 0;
+    const Gen *base = nullptr;
+
     Gen(const Sys &s, int p, bool ha) : S(s), P(p), high_acc(ha), perlane(s.nconsts, 0) {}
 
     static std::string cv(uint32_t node, int k) { return "c" + std::to_string(node) + "_" + std::to_string(k); }
@@ -202,6 +207,7 @@ struct Gen {
     // Coefficient k of node idx: a register, or for a state variable in LDS an older row.
     std::string nv(uint32_t idx, int k) const
     {
+        if (base && idx < base->S.n) return base->nv(idx, k);
         if (lds_state && idx < S.n && k != cur_k) return xs(idx, k);
         return cv(idx, k);
     }
@@ -555,7 +561,8 @@ struct Gen {
         o << "template <bool WTC>\n"
              "__device__ __forceinline__ void hyx_taylor_step(double (&x)[HYX_N], const double (&pv)[HYX_NPAR], "
              "double t0, double max_dt, double &h_out, bool &nf_out, double *__restrict__ tc, hyx_u64 tcs, "
-             "double *__restrict__ tape, hyx_u64 tps)\n{\n";
+             "double *__restrict__ tape, hyx_u64 tps"
+          << (E ? ", unsigned long long &hyx_evm" : "") << ")\n{\n";
         o << "    (void)pv; (void)t0; (void)tape; (void)tps; (void)tc; (void)tcs;\n";
         if (lds_state) o << "    double *const xs = hyx_xs_lane();\n";
         // The rows of order k go to LDS as soon as they exist; the empty asm at the
@@ -596,6 +603,22 @@ struct Gen {
             for (int k = 0; k <= P; ++k)
                 o << "        tc[(hyx_u64)" << (i * (P + 1) + k) << " * tcs] = " << nv(i, k) << ";\n";
         o << "    }\n";
+        if (E) {
+            // The event functions' coefficients, from the state's registers: a nested scope
+            // (the event op-list's names shadow the system's), results in hyx_evg.
+            Gen eg(*E, P, high_acc);
+            eg.base = this;
+            eg.literal_pow = literal_pow;
+            o << "    double hyx_evg[" << nev << "][" << (P + 1) << "];\n    {\n";
+            for (uint32_t j = E->n; j < E->nnodes; ++j) eg.order0(j);
+            for (int k = 1; k <= P; ++k)
+                for (uint32_t j = E->n; j < E->nnodes; ++j) eg.orderk(j, k);
+            for (uint32_t e = 0; e < nev; ++e)
+                for (int k = 0; k <= P; ++k)
+                    eg.o << "    hyx_evg[" << e << "][" << k << "] = " << (eg.zero(E->rhs[e], k) ? "0.0" : eg.val(E->rhs[e], k))
+                         << ";\n";
+            o << eg.o.str() << "    }\n";
+        }
         // Step size: infinity norms of the state and of the last two orders.
         o << "    double mx = 0.0, mo = 0.0, mom1 = 0.0, ab;\n";
         for (uint32_t i = 0; i < n; ++i) {
@@ -603,7 +626,10 @@ struct Gen {
             o << "    ab = fabs(" << nv(i, P) << "); mo = (ab > mo) ? ab : mo;\n";
             o << "    ab = fabs(" << nv(i, P - 1) << "); mom1 = (ab > mom1) ? ab : mom1;\n";
         }
-        o << EV_NORMS_HOOK;
+        for (uint32_t e = 0; E && e < nev; ++e) {
+            o << "    ab = fabs(hyx_evg[" << e << "][" << P << "]); mo = (ab > mo) ? ab : mo;\n";
+            o << "    ab = fabs(hyx_evg[" << e << "][" << (P - 1) << "]); mom1 = (ab > mom1) ? ab : mom1;\n";
+        }
         o << "    const double num = (mx <= 1.0) ? 1.0 : mx;\n";
         o << "    const double rho_o = pow(num / mo, " << lit(1.0 / P) << ");\n";
         o << "    const double rho_om1 = pow(num / mom1, " << lit(1.0 / (P - 1)) << ");\n";
@@ -614,6 +640,31 @@ struct Gen {
         o << "    const double lim = fabs(max_dt);\n";
         o << "    h = (h < lim) ? h : lim;\n";
         o << "    if (max_dt < 0.0) h = -h;\n";
+        if (E) {
+            // Fast exclusion per event over the step (the same test the root scan starts
+            // with); only steps it cannot clear keep their coefficients in memory.
+            const uint64_t off = (uint64_t)n * (P + 1) + (uint64_t)(E->nnodes - E->n) * (P + 1);
+            o << "    hyx_evm = 0ull;\n";
+            o << "#pragma unroll\n    for (int e = 0; e < " << nev << "; ++e) {\n"
+                 "        double hk = 1.0, mag = 0.0, lo = 0.0, hi = 0.0;\n"
+                 "#pragma unroll\n        for (int k = 0; k <= " << P << "; ++k) {\n"
+                 "            const double c = hyx_evg[e][k] * hk;\n            hk *= h;\n            mag += fabs(c);\n"
+                 "            if (k > 0) { lo += fmin(c, 0.0); hi += fmax(c, 0.0); }\n        }\n"
+                 "        const double slack = " << lit(4.0 * (P + 1) * 1.1102230246251565e-16) << " * mag;\n"
+                 "        const double c0 = hyx_evg[e][0];\n"
+                 "        if (mag > 0.0 && isfinite(mag) && !(c0 + lo > slack || c0 + hi < -slack)) hyx_evm |= 1ull << e;\n"
+                 "    }\n";
+            o << "    if (hyx_evm != 0ull) {\n";
+            for (uint32_t e = 0; e < nev; ++e)
+                for (int k = 0; k <= P; ++k)
+                    o << "        tc[(hyx_u64)" << (off + (uint64_t)e * (P + 1) + k) << " * tcs] = hyx_evg[" << e << "][" << k
+                      << "];\n";
+            o << "        if constexpr (!WTC) {\n";
+            for (uint32_t i = 0; i < n; ++i)
+                for (int k = 0; k <= P; ++k)
+                    o << "            tc[(hyx_u64)" << (i * (P + 1) + k) << " * tcs] = " << nv(i, k) << ";\n";
+            o << "        }\n    }\n";
+        }
         o << "    bool nf = false;\n";
         for (uint32_t i = 0; i < n; ++i) {
             if (!high_acc) {
@@ -651,6 +702,8 @@ struct CompactGen {
     // of __shfl_xor adds the partial sums (every lane gets the same value, because the
     // two operands of each addition only swap between partner lanes).
     int sub = 1;
+    // The system has events: the step takes hyx_evm and always writes tc (see run()).
+    bool events = false;
 
     CompactGen(const Sys &s, int p, bool ha) : S(s), P(p), high_acc(ha) {}
 
@@ -860,8 +913,12 @@ struct CompactGen {
         o << "template <bool WTC>\n"
              "__device__ __forceinline__ void hyx_taylor_step(double (&x)[HYX_N], const double (&pv)[HYX_NPAR], "
              "double t0, double max_dt, double &h_out, bool &nf_out, double *__restrict__ tc, hyx_u64 tcs, "
-             "double *__restrict__ tape, hyx_u64 tps)\n{\n";
+             "double *__restrict__ tape, hyx_u64 tps"
+          << (events ? ", unsigned long long &hyx_evm" : "") << ")\n{\n";
         o << "    (void)pv; (void)t0; (void)tc; (void)tcs;\n";
+        // With events every step keeps its coefficients and forms the event functions'
+        // from them (hyx_ev_norms); the root scan decides per event.
+        if (events) o << "    hyx_evm = ~0ull;\n";
         if (sub > 1)
             o << "    const int hyx_sub = (int)(__lane_id() & (HYX_SUB - 1));\n";
         else
@@ -879,7 +936,7 @@ struct CompactGen {
         o << "        if (k == HYX_P) break;\n";
         for (uint32_t j = n; j < S.nnodes; ++j) orderk(j);
         o << "    }\n";
-        o << "    if constexpr (WTC) {\n";
+        o << (events ? "    {\n" : "    if constexpr (WTC) {\n");
         o << "        for (int i = 0; i < HYX_N; ++i)\n"
              "            for (int k = 0; k <= HYX_P; ++k) tc[(hyx_u64)(i * (HYX_P + 1) + k) * tcs] = T(i, k);\n";
         o << "    }\n";
@@ -889,7 +946,7 @@ struct CompactGen {
              "        ab = fabs(T(i, HYX_P)); mo = (ab > mo) ? ab : mo;\n"
              "        ab = fabs(T(i, HYX_P - 1)); mom1 = (ab > mom1) ? ab : mom1;\n"
              "    }\n";
-        o << EV_NORMS_HOOK;
+        if (events) o << "    hyx_ev_norms(tc, tcs, pv, t0, mo, mom1);\n";
         o << "    const double num = (mx <= 1.0) ? 1.0 : mx;\n";
         o << "    const double rho_o = pow(num / mo, " << lit(1.0 / P) << ");\n";
         o << "    const double rho_om1 = pow(num / mom1, " << lit(1.0 / (P - 1)) << ");\n";
@@ -1589,8 +1646,8 @@ std::string generate_events_prelude(const Sys &E, uint32_t n_sys, int P, const E
         o << "    }\n";
     }
     o << "}\n";
-    // Coefficient k of event e: a node row, a state row, or a number / parameter (order 0 only).
-    o << "__device__ __forceinline__ double hyx_ev_coeff(int e, int k, const double *__restrict__ tc, "
+    // Event e's coefficient k from the jet: a node row, a state row, or a number / parameter (order 0 only).
+    o << "__device__ __forceinline__ double hyx_ev_src(int e, int k, const double *__restrict__ tc, "
          "const double *__restrict__ et, hyx_u64 ld, const double *__restrict__ pv)\n{\n";
     o << "    (void)tc; (void)et; (void)ld; (void)pv;\n    switch (e) {\n";
     for (uint32_t e = 0; e < nev; ++e) {
@@ -1606,14 +1663,20 @@ std::string generate_events_prelude(const Sys &E, uint32_t n_sys, int P, const E
         o << ";\n";
     }
     o << "    }\n    return 0.0;\n}\n#undef T\n";
-    // The event-node rows follow the state rows in tc.
+    // Rows in tc after the state's: the event nodes' (compact form), then the event
+    // functions' coefficients [e][k], which the root scan reads (hyx_ev_coeff).
+    o << "#define HYX_EV_OUT_ROW ((hyx_u64)HYX_N * (HYX_P + 1) + HYX_EV_ROWS)\n";
     o << "__device__ __forceinline__ double *hyx_ev_rows(double *tc, hyx_u64 ld) { return tc + (hyx_u64)HYX_N * "
          "(HYX_P + 1) * ld; }\n";
+    o << "__device__ __forceinline__ double hyx_ev_coeff(int e, int k, const double *__restrict__ tc, hyx_u64 ld)\n"
+         "{\n    return tc[(HYX_EV_OUT_ROW + (hyx_u64)e * (HYX_P + 1) + (hyx_u64)k) * ld];\n}\n";
     o << "__device__ __forceinline__ void hyx_ev_norms(double *tc, hyx_u64 ld, const double *pv, double t0, double &mo, "
          "double &mom1)\n{\n    double *et = hyx_ev_rows(tc, ld);\n    hyx_event_jet(tc, et, ld, pv, t0);\n"
          "    for (int e = 0; e < HYX_NEV; ++e) {\n"
-         "        const double a = fabs(hyx_ev_coeff(e, HYX_P, tc, et, ld, pv));\n"
-         "        const double b = fabs(hyx_ev_coeff(e, HYX_P - 1, tc, et, ld, pv));\n"
+         "        for (int k = 0; k <= HYX_P; ++k)\n"
+         "            tc[(HYX_EV_OUT_ROW + (hyx_u64)e * (HYX_P + 1) + (hyx_u64)k) * ld] = hyx_ev_src(e, k, tc, et, ld, pv);\n"
+         "        const double a = fabs(hyx_ev_coeff(e, HYX_P, tc, ld));\n"
+         "        const double b = fabs(hyx_ev_coeff(e, HYX_P - 1, tc, ld));\n"
          "        mo = (a > mo) ? a : mo;\n        mom1 = (b > mom1) ? b : mom1;\n    }\n}\n";
     return o.str();
 }
@@ -1658,16 +1721,20 @@ bool choose_compact(const Sys &S, int order)
     return (long)history_nodes(S) * (order + 1) > 320;
 }
 
-std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compact, int split_min, int sub)
+std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compact, int split_min, int sub,
+                             const Sys *E, uint32_t nev)
 {
     if (order < 2) throw std::invalid_argument("the Taylor order must be at least 2");
     if (compact) {
         CompactGen g(S, order, high_acc);
         g.sub = sub;
+        g.events = E != nullptr;
         return g.run();
     }
     Gen g(S, order, high_acc);
     g.split_min = split_min;
+    g.E = E;
+    g.nev = nev;
     // State rows go to LDS when that brings the remaining histories within about 200
     // VGPRs (two waves per SIMD) and the rows of a 64-lane block fit in 64 KB. When the
     // node histories alone exceed that, the extra LDS traffic buys no occupancy.

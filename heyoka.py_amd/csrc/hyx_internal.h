@@ -27,7 +27,10 @@ int parse_oplist(const uint32_t *w, uint32_t n_words, const double *consts, uint
 // HIP source of hyx_taylor_step<WTC>() for S at the given order, unrolled or compact.
 // split_min: fma chains of at least this many terms use two accumulators (0: never).
 // sub: lanes per instance of the compact form (1, 2, 4 or 8).
-std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compact, int split_min = 0, int sub = 1);
+// With events (E: the event op-list, nev events) the step takes an extra
+// `unsigned long long &hyx_evm`: per event, 1 when its root scan has to run this step.
+std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compact, int split_min = 0, int sub = 1,
+                             const Sys *E = nullptr, uint32_t nev = 0);
 // Number of nodes whose coefficient history the recurrences read.
 int history_nodes(const Sys &S);
 // Default choice of code form: compact when the histories exceed the register file.

@@ -3,10 +3,12 @@
 //   HYX_NEV, HYX_NTE            events, terminal ones first
 //   hyx_ev_dir[], hyx_ev_cd[]   direction of every event, cooldown of every terminal event
 //   HYX_TOL                     the integrator's tolerance
-//   hyx_event_jet(tc, et, ...)  the event functions' Taylor coefficients at orders 0..P
-//   hyx_ev_coeff(e, k, ...)     coefficient k of event e
-//   hyx_ev_norms(...)           called by the step: the event jet, folded into the
-//                               step-size norms of the last two orders
+//   hyx_ev_coeff(e, k, tc, ld)  coefficient k of event e, from its rows in tc
+//   hyx_ev_norms(...)           compact form: the event jet from the tape, folded into
+//                               the step-size norms of the last two orders
+// The unrolled step forms the event jet in registers itself, folds it into its
+// norms, runs the fast exclusion test per event and writes the coefficients to tc
+// only for the events (bits of hyx_evm) the test does not clear.
 //
 // Semantics follow heyoka's events (heyoka/taylor_expose_events.cpp:195-300; the
 // batch tests heyoka/test.py:693-1000 and heyoka/_test_batch_integrator.py:555-600;
@@ -209,11 +211,13 @@ __device__ __forceinline__ void hyx_ev_scan(const double (&g)[HYX_P + 1], double
 // identical records.
 __device__ __forceinline__ double hyx_ev_step(const hyx_kargs &a, hyx_u64 i, const double *pv, double thi,
                                               double tlo, double h, double (&x)[HYX_N], bool &nf, int &te,
-                                              int &te_dsgn, unsigned &ntc_out)
+                                              int &te_dsgn, unsigned &ntc_out, unsigned long long evm)
 {
-    // The step already formed the event functions' coefficients (hyx_ev_norms).
+    // The step formed the event functions' coefficients and cleared, per event, the
+    // steps its fast exclusion test rules out (bit e of evm is 0); for the others the
+    // coefficients (and the state's) are in tc.
+    (void)pv;
     double *tc = a.tc + i;
-    const double *et = hyx_ev_rows(tc, a.ld);
     const hyx_u64 ld = a.ld;
     te = -1;
     te_dsgn = 0;
@@ -223,8 +227,9 @@ __device__ __forceinline__ double hyx_ev_step(const hyx_kargs &a, hyx_u64 i, con
     if (h != 0.0) {
         double best = fabs(h);
         for (int e = 0; e < HYX_NTE; ++e) {
+            if (!((evm >> e) & 1ull)) continue;
 #pragma unroll
-            for (int k = 0; k <= HYX_P; ++k) g[k] = hyx_ev_coeff(e, k, tc, et, ld, pv);
+            for (int k = 0; k <= HYX_P; ++k) g[k] = hyx_ev_coeff(e, k, tc, ld);
             const double cdl = a.evcd[((hyx_u64)e * 3 + 2) * ld + i];
             const double dt0 = (cdl >= 0.0) ? hyx_dfl_sub_hi(thi, tlo, a.evcd[((hyx_u64)e * 3) * ld + i],
                                                              a.evcd[((hyx_u64)e * 3 + 1) * ld + i])
@@ -242,8 +247,9 @@ __device__ __forceinline__ double hyx_ev_step(const hyx_kargs &a, hyx_u64 i, con
             });
         }
         for (int e = HYX_NTE; e < HYX_NEV; ++e) {
+            if (!((evm >> e) & 1ull)) continue;
 #pragma unroll
-            for (int k = 0; k <= HYX_P; ++k) g[k] = hyx_ev_coeff(e, k, tc, et, ld, pv);
+            for (int k = 0; k <= HYX_P; ++k) g[k] = hyx_ev_coeff(e, k, tc, ld);
             const int dir = hyx_ev_dir[e];
             hyx_ev_scan(g, h, [&](double tau, int ds) {
                 if (fabs(tau) >= best) return true;
@@ -287,7 +293,7 @@ __device__ __forceinline__ double hyx_ev_step(const hyx_kargs &a, hyx_u64 i, con
         }
         nf = bad;
 #pragma unroll
-        for (int k = 0; k <= HYX_P; ++k) g[k] = hyx_ev_coeff(te, k, tc, et, ld, pv);
+        for (int k = 0; k <= HYX_P; ++k) g[k] = hyx_ev_coeff(te, k, tc, ld);
         cd = hyx_ev_cd[te];
         if (cd < 0.0) {
             cd = 10.0 * (HYX_TOL * mx) / fabs(hyx_ev_deval(g, hend));

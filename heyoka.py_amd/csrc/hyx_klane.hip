@@ -12,13 +12,21 @@
 // inside the step (dense output, heyoka's propagate_grid:
 // heyoka/expose_batch_integrators.cpp:315-392).
 //
-// With events (HYX_EVENTS, hyx_kevent.hip), every step writes its coefficients to
-// a.tc as well; event processing may shorten the step to a terminal trigger, and
-// a propagation ends at a terminal event with outcome -(index + 1).
-#if defined(HYX_GRID) || defined(HYX_EVENTS)
+// With events (HYX_EVENTS, hyx_kevent.hip), the step reports which events need a
+// root scan (and then leaves the coefficients in a.tc); event processing may
+// shorten the step to a terminal trigger, and a propagation ends at a terminal
+// event with outcome -(index + 1).
+#ifdef HYX_GRID
 #define HYX_PROP_WTC true
 #else
 #define HYX_PROP_WTC false
+#endif
+#ifdef HYX_EVENTS
+#define HYX_EV_ARGS , evm
+#define HYX_TC_ON true
+#else
+#define HYX_EV_ARGS
+#define HYX_TC_ON HYX_PROP_WTC
 #endif
 //
 // The compact prelude may set HYX_SUB > 1: HYX_SUB adjacent lanes then hold the
@@ -95,14 +103,16 @@ __device__ __forceinline__ bool hyx_prop_iter(hyx_inst &s, const hyx_kargs &a, h
     const double lim = s.tdir ? ((s.mdt < rem) ? s.mdt : rem) : ((-s.mdt > rem) ? -s.mdt : rem);
     double h;
     bool nf;
-    hyx_taylor_step<HYX_PROP_WTC>(s.x, s.pv, s.thi, lim, h, nf, HYX_PROP_WTC ? a.tc + i : nullptr, a.ld,
-                                  a.tape ? a.tape + lane_tape : nullptr, a.tape_stride);
+    unsigned long long evm = 0;
+    (void)evm;
+    hyx_taylor_step<HYX_PROP_WTC>(s.x, s.pv, s.thi, lim, h, nf, HYX_TC_ON ? a.tc + i : nullptr, a.ld,
+                                  a.tape ? a.tape + lane_tape : nullptr, a.tape_stride HYX_EV_ARGS);
     int te = -1;
     bool park = false;
 #ifdef HYX_EVENTS
     int te_dsgn;
     unsigned ntc;
-    h = hyx_ev_step(a, i, s.pv, s.thi, s.tlo, h, s.x, nf, te, te_dsgn, ntc);
+    h = hyx_ev_step(a, i, s.pv, s.thi, s.tlo, h, s.x, nf, te, te_dsgn, ntc, evm);
     park = a.ev_pause && ((te >= 0 && !hyx_ev_stop[te]) || ntc > 0u);
 #endif
     hyx_dfl_add(s.thi, s.tlo, h, 0.0);
@@ -193,11 +203,12 @@ __device__ __forceinline__ void hyx_step_body(const hyx_kargs &a)
     double h;
     bool nf;
 #ifdef HYX_EVENTS
-    // Events need the coefficients of every step.
-    hyx_taylor_step<true>(s.x, s.pv, s.thi, mdt, h, nf, a.tc + i, a.ld, a.tape ? a.tape + i : nullptr, a.tape_stride);
+    unsigned long long evm = 0;
+    hyx_taylor_step<WTC>(s.x, s.pv, s.thi, mdt, h, nf, a.tc + i, a.ld, a.tape ? a.tape + i : nullptr, a.tape_stride,
+                         evm);
     int te, te_dsgn;
     unsigned ntc;
-    h = hyx_ev_step(a, i, s.pv, s.thi, s.tlo, h, s.x, nf, te, te_dsgn, ntc);
+    h = hyx_ev_step(a, i, s.pv, s.thi, s.tlo, h, s.x, nf, te, te_dsgn, ntc, evm);
 #else
     hyx_taylor_step<WTC>(s.x, s.pv, s.thi, mdt, h, nf, WTC ? a.tc + i : nullptr, a.ld, a.tape ? a.tape + i : nullptr,
                          a.tape_stride);

@@ -347,8 +347,11 @@ int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint3
                 sub = (flags & HYX_FLAG_SUB1) ? 1 : (flags & HYX_FLAG_SUB4) ? 4 : (flags & HYX_FLAG_SUB8) ? 8 : 2;
             s.inst_per_block = 64u / (uint32_t)sub;
             s.sub = sub;
+            if (s.has_events && s.ev.dir.size() > 64) return fail(HYX_EINVAL, "at most 64 events are supported");
             s.src = (s.has_events ? hyx::events_forward_decl() : std::string()) +
-                    hyx::generate_prelude(s.S, (int)order, ha, s.compact, split, sub) + "\n";
+                    hyx::generate_prelude(s.S, (int)order, ha, s.compact, split, sub, s.has_events ? &s.E : nullptr,
+                                          (uint32_t)s.ev.dir.size()) +
+                    "\n";
             if (s.has_events)
                 s.src += hyx::generate_events_prelude(s.E, s.S.n, (int)order, s.ev, ha) + "\n" + k_hyx_kcommon + "\n" +
                          k_hyx_kevent + "\n";
@@ -798,7 +801,7 @@ int hyx_batch_create(hyx_sys *s, uint64_t n_inst, hyx_batch **out)
             void **p;
             size_t bytes;
         } ev_allocs[] = {
-            {(void **)&b->tc, (n * P1 + s->ev_rows) * N * 8},
+            {(void **)&b->tc, (n * P1 + s->ev_rows + (uint64_t)s->ev.dir.size() * P1) * N * 8},
             {(void **)&b->evcd, 3 * nte * N * 8},
             {(void **)&b->te, N * 8},
             {(void **)&b->ntc, N * 4},
