@@ -304,8 +304,10 @@ def main():
 
 def events_bench(args):
     """Event detection at ensemble scale: the CR3BP ensemble (65,536 instances, t in
-    [0, 20], tol 1e-15) propagated (A) without events, (B) with a terminal escape event
-    (r^2 = 4, no callback: stops on the device), (C) with B plus a non-terminal section
+    [0, 100] (--dt sets the horizon), tol 1e-15) propagated (A) without events, (B0) with a
+    terminal event that never triggers (r^2 = 1e4: the per-step detection cost), (B) with a
+    terminal escape event (r^2 = 4, no callback: stops on the device), (C) with B plus a
+    non-terminal section
     event x = 1/2 - mu (direction positive) whose Python callback logs every crossing:
     instances park after a step with crossings and resume after their callbacks.
     Reports instance-steps/s of each, the crossings per second and the kernel share."""
@@ -314,17 +316,20 @@ def events_bench(args):
 
     N = args.n or 65536
     wl = W.cr3bp(N)
-    T = 20.0
+    T = args.dt if args.dt != 25.0 else 100.0
     x, y, vx, vy = hy.make_vars("x", "y", "vx", "vy")
     results = {}
-    for label in ("A_no_events", "B_terminal_device", "C_terminal_and_section_parked"):
+    for label in ("A_no_events", "B0_terminal_never_triggers", "B_terminal_device", "C_terminal_and_section_parked"):
         crossings = [0]
 
         def cb(ta, t, d_sgn, bidx):
             crossings[0] += 1
 
         kw = {}
-        if label != "A_no_events":
+        if label.startswith("B0"):
+            # Same detection work per step, no trigger: the per-step cost of event detection.
+            kw["t_events"] = [hy.t_event_batch(x * x + y * y - 1e4, direction=hy.event_direction.positive)]
+        elif label != "A_no_events":
             kw["t_events"] = [hy.t_event_batch(x * x + y * y - 4.0, direction=hy.event_direction.positive)]
         if label.startswith("C"):
             kw["nt_events"] = [hy.nt_event_batch(x - 0.5 + hy.par[0], cb, direction=hy.event_direction.positive)]
@@ -341,6 +346,7 @@ def events_bench(args):
         ms, launches = ta.last_kernel_timing()
         stopped = int(sum(1 for r in ta.propagate_res if int(r[0]) == -1))
         results[label] = {"instance_steps_per_s": ns / wall, "instance_steps": ns, "wall_s": wall,
+                          "kernel_ms_last_launch_group": ms, "launches_last_call": launches,
                           "escapes_stopped": stopped, "section_crossings": crossings[0],
                           "crossings_per_s": crossings[0] / wall if crossings[0] else None}
         print(f"[bench] events {label}: {ns / wall:.3e} instance-steps/s, {crossings[0]} crossings, "
