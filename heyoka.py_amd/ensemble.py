@@ -11,7 +11,9 @@ instances, propagated in one persistent launch, and unpacked into the member
 integrators. Instances are independent and their arithmetic does not depend on
 where they sit in the batch, so every member ends bit-for-bit where its own
 ``propagate_until`` would have taken it (tests/test_gpu_ensemble.py checks this).
-Members with a step callback are propagated one at a time instead.
+Members with a step callback, or whose events call back into Python, are
+propagated one at a time instead; terminal events without callbacks stop
+instances inside the packed launch (cooldowns are packed with the members).
 ``ensemble_propagate_grid[_batch]`` splats the 1-D grid over every instance
 (``_ensemble_impl.py:13-19``) and returns ``(ta_i, None, out_i)`` with
 ``out_i`` of shape (n_pts, n, batch_size), cut from one device-side
@@ -60,6 +62,9 @@ def _ensemble_generic(tp, ta, arg, n_iter, gen, **kwargs):
         raise NotImplementedError("Continuous output is not supported by the GPU batch integrator")
     if "callback" in kwargs and kwargs["callback"] is not None:
         return _ensemble_sequential(tp, ta, arg, n_iter, gen, kwargs)
+    if isinstance(ta, taylor_adaptive_batch) and not ta._device_events_ok():
+        # Event callbacks receive their own member integrator: one member at a time.
+        return _ensemble_sequential(tp, ta, arg, n_iter, gen, kwargs)
     return _ensemble_packed(tp, ta, arg, n_iter, gen, kwargs)
 
 
@@ -72,7 +77,8 @@ def _ensemble_sequential(tp, ta, arg, n_iter, gen, kwargs):
     out = []
     for i in range(n_iter):
         kw = copy(kwargs)
-        kw["callback"] = deepcopy(kwargs["callback"])
+        if kwargs.get("callback") is not None:
+            kw["callback"] = deepcopy(kwargs["callback"])
         local = gen(deepcopy(ta), i)
         if tp == "until":
             ret = local.propagate_until(arg, **kw)
@@ -145,6 +151,10 @@ def _ensemble_packed(tp, ta, arg, n_iter, gen, kwargs):
         batch.write(_lib.FIELD_PARS, pa)
     batch.write(_lib.FIELD_TIME_HI, thi)
     batch.write(_lib.FIELD_TIME_LO, tlo)
+    if ta._hyx_tes:
+        # Terminal events without callbacks stop instances on the device; the members'
+        # cooldowns travel with them.
+        batch.set_cooldowns(np.concatenate([m._hyx_batch.cooldowns() for m in members], axis=2))
     if tp == "grid":
         r, gout = batch.propagate_grid(grid, md, max_steps, _MODES[ta._hyx_flags[3]])
     else:
@@ -152,9 +162,12 @@ def _ensemble_packed(tp, ta, arg, n_iter, gen, kwargs):
     st = batch.read(_lib.FIELD_STATE, (n, N))
     thi = batch.read(_lib.FIELD_TIME_HI, (N,))
     tlo = batch.read(_lib.FIELD_TIME_LO, (N,))
+    cds = batch.cooldowns() if ta._hyx_tes else None
     out = []
     for i, m in enumerate(members):
         sl = slice(i * B, (i + 1) * B)
+        if cds is not None:
+            m._hyx_batch.set_cooldowns(cds[:, :, sl])
         m._hyx_state[...] = st[:, sl]
         m._hyx_thi[...] = thi[sl]
         m._hyx_tlo[...] = tlo[sl]

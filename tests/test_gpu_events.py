@@ -379,3 +379,32 @@ def test_parked_grid_and_large_batch():
     np.testing.assert_allclose(ta.state[:, sub], ref["state"], rtol=0, atol=1e-13)
     # The events change the step sizes (their norms join the step's), not the solution.
     np.testing.assert_allclose(out, o2, rtol=0, atol=1e-12)
+
+
+def test_ensembles_with_events():
+    print("[gpu] ensembles with events", flush=True)
+    ic = np.array([[0.05, 0.2], [0.0, 0.4]])
+
+    def gen(t, i):
+        t.state[:] = ic + 0.01 * i
+        return t
+
+    # Stopping terminal events: one packed launch, members equal to their own runs.
+    ta = hy.taylor_adaptive_batch(PEND, ic, t_events=[hy.t_event_batch(x - 0.1)])
+    ret = hy.ensemble_propagate_until_batch(ta, 5.0, 4, gen)
+    for i, (m, _, _) in enumerate(ret):
+        one = hy.taylor_adaptive_batch(PEND, ic + 0.01 * i, t_events=[hy.t_event_batch(x - 0.1)])
+        one.propagate_until(5.0)
+        np.testing.assert_array_equal(m.state, one.state)
+        assert m.propagate_res == one.propagate_res
+        assert m.te_cooldowns == one.te_cooldowns
+    # Non-terminal events with callbacks: each member's callback sees its own integrator.
+    seen = {}
+
+    class cb:
+        def __call__(self, t_, t, d, b):
+            seen.setdefault(id(t_), []).append(t)
+
+    ta = hy.taylor_adaptive_batch(PEND, ic, nt_events=[hy.nt_event_batch(v, cb())])
+    ret = hy.ensemble_propagate_until_batch(ta, 5.0, 3, gen)
+    assert len(seen) == 3 and all(id(m) in seen for m, _, _ in ret)
