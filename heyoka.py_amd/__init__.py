@@ -17,6 +17,7 @@ from .var_ode_sys import var_ode_sys, var_args  # noqa: F401
 from .cfunc import cfunc  # noqa: F401
 from .integrator import taylor_adaptive_batch as _tab, taylor_outcome  # noqa: F401
 from .events import event_direction, t_event_batch, nt_event_batch  # noqa: F401
+from .c_output import continuous_output_batch, continuous_output_batch_dbl  # noqa: F401
 from .ensemble import (  # noqa: F401
     ensemble_propagate_until,
     ensemble_propagate_for,

@@ -58,9 +58,7 @@ def _ensemble_generic(tp, ta, arg, n_iter, gen, **kwargs):
         kwargs.pop("chunksize", None)
     if "chunksize" in kwargs:
         raise TypeError("propagate_until() got an unexpected keyword argument 'chunksize'")
-    if kwargs.get("c_output"):
-        raise NotImplementedError("Continuous output is not supported by the GPU batch integrator")
-    if "callback" in kwargs and kwargs["callback"] is not None:
+    if ("callback" in kwargs and kwargs["callback"] is not None) or kwargs.get("c_output"):
         return _ensemble_sequential(tp, ta, arg, n_iter, gen, kwargs)
     if isinstance(ta, taylor_adaptive_batch) and not ta._device_events_ok():
         # Event callbacks receive their own member integrator: one member at a time.

@@ -25,6 +25,7 @@ import numpy as np
 
 from . import _lib
 from .decompose import taylor_decompose, order_from_tol
+from .c_output import continuous_output_batch
 from .events import nt_event_batch, t_event_batch, _event_ref
 from .expr import _ex, expression, get_variables
 from .var_ode_sys import var_ode_sys as _var_ode_sys
@@ -416,13 +417,13 @@ class taylor_adaptive_batch:
         return mdt
 
     def _prop_impl(self, tf_hi, tf_lo, max_steps, max_delta_t, callback, write_tc, c_output):
-        if c_output:
-            raise NotImplementedError("Continuous output is not supported by the GPU batch integrator")
         mdt = self._prop_args(max_steps, max_delta_t)
         if not np.all(np.isfinite(tf_hi)) or not np.all(np.isfinite(self._hyx_thi)):
             raise ValueError("A non-finite time was passed to the propagate_until() function of an adaptive Taylor "
                              "integrator")
         cb = self._check_cb(callback)
+        if c_output:
+            return self._prop_c_output(tf_hi, tf_lo, int(max_steps), mdt, cb)
         if cb is None and not write_tc and not self._device_events_ok():
             self._push()
             self._prop_parked(lambda: self._hyx_batch.propagate_until(tf_hi, tf_lo, mdt, int(max_steps),
@@ -437,6 +438,28 @@ class taylor_adaptive_batch:
                                   for o, a, b, c in zip(r["outcome"], r["min_h"], r["max_h"], r["nsteps"])]
             return None, None
         return self._prop_stepwise(tf_hi, tf_lo, int(max_steps), mdt, cb, write_tc)
+
+    def _prop_c_output(self, tf_hi, tf_lo, max_steps, mdt, cb):
+        """propagate with c_output=True: host-driven steps, each keeping its Taylor
+        coefficients and start time per batch element (continuous_output_batch)."""
+        B = self._hyx_B
+        t0 = self._hyx_thi + self._hyx_tlo
+        rec = [([], [], [], []) for _ in range(B)]
+
+        def after_step(h, live, fin, tdir):
+            th, tl = self._hyx_t_before
+            for b in np.nonzero(live)[0]:
+                if h[b] != 0.0 or not rec[b][2]:
+                    r = rec[b]
+                    r[0].append(th[b])
+                    r[1].append(tl[b])
+                    r[2].append(h[b])
+                    r[3].append(self._hyx_tc[:, :, b].copy())
+
+        _, cb = self._prop_stepwise(tf_hi, tf_lo, max_steps, mdt, cb, True, after_step)
+        steps = [(np.array(a), np.array(c), np.array(d), np.array(e)) for a, c, d, e in rec]
+        n = self._hyx_state.shape[0]
+        return continuous_output_batch._from_steps(steps, n, self._hyx_order, t0, self._hyx_thi + self._hyx_tlo), cb
 
     def _device_events_ok(self):
         """Propagations run entirely on the device unless an event needs the host: every
@@ -462,6 +485,7 @@ class taylor_adaptive_batch:
             rem = dfl_add(tf_hi, tf_lo, -self._hyx_thi, -self._hyx_tlo)[0]
             lim = np.where(tdir, np.minimum(mdt, rem), np.maximum(-mdt, rem))
             lim = np.where(done, 0.0, lim)
+            self.__dict__["_hyx_t_before"] = (self._hyx_thi.copy(), self._hyx_tlo.copy())
             oc, h = self._step_impl(lim, write_tc)
             it += 1
             act = ~done

@@ -163,6 +163,13 @@ def main():
     fl = _floats(txt)
     g["var2_displaced"] = {"src": f"doc/notebooks/{vo}:650-651", "dx": 1e-4, "dv": -2e-4, "nonvar_state": fl[0:2], "digits": 8}
 
+    # Continuous output of the forced damped pendulum batch, propagate_until([10, 10.1, 10.2, 10.3]).
+    txt = _out_text([c for c in _cells(bm) if "".join(c["source"]).strip() == "c_arr[0, 1, 2]"][0])
+    g["fdp_c_output"] = {"src": f"doc/notebooks/{bm}:{_line_of(bm, '1.8917796783928316')}",
+                         "t_final": [10.0, 10.1, 10.2, 10.3],
+                         "times": [[0.01, 0.02, 0.03, 0.04], [1.01, 1.02, 1.03, 1.04], [2.01, 2.02, 2.03, 2.04]],
+                         "c_arr_0_1_2": _floats(txt)[0]}
+
     # Event detection (pendulum x' = v, v' = -9.8 sin(x), x0 = -0.05, v0 = 0).
     ed = "Event detection.ipynb"
     txt = _out_text(_cell_by_source(ed, "print(zero_vel_times)", 0))
