@@ -124,6 +124,8 @@ def lib():
             "hyx_events_gather": (ct.c_int, [vp, P(u32), u32, P(i64), P(u32), P(d), P(i64), P(d), P(d)]),
             "hyx_events_resume": (ct.c_int, [vp, P(u32), u32, P(i64)]),
             "hyx_propagate_resume": (ct.c_int, [vp, u64, ct.c_int, P(prop_res), P(d)]),
+            "hyx_batch_set_clog": (ct.c_int, [vp, u32]),
+            "hyx_clog_drain": (ct.c_int, [vp, P(u32), u32, P(u32), P(d), P(d)]),
             "hyx_cooldowns_read": (ct.c_int, [vp, P(d)]),
             "hyx_cooldowns_write": (ct.c_int, [vp, P(d)]),
             "hyx_last_error": (ct.c_char_p, []),
@@ -151,7 +153,7 @@ EXPORTED = (
     "hyx_cfunc_create", "hyx_cfunc_source", "hyx_cfunc_eval", "hyx_cfunc_eval_batch", "hyx_cfunc_destroy",
     "hyx_sys_create_events", "hyx_sys_n_events", "hyx_events_read", "hyx_cooldowns_read", "hyx_cooldowns_write",
     "hyx_batch_set_event_pause", "hyx_events_parked", "hyx_events_gather", "hyx_events_resume",
-    "hyx_propagate_resume",
+    "hyx_propagate_resume", "hyx_batch_set_clog", "hyx_clog_drain",
     "hyx_last_error", "hyx_device_count",
 )
 
@@ -343,6 +345,20 @@ class Batch:
         sl = np.ascontiguousarray(lanes, dtype=np.uint32)
         st = np.ascontiguousarray(stop, dtype=np.int64)
         check(self._L.hyx_events_resume(self.h, dptr(sl, ct.c_uint32), len(sl), dptr(st, ct.c_int64)))
+
+    def set_clog(self, cap: int):
+        check(self._L.hyx_batch_set_clog(self.h, int(cap)))
+        self._clog_cap = int(cap)
+
+    def clog_drain(self, lanes):
+        """Empty the step logs of the listed instances: (nrec [k], tc [cap, n, p+1, k], tt [cap, 3, k])."""
+        sl = np.ascontiguousarray(lanes, dtype=np.uint32)
+        k, cap = len(sl), self._clog_cap
+        nrec = np.empty(k, dtype=np.uint32)
+        tc = np.empty((cap, self.sys.n, self.sys.order + 1, k), dtype=np.float64)
+        tt = np.empty((cap, 3, k), dtype=np.float64)
+        check(self._L.hyx_clog_drain(self.h, dptr(sl, ct.c_uint32), k, dptr(nrec, ct.c_uint32), dptr(tc), dptr(tt)))
+        return nrec, tc, tt
 
     def propagate_resume(self, max_steps=0, mode=MODE_PERSISTENT, grid_out=None):
         res = (prop_res * self.N)()

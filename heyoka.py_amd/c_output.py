@@ -47,8 +47,9 @@ class continuous_output_batch:
         self._order = 0
 
     @classmethod
-    def _from_steps(cls, steps, n, order, t0, t1):
+    def _from_steps(cls, steps, n, order, t0, t1, final=None):
         c = cls()
+        c._final = np.zeros((n, len(steps))) if final is None else np.array(final, dtype=np.float64)
         c._B = len(steps)
         c._steps = steps
         c._n, c._order = n, order
@@ -60,7 +61,11 @@ class continuous_output_batch:
     def _eval_row(self, tm, out):
         for b, (thi, tlo, h, tc) in enumerate(self._steps):
             s = len(h)
-            fwd = not (s and h[0] < 0)
+            if s == 0:
+                # No step taken (already at the final time): the solution is the state there.
+                out[:, b] = self._final[:, b]
+                continue
+            fwd = not (h[0] < 0)
             t = float(tm[b])
             # The step whose interval holds t; before the start the first, after the end the last.
             ends = thi + h  # step end times (hi parts), monotonic in the direction of integration
@@ -118,7 +123,10 @@ class continuous_output_batch:
             times[0, b] = thi[0] + tlo[0] if s else self._bounds[0][b]
             times[1:s + 1, b] = thi + tlo + h
             tcs[:s, :, :, b] = tc
-            if s < ns:
+            if s == 0:
+                times[1:, b] = times[0, b]
+                tcs[:, :, 0, b] = self._final[:, b]
+            elif s < ns:
                 end = times[s, b]
                 times[s + 1:, b] = end
                 # Zero-length steps at the final state.

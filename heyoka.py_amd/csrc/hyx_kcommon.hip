@@ -67,6 +67,11 @@ struct hyx_kargs {
     hyx_i64 *nt_code;       // events: [nt_cap][ld] 4 * (non-terminal index) + d_sgn + 1
     unsigned nt_cap;        // events: records kept per instance (0: none)
     unsigned ev_pause;      // events: propagations park a lane after a step that needs the host
+    double *clog;           // continuous output: [clog_cap][n * (P+1)][ld] coefficients of each step
+    double *clog_t;         // continuous output: [clog_cap][3][ld] start time hi, lo and size of each step
+    unsigned *clog_n;       // continuous output: [N] steps logged since the host last drained
+    unsigned clog_cap;      // continuous output: steps per instance before it parks
+    unsigned pad3_;
 };
 
 // Propagation status of an instance: 0 / 2 running forward / backward, 1 done,

@@ -16,7 +16,12 @@
 // root scan (and then leaves the coefficients in a.tc); event processing may
 // shorten the step to a terminal trigger, and a propagation ends at a terminal
 // event with outcome -(index + 1).
-#ifdef HYX_GRID
+//
+// Built with HYX_CLOG defined (a third code object, for continuous output), every
+// step of a propagation writes its coefficients and its start time and size to
+// the next slot of the instance's log (a.clog, a.clog_t); an instance whose log
+// is full parks (as for events) until the host has drained it.
+#if defined(HYX_GRID) || defined(HYX_CLOG)
 #define HYX_PROP_WTC true
 #else
 #define HYX_PROP_WTC false
@@ -46,6 +51,7 @@ struct hyx_inst {
     double hlast; // the last step's size (kept for the host when the instance parks)
     int tdir;
     unsigned gc; // propagate_grid: next grid point
+    unsigned cn; // continuous output: log slots in use
 };
 
 __device__ __forceinline__ void hyx_load(hyx_inst &s, const hyx_kargs &a, hyx_u64 i)
@@ -105,8 +111,15 @@ __device__ __forceinline__ bool hyx_prop_iter(hyx_inst &s, const hyx_kargs &a, h
     bool nf;
     unsigned long long evm = 0;
     (void)evm;
-    hyx_taylor_step<HYX_PROP_WTC>(s.x, s.pv, s.thi, lim, h, nf, HYX_TC_ON ? a.tc + i : nullptr, a.ld,
-                                  a.tape ? a.tape + lane_tape : nullptr, a.tape_stride HYX_EV_ARGS);
+#ifdef HYX_CLOG
+    const unsigned slot = s.cn; // < a.clog_cap: an instance with a full log parks
+    double *tcp = a.clog + (hyx_u64)slot * (hyx_u64)(HYX_N * (HYX_P + 1)) * a.ld + i;
+    const double th0 = s.thi, tl0 = s.tlo;
+#else
+    double *tcp = HYX_TC_ON ? a.tc + i : nullptr;
+#endif
+    hyx_taylor_step<HYX_PROP_WTC>(s.x, s.pv, s.thi, lim, h, nf, tcp, a.ld, a.tape ? a.tape + lane_tape : nullptr,
+                                  a.tape_stride HYX_EV_ARGS);
     int te = -1;
     bool park = false;
 #ifdef HYX_EVENTS
@@ -117,6 +130,12 @@ __device__ __forceinline__ bool hyx_prop_iter(hyx_inst &s, const hyx_kargs &a, h
 #endif
     hyx_dfl_add(s.thi, s.tlo, h, 0.0);
     s.hlast = h;
+#ifdef HYX_CLOG
+    a.clog_t[((hyx_u64)slot * 3) * a.ld + i] = th0;
+    a.clog_t[((hyx_u64)slot * 3 + 1) * a.ld + i] = tl0;
+    a.clog_t[((hyx_u64)slot * 3 + 2) * a.ld + i] = h;
+    s.cn = slot + 1;
+#endif
     nf = nf || !isfinite(s.thi);
     s.iter += 1;
     s.nst += (h != 0.0) ? 1 : 0;
@@ -151,6 +170,12 @@ __device__ __forceinline__ bool hyx_prop_iter(hyx_inst &s, const hyx_kargs &a, h
         *res = HYX_STEP_LIMIT;
         return true;
     }
+#ifdef HYX_CLOG
+    if (s.cn >= a.clog_cap) {
+        *res = HYX_PARKED; // the host drains the log, then resumes the instance
+        return true;
+    }
+#endif
     return false;
 }
 
@@ -169,6 +194,9 @@ __device__ __forceinline__ void hyx_resume(hyx_inst &s, const hyx_kargs &a, hyx_
 #ifdef HYX_GRID
     s.gc = a.gcur[i];
 #endif
+#ifdef HYX_CLOG
+    s.cn = a.clog_n[i];
+#endif
 }
 
 // Write an instance back; done instances also record their outcome.
@@ -181,6 +209,9 @@ __device__ __forceinline__ void hyx_suspend(const hyx_inst &s, const hyx_kargs &
     a.nsteps[i] = s.nst;
 #ifdef HYX_GRID
     a.gcur[i] = s.gc;
+#endif
+#ifdef HYX_CLOG
+    a.clog_n[i] = s.cn;
 #endif
     if (done && res == HYX_PARKED) {
         a.status[i] = HYX_ST_PARKED | (s.tdir ? 0u : 2u);

@@ -91,6 +91,11 @@ struct KArgs {
     int64_t *nt_code;
     unsigned nt_cap;
     unsigned ev_pause;
+    double *clog;
+    double *clog_t;
+    unsigned *clog_n;
+    unsigned clog_cap;
+    unsigned pad3_;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -228,6 +233,9 @@ struct hyx_sys {
     std::mutex gmu;
     hipModule_t gmod = nullptr;
     hipFunction_t kg_init = nullptr, kg_iter = nullptr, kg_persist = nullptr;
+    // Continuous output: the same source built with HYX_CLOG, loaded on first use.
+    hipModule_t cmod = nullptr;
+    hipFunction_t kc_init = nullptr, kc_iter = nullptr, kc_persist = nullptr;
     // Events: the event op-list (rows = state variables, right-hand sides = event
     // functions, terminal ones first) and its specification.
     bool has_events = false;
@@ -274,6 +282,10 @@ struct hyx_batch {
     unsigned nt_cap = 0;
     bool ev_pause = false;       // propagations park instances whose events need the host
     bool last_grid = false;      // the last propagation was a propagate_grid (for resumes)
+    // Continuous output: per-instance logs of step coefficients / start times and sizes.
+    double *clog = nullptr, *clog_t = nullptr;
+    unsigned *clog_n = nullptr;
+    unsigned clog_cap = 0;
 };
 
 namespace
@@ -404,6 +416,10 @@ KArgs make_args(hyx_batch *b)
     // Records are kept by hyx_step and by propagations that park.
     a.nt_cap = b->ev_pause ? b->nt_cap : 0;
     a.ev_pause = b->ev_pause ? 1u : 0u;
+    a.clog = b->clog;
+    a.clog_t = b->clog_t;
+    a.clog_n = b->clog_n;
+    a.clog_cap = b->clog_cap;
     return a;
 }
 
@@ -751,6 +767,7 @@ int hyx_sys_kernel_info(const hyx_sys *s, int *vgprs, int *scratch_bytes)
 void hyx_sys_destroy(hyx_sys *s)
 {
     if (!s) return;
+    if (s->cmod) hipModuleUnload(s->cmod);
     if (s->gmod) hipModuleUnload(s->gmod);
     if (s->mod) hipModuleUnload(s->mod);
     delete s;
@@ -834,7 +851,8 @@ void hyx_batch_destroy(hyx_batch *b)
     void *ptrs[] = {b->state, b->pars, b->thi,    b->tlo,   b->tfhi,   b->tflo,  b->mdt,
                     b->hout,  b->minh, b->maxh,   b->tc,    b->oc,     b->nsteps, b->iters,
                     b->work,  b->status, b->active, b->unfinished, b->queue, b->tape, b->stage, b->stage_idx,
-                    b->grid,  b->gout, b->gcur, b->evcd, b->te, b->ntc, b->nt_time, b->nt_code};
+                    b->grid,  b->gout, b->gcur, b->evcd, b->te, b->ntc, b->nt_time, b->nt_code,
+                    b->clog, b->clog_t, b->clog_n};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (b->ev0) hipEventDestroy(b->ev0);
@@ -995,15 +1013,45 @@ int ensure_grid_module(hyx_sys *s)
     return 0;
 }
 
+// Loads the continuous-output code object of a system (HYX_CLOG) on first use.
+int ensure_clog_module(hyx_sys *s)
+{
+    std::lock_guard<std::mutex> lk(s->gmu);
+    if (s->cmod) return 0;
+    std::vector<char> code;
+    if (int rc = get_code("#define HYX_CLOG 1\n" + s->src, device_arch(s->device), code)) return rc;
+    hipModule_t m = nullptr;
+    HIPCHK(hipModuleLoadData(&m, code.data()));
+    struct {
+        hipFunction_t *f;
+        const char *name;
+    } fns[] = {{&s->kc_init, "hyx_iter_init_k"}, {&s->kc_iter, "hyx_iter_k"}, {&s->kc_persist, "hyx_persist_k"}};
+    for (auto &f : fns) {
+        hipError_t e = hipModuleGetFunction(f.f, m, f.name);
+        if (e != hipSuccess) {
+            hipModuleUnload(m);
+            return fail(HYX_EHIP, std::string("hipModuleGetFunction(") + f.name + "): " + hipGetErrorString(e));
+        }
+    }
+    s->cmod = m;
+    return 0;
+}
+
 // The propagate loop shared by propagate_until and propagate_grid: final
 // times, max_delta_t (and the grid buffers) are already on the device.
 int run_propagation(hyx_batch *b, uint64_t max_steps, int mode, hyx_prop_res *res, bool grid_run, bool init = true)
 {
     const uint64_t N = b->N;
     hyx_sys *s = b->sys;
-    hipFunction_t k_init = grid_run ? s->kg_init : s->k_init;
-    hipFunction_t k_iter = grid_run ? s->kg_iter : s->k_iter;
-    hipFunction_t k_persist = grid_run ? s->kg_persist : s->k_persist;
+    const bool clog = b->clog_cap > 0;
+    if (clog && grid_run) return fail(HYX_EINVAL, "continuous output does not combine with propagate_grid");
+    if (clog) {
+        if (int rc = ensure_clog_module(s)) return rc;
+    }
+    hipFunction_t k_init = grid_run ? s->kg_init : clog ? s->kc_init : s->k_init;
+    hipFunction_t k_iter = grid_run ? s->kg_iter : clog ? s->kc_iter : s->k_iter;
+    hipFunction_t k_persist = grid_run ? s->kg_persist : clog ? s->kc_persist : s->k_persist;
+    if (clog && mode == HYX_MODE_GRAPH) mode = HYX_MODE_PLAIN; // the captured graph uses the plain code object
     if (grid_run && mode == HYX_MODE_GRAPH) mode = HYX_MODE_PLAIN; // no captured graph for the grid code object
     if (b->ev_pause && mode == HYX_MODE_GRAPH) mode = HYX_MODE_PLAIN; // the graph holds arguments without parking
     uint64_t pgrid = 0;
@@ -1700,6 +1748,66 @@ int hyx_propagate_resume(hyx_batch *b, uint64_t max_steps, int mode, hyx_prop_re
         HIPCHK(hipMemcpyAsync(grid_out, b->gout, b->npts * b->sys->S.n * b->N * 8, hipMemcpyDeviceToHost, b->stream));
         HIPCHK(hipStreamSynchronize(b->stream));
     }
+    return 0;
+}
+
+// ---- continuous output: per-instance step logs ------------------------------------------
+
+int hyx_batch_set_clog(hyx_batch *b, uint32_t cap)
+{
+    if (!b) return fail(HYX_EINVAL, "null batch");
+    if (cap && b->sys->has_events) return fail(HYX_EINVAL, "the step log does not combine with events");
+    if (cap && b->sys->form == HYX_FORM_GROUP) return fail(HYX_EINVAL, "the group form has no step log");
+    void *ptrs[] = {b->clog, b->clog_t, b->clog_n};
+    for (void *p : ptrs)
+        if (p) HIPCHK(hipFree(p));
+    b->clog = b->clog_t = nullptr;
+    b->clog_n = nullptr;
+    b->clog_cap = 0;
+    if (cap == 0) return 0;
+    const uint64_t N = b->N, rows = (uint64_t)b->sys->S.n * (b->sys->order + 1);
+    if (hipMalloc((void **)&b->clog, (size_t)cap * rows * N * 8) != hipSuccess ||
+        hipMalloc((void **)&b->clog_t, (size_t)cap * 3 * N * 8) != hipSuccess ||
+        hipMalloc((void **)&b->clog_n, (size_t)N * 4) != hipSuccess) {
+        hyx_batch_set_clog(b, 0);
+        return fail(HYX_ENOMEM, "hipMalloc of the step log failed");
+    }
+    HIPCHK(hipMemsetAsync(b->clog_n, 0, N * 4, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->clog_cap = cap;
+    return 0;
+}
+
+int hyx_clog_drain(hyx_batch *b, const uint32_t *lanes, uint32_t count, uint32_t *nrec, double *tc, double *tt)
+{
+    if (!b) return fail(HYX_EINVAL, "null batch");
+    if (!b->clog_cap) return fail(HYX_EINVAL, "no step log: call hyx_batch_set_clog first");
+    if (int rc = check_slots(b, lanes, count)) return rc;
+    if (count == 0) return 0;
+    const unsigned rows = b->sys->S.n * (b->sys->order + 1);
+    if (nrec) {
+        if (int rc = stage_reserve(b, (size_t)count * 8, count)) return rc;
+        HIPCHK(hipMemcpyAsync(b->stage_idx, lanes, (size_t)count * 4, hipMemcpyHostToDevice, b->stream));
+        unsigned *src = b->clog_n, *dst = (unsigned *)b->stage, *idx = b->stage_idx;
+        void *params[] = {&src, &dst, &idx, &count};
+        HIPCHK(launch_raw(b, b->sys->k_gather32, count, params));
+        HIPCHK(hipMemcpyAsync(nrec, b->stage, (size_t)count * 4, hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+    }
+    if (tc) {
+        if (int rc = gather64(b, b->clog, b->clog_cap * rows, lanes, count, tc)) return rc;
+    }
+    if (tt) {
+        if (int rc = gather64(b, b->clog_t, b->clog_cap * 3, lanes, count, tt)) return rc;
+    }
+    // The drained logs start over.
+    if (int rc = stage_reserve(b, (size_t)count * 8, count)) return rc;
+    HIPCHK(hipMemcpyAsync(b->stage_idx, lanes, (size_t)count * 4, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemsetAsync(b->stage, 0, (size_t)count * 4, b->stream));
+    unsigned *dst = b->clog_n, *src = (unsigned *)b->stage, *idx = b->stage_idx;
+    void *params[] = {&dst, &src, &idx, &count};
+    HIPCHK(launch_raw(b, b->sys->k_scatter32, count, params));
+    HIPCHK(hipStreamSynchronize(b->stream));
     return 0;
 }
 

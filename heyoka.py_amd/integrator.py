@@ -440,10 +440,15 @@ class taylor_adaptive_batch:
         return self._prop_stepwise(tf_hi, tf_lo, int(max_steps), mdt, cb, write_tc)
 
     def _prop_c_output(self, tf_hi, tf_lo, max_steps, mdt, cb):
-        """propagate with c_output=True: host-driven steps, each keeping its Taylor
-        coefficients and start time per batch element (continuous_output_batch)."""
+        """propagate with c_output=True (continuous_output_batch): every step's Taylor
+        coefficients and start time per batch element. On the device (a step log per
+        instance, drained when full) unless a step callback, events or the group form
+        need host-driven steps."""
         B = self._hyx_B
         t0 = self._hyx_thi + self._hyx_tlo
+        n = self._hyx_state.shape[0]
+        if cb is None and not self.with_events and self._hyx_system.form != "group":
+            return self._prop_c_output_device(tf_hi, tf_lo, max_steps, mdt, t0), None
         rec = [([], [], [], []) for _ in range(B)]
 
         def after_step(h, live, fin, tdir):
@@ -458,8 +463,50 @@ class taylor_adaptive_batch:
 
         _, cb = self._prop_stepwise(tf_hi, tf_lo, max_steps, mdt, cb, True, after_step)
         steps = [(np.array(a), np.array(c), np.array(d), np.array(e)) for a, c, d, e in rec]
-        n = self._hyx_state.shape[0]
         return continuous_output_batch._from_steps(steps, n, self._hyx_order, t0, self._hyx_thi + self._hyx_tlo), cb
+
+    def _prop_c_output_device(self, tf_hi, tf_lo, max_steps, mdt, t0):
+        b = self._hyx_batch
+        B, n, P = self._hyx_B, self._hyx_state.shape[0], self._hyx_order
+        mode = _MODES[self._hyx_flags[3]]
+        # Log slots per instance: up to 64 steps, within about 1 GiB of device memory.
+        cap = int(max(1, min(64, (1 << 30) // (B * n * (P + 1) * 8))))
+        parts = [[] for _ in range(B)]
+
+        def drain(lanes):
+            nrec, tc, tt = b.clog_drain(lanes)
+            for j, lane in enumerate(lanes):
+                k = int(nrec[j])
+                if k:
+                    parts[lane].append((tt[:k, 0, j].copy(), tt[:k, 1, j].copy(), tt[:k, 2, j].copy(),
+                                        np.moveaxis(tc[:k, :, :, j], 0, 0).copy()))
+
+        b.set_clog(cap)
+        try:
+            self._push()
+            r = b.propagate_until(tf_hi, tf_lo, mdt, max_steps, mode)
+            while True:
+                lanes = b.events_parked()
+                if lanes.size == 0:
+                    break
+                drain(lanes)
+                b.events_resume(lanes, np.zeros(lanes.size, dtype=np.int64))
+                r = b.propagate_resume(max_steps, mode)
+            drain(np.arange(B, dtype=np.uint32))
+        finally:
+            b.set_clog(0)
+        self._pull()
+        self._hyx_prop_res = [(_outcome(o), float(a), float(c), int(d))
+                              for o, a, c, d in zip(r["outcome"], r["min_h"], r["max_h"], r["nsteps"])]
+        steps = []
+        for lane in range(B):
+            ps = parts[lane]
+            if ps:
+                steps.append(tuple(np.concatenate([p[q] for p in ps]) for q in range(4)))
+            else:
+                steps.append((np.zeros(0), np.zeros(0), np.zeros(0), np.zeros((0, n, P + 1))))
+        return continuous_output_batch._from_steps(steps, n, P, t0, self._hyx_thi + self._hyx_tlo,
+                                                   final=self._hyx_state)
 
     def _device_events_ok(self):
         """Propagations run entirely on the device unless an event needs the host: every

@@ -197,6 +197,17 @@ int hyx_events_gather(hyx_batch *b, const uint32_t *lanes, uint32_t count, int64
                       double *nt_time, int64_t *nt_code, double *h, double *tc);
 int hyx_events_resume(hyx_batch *b, const uint32_t *lanes, uint32_t count, const int64_t *stop);
 int hyx_propagate_resume(hyx_batch *b, uint64_t max_steps, int mode, hyx_prop_res *res, double *grid_out);
+
+/* Continuous output on the device (c_output=True of propagate_until / propagate_for,
+ * heyoka/taylor_expose_c_output.cpp:269-520): with a step log of cap slots per
+ * instance, propagations write every step's Taylor coefficients and its start time
+ * (hi, lo) and size to the instance's log, and an instance whose log is full parks
+ * (hyx_events_parked / hyx_events_resume with stop = NULL). hyx_clog_drain copies the
+ * logs of the listed instances out and empties them: nrec [count] steps logged, tc
+ * [cap][n][p+1][count] and tt [cap][3][count] (any may be NULL). cap = 0 removes the
+ * log. Lane code forms, systems without events. */
+int hyx_batch_set_clog(hyx_batch *b, uint32_t cap);
+int hyx_clog_drain(hyx_batch *b, const uint32_t *lanes, uint32_t count, uint32_t *nrec, double *tc, double *tt);
 /* te_cooldowns / reset_cooldowns (expose_batch_integrators.cpp:652-654): [n_te][3][N]
  * trigger time (hi, lo) and cooldown length; a negative length = no cooldown. */
 int hyx_cooldowns_read(hyx_batch *b, double *dst);

@@ -136,3 +136,42 @@ def test_c_output_matches_grid_and_backward():
     assert "backward" in repr(back)
     np.testing.assert_allclose(back(np.full(6, 0.0)), ic, rtol=0, atol=1e-13)
     np.testing.assert_allclose(back(grid[::-1]), out[::-1], rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("mode", ["persistent", "plain"])
+def test_device_log_matches_host_steps(mode):
+    # The device path (per-instance step logs, instances park when their 64-slot log is
+    # full) against host-driven steps (forced by a step callback): same steps, same
+    # coefficients, same evaluation, bit for bit.
+    print(f"[gpu] c_output device log vs host steps mode={mode}", flush=True)
+    rng = np.random.default_rng(9)
+    B = 24
+    ic = np.stack([rng.uniform(-1.0, 1.0, B), rng.uniform(-1.0, 1.0, B)])
+    tf = rng.uniform(40.0, 80.0, B)
+    ta = hy.taylor_adaptive_batch(PEND, ic, gpu_mode=mode)
+    c1, _ = ta.propagate_until(tf, c_output=True)
+    tb = hy.taylor_adaptive_batch(PEND, ic, gpu_mode=mode)
+    c2, _ = tb.propagate_until(tf, c_output=True, callback=lambda t: True)
+    assert c1.n_steps > 64  # several drains per instance
+    np.testing.assert_array_equal(ta.state, tb.state)
+    assert ta.propagate_res == tb.propagate_res
+    np.testing.assert_array_equal(c1.tcs, c2.tcs)
+    np.testing.assert_array_equal(c1.times, c2.times)
+    tq = np.stack([rng.uniform(0.0, tf) for _ in range(7)])
+    np.testing.assert_array_equal(c1(tq), c2(tq))
+    # Step counts match propagate_res per element.
+    ns = [r[3] for r in ta.propagate_res]
+    assert max(ns) == c1.n_steps
+
+
+def test_large_batch_c_output_vs_grid():
+    print("[gpu] c_output large batch", flush=True)
+    B = 8192
+    rng = np.random.default_rng(4)
+    ic = np.stack([rng.uniform(-1.0, 1.0, B), rng.uniform(-1.0, 1.0, B)])
+    ta = hy.taylor_adaptive_batch(PEND, ic)
+    c_out, _ = ta.propagate_until(np.full(B, 20.0), c_output=True)
+    tb = hy.taylor_adaptive_batch(PEND, ic)
+    grid = np.repeat(np.linspace(0.0, 20.0, 9), B).reshape(9, B)
+    _, out = tb.propagate_grid(grid)
+    np.testing.assert_allclose(c_out(grid), out, rtol=0, atol=1e-13)
