@@ -197,6 +197,9 @@ struct Gen {
     const Sys *E = nullptr;
     uint32_t nev = 0;
     const Gen *base = nullptr;
+    // Group form events: the state variables' coefficients are read from the LDS rows
+    // fl[lds_xb + k * lds_nx + i] (lds_xb < 0: not used).
+    int lds_xb = -1, lds_nx = 0;
 
     Gen(const Sys &s, int p, bool ha) : S(s), P(p), high_acc(ha), perlane(s.nconsts, 0) {}
 
@@ -207,6 +210,7 @@ struct Gen {
     // Coefficient k of node idx: a register, or for a state variable in LDS an older row.
     std::string nv(uint32_t idx, int k) const
     {
+        if (lds_xb >= 0 && idx < S.n) return "fl[" + std::to_string(lds_xb + k * lds_nx + (int)idx) + "]";
         if (base && idx < base->S.n) return base->nv(idx, k);
         if (lds_state && idx < S.n && k != cur_k) return xs(idx, k);
         return cv(idx, k);
@@ -1272,7 +1276,7 @@ bool plan_groups(const Sys &S, GroupPlan &gp, std::string &why)
 }
 
 std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, bool high_acc, int split_min,
-                                   bool pipeline)
+                                   bool pipeline, const Sys *E, uint32_t nev)
 {
     const int G = gp.G, own = gp.own, nin = gp.nin, nout = gp.nout;
     const int n = (int)S.n, nx = n + 1;
@@ -1426,7 +1430,8 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
     o << "template <bool WTC, class TAB>\n"
          "__device__ __forceinline__ void hyx_gstep(double (&xo)[HYX_OWN], const double (&pv)[HYX_NPAR], double t0, "
          "double max_dt, double &h_out, bool &nf_out, int g, double *fl, const TAB &tab, double *__restrict__ tc, "
-         "hyx_u64 tcs)\n{\n";
+         "hyx_u64 tcs"
+      << (E ? ", unsigned long long &hyx_evm, double &hyx_mx" : "") << ")\n{\n";
     o << "    (void)pv; (void)t0; (void)tc; (void)tcs; (void)g;\n";
     // Column of each owned variable in the x rows (column n reads zero for empty slots).
     for (int oo = 0; oo < own; ++oo)
@@ -1549,11 +1554,31 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
           << " + k) * tcs] = fl[HYX_XB + k * HYX_NX + " << vcol(oo) << "];\n";
     }
     o << "    }\n";
+    if (E) {
+        // Event functions: every lane of the group forms the whole jet from the state
+        // rows in LDS (broadcast reads), so the group agrees without an exchange.
+        Gen eg(*E, P, high_acc);
+        eg.lds_xb = xb;
+        eg.lds_nx = nx;
+        o << "    double hyx_evg[" << nev << "][" << (P + 1) << "];\n    {\n";
+        for (uint32_t j = E->n; j < E->nnodes; ++j) eg.order0(j);
+        for (int k = 1; k <= P; ++k)
+            for (uint32_t j = E->n; j < E->nnodes; ++j) eg.orderk(j, k);
+        for (uint32_t e = 0; e < nev; ++e)
+            for (int k = 0; k <= P; ++k)
+                eg.o << "    hyx_evg[" << e << "][" << k << "] = " << (eg.zero(E->rhs[e], k) ? "0.0" : eg.val(E->rhs[e], k))
+                     << ";\n";
+        o << eg.o.str() << "    }\n";
+    }
     o << "    double mx = 0.0, mo = 0.0, mom1 = 0.0, ab;\n";
     for (int oo = 0; oo < own; ++oo) {
         o << "    ab = fabs(" << xv(oo, 0) << "); mx = (ab > mx) ? ab : mx;\n";
         o << "    ab = fabs(" << xv(oo, P) << "); mo = (ab > mo) ? ab : mo;\n";
         o << "    ab = fabs(" << xv(oo, P - 1) << "); mom1 = (ab > mom1) ? ab : mom1;\n";
+    }
+    for (uint32_t e = 0; E && e < nev; ++e) {
+        o << "    ab = fabs(hyx_evg[" << e << "][" << P << "]); mo = (ab > mo) ? ab : mo;\n";
+        o << "    ab = fabs(hyx_evg[" << e << "][" << (P - 1) << "]); mom1 = (ab > mom1) ? ab : mom1;\n";
     }
     o << "#pragma unroll\n    for (int off = 1; off < HYX_G; off <<= 1) {\n"
          "        ab = __shfl_xor(mx, off); mx = (ab > mx) ? ab : mx;\n"
@@ -1567,6 +1592,35 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
     const double rhofac = std::exp(-7.0 / 10.0 / (double)(P - 1)) / (e1 * e1);
     o << "    double h = rho_m * " << lit(rhofac) << ";\n";
     o << "    const double lim = fabs(max_dt);\n    h = (h < lim) ? h : lim;\n    if (max_dt < 0.0) h = -h;\n";
+    if (E) {
+        // The same fast exclusion as the lane forms; the coefficients of the steps it
+        // cannot clear go to tc (event rows by every lane, identical values; state rows
+        // by their owners).
+        const uint64_t off = (uint64_t)n * (P + 1) + (uint64_t)(E->nnodes - E->n) * (P + 1);
+        o << "    hyx_evm = 0ull;\n";
+        o << "#pragma unroll\n    for (int e = 0; e < " << nev << "; ++e) {\n"
+             "        double hk = 1.0, emag = 0.0, lo = 0.0, hi = 0.0;\n"
+             "#pragma unroll\n        for (int k = 0; k <= " << P << "; ++k) {\n"
+             "            const double c = hyx_evg[e][k] * hk;\n            hk *= h;\n            emag += fabs(c);\n"
+             "            if (k > 0) { lo += fmin(c, 0.0); hi += fmax(c, 0.0); }\n        }\n"
+             "        const double slack = " << lit(4.0 * (P + 1) * 1.1102230246251565e-16) << " * emag;\n"
+             "        const double c0 = hyx_evg[e][0];\n"
+             "        if (emag > 0.0 && isfinite(emag) && !(c0 + lo > slack || c0 + hi < -slack)) hyx_evm |= 1ull << e;\n"
+             "    }\n";
+        o << "    if (hyx_evm != 0ull && tc) {\n";
+        for (uint32_t e = 0; e < nev; ++e)
+            for (int k = 0; k <= P; ++k)
+                o << "        tc[(hyx_u64)" << (off + (uint64_t)e * (P + 1) + k) << " * tcs] = hyx_evg[" << e << "][" << k
+                  << "];\n";
+        o << "        if constexpr (!WTC) {\n";
+        for (int oo = 0; oo < own; ++oo) {
+            o << "            if (tab.vo[" << oo << "] >= 0)\n";
+            o << "                for (int k = 0; k <= HYX_P; ++k) tc[((hyx_u64)tab.vo[" << oo << "] * " << (P + 1)
+              << " + k) * tcs] = fl[HYX_XB + k * HYX_NX + " << vcol(oo) << "];\n";
+        }
+        o << "        }\n    }\n";
+        o << "    hyx_mx = mx;\n";
+    }
     o << "    int nf = 0;\n";
     // State update from the coefficient rows in LDS.
     for (int oo = 0; oo < own; ++oo) {

@@ -58,8 +58,10 @@ struct GroupPlan {
 
 bool plan_groups(const Sys &S, GroupPlan &gp, std::string &why);
 // pipeline: one LDS exchange per order when every cluster input is a lagged state variable.
+// With events (E, nev) the step takes `unsigned long long &hyx_evm, double &hyx_mx`
+// (per-event root-scan flags, and the group's max |x| at the start of the step).
 std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int order, bool high_acc, int split_min = 0,
-                                   bool pipeline = true);
+                                   bool pipeline = true, const Sys *E = nullptr, uint32_t nev = 0);
 
 // Compiled function: the order-0 values of the first n_out right-hand sides (hyx_cfunc_eval).
 std::string generate_cfunc_prelude(const Sys &S, uint32_t n_out);

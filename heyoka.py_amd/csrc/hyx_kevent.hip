@@ -201,30 +201,27 @@ __device__ __forceinline__ void hyx_ev_scan(const double (&g)[HYX_P + 1], double
     }
 }
 
-// Event processing after hyx_taylor_step<true> took a step of size h from
-// (thi, tlo): tc holds the step's state coefficients (column i, stride ld) and x
-// the state advanced by h. Returns the step actually taken: h, or the offset of
-// the earliest terminal trigger (x is then re-evaluated there and nf updated).
-// te / te_dsgn: that event (-1: none) and its direction; ntc_out: the non-terminal
-// triggers found, whose times and codes go to a.nt_time / a.nt_code (the first
-// a.nt_cap of them). The sub-lanes of an instance hold identical values and write
-// identical records.
-__device__ __forceinline__ double hyx_ev_step(const hyx_kargs &a, hyx_u64 i, const double *pv, double thi,
-                                              double tlo, double h, double (&x)[HYX_N], bool &nf, int &te,
-                                              int &te_dsgn, unsigned &ntc_out, unsigned long long evm)
+// Event detection after a step of size h from (thi, tlo). Bit e of evm is set for
+// the events whose fast exclusion test the step could not clear; their
+// coefficients are in tc (column i). Records the triggers, updates the cooldowns
+// and returns the step actually taken: h, or the offset hend of the earliest
+// terminal trigger (te / te_dsgn: that event, -1 if none). mx: max |x| at the start
+// of the step (the automatic cooldown); ntc_out: the non-terminal triggers found,
+// whose times and codes go to a.nt_time / a.nt_code (the first a.nt_cap of them).
+// Every lane holding the instance computes the same values and writes the same
+// records. The caller re-evaluates the state at hend when te >= 0.
+__device__ __forceinline__ double hyx_ev_detect(const hyx_kargs &a, hyx_u64 i, double thi, double tlo, double h,
+                                                unsigned long long evm, double mx, int &te, int &te_dsgn,
+                                                unsigned &ntc_out)
 {
-    // The step formed the event functions' coefficients and cleared, per event, the
-    // steps its fast exclusion test rules out (bit e of evm is 0); for the others the
-    // coefficients (and the state's) are in tc.
-    (void)pv;
-    double *tc = a.tc + i;
+    const double *tc = a.tc + i;
     const hyx_u64 ld = a.ld;
     te = -1;
     te_dsgn = 0;
     unsigned ntc = 0;
     double hend = h;
     double g[HYX_P + 1];
-    if (h != 0.0) {
+    if (h != 0.0 && evm != 0ull) {
         double best = fabs(h);
         for (int e = 0; e < HYX_NTE; ++e) {
             if (!((evm >> e) & 1ull)) continue;
@@ -267,31 +264,6 @@ __device__ __forceinline__ double hyx_ev_step(const hyx_kargs &a, hyx_u64 i, con
     }
     double cd = 0.0;
     if (te >= 0) {
-        // The state at the trigger, from the step's coefficients.
-        bool bad = false;
-        double mx = 0.0;
-#pragma unroll 1
-        for (int q = 0; q < HYX_N; ++q) {
-            const double *cq = tc + (hyx_u64)q * (HYX_P + 1) * ld;
-            const double a0 = fabs(cq[0]);
-            mx = (a0 > mx) ? a0 : mx;
-#ifdef HYX_HIGH_ACC
-            double r = cq[0], cp = 0.0, hk = 1.0;
-            for (int k = 1; k <= HYX_P; ++k) {
-                hk *= hend;
-                const double y = cq[(hyx_u64)k * ld] * hk - cp;
-                const double tt = r + y;
-                cp = (tt - r) - y;
-                r = tt;
-            }
-#else
-            double r = cq[(hyx_u64)HYX_P * ld];
-            for (int k = HYX_P - 1; k >= 0; --k) r = __builtin_fma(r, hend, cq[(hyx_u64)k * ld]);
-#endif
-            x[q] = r;
-            bad = bad || !isfinite(r);
-        }
-        nf = bad;
 #pragma unroll
         for (int k = 0; k <= HYX_P; ++k) g[k] = hyx_ev_coeff(te, k, tc, ld);
         cd = hyx_ev_cd[te];
@@ -317,5 +289,51 @@ __device__ __forceinline__ double hyx_ev_step(const hyx_kargs &a, hyx_u64 i, con
     a.te[i] = (te >= 0) ? (hyx_i64)(4 * te + te_dsgn + 1) : -1;
     a.ntc[i] = ntc;
     ntc_out = ntc;
+    return hend;
+}
+
+// Lane forms: the step left the coefficients of the steps that need a root scan in
+// tc (state rows included), so max |x| and the state at a terminal trigger come
+// from there.
+__device__ __forceinline__ double hyx_ev_step(const hyx_kargs &a, hyx_u64 i, const double *pv, double thi,
+                                              double tlo, double h, double (&x)[HYX_N], bool &nf, int &te,
+                                              int &te_dsgn, unsigned &ntc_out, unsigned long long evm)
+{
+    (void)pv;
+    const double *tc = a.tc + i;
+    const hyx_u64 ld = a.ld;
+    double mx = 0.0;
+    if (evm != 0ull) {
+#pragma unroll 1
+        for (int q = 0; q < HYX_N; ++q) {
+            const double a0 = fabs(tc[(hyx_u64)q * (HYX_P + 1) * ld]);
+            mx = (a0 > mx) ? a0 : mx;
+        }
+    }
+    const double hend = hyx_ev_detect(a, i, thi, tlo, h, evm, mx, te, te_dsgn, ntc_out);
+    if (te >= 0) {
+        // The state at the trigger, from the step's coefficients.
+        bool bad = false;
+#pragma unroll 1
+        for (int q = 0; q < HYX_N; ++q) {
+            const double *cq = tc + (hyx_u64)q * (HYX_P + 1) * ld;
+#ifdef HYX_HIGH_ACC
+            double r = cq[0], cp = 0.0, hk = 1.0;
+            for (int k = 1; k <= HYX_P; ++k) {
+                hk *= hend;
+                const double y = cq[(hyx_u64)k * ld] * hk - cp;
+                const double tt = r + y;
+                cp = (tt - r) - y;
+                r = tt;
+            }
+#else
+            double r = cq[(hyx_u64)HYX_P * ld];
+            for (int k = HYX_P - 1; k >= 0; --k) r = __builtin_fma(r, hend, cq[(hyx_u64)k * ld]);
+#endif
+            x[q] = r;
+            bad = bad || !isfinite(r);
+        }
+        nf = bad;
+    }
     return hend;
 }

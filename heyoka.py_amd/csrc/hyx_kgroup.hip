@@ -24,15 +24,60 @@
 // kernels also evaluate the step's Taylor polynomials at the grid times inside
 // the step; each lane evaluates its owned state variables from the
 // coefficient rows in LDS, which stay intact until the next hyx_gstep.
+//
+// With events (HYX_EVENTS), hyx_gstep also forms the event functions' jet in every
+// lane of the group from the LDS rows and reports which events need a root scan;
+// the group then runs the detection of hyx_kevent.hip (every lane the same) and,
+// at a terminal trigger, re-evaluates its owned variables from the LDS rows.
 
 struct hyx_ginst {
     double xo[HYX_OWN];
     double pv[HYX_NPAR];
     double thi, tlo, tfhi, tflo, mdt, minh, maxh;
     hyx_u64 iter, nst;
+    double hlast; // the last step's size (kept for the host when the instance parks)
     int tdir;
     unsigned gc; // propagate_grid: next grid point
 };
+
+#ifdef HYX_EVENTS
+// Event detection for the group's instance i after the step of size h from
+// (thi, tlo): shortens h to a terminal trigger and re-evaluates the owned
+// variables there from the LDS rows. Every lane of the group computes the same.
+__device__ __forceinline__ double hyx_gevents(hyx_ginst &s, const hyx_kargs &a, hyx_u64 i, const double *fl,
+                                              const hyx_gtab &t, double thi, double tlo, double h,
+                                              unsigned long long evm, double mx, bool &nf, int &te, unsigned &ntc)
+{
+    int te_dsgn;
+    const double hend = hyx_ev_detect(a, i, thi, tlo, h, evm, mx, te, te_dsgn, ntc);
+    if (te < 0) return h;
+    int bad = 0;
+#pragma unroll
+    for (int o = 0; o < HYX_OWN; ++o) {
+        if (t.vo[o] < 0) continue;
+        const double *row = fl + HYX_XB + t.vo[o];
+#ifdef HYX_HIGH_ACC
+        double r = row[0], cp = 0.0, hk = 1.0;
+        for (int k = 1; k <= HYX_P; ++k) {
+            hk *= hend;
+            const double y = row[k * HYX_NX] * hk - cp;
+            const double tt = r + y;
+            cp = (tt - r) - y;
+            r = tt;
+        }
+#else
+        double r = row[HYX_P * HYX_NX];
+        for (int k = HYX_P - 1; k >= 0; --k) r = __builtin_fma(r, hend, row[k * HYX_NX]);
+#endif
+        s.xo[o] = r;
+        bad |= isfinite(r) ? 0 : 1;
+    }
+#pragma unroll
+    for (int off = 1; off < HYX_G; off <<= 1) bad |= __shfl_xor(bad, off);
+    nf = bad != 0;
+    return hend;
+}
+#endif
 
 __device__ __forceinline__ void hyx_gload(hyx_ginst &s, const hyx_kargs &a, hyx_u64 i, int g, const hyx_gtab &t)
 {
@@ -86,8 +131,11 @@ __device__ __forceinline__ void hyx_gsuspend(const hyx_ginst &s, const hyx_kargs
 #ifdef HYX_GRID
         a.gcur[i] = s.gc;
 #endif
-        if (done) {
-            a.status[i] = 1u;
+        if (done && res == HYX_PARKED) {
+            a.status[i] = HYX_ST_PARKED | (s.tdir ? 0u : 2u);
+            a.h_out[i] = s.hlast;
+        } else if (done) {
+            a.status[i] = HYX_ST_DONE;
             a.outcome[i] = res;
         }
     }
@@ -133,12 +181,26 @@ __device__ __forceinline__ bool hyx_gprop_iter(hyx_ginst &s, const hyx_kargs &a,
     const double lim = s.tdir ? ((s.mdt < rem) ? s.mdt : rem) : ((-s.mdt > rem) ? -s.mdt : rem);
     double h;
     bool nf;
+    int te = -1;
+    bool park = false;
+#ifdef HYX_EVENTS
+    unsigned long long evm = 0;
+    double gmx = 0.0;
+    hyx_gstep<false>(s.xo, s.pv, s.thi, lim, h, nf, g, fl, t, wr ? a.tc + i : nullptr, a.ld, evm, gmx);
+    if (wr) {
+        unsigned ntc = 0;
+        h = hyx_gevents(s, a, i, fl, t, s.thi, s.tlo, h, evm, gmx, nf, te, ntc);
+        park = a.ev_pause && ((te >= 0 && !hyx_ev_stop[te]) || ntc > 0u);
+    }
+#else
     hyx_gstep<false>(s.xo, s.pv, s.thi, lim, h, nf, g, fl, t, nullptr, 0);
+#endif
     hyx_dfl_add(s.thi, s.tlo, h, 0.0);
+    s.hlast = h;
     nf = nf || !isfinite(s.thi);
     s.iter += 1;
     s.nst += (h != 0.0) ? 1 : 0;
-    const hyx_i64 oc = nf ? HYX_ERR_NF_STATE : ((h == lim) ? HYX_TIME_LIMIT : HYX_SUCCESS);
+    const hyx_i64 oc = nf ? HYX_ERR_NF_STATE : ((te < 0 && h == lim) ? HYX_TIME_LIMIT : HYX_SUCCESS);
     if (oc == HYX_SUCCESS) {
         const double ah = fabs(h);
         s.minh = (ah < s.minh) ? ah : s.minh;
@@ -152,6 +214,14 @@ __device__ __forceinline__ bool hyx_gprop_iter(hyx_ginst &s, const hyx_kargs &a,
 #ifdef HYX_GRID
     if (wr) hyx_ggrid(s, a, i, fl, t, h, fin);
 #endif
+    if (park) {
+        *res = HYX_PARKED;
+        return true;
+    }
+    if (te >= 0) {
+        *res = -(hyx_i64)te - 1;
+        return true;
+    }
     if (fin) {
         *res = HYX_TIME_LIMIT;
         return true;
@@ -187,14 +257,26 @@ __device__ __forceinline__ void hyx_gstep_body(const hyx_kargs &a)
     const double mdt = a.max_dt[ii];
     double h;
     bool nf;
+    int te = -1;
+#ifdef HYX_EVENTS
+    unsigned long long evm = 0;
+    double gmx = 0.0;
+    hyx_gstep<WTC>(s.xo, s.pv, s.thi, mdt, h, nf, g, fl, tab, valid ? a.tc + i : nullptr, a.ld, evm, gmx);
+    if (valid) {
+        unsigned ntc = 0;
+        h = hyx_gevents(s, a, i, fl, tab, s.thi, s.tlo, h, evm, gmx, nf, te, ntc);
+    }
+#else
     hyx_gstep<WTC>(s.xo, s.pv, s.thi, mdt, h, nf, g, fl, tab, (WTC && valid) ? a.tc + i : nullptr, a.ld);
+#endif
     hyx_dfl_add(s.thi, s.tlo, h, 0.0);
     nf = nf || !isfinite(s.thi);
     if (valid) {
         hyx_gstore_state(s, a, i, g, tab);
         if (g == 0) {
             a.h_out[i] = h;
-            a.outcome[i] = nf ? HYX_ERR_NF_STATE : ((h == mdt) ? HYX_TIME_LIMIT : HYX_SUCCESS);
+            a.outcome[i] = nf ? HYX_ERR_NF_STATE
+                              : (te >= 0 ? -(hyx_i64)te - 1 : ((h == mdt) ? HYX_TIME_LIMIT : HYX_SUCCESS));
         }
     }
 }
@@ -222,7 +304,7 @@ extern "C" __global__ void __launch_bounds__(64) hyx_iter_k(hyx_kargs a)
     HYX_GROUP_SETUP
     const hyx_u64 i = (hyx_u64)blockIdx.x * HYX_I + grp;
     const unsigned st = (i < a.N) ? a.status[i] : 1u;
-    const bool valid = st != 1u;
+    const bool valid = hyx_runnable(st);
     const hyx_u64 ii = (i < a.N) ? i : 0;
     hyx_ginst s;
     hyx_gresume(s, a, ii, st, g, tab);
@@ -265,6 +347,7 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
     s.maxh = 0.0;
     s.iter = 0;
     s.nst = 0;
+    s.hlast = 0.0;
     s.gc = 0;
     hyx_u64 inst = 0, budget = 0;
     bool active = false;
@@ -286,7 +369,7 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
                 if (gpos < a.N) {
                     inst = a.queue ? (hyx_u64)a.queue[gpos] : gpos;
                     const unsigned st = a.status[inst];
-                    if (st != 1u) {
+                    if (hyx_runnable(st)) {
                         hyx_gresume(s, a, inst, st, g, tab);
                         budget = 0;
                         active = true;

@@ -325,7 +325,6 @@ int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint3
             s.ev.stopping.push_back(ev->stopping ? (ev->stopping[e] != 0) : 1);
         }
         if (!(ev->tol > 0.0)) return fail(HYX_EINVAL, "the event specification needs the tolerance");
-        if (flags & HYX_FLAG_FORCE_GROUP) return fail(HYX_EINVAL, "the group form does not support events");
         s.has_events = true;
         s.ev_rows = (uint64_t)(s.E.nnodes - s.E.n) * (order + 1);
     }
@@ -341,7 +340,7 @@ int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint3
         else if (flags & HYX_FLAG_FORCE_GROUP) {
             if (!hyx::plan_groups(s.S, gp, why)) return fail(HYX_EINVAL, "the group form does not apply: " + why);
             s.form = HYX_FORM_GROUP;
-        } else if (lane_compact && !s.has_events && hyx::plan_groups(s.S, gp, why))
+        } else if (lane_compact && hyx::plan_groups(s.S, gp, why))
             s.form = HYX_FORM_GROUP;
         else
             s.form = lane_compact ? HYX_FORM_COMPACT : HYX_FORM_UNROLLED;
@@ -349,7 +348,17 @@ int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint3
         const int split = (flags & HYX_FLAG_SPLIT_CHAINS) ? 8 : 0;
         if (s.form == HYX_FORM_GROUP) {
             s.inst_per_block = 64u / (uint32_t)gp.G;
-            s.src = hyx::generate_group_prelude(s.S, gp, (int)order, ha, split, !(flags & HYX_FLAG_NO_PIPELINE)) + "\n" + k_hyx_kcommon + "\n" + k_hyx_kgroup;
+            if (s.has_events && s.ev.dir.size() > 64) return fail(HYX_EINVAL, "at most 64 events are supported");
+            s.src = (s.has_events ? hyx::events_forward_decl() : std::string()) +
+                    hyx::generate_group_prelude(s.S, gp, (int)order, ha, split, !(flags & HYX_FLAG_NO_PIPELINE),
+                                                s.has_events ? &s.E : nullptr, (uint32_t)s.ev.dir.size()) +
+                    "\n";
+            if (s.has_events)
+                s.src += hyx::generate_events_prelude(s.E, s.S.n, (int)order, s.ev, ha) + "\n" + k_hyx_kcommon + "\n" +
+                         k_hyx_kevent + "\n";
+            else
+                s.src += std::string(k_hyx_kcommon) + "\n";
+            s.src += k_hyx_kgroup;
         } else {
             // Compact form: lanes per instance (default 2: measured best on the J2 +
             // variational ensemble, 1.5x over one lane), each summing every sub-th

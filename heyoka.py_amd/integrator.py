@@ -222,8 +222,6 @@ class taylor_adaptive_batch:
         tes, ntes = events
         spec = None
         if tes or ntes:
-            if compact == "group":
-                raise ValueError("The group code form does not support events: use the unrolled or compact form")
             spec = _event_spec(dec, tes, ntes, tol)
             if spec.dec.n_par > dec.n_par:
                 dec.n_par = spec.dec.n_par

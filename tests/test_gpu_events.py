@@ -408,3 +408,50 @@ def test_ensembles_with_events():
     ta = hy.taylor_adaptive_batch(PEND, ic, nt_events=[hy.nt_event_batch(v, cb())])
     ret = hy.ensemble_propagate_until_batch(ta, 5.0, 3, gen)
     assert len(seen) == 3 and all(id(m) in seen for m, _, _ in ret)
+
+
+def test_group_form_events_vs_compact_and_oracle():
+    # The 6-body ensemble keeps the group form (16 lanes per instance) with events: the
+    # event jet comes from the state rows in LDS, the group detects together and
+    # re-evaluates its owned variables at a terminal trigger.
+    print("[gpu] group form events", flush=True)
+    from heyoka_amd import workloads as W
+    rng = np.random.default_rng(21)
+    B = 64
+    wl = W.nbody6(B)
+    nm = hy.taylor_decompose(wl.sys).state_names
+    x0, y0, x1, y1 = (hy.expression(nm[i]) for i in (0, 1, 6, 7))
+    pars = rng.uniform(-0.9, 0.9, (1, B))
+    ev = x1 - x0 - hy.par[0]
+    runs = {}
+    for form in ("group", "compact"):
+        ta = hy.taylor_adaptive_batch(wl.sys, wl.state, pars=pars, tol=1e-15, code_form=form,
+                                      t_events=[hy.t_event_batch(ev, direction=hy.event_direction.positive)])
+        assert ta.code_form == form
+        ta.propagate_until(np.full(B, 5.0))
+        runs[form] = ta
+    g, c = runs["group"], runs["compact"]
+    og = [int(r[0]) for r in g.propagate_res]
+    assert og == [int(r[0]) for r in c.propagate_res]
+    assert og.count(-1) > B // 4 and og.count(int(to.time_limit)) > 0
+    np.testing.assert_allclose(g.time, c.time, rtol=1e-12)
+    scale = np.max(np.abs(c.state), axis=1, keepdims=True)
+    assert np.max(np.abs(g.state - c.state) / scale) < 1e-11
+    stopped = np.array(og) == -1
+    np.testing.assert_allclose((g.state[6] - g.state[0])[stopped], pars[0, stopped], atol=1e-13)
+    ref = OE.propagate(hy, wl.sys, wl.state[:, :4], 5.0, pars=pars[:, :4], tol=1e-15,
+                       t_events=[(ev, 1, -1.0, True)])
+    assert og[:4] == list(ref["outcome"])
+    np.testing.assert_allclose(g.time[:4], ref["t_hi"], rtol=1e-12)
+    assert [r[3] for r in g.propagate_res[:4]] == list(ref["nsteps"])
+    # Non-terminal events with a Python callback: the parked path in the group form.
+    logs = {}
+    for form in ("group", "compact"):
+        log = []
+        ta = hy.taylor_adaptive_batch(wl.sys, wl.state, pars=pars, tol=1e-15, code_form=form,
+                                      nt_events=[hy.nt_event_batch(y1 - y0, lambda t_, t, d, b: log.append((b, t, d)))])
+        ta.propagate_until(np.full(B, 5.0))
+        logs[form] = sorted(log)
+    assert len(logs["group"]) == len(logs["compact"]) > 0
+    for (bg, tg_, dg), (bc, tc_, dc) in zip(logs["group"], logs["compact"]):
+        assert bg == bc and dg == dc and abs(tg_ - tc_) <= 1e-12 * max(1.0, abs(tc_))
