@@ -352,6 +352,32 @@ def events_bench(args):
         print(f"[bench] events {label}: {ns / wall:.3e} instance-steps/s, {crossings[0]} crossings, "
               f"{stopped} stopped, {wall:.2f} s", file=sys.stderr, flush=True)
     base = results["A_no_events"]["instance_steps_per_s"]
+    # 6-body ensemble, group form: the same comparison for a terminal close-approach
+    # event between bodies 1 and 2 (|r12| = 0.1, never reached: the per-step cost) and
+    # for a stopping event that does trigger (body 1 drifting past x = x0 - 2).
+    wl6 = W.nbody6(N)
+    nm = hy.taylor_decompose(wl6.sys).state_names
+    p = [hy.expression(v) for v in nm]
+    r12 = (p[6] - p[12]) ** 2 + (p[7] - p[13]) ** 2 + (p[8] - p[14]) ** 2
+    nb = {}
+    for label, tes in (("A_no_events", []), ("B0_close_approach_never", [hy.t_event_batch(r12 - 0.01)]),
+                       ("B_crossing_stops", [hy.t_event_batch(p[6] - p[0] + 2.0,
+                                                              direction=hy.event_direction.negative)])):
+        ta = hy.taylor_adaptive_batch(wl6.sys, wl6.state, tol=1e-15, t_events=tes)
+        ta.propagate_until(np.full(N, 1.0))
+        ta.set_time(np.zeros(N))
+        ta.state[:] = wl6.state
+        ta.reset_cooldowns()
+        t0 = time.perf_counter()
+        ta.propagate_until(np.full(N, 10.0))
+        wall = time.perf_counter() - t0
+        ns = int(sum(r[3] for r in ta.propagate_res))
+        ms, launches = ta.last_kernel_timing()
+        nb[label] = {"instance_steps_per_s": ns / wall, "instance_steps": ns, "wall_s": wall, "kernel_ms": ms,
+                     "code_form": ta.code_form,
+                     "stopped": int(sum(1 for r in ta.propagate_res if int(r[0]) == -1))}
+        print(f"[bench] events nbody6 {label}: {ns / wall:.3e} instance-steps/s, kernel {ms:.1f} ms, "
+              f"{nb[label]['stopped']} stopped ({ta.code_form})", file=sys.stderr, flush=True)
     out = {
         "metric": METRIC,
         "value": results["C_terminal_and_section_parked"]["instance_steps_per_s"],
@@ -370,6 +396,9 @@ def events_bench(args):
                                "x = 1/2 - mu (direction positive)"},
         "events": results,
         "relative_to_no_events": {k: v["instance_steps_per_s"] / base for k, v in results.items()},
+        "nbody6_group_events": nb,
+        "nbody6_kernel_ms_relative_to_no_events": {k: v["kernel_ms"] / nb["A_no_events"]["kernel_ms"]
+                                                   for k, v in nb.items()},
     }
     print(json.dumps(out), flush=True)
 
