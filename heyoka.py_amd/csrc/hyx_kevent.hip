@@ -201,6 +201,82 @@ __device__ __forceinline__ void hyx_ev_scan(const double (&g)[HYX_P + 1], double
     }
 }
 
+// What the root scan of one step found: the step actually taken (h, or the offset of
+// the earliest accepted terminal trigger), that event (-1: none) with its direction
+// and cooldown, and the number of non-terminal triggers.
+struct hyx_ev_hit {
+    double hend, cd;
+    int te, te_dsgn;
+    unsigned ntc;
+};
+
+// The root scan of the events whose bit is set in evm (only on steps the fast
+// exclusion test cannot clear). tc, evcd, nt_time, nt_code: the instance's columns
+// (stride ld). Measured: taking it out of line does not lower the per-step cost
+// and makes trigger steps slower, so it stays inline.
+__device__ __forceinline__ hyx_ev_hit hyx_ev_scan_all(const double *tc, const double *evcd, double *nt_time,
+                                                              hyx_i64 *nt_code, unsigned nt_cap, hyx_u64 ld,
+                                                              double thi, double tlo, double h,
+                                                              unsigned long long evm, double mx)
+{
+    hyx_ev_hit r;
+    r.hend = h;
+    r.cd = 0.0;
+    r.te = -1;
+    r.te_dsgn = 0;
+    r.ntc = 0;
+    double g[HYX_P + 1];
+    double best = fabs(h);
+    for (int e = 0; e < HYX_NTE; ++e) {
+        if (!((evm >> e) & 1ull)) continue;
+#pragma unroll
+        for (int k = 0; k <= HYX_P; ++k) g[k] = hyx_ev_coeff(e, k, tc, ld);
+        const double cdl = evcd[((hyx_u64)e * 3 + 2) * ld];
+        const double dt0 =
+            (cdl >= 0.0) ? hyx_dfl_sub_hi(thi, tlo, evcd[((hyx_u64)e * 3) * ld], evcd[((hyx_u64)e * 3 + 1) * ld]) : 0.0;
+        const int dir = hyx_ev_dir[e];
+        hyx_ev_scan(g, h, [&](double tau, int ds) {
+            if (fabs(tau) >= best) return true;
+            if (dir != 0 && ds != dir) return false;
+            if (cdl >= 0.0 && fabs(dt0 + tau) < cdl) return false;
+            best = fabs(tau);
+            r.te = e;
+            r.te_dsgn = ds;
+            r.hend = tau;
+            return true;
+        });
+    }
+    for (int e = HYX_NTE; e < HYX_NEV; ++e) {
+        if (!((evm >> e) & 1ull)) continue;
+#pragma unroll
+        for (int k = 0; k <= HYX_P; ++k) g[k] = hyx_ev_coeff(e, k, tc, ld);
+        const int dir = hyx_ev_dir[e];
+        hyx_ev_scan(g, h, [&](double tau, int ds) {
+            if (fabs(tau) >= best) return true;
+            if (dir != 0 && ds != dir) return false;
+            if (r.ntc < nt_cap) {
+                double th = thi, tl = tlo;
+                hyx_dfl_add(th, tl, tau, 0.0);
+                nt_time[(hyx_u64)r.ntc * ld] = th;
+                nt_code[(hyx_u64)r.ntc * ld] = 4 * (e - HYX_NTE) + ds + 1;
+            }
+            ++r.ntc;
+            return false;
+        });
+    }
+    if (r.te >= 0) {
+#pragma unroll
+        for (int k = 0; k <= HYX_P; ++k) g[k] = hyx_ev_coeff(r.te, k, tc, ld);
+        double cd = hyx_ev_cd[r.te];
+        if (cd < 0.0) {
+            cd = 10.0 * (HYX_TOL * mx) / fabs(hyx_ev_deval(g, r.hend));
+            if (!isfinite(cd)) cd = 0.0;
+        }
+        r.cd = cd;
+    }
+    return r;
+}
+
 // Event detection after a step of size h from (thi, tlo). Bit e of evm is set for
 // the events whose fast exclusion test the step could not clear; their
 // coefficients are in tc (column i). Records the triggers, updates the cooldowns
@@ -210,86 +286,63 @@ __device__ __forceinline__ void hyx_ev_scan(const double (&g)[HYX_P + 1], double
 // whose times and codes go to a.nt_time / a.nt_code (the first a.nt_cap of them).
 // Every lane holding the instance computes the same values and writes the same
 // records. The caller re-evaluates the state at hend when te >= 0.
+// Terminal events whose cooldown is running, from the instance's cooldown columns (a
+// bit per event); the instance keeps the mask in registers between steps.
+__device__ __forceinline__ unsigned long long hyx_ev_cdmask(const hyx_kargs &a, hyx_u64 i)
+{
+    unsigned long long m = 0ull;
+    for (int e = 0; e < HYX_NTE; ++e)
+        if (a.evcd[((hyx_u64)e * 3 + 2) * a.ld + i] >= 0.0) m |= 1ull << e;
+    return m;
+}
+
 __device__ __forceinline__ double hyx_ev_detect(const hyx_kargs &a, hyx_u64 i, double thi, double tlo, double h,
                                                 unsigned long long evm, double mx, int &te, int &te_dsgn,
-                                                unsigned &ntc_out)
+                                                unsigned &ntc_out, unsigned long long &cdact, bool always_record)
 {
-    const double *tc = a.tc + i;
     const hyx_u64 ld = a.ld;
-    te = -1;
-    te_dsgn = 0;
-    unsigned ntc = 0;
-    double hend = h;
-    double g[HYX_P + 1];
-    if (h != 0.0 && evm != 0ull) {
-        double best = fabs(h);
+    hyx_ev_hit r;
+    r.hend = h;
+    r.cd = 0.0;
+    r.te = -1;
+    r.te_dsgn = 0;
+    r.ntc = 0;
+    if (h != 0.0 && evm != 0ull)
+        r = hyx_ev_scan_all(a.tc + i, a.evcd + i, a.nt_time + i, a.nt_code + i, a.nt_cap, ld, thi, tlo, h, evm, mx);
+    te = r.te;
+    te_dsgn = r.te_dsgn;
+    // Cooldowns (only those running, per cdact): expire those the step moved past,
+    // then start the triggered one's.
+    if (cdact != 0ull || r.te >= 0) {
+        double nhi = thi, nlo = tlo;
+        hyx_dfl_add(nhi, nlo, r.hend, 0.0);
         for (int e = 0; e < HYX_NTE; ++e) {
-            if (!((evm >> e) & 1ull)) continue;
-#pragma unroll
-            for (int k = 0; k <= HYX_P; ++k) g[k] = hyx_ev_coeff(e, k, tc, ld);
-            const double cdl = a.evcd[((hyx_u64)e * 3 + 2) * ld + i];
-            const double dt0 = (cdl >= 0.0) ? hyx_dfl_sub_hi(thi, tlo, a.evcd[((hyx_u64)e * 3) * ld + i],
-                                                             a.evcd[((hyx_u64)e * 3 + 1) * ld + i])
-                                            : 0.0;
-            const int dir = hyx_ev_dir[e];
-            hyx_ev_scan(g, h, [&](double tau, int ds) {
-                if (fabs(tau) >= best) return true;
-                if (dir != 0 && ds != dir) return false;
-                if (cdl >= 0.0 && fabs(dt0 + tau) < cdl) return false;
-                best = fabs(tau);
-                te = e;
-                te_dsgn = ds;
-                hend = tau;
-                return true;
-            });
+            if (!((cdact >> e) & 1ull)) continue;
+            const hyx_u64 o = (hyx_u64)e * 3 * ld + i;
+            const double cdl = a.evcd[o + 2 * ld];
+            if (!(cdl >= 0.0)) {
+                cdact &= ~(1ull << e);
+            } else if (!(fabs(hyx_dfl_sub_hi(nhi, nlo, a.evcd[o], a.evcd[o + ld])) < cdl)) {
+                a.evcd[o + 2 * ld] = -1.0;
+                cdact &= ~(1ull << e);
+            }
         }
-        for (int e = HYX_NTE; e < HYX_NEV; ++e) {
-            if (!((evm >> e) & 1ull)) continue;
-#pragma unroll
-            for (int k = 0; k <= HYX_P; ++k) g[k] = hyx_ev_coeff(e, k, tc, ld);
-            const int dir = hyx_ev_dir[e];
-            hyx_ev_scan(g, h, [&](double tau, int ds) {
-                if (fabs(tau) >= best) return true;
-                if (dir != 0 && ds != dir) return false;
-                if (ntc < a.nt_cap) {
-                    double th = thi, tl = tlo;
-                    hyx_dfl_add(th, tl, tau, 0.0);
-                    a.nt_time[(hyx_u64)ntc * ld + i] = th;
-                    a.nt_code[(hyx_u64)ntc * ld + i] = 4 * (e - HYX_NTE) + ds + 1;
-                }
-                ++ntc;
-                return false;
-            });
+        if (r.te >= 0) {
+            const hyx_u64 o = (hyx_u64)r.te * 3 * ld + i;
+            a.evcd[o] = nhi;
+            a.evcd[o + ld] = nlo;
+            a.evcd[o + 2 * ld] = r.cd;
+            cdact |= 1ull << r.te;
         }
     }
-    double cd = 0.0;
-    if (te >= 0) {
-#pragma unroll
-        for (int k = 0; k <= HYX_P; ++k) g[k] = hyx_ev_coeff(te, k, tc, ld);
-        cd = hyx_ev_cd[te];
-        if (cd < 0.0) {
-            cd = 10.0 * (HYX_TOL * mx) / fabs(hyx_ev_deval(g, hend));
-            if (!isfinite(cd)) cd = 0.0;
-        }
+    // Nothing can be found on a step the exclusion test cleared: the records are read
+    // after step() (always written) or for parked instances (which had triggers).
+    if (always_record || evm != 0ull) {
+        a.te[i] = (r.te >= 0) ? (hyx_i64)(4 * r.te + r.te_dsgn + 1) : -1;
+        a.ntc[i] = r.ntc;
     }
-    // Cooldowns: expire those the step moved past, then start the triggered one's.
-    double nhi = thi, nlo = tlo;
-    hyx_dfl_add(nhi, nlo, hend, 0.0);
-    for (int e = 0; e < HYX_NTE; ++e) {
-        const hyx_u64 o = (hyx_u64)e * 3 * ld + i;
-        const double cdl = a.evcd[o + 2 * ld];
-        if (cdl >= 0.0 && !(fabs(hyx_dfl_sub_hi(nhi, nlo, a.evcd[o], a.evcd[o + ld])) < cdl)) a.evcd[o + 2 * ld] = -1.0;
-    }
-    if (te >= 0) {
-        const hyx_u64 o = (hyx_u64)te * 3 * ld + i;
-        a.evcd[o] = nhi;
-        a.evcd[o + ld] = nlo;
-        a.evcd[o + 2 * ld] = cd;
-    }
-    a.te[i] = (te >= 0) ? (hyx_i64)(4 * te + te_dsgn + 1) : -1;
-    a.ntc[i] = ntc;
-    ntc_out = ntc;
-    return hend;
+    ntc_out = r.ntc;
+    return r.hend;
 }
 
 // Lane forms: the step left the coefficients of the steps that need a root scan in
@@ -297,7 +350,8 @@ __device__ __forceinline__ double hyx_ev_detect(const hyx_kargs &a, hyx_u64 i, d
 // from there.
 __device__ __forceinline__ double hyx_ev_step(const hyx_kargs &a, hyx_u64 i, const double *pv, double thi,
                                               double tlo, double h, double (&x)[HYX_N], bool &nf, int &te,
-                                              int &te_dsgn, unsigned &ntc_out, unsigned long long evm)
+                                              int &te_dsgn, unsigned &ntc_out, unsigned long long evm,
+                                              unsigned long long &cdact, bool always_record)
 {
     (void)pv;
     const double *tc = a.tc + i;
@@ -310,7 +364,7 @@ __device__ __forceinline__ double hyx_ev_step(const hyx_kargs &a, hyx_u64 i, con
             mx = (a0 > mx) ? a0 : mx;
         }
     }
-    const double hend = hyx_ev_detect(a, i, thi, tlo, h, evm, mx, te, te_dsgn, ntc_out);
+    const double hend = hyx_ev_detect(a, i, thi, tlo, h, evm, mx, te, te_dsgn, ntc_out, cdact, always_record);
     if (te >= 0) {
         // The state at the trigger, from the step's coefficients.
         bool bad = false;

@@ -38,6 +38,9 @@ struct hyx_ginst {
     double hlast; // the last step's size (kept for the host when the instance parks)
     int tdir;
     unsigned gc; // propagate_grid: next grid point
+#ifdef HYX_EVENTS
+    unsigned long long cdact; // terminal events with a running cooldown
+#endif
 };
 
 #ifdef HYX_EVENTS
@@ -46,10 +49,11 @@ struct hyx_ginst {
 // variables there from the LDS rows. Every lane of the group computes the same.
 __device__ __forceinline__ double hyx_gevents(hyx_ginst &s, const hyx_kargs &a, hyx_u64 i, const double *fl,
                                               const hyx_gtab &t, double thi, double tlo, double h,
-                                              unsigned long long evm, double mx, bool &nf, int &te, unsigned &ntc)
+                                              unsigned long long evm, double mx, bool &nf, int &te, unsigned &ntc,
+                                              bool always_record)
 {
     int te_dsgn;
-    const double hend = hyx_ev_detect(a, i, thi, tlo, h, evm, mx, te, te_dsgn, ntc);
+    const double hend = hyx_ev_detect(a, i, thi, tlo, h, evm, mx, te, te_dsgn, ntc, s.cdact, always_record);
     if (te < 0) return h;
     int bad = 0;
 #pragma unroll
@@ -116,6 +120,9 @@ __device__ __forceinline__ void hyx_gresume(hyx_ginst &s, const hyx_kargs &a, hy
     s.nst = a.nsteps[i];
 #ifdef HYX_GRID
     s.gc = a.gcur[i];
+#endif
+#ifdef HYX_EVENTS
+    s.cdact = hyx_ev_cdmask(a, i);
 #endif
 }
 
@@ -189,7 +196,7 @@ __device__ __forceinline__ bool hyx_gprop_iter(hyx_ginst &s, const hyx_kargs &a,
     hyx_gstep<false>(s.xo, s.pv, s.thi, lim, h, nf, g, fl, t, wr ? a.tc + i : nullptr, a.ld, evm, gmx);
     if (wr) {
         unsigned ntc = 0;
-        h = hyx_gevents(s, a, i, fl, t, s.thi, s.tlo, h, evm, gmx, nf, te, ntc);
+        h = hyx_gevents(s, a, i, fl, t, s.thi, s.tlo, h, evm, gmx, nf, te, ntc, false);
         park = a.ev_pause && ((te >= 0 && !hyx_ev_stop[te]) || ntc > 0u);
     }
 #else
@@ -264,7 +271,8 @@ __device__ __forceinline__ void hyx_gstep_body(const hyx_kargs &a)
     hyx_gstep<WTC>(s.xo, s.pv, s.thi, mdt, h, nf, g, fl, tab, valid ? a.tc + i : nullptr, a.ld, evm, gmx);
     if (valid) {
         unsigned ntc = 0;
-        h = hyx_gevents(s, a, i, fl, tab, s.thi, s.tlo, h, evm, gmx, nf, te, ntc);
+        s.cdact = hyx_ev_cdmask(a, i);
+        h = hyx_gevents(s, a, i, fl, tab, s.thi, s.tlo, h, evm, gmx, nf, te, ntc, true);
     }
 #else
     hyx_gstep<WTC>(s.xo, s.pv, s.thi, mdt, h, nf, g, fl, tab, (WTC && valid) ? a.tc + i : nullptr, a.ld);
@@ -349,6 +357,9 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
     s.nst = 0;
     s.hlast = 0.0;
     s.gc = 0;
+#ifdef HYX_EVENTS
+    s.cdact = 0ull;
+#endif
     hyx_u64 inst = 0, budget = 0;
     bool active = false;
     bool exhausted = false;

@@ -52,6 +52,9 @@ struct hyx_inst {
     int tdir;
     unsigned gc; // propagate_grid: next grid point
     unsigned cn; // continuous output: log slots in use
+#ifdef HYX_EVENTS
+    unsigned long long cdact; // terminal events with a running cooldown
+#endif
 };
 
 __device__ __forceinline__ void hyx_load(hyx_inst &s, const hyx_kargs &a, hyx_u64 i)
@@ -125,7 +128,7 @@ __device__ __forceinline__ bool hyx_prop_iter(hyx_inst &s, const hyx_kargs &a, h
 #ifdef HYX_EVENTS
     int te_dsgn;
     unsigned ntc;
-    h = hyx_ev_step(a, i, s.pv, s.thi, s.tlo, h, s.x, nf, te, te_dsgn, ntc, evm);
+    h = hyx_ev_step(a, i, s.pv, s.thi, s.tlo, h, s.x, nf, te, te_dsgn, ntc, evm, s.cdact, false);
     park = a.ev_pause && ((te >= 0 && !hyx_ev_stop[te]) || ntc > 0u);
 #endif
     hyx_dfl_add(s.thi, s.tlo, h, 0.0);
@@ -197,6 +200,9 @@ __device__ __forceinline__ void hyx_resume(hyx_inst &s, const hyx_kargs &a, hyx_
 #ifdef HYX_CLOG
     s.cn = a.clog_n[i];
 #endif
+#ifdef HYX_EVENTS
+    s.cdact = hyx_ev_cdmask(a, i);
+#endif
 }
 
 // Write an instance back; done instances also record their outcome.
@@ -239,7 +245,8 @@ __device__ __forceinline__ void hyx_step_body(const hyx_kargs &a)
                          evm);
     int te, te_dsgn;
     unsigned ntc;
-    h = hyx_ev_step(a, i, s.pv, s.thi, s.tlo, h, s.x, nf, te, te_dsgn, ntc, evm);
+    s.cdact = hyx_ev_cdmask(a, i);
+    h = hyx_ev_step(a, i, s.pv, s.thi, s.tlo, h, s.x, nf, te, te_dsgn, ntc, evm, s.cdact, true);
 #else
     hyx_taylor_step<WTC>(s.x, s.pv, s.thi, mdt, h, nf, WTC ? a.tc + i : nullptr, a.ld, a.tape ? a.tape + i : nullptr,
                          a.tape_stride);

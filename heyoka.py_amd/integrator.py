@@ -57,10 +57,30 @@ _MODES = {"persistent": _lib.MODE_PERSISTENT, "plain": _lib.MODE_PLAIN, "graph":
 _PAD = "__hyx_event_pad_"
 
 
+def _squares(e, memo=None):
+    """e with every product of a subexpression with itself written as its square (the
+    square recurrence sums half the terms of the product's)."""
+    memo = {} if memo is None else memo
+    key = id(e)
+    if key in memo:
+        return memo[key]
+    r = e
+    if e.is_func():
+        args = [_squares(a, memo) for a in e.args]
+        if e.name == "mul" and len(args) == 2 and args[0] == args[1]:
+            r = expression._func("square", args[0])
+        elif e.name == "pow" and len(args) == 2 and args[1].is_number() and args[1].value == 2.0:
+            r = expression._func("square", args[0])
+        elif any(a is not b for a, b in zip(args, e.args)):
+            r = expression._func(e.name, *args)
+    memo[key] = r
+    return r
+
+
 def _event_spec(dec, tes, ntes, tol):
     """The events' op-list: a row per state variable (then padding rows), the event
     functions as right-hand sides, terminal events first."""
-    exprs = [e.expression for e in tes] + [e.expression for e in ntes]
+    exprs = [_squares(e.expression) for e in tes] + [_squares(e.expression) for e in ntes]
     names = list(dec.state_names)
     missing = sorted(set(get_variables(exprs)) - set(names))
     if missing:
