@@ -455,3 +455,75 @@ def test_group_form_events_vs_compact_and_oracle():
     assert len(logs["group"]) == len(logs["compact"]) > 0
     for (bg, tg_, dg), (bc, tc_, dc) in zip(logs["group"], logs["compact"]):
         assert bg == bc and dg == dc and abs(tg_ - tc_) <= 1e-12 * max(1.0, abs(tc_))
+
+
+def _wavy_ramp(log):
+    # doc/notebooks/Batch mode overview.ipynb cell 46: a ball bouncing on a wavy ramp and on
+    # the ground; terminal events whose callbacks reflect the velocity and continue.
+    px, py, vx, vy = hy.make_vars("x", "y", "vx", "vy")
+    eqns = [(px, vx), (py, vy), (vx, hy.expression(0.0)), (vy, hy.expression(-9.8))]
+    eq_curve = py - (1.0 - px + 0.05 * hy.cos(11 * np.pi * px))
+    CR = 0.8
+    last = {}
+
+    def cb_curve(ta, d_sgn, b):
+        if ta.time[b] - last.get(b, -1.0) < 1e-6:
+            return False
+        last[b] = ta.time[b]
+        x_, y_ = ta.state[0:2, b]
+        log.append((b, "curve", float(ta.time[b]), float(x_), float(y_)))
+        grad = np.array([1 + 0.05 * 11 * np.pi * np.sin(11 * np.pi * x_), 1])
+        u = grad / np.linalg.norm(grad)
+        vel = ta.state[2:4, b]
+        vel += (1.0 + CR) * (-np.dot(vel, u) * u)
+        return True
+
+    def cb_bottom(ta, d_sgn, b):
+        if ta.time[b] - last.get(b, -1.0) < 1e-6:
+            return False
+        last[b] = ta.time[b]
+        log.append((b, "ground", float(ta.time[b]), float(ta.state[0, b]), float(ta.state[1, b])))
+        ta.state[3, b] = -CR * ta.state[3, b]
+        return True
+
+    B = 4
+    ic = [[0.0] * B, [1.2 - i / 50 for i in range(B)], [0.0] * B, [0.0] * B]
+    tes = [hy.t_event_batch(eq_curve, callback=cb_curve, direction=hy.event_direction.negative),
+           hy.t_event_batch(py, callback=cb_bottom, direction=hy.event_direction.negative)]
+    return eqns, ic, tes
+
+
+def test_wavy_ramp_batch():
+    print("[gpu] wavy ramp batch", flush=True)
+    logs = ([], [])
+    outs = []
+    grid = np.repeat(np.linspace(0, 10, 2001), 4).reshape(2001, -1)
+    for k, log in enumerate(logs):
+        eqns, ic, tes = _wavy_ramp(log)
+        ta = hy.taylor_adaptive_batch(eqns, ic, t_events=tes)
+        if k == 0:
+            _, res = ta.propagate_grid(grid)
+        else:
+            _, res = ta.propagate_grid(grid, callback=lambda t: True)  # host-driven steps
+        outs.append((res, ta.state.copy(), list(ta.propagate_res)))
+    (r0, s0, p0), (r1, s1, p1) = outs
+    # Parked device path and host-driven steps take the same steps and bounces bit for bit;
+    # the grid points are evaluated on the device from the step start and on the host from
+    # the step end, so they differ by rounding.
+    np.testing.assert_array_equal(s0, s1)
+    np.testing.assert_allclose(r0, r1, rtol=0, atol=1e-13)
+    assert p0 == p1 and sorted(logs[0]) == sorted(logs[1])
+    curve = lambda x: 1.0 - x + 0.05 * np.cos(11 * np.pi * x)
+    bounces = logs[0]
+    assert sum(1 for e in bounces if e[1] == "curve") >= 8
+    for b, kind, t, x_, y_ in bounces:
+        # Each bounce sits on its surface.
+        if kind == "curve":
+            assert abs(y_ - curve(x_)) < 1e-12
+        else:
+            assert abs(y_) < 1e-12
+    # The ball never goes through the ramp or the ground.
+    fin = np.isfinite(r0[:, 0, :])
+    assert np.all(r0[:, 1, :][fin] >= -1e-9)
+    above = r0[:, 0, :][fin] < 0.9
+    assert np.all((r0[:, 1, :][fin] - curve(r0[:, 0, :][fin]))[above] >= -1e-9)
