@@ -108,7 +108,8 @@ def main():
     ap.add_argument("--duration", type=float, default=4.0, help="serving: seconds of arrivals per rate")
     ap.add_argument("--n", type=int, default=None, help="instances (default: the config's)")
     ap.add_argument("--dt", type=float, default=25.0, help="nbody6: time units per bench step")
-    ap.add_argument("--code-form", default=None, choices=[None, "compact", "unrolled", "group"])
+    ap.add_argument("--code-form", default=None, choices=[None, "compact", "unrolled", "group", "wave", "wave32",
+                                                           "wave16"])
     ap.add_argument("--split-chains", action="store_true",
                     help="two accumulators per long convolution sum instead of one (A/B of the code variant)")
     ap.add_argument("--no-pipeline", action="store_true",
@@ -238,7 +239,7 @@ def main():
             "code_form": system.form,
             "sum_chains": "split" if args.split_chains else "serial",
             "group_exchanges_per_order": ((2 if args.no_pipeline else 1) if system.form == "group" else None),
-            "lanes_per_instance": {"group": None, "compact": args.sub, "unrolled": 1}[system.form],
+            "lanes_per_instance": {"group": None, "wave": None, "compact": args.sub, "unrolled": 1}[system.form],
             "form": "persistent",
             "algorithmic_flops_per_instance_step": flops,
             "hiprtc_compile_s": round(t_compile, 3),

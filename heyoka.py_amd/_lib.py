@@ -22,7 +22,10 @@ FIELD_STATE, FIELD_PARS, FIELD_TIME_HI, FIELD_TIME_LO = 0, 1, 2, 3
 MODE_PERSISTENT, MODE_PLAIN, MODE_GRAPH = 0, 1, 2
 FLAG_HIGH_ACCURACY, FLAG_COMPILE_ONLY, FLAG_FORCE_COMPACT, FLAG_FORCE_UNROLLED, FLAG_SOURCE_ONLY = 1, 2, 4, 8, 16
 FLAG_FORCE_GROUP = 32
-FORMS = {0: "unrolled", 1: "compact", 2: "group"}
+FLAG_FORCE_WAVE = 4096
+FLAG_WAVE32 = 8192
+FLAG_WAVE16 = 16384
+FORMS = {0: "unrolled", 1: "compact", 2: "group", 3: "wave"}
 FLAG_GRID = 64
 FLAG_SPLIT_CHAINS = 128
 FLAG_NO_PIPELINE = 256
@@ -175,7 +178,8 @@ def oplist_args(dec):
 
 
 def _flags(high_accuracy=False, compact=None):
-    """compact: None (automatic), True / "compact", False / "unrolled", or "group"."""
+    """compact: None (automatic), True / "compact", False / "unrolled", "group", or "wave" (64 lanes per
+    instance; "wave32" / "wave16": 32 or 16)."""
     f = FLAG_HIGH_ACCURACY if high_accuracy else 0
     if compact is True or compact == "compact":
         f |= FLAG_FORCE_COMPACT
@@ -183,6 +187,8 @@ def _flags(high_accuracy=False, compact=None):
         f |= FLAG_FORCE_UNROLLED
     elif compact == "group":
         f |= FLAG_FORCE_GROUP
+    elif compact in ("wave", "wave32", "wave16"):
+        f |= FLAG_FORCE_WAVE | {"wave": 0, "wave32": FLAG_WAVE32, "wave16": FLAG_WAVE16}[compact]
     return f
 
 

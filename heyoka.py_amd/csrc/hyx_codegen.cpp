@@ -16,6 +16,7 @@
 // Divisions inside the recurrences become multiplications by reciprocals
 // computed once per step; 1/k and j/k factors are literals.
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -1796,6 +1797,600 @@ std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compac
     g.lds_state = hist * (order + 1) > 160 && (hist - nst) * (order + 1) <= 96
                   && (long)(order + 1) * S.n * 64 * 8 <= 65536;
     return g.run();
+}
+
+// ---- Wave form ----------------------------------------------------------------------------
+//
+// G lanes per instance (64: one instance per wavefront, or 32 / 16). The nodes of the
+// decomposition are spread over the lanes of the group by a static schedule: each node
+// gets the level 1 + the highest level of its node operands (state variables are level
+// 0), and the nodes of one level and one recurrence kind form slots of at most G nodes,
+// lane q of a slot evaluating its q-th node. Per order, the state rows come first, then
+// the slots in level order, separated by wavefront-scope fences. Every coefficient
+// lives in the instance's LDS block:
+//   T[k][r] = fl[k * R + r]: every order of the state variables (rows 0..n-1, the rows
+//     the group kernels' state update, grid output and event code read), a zero row
+//     (n), the nodes whose history a recurrence reads, and the constants and
+//     parameters used by the nodes (order 0: their value; higher orders: zero);
+//   cur[c] = fl[(P + 1) * R + c]: the current order of every state variable (c = i),
+//     the zero entry (c = n), every node (c = node + 1) and every constant or
+//     parameter, read by the linear nodes and the right-hand sides.
+// A lane's operand rows come from per-lane tables loaded once into registers. Each
+// recurrence does the arithmetic of the general recurrences of the register forms (the
+// same terms in the same order); terms that those skip because a factor is zero by
+// degree are read here as exact zeros.
+namespace
+{
+enum WaveKind { WK_LIN, WK_MUL, WK_MULS, WK_SQ, WK_POW, WK_DIV, WK_SQRT, WK_EXP, WK_LOG, WK_TRIG, WK_TIME, WK_COUNT };
+
+struct WaveSlot {
+    int kind = 0;
+    int m = 0;                   // linear slots: operands per lane
+    std::vector<uint32_t> nodes; // the node of each lane (lane q < nodes.size())
+};
+} // namespace
+
+std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, const Sys *E, uint32_t nev,
+                                  std::string &why)
+{
+    const int n = (int)S.n, NN = (int)S.nnodes;
+    if (NN == n) {
+        why = "no nodes";
+        return "";
+    }
+    if (G != 16 && G != 32 && G != 64) throw std::invalid_argument("wave form: G must be 16, 32 or 64");
+    auto is_node = [](uint32_t w) { return HYX_OPERAND_KIND(w) == HYX_KIND_NODE; };
+    std::vector<int> kind(NN, -1);
+    for (int j = n; j < NN; ++j) {
+        const auto &A = S.args[j];
+        switch (S.op[j]) {
+        case HYX_OP_ADD:
+        case HYX_OP_SUB:
+        case HYX_OP_NEG:
+        case HYX_OP_SUM: kind[j] = WK_LIN; break;
+        case HYX_OP_MUL: kind[j] = (is_node(A[0]) && is_node(A[1])) ? WK_MUL : WK_MULS; break;
+        case HYX_OP_SQUARE: kind[j] = WK_SQ; break;
+        case HYX_OP_POW: kind[j] = WK_POW; break;
+        case HYX_OP_DIV: kind[j] = WK_DIV; break;
+        case HYX_OP_SQRT: kind[j] = WK_SQRT; break;
+        case HYX_OP_EXP: kind[j] = WK_EXP; break;
+        case HYX_OP_LOG: kind[j] = WK_LOG; break;
+        case HYX_OP_SIN:
+        case HYX_OP_COS: kind[j] = WK_TRIG; break;
+        case HYX_OP_TIME: kind[j] = WK_TIME; break;
+        default: why = "an operation the wave form does not implement"; return "";
+        }
+        for (auto a : A)
+            if (is_node(a) && (int)HYX_OPERAND_INDEX(a) >= j) {
+                why = "a node operand that is not an earlier node";
+                return "";
+            }
+    }
+    // Rows with a history: node operands of every recurrence but the linear ones, and
+    // the nodes whose recurrence reads their own (or their partner's) history.
+    std::vector<char> hist(NN, 0);
+    for (int j = n; j < NN; ++j) {
+        if (kind[j] != WK_LIN)
+            for (auto a : S.args[j])
+                if (is_node(a) && (int)HYX_OPERAND_INDEX(a) >= n) hist[HYX_OPERAND_INDEX(a)] = 1;
+        const int k = kind[j];
+        if (k == WK_POW || k == WK_DIV || k == WK_SQRT || k == WK_EXP || k == WK_LOG || k == WK_TRIG) hist[j] = 1;
+    }
+    // Constants and parameters used by nodes or right-hand sides.
+    std::vector<uint32_t> scal;
+    auto scal_index = [&](uint32_t w) {
+        for (size_t q = 0; q < scal.size(); ++q)
+            if (scal[q] == w) return (int)q;
+        scal.push_back(w);
+        return (int)scal.size() - 1;
+    };
+    for (int j = n; j < NN; ++j)
+        for (auto a : S.args[j])
+            if (!is_node(a)) scal_index(a);
+    for (int i = 0; i < n; ++i)
+        if (!is_node(S.rhs[i])) scal_index(S.rhs[i]);
+    const int NS = (int)scal.size();
+    // T rows and cur entries.
+    std::vector<int> hrow(NN, -1);
+    int R = n + 1;
+    for (int j = n; j < NN; ++j)
+        if (hist[j]) hrow[j] = R++;
+    const int srow0 = R;
+    R += NS;
+    const int CB = (P + 1) * R;
+    const int NC = NN + 1 + NS;
+    const int flat = CB + NC;
+    const int ipb = 64 / G;
+    if ((long)flat * ipb * 8 > 65536) {
+        why = "the coefficient rows exceed 64 KB of LDS per wavefront";
+        return "";
+    }
+    auto trow = [&](uint32_t w) -> int {
+        const int idx = (int)HYX_OPERAND_INDEX(w);
+        if (!is_node(w)) return srow0 + scal_index(w);
+        if (idx < n) return idx;
+        if (hrow[idx] < 0) throw std::logic_error("wave form: operand without a history row");
+        return hrow[idx];
+    };
+    auto cid = [&](uint32_t w) -> int {
+        const int idx = (int)HYX_OPERAND_INDEX(w);
+        if (!is_node(w)) return NN + 1 + scal_index(w);
+        return idx < n ? idx : idx + 1;
+    };
+    // Levels and slots.
+    std::vector<int> lvl(NN, 0);
+    int maxl = 0;
+    for (int j = n; j < NN; ++j) {
+        int l = 1;
+        for (auto a : S.args[j])
+            if (is_node(a) && (int)HYX_OPERAND_INDEX(a) >= n) l = std::max(l, lvl[HYX_OPERAND_INDEX(a)] + 1);
+        lvl[j] = l;
+        maxl = std::max(maxl, l);
+    }
+    std::vector<WaveSlot> slots;
+    for (int l = 1; l <= maxl; ++l)
+        for (int kd = 0; kd < WK_COUNT; ++kd) {
+            std::vector<uint32_t> ns;
+            for (int j = n; j < NN; ++j)
+                if (lvl[j] == l && kind[j] == kd) ns.push_back((uint32_t)j);
+            for (size_t q0 = 0; q0 < ns.size(); q0 += G) {
+                WaveSlot s;
+                s.kind = kd;
+                s.nodes.assign(ns.begin() + q0, ns.begin() + std::min(ns.size(), q0 + G));
+                for (auto j : s.nodes) s.m = std::max(s.m, (int)S.args[j].size());
+                slots.push_back(s);
+            }
+        }
+    const int own = (n + G - 1) / G;
+    if (own > 8) {
+        why = "too many state variables per lane";
+        return "";
+    }
+
+    // Per-lane tables: int fields w[f][g], double fields d[f][g].
+    std::vector<std::vector<int>> W;
+    std::vector<std::vector<double>> D;
+    auto addw = [&](std::vector<int> v) {
+        v.resize(G, v.empty() ? 0 : v.back());
+        W.push_back(v);
+        return (int)W.size() - 1;
+    };
+    auto addd = [&](std::vector<double> v) {
+        v.resize(G, 0.0);
+        D.push_back(v);
+        return (int)D.size() - 1;
+    };
+    std::vector<int> vo((size_t)G * own, -1);
+    for (int i = 0; i < n; ++i) vo[(size_t)(i % G) * own + i / G] = i;
+    // Right-hand side of each owned variable: its cur entry (the zero entry for none).
+    std::vector<int> f_rhs(own);
+    for (int o = 0; o < own; ++o) {
+        std::vector<int> v(G, n);
+        for (int g = 0; g < G; ++g)
+            if (vo[(size_t)g * own + o] >= 0) v[g] = cid(S.rhs[vo[(size_t)g * own + o]]);
+        f_rhs[o] = addw(v);
+    }
+
+    std::ostringstream h, o;
+    h << "#define HYX_N " << n << "\n#define HYX_NPAR " << std::max<uint32_t>(S.npar, 1) << "\n#define HYX_P " << P
+      << "\n#define HYX_GROUP 1\n#define HYX_WAVE 1\n#define HYX_G " << G << "\n#define HYX_I " << ipb
+      << "\n#define HYX_OWN " << own << "\n#define HYX_NX " << R << "\n#define HYX_XB 0\n#define HYX_FLAT " << flat
+      << "\n#define HYX_WAVE_SLOTS " << slots.size() << "\n";
+    h << "typedef unsigned long long hyx_u64;\n";
+    h << "#ifndef HYX_CPU_EMU\n"
+         "// A block is one wavefront, and the LDS operations of a wavefront complete in issue\n"
+         "// order: the exchanges between slots only need the compiler to keep memory accesses\n"
+         "// on their side of the fence (no s_waitcnt, no s_barrier).\n"
+         "__device__ __forceinline__ void hyx_wsync()\n{\n"
+         "    __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"wavefront\");\n"
+         "    __builtin_amdgcn_wave_barrier();\n"
+         "    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, \"wavefront\");\n}\n"
+         "#endif\n";
+
+    // Slot code. Operand rows are per-lane registers tab.w[f]; T[k][row] is fl[row + k * R].
+    const std::string Rs = std::to_string(R);
+    auto Tk = [&](const std::string &row, int k) {
+        return k == 0 ? "fl[" + row + "]" : "fl[" + row + " + " + std::to_string(k * R) + "]";
+    };
+    auto cur = [&](const std::string &c) { return "fl[" + std::to_string(CB) + " + " + c + "]"; };
+    struct SlotTab {
+        int a = -1, b = -1, oc = -1, orow = -1, sgn = -1, al = -1, al1 = -1, cosf = -1;
+        std::vector<int> lin;
+        bool cur_any = false, hist_all = false, hist_any = false;
+    };
+    std::vector<SlotTab> st(slots.size());
+    // Readers of each node's cur entry: linear nodes and right-hand sides.
+    std::vector<char> cur_read(NN + 1 + NS, 0);
+    for (int j = n; j < NN; ++j)
+        if (kind[j] == WK_LIN)
+            for (auto a : S.args[j]) cur_read[cid(a)] = 1;
+    for (int i = 0; i < n; ++i) cur_read[cid(S.rhs[i])] = 1;
+    for (size_t s = 0; s < slots.size(); ++s) {
+        const auto &sl = slots[s];
+        auto &t = st[s];
+        const int w = (int)sl.nodes.size();
+        auto lanes = [&](auto f) {
+            std::vector<int> v(G, 0);
+            for (int q = 0; q < w; ++q) v[q] = f(sl.nodes[q]);
+            for (int q = w; q < G; ++q) v[q] = v[0];
+            return v;
+        };
+        t.hist_all = true;
+        for (auto j : sl.nodes) {
+            t.hist_all = t.hist_all && hist[j];
+            t.hist_any = t.hist_any || hist[j];
+            t.cur_any = t.cur_any || cur_read[j + 1];
+        }
+        if (t.cur_any) t.oc = addw(lanes([&](uint32_t j) { return (int)j + 1; }));
+        if (t.hist_any) t.orow = addw(lanes([&](uint32_t j) { return hist[j] ? hrow[j] : -1; }));
+        switch (sl.kind) {
+        case WK_LIN: {
+            for (int q = 0; q < sl.m; ++q)
+                t.lin.push_back(addw(lanes([&](uint32_t j) {
+                    return q < (int)S.args[j].size() ? cid(S.args[j][q]) : n;
+                })));
+            t.sgn = addw(lanes([&](uint32_t j) {
+                int m = 0;
+                for (size_t q = 0; q < S.args[j].size(); ++q)
+                    if (S.op[j] == HYX_OP_NEG || (S.op[j] == HYX_OP_SUB && q == 1)) m |= 1 << q;
+                return m;
+            }));
+            break;
+        }
+        case WK_MULS:
+            // a: the operand with a series (if any); b: the constant or parameter.
+            t.a = addw(lanes([&](uint32_t j) { return trow(is_node(S.args[j][0]) ? S.args[j][0] : S.args[j][1]); }));
+            t.b = addw(lanes([&](uint32_t j) { return trow(is_node(S.args[j][0]) ? S.args[j][1] : S.args[j][0]); }));
+            break;
+        case WK_TIME: break;
+        default:
+            t.a = addw(lanes([&](uint32_t j) { return trow(S.args[j][0]); }));
+            if (sl.kind == WK_MUL || sl.kind == WK_DIV) t.b = addw(lanes([&](uint32_t j) { return trow(S.args[j][1]); }));
+            if (sl.kind == WK_TRIG) {
+                t.b = addw(lanes([&](uint32_t j) { return hrow[S.aux[j]]; }));
+                t.cosf = addw(lanes([&](uint32_t j) { return S.op[j] == HYX_OP_COS ? 1 : 0; }));
+            }
+            if (sl.kind == WK_POW) {
+                std::vector<double> al(G, 0.0), al1(G, 0.0);
+                for (int q = 0; q < w; ++q) {
+                    al[q] = S.consts[S.aux[sl.nodes[q]]];
+                    al1[q] = al[q] + 1.0;
+                }
+                t.al = addd(al);
+                t.al1 = addd(al1);
+            }
+            break;
+        }
+    }
+    auto wv = [](int f) { return "tab.w[" + std::to_string(f) + "]"; };
+    auto dv = [](int f) { return "tab.d[" + std::to_string(f) + "]"; };
+    // Per-slot registers that live across orders (reciprocals of order-0 values).
+    auto rv = [](size_t s, const char *nm) { return std::string(nm) + std::to_string(s); };
+
+    auto emit_slot = [&](size_t s, int k) {
+        const auto &sl = slots[s];
+        const auto &t = st[s];
+        const int w = (int)sl.nodes.size();
+        const std::string a = t.a >= 0 ? wv(t.a) : "", b = t.b >= 0 ? wv(t.b) : "";
+        o << "    " << (w < G ? "if (g < " + std::to_string(w) + ") " : "") << "{\n";
+        auto chain = [&](const std::vector<std::pair<std::string, std::string>> &tm, const std::string &nm) {
+            o << "        double " << nm << " = " << tm[0].first << " * " << tm[0].second << ";\n";
+            for (size_t q = 1; q < tm.size(); ++q)
+                o << "        " << nm << " = __builtin_fma(" << tm[q].first << ", " << tm[q].second << ", " << nm << ");\n";
+        };
+        std::vector<std::pair<std::string, std::string>> tm, tm2;
+        switch (sl.kind) {
+        case WK_LIN: {
+            o << "        const int sm = " << wv(t.sgn) << ";\n";
+            for (int q = 0; q < sl.m; ++q) {
+                const std::string x = cur(wv(t.lin[q]));
+                o << "        const double l" << q << " = ((sm >> " << q << ") & 1) ? -" << x << " : " << x << ";\n";
+            }
+            o << "        double v = l0;\n";
+            for (int q = 1; q < sl.m; ++q) o << "        v = v + l" << q << ";\n";
+            break;
+        }
+        case WK_MUL:
+            if (k == 0)
+                o << "        const double v = " << Tk(a, 0) << " * " << Tk(b, 0) << ";\n";
+            else {
+                for (int q = 0; q <= k; ++q) tm.push_back({Tk(a, q), Tk(b, k - q)});
+                chain(tm, "v");
+            }
+            break;
+        case WK_MULS: o << "        const double v = " << Tk(a, k) << " * " << Tk(b, 0) << ";\n"; break;
+        case WK_SQ:
+            if (k == 0)
+                o << "        const double v = " << Tk(a, 0) << " * " << Tk(a, 0) << ";\n";
+            else {
+                for (int q = 0; 2 * q < k; ++q) tm.push_back({Tk(a, q), Tk(a, k - q)});
+                chain(tm, "acc");
+                if (k % 2)
+                    o << "        const double v = acc + acc;\n";
+                else
+                    o << "        const double v = __builtin_fma(" << Tk(a, k / 2) << ", " << Tk(a, k / 2) << ", acc + acc);\n";
+            }
+            break;
+        case WK_POW: {
+            const std::string c = wv(t.orow);
+            if (k == 0) {
+                o << "        const double a0 = " << Tk(a, 0) << ";\n        const double v = pow(a0, " << dv(t.al) << ");\n";
+                o << "        const double ra = 1.0 / a0;\n        " << rv(s, "raa") << " = ra * " << dv(t.al) << ";\n        "
+                  << rv(s, "rab") << " = ra * " << dv(t.al1) << ";\n";
+            } else {
+                for (int q = 0; q < k; ++q) tm.push_back({Tk(a, k - q), Tk(c, q)});
+                chain(tm, "s1");
+                if (k == 1)
+                    o << "        const double v = s1 * " << rv(s, "raa") << ";\n";
+                else {
+                    for (int q = 1; q < k; ++q) {
+                        o << "        const double z" << q << " = " << lit((double)q) << " * " << Tk(c, q) << ";\n";
+                        tm2.push_back({Tk(a, k - q), "z" + std::to_string(q)});
+                    }
+                    chain(tm2, "s2");
+                    o << "        const double v = __builtin_fma(s1, " << rv(s, "raa") << ", -(s2 * " << rv(s, "rab") << ") * "
+                      << lit(1.0 / k) << ");\n";
+                }
+            }
+            break;
+        }
+        case WK_DIV: {
+            const std::string c = wv(t.orow);
+            if (k == 0) {
+                o << "        const double v = " << Tk(a, 0) << " / " << Tk(b, 0) << ";\n        " << rv(s, "rb") << " = 1.0 / "
+                  << Tk(b, 0) << ";\n";
+            } else {
+                for (int q = 1; q <= k; ++q) tm.push_back({Tk(b, q), Tk(c, k - q)});
+                chain(tm, "acc");
+                o << "        const double v = (" << Tk(a, k) << " - acc) * " << rv(s, "rb") << ";\n";
+            }
+            break;
+        }
+        case WK_SQRT: {
+            const std::string c = wv(t.orow);
+            if (k == 0) {
+                o << "        const double v = sqrt(" << Tk(a, 0) << ");\n        " << rv(s, "rs") << " = 0.5 / v;\n";
+            } else {
+                for (int q = 1; 2 * q < k; ++q) tm.push_back({Tk(c, q), Tk(c, k - q)});
+                std::string tot;
+                if (!tm.empty()) {
+                    chain(tm, "acc");
+                    tot = "acc + acc";
+                }
+                if (k % 2 == 0) {
+                    const std::string m = Tk(c, k / 2);
+                    tot = tot.empty() ? (m + " * " + m) : ("__builtin_fma(" + m + ", " + m + ", " + tot + ")");
+                }
+                o << "        const double v = (" << Tk(a, k) << (tot.empty() ? "" : " - (" + tot + ")") << ") * "
+                  << rv(s, "rs") << ";\n";
+            }
+            break;
+        }
+        case WK_EXP: {
+            const std::string c = wv(t.orow);
+            if (k == 0)
+                o << "        const double v = exp(" << Tk(a, 0) << ");\n";
+            else {
+                for (int q = 1; q <= k; ++q) {
+                    std::string d = Tk(a, q);
+                    if (q > 1) {
+                        o << "        const double d" << q << " = " << lit((double)q) << " * " << d << ";\n";
+                        d = "d" + std::to_string(q);
+                    }
+                    tm.push_back({d, Tk(c, k - q)});
+                }
+                chain(tm, "acc");
+                o << "        const double v = acc * " << lit(1.0 / k) << ";\n";
+            }
+            break;
+        }
+        case WK_LOG: {
+            const std::string c = wv(t.orow);
+            if (k == 0)
+                o << "        const double v = log(" << Tk(a, 0) << ");\n        " << rv(s, "rl") << " = 1.0 / " << Tk(a, 0)
+                  << ";\n";
+            else {
+                for (int q = 1; q < k; ++q) tm.push_back({lit((double)q / k), "(" + Tk(c, q) + " * " + Tk(a, k - q) + ")"});
+                if (!tm.empty()) chain(tm, "acc");
+                o << "        const double v = (" << Tk(a, k) << (tm.empty() ? "" : " - acc") << ") * " << rv(s, "rl") << ";\n";
+            }
+            break;
+        }
+        case WK_TRIG: {
+            if (k == 0) {
+                o << "        double sn, cs;\n        sincos(" << Tk(a, 0) << ", &sn, &cs);\n        const double v = "
+                  << wv(t.cosf) << " ? cs : sn;\n";
+            } else {
+                for (int q = 1; q <= k; ++q) {
+                    std::string d = Tk(a, q);
+                    if (q > 1) {
+                        o << "        const double d" << q << " = " << lit((double)q) << " * " << d << ";\n";
+                        d = "d" + std::to_string(q);
+                    }
+                    tm.push_back({d, Tk(b, k - q)});
+                }
+                chain(tm, "acc");
+                o << "        const double u = acc * " << lit(1.0 / k) << ";\n        const double v = " << wv(t.cosf)
+                  << " ? -u : u;\n";
+            }
+            break;
+        }
+        case WK_TIME: o << "        const double v = " << (k == 0 ? "t0" : k == 1 ? "1.0" : "0.0") << ";\n"; break;
+        }
+        if (t.cur_any) o << "        " << cur(wv(t.oc)) << " = v;\n";
+        if (t.hist_all)
+            o << "        " << Tk(wv(t.orow), k) << " = v;\n";
+        else if (t.hist_any)
+            o << "        if (" << wv(t.orow) << " >= 0) " << Tk(wv(t.orow), k) << " = v;\n";
+        o << "    }\n    hyx_wsync();\n";
+    };
+
+    // The per-lane table struct and its loader.
+    h << "struct hyx_gtab {\n    int vo[HYX_OWN];\n    int w[" << std::max<size_t>(W.size(), 1) << "];\n    double d["
+      << std::max<size_t>(D.size(), 1) << "];\n};\n";
+    {
+        std::vector<int> wf;
+        for (auto &f : W) wf.insert(wf.end(), f.begin(), f.end());
+        if (wf.empty()) wf.push_back(0);
+        h << "__constant__ int hyx_wtab_w[" << wf.size() << "] = {";
+        for (size_t q = 0; q < wf.size(); ++q) h << (q ? "," : "") << wf[q];
+        h << "};\n";
+        std::vector<double> df;
+        for (auto &f : D) df.insert(df.end(), f.begin(), f.end());
+        if (df.empty()) df.push_back(0.0);
+        h << "__constant__ double hyx_wtab_d[" << df.size() << "] = {";
+        for (size_t q = 0; q < df.size(); ++q) h << (q ? "," : "") << lit(df[q]);
+        h << "};\n";
+        h << "__constant__ int hyx_wtab_vo[" << vo.size() << "] = {";
+        for (size_t q = 0; q < vo.size(); ++q) h << (q ? "," : "") << vo[q];
+        h << "};\n";
+    }
+    h << "__device__ __forceinline__ void hyx_gtab_load(hyx_gtab &t, int g)\n{\n"
+         "    for (int q = 0; q < HYX_OWN; ++q) t.vo[q] = hyx_wtab_vo[g * HYX_OWN + q];\n";
+    if (!W.empty()) h << "    for (int f = 0; f < " << W.size() << "; ++f) t.w[f] = hyx_wtab_w[f * HYX_G + g];\n";
+    if (!D.empty()) h << "    for (int f = 0; f < " << D.size() << "; ++f) t.d[f] = hyx_wtab_d[f * HYX_G + g];\n";
+    h << "}\n";
+
+    auto xv = [](int oo, int k) { return "x" + std::to_string(oo) + "_" + std::to_string(k); };
+    auto vcol = [](int oo) { return "vc" + std::to_string(oo); };
+    o << "template <bool WTC, class TAB>\n"
+         "__device__ __forceinline__ void hyx_gstep(double (&xo)[HYX_OWN], const double (&pv)[HYX_NPAR], double t0, "
+         "double max_dt, double &h_out, bool &nf_out, int g, double *fl, const TAB &tab, double *__restrict__ tc, "
+         "hyx_u64 tcs"
+      << (E ? ", unsigned long long &hyx_evm, double &hyx_mx" : "") << ")\n{\n";
+    o << "    (void)pv; (void)t0; (void)tc; (void)tcs; (void)g;\n";
+    for (size_t s = 0; s < slots.size(); ++s) {
+        const int kd = slots[s].kind;
+        if (kd == WK_POW) o << "    double " << rv(s, "raa") << " = 0.0, " << rv(s, "rab") << " = 0.0;\n";
+        if (kd == WK_DIV) o << "    double " << rv(s, "rb") << " = 0.0;\n";
+        if (kd == WK_SQRT) o << "    double " << rv(s, "rs") << " = 0.0;\n";
+        if (kd == WK_LOG) o << "    double " << rv(s, "rl") << " = 0.0;\n";
+    }
+    for (int oo = 0; oo < own; ++oo)
+        o << "    const int " << vcol(oo) << " = (tab.vo[" << oo << "] >= 0) ? tab.vo[" << oo << "] : HYX_N;\n";
+    o << "    hyx_wsync();\n";
+    // Order 0: state rows, constants and parameters (lane 0), then the slots.
+    for (int oo = 0; oo < own; ++oo) {
+        o << "    const double " << xv(oo, 0) << " = xo[" << oo << "];\n";
+        o << "    fl[" << vcol(oo) << "] = " << xv(oo, 0) << ";\n";
+        o << "    " << cur(vcol(oo)) << " = " << xv(oo, 0) << ";\n";
+    }
+    if (NS) {
+        o << "    if (g == 0) {\n";
+        for (int q = 0; q < NS; ++q) {
+            const uint32_t w = scal[q];
+            const std::string val = HYX_OPERAND_KIND(w) == HYX_KIND_CONST ? lit(S.consts[HYX_OPERAND_INDEX(w)])
+                                                                           : "pv[" + std::to_string(HYX_OPERAND_INDEX(w)) + "]";
+            o << "        fl[" << (srow0 + q) << "] = " << val << ";\n";
+            o << "        fl[" << (CB + NN + 1 + q) << "] = " << val << ";\n";
+        }
+        o << "    }\n";
+    }
+    o << "    hyx_wsync();\n";
+    for (size_t s = 0; s < slots.size(); ++s) emit_slot(s, 0);
+    for (int k = 1; k <= P; ++k) {
+        o << "    // order " << k << "\n";
+        for (int oo = 0; oo < own; ++oo) o << "    const double r" << oo << "_" << k << " = " << cur(wv(f_rhs[oo])) << ";\n";
+        o << "    hyx_wsync();\n";
+        for (int oo = 0; oo < own; ++oo) {
+            o << "    const double " << xv(oo, k) << " = r" << oo << "_" << k << " * " << lit(1.0 / k) << ";\n";
+            o << "    fl[" << vcol(oo) << " + " << k * R << "] = " << xv(oo, k) << ";\n";
+            if (k < P) o << "    " << cur(vcol(oo)) << " = " << xv(oo, k) << ";\n";
+        }
+        if (k == 1 && NS) {
+            o << "    if (g == 0) {\n";
+            for (int q = 0; q < NS; ++q) o << "        fl[" << (CB + NN + 1 + q) << "] = 0.0;\n";
+            o << "    }\n";
+        }
+        o << "    hyx_wsync();\n";
+        if (k < P)
+            for (size_t s = 0; s < slots.size(); ++s) emit_slot(s, k);
+    }
+    o << "    if constexpr (WTC) if (tc) {\n";
+    for (int oo = 0; oo < own; ++oo) {
+        o << "        if (tab.vo[" << oo << "] >= 0)\n";
+        o << "            for (int k = 0; k <= HYX_P; ++k) tc[((hyx_u64)tab.vo[" << oo << "] * " << (P + 1)
+          << " + k) * tcs] = fl[HYX_XB + k * HYX_NX + " << vcol(oo) << "];\n";
+    }
+    o << "    }\n";
+    if (E) {
+        Gen eg(*E, P, high_acc);
+        eg.lds_xb = 0;
+        eg.lds_nx = R;
+        o << "    double hyx_evg[" << nev << "][" << (P + 1) << "];\n    {\n";
+        for (uint32_t j = E->n; j < E->nnodes; ++j) eg.order0(j);
+        for (int k = 1; k <= P; ++k)
+            for (uint32_t j = E->n; j < E->nnodes; ++j) eg.orderk(j, k);
+        for (uint32_t e = 0; e < nev; ++e)
+            for (int k = 0; k <= P; ++k)
+                eg.o << "    hyx_evg[" << e << "][" << k << "] = " << (eg.zero(E->rhs[e], k) ? "0.0" : eg.val(E->rhs[e], k))
+                     << ";\n";
+        o << eg.o.str() << "    }\n";
+    }
+    o << "    double mx = 0.0, mo = 0.0, mom1 = 0.0, ab;\n";
+    for (int oo = 0; oo < own; ++oo) {
+        o << "    ab = fabs(" << xv(oo, 0) << "); mx = (ab > mx) ? ab : mx;\n";
+        o << "    ab = fabs(" << xv(oo, P) << "); mo = (ab > mo) ? ab : mo;\n";
+        o << "    ab = fabs(" << xv(oo, P - 1) << "); mom1 = (ab > mom1) ? ab : mom1;\n";
+    }
+    for (uint32_t e = 0; E && e < nev; ++e) {
+        o << "    ab = fabs(hyx_evg[" << e << "][" << P << "]); mo = (ab > mo) ? ab : mo;\n";
+        o << "    ab = fabs(hyx_evg[" << e << "][" << (P - 1) << "]); mom1 = (ab > mom1) ? ab : mom1;\n";
+    }
+    o << "#pragma unroll\n    for (int off = 1; off < HYX_G; off <<= 1) {\n"
+         "        ab = __shfl_xor(mx, off); mx = (ab > mx) ? ab : mx;\n"
+         "        ab = __shfl_xor(mo, off); mo = (ab > mo) ? ab : mo;\n"
+         "        ab = __shfl_xor(mom1, off); mom1 = (ab > mom1) ? ab : mom1;\n    }\n";
+    o << "    const double num = (mx <= 1.0) ? 1.0 : mx;\n";
+    o << "    const double rho_o = pow(num / mo, " << lit(1.0 / P) << ");\n";
+    o << "    const double rho_om1 = pow(num / mom1, " << lit(1.0 / (P - 1)) << ");\n";
+    o << "    const double rho_m = (rho_om1 < rho_o) ? rho_om1 : rho_o;\n";
+    const double e1 = std::exp(1.0);
+    const double rhofac = std::exp(-7.0 / 10.0 / (double)(P - 1)) / (e1 * e1);
+    o << "    double h = rho_m * " << lit(rhofac) << ";\n";
+    o << "    const double lim = fabs(max_dt);\n    h = (h < lim) ? h : lim;\n    if (max_dt < 0.0) h = -h;\n";
+    if (E) {
+        const uint64_t off = (uint64_t)n * (P + 1) + (uint64_t)(E->nnodes - E->n) * (P + 1);
+        o << "    hyx_evm = 0ull;\n";
+        o << "#pragma unroll\n    for (int e = 0; e < " << nev << "; ++e) {\n"
+             "        double hk = 1.0, emag = 0.0, lo = 0.0, hi = 0.0;\n"
+             "#pragma unroll\n        for (int k = 0; k <= " << P << "; ++k) {\n"
+             "            const double c = hyx_evg[e][k] * hk;\n            hk *= h;\n            emag += fabs(c);\n"
+             "            if (k > 0) { lo += fmin(c, 0.0); hi += fmax(c, 0.0); }\n        }\n"
+             "        const double slack = " << lit(4.0 * (P + 1) * 1.1102230246251565e-16) << " * emag;\n"
+             "        const double c0 = hyx_evg[e][0];\n"
+             "        if (emag > 0.0 && isfinite(emag) && !(c0 + lo > slack || c0 + hi < -slack)) hyx_evm |= 1ull << e;\n"
+             "    }\n";
+        o << "    if (hyx_evm != 0ull && tc) {\n";
+        for (uint32_t e = 0; e < nev; ++e)
+            for (int k = 0; k <= P; ++k)
+                o << "        tc[(hyx_u64)" << (off + (uint64_t)e * (P + 1) + k) << " * tcs] = hyx_evg[" << e << "][" << k
+                  << "];\n";
+        o << "        if constexpr (!WTC) {\n";
+        for (int oo = 0; oo < own; ++oo) {
+            o << "            if (tab.vo[" << oo << "] >= 0)\n";
+            o << "                for (int k = 0; k <= HYX_P; ++k) tc[((hyx_u64)tab.vo[" << oo << "] * " << (P + 1)
+              << " + k) * tcs] = fl[HYX_XB + k * HYX_NX + " << vcol(oo) << "];\n";
+        }
+        o << "        }\n    }\n";
+        o << "    hyx_mx = mx;\n";
+    }
+    o << "    int nf = 0;\n";
+    for (int oo = 0; oo < own; ++oo) {
+        if (!high_acc) {
+            o << "    { double r = " << xv(oo, P) << ";\n";
+            o << "#pragma unroll\n      for (int k = HYX_P - 1; k >= 0; --k) r = __builtin_fma(r, h, fl[HYX_XB + k * HYX_NX + "
+              << vcol(oo) << "]);\n";
+        } else {
+            o << "    { double r = " << xv(oo, 0) << ", cp = 0.0, hk = 1.0;\n";
+            o << "#pragma unroll\n      for (int k = 1; k <= HYX_P; ++k) { hk *= h; const double y = fl[HYX_XB + k * HYX_NX + "
+              << vcol(oo) << "] * hk - cp; const double tt = r + y; cp = (tt - r) - y; r = tt; }\n";
+        }
+        o << "      xo[" << oo << "] = r; nf |= (tab.vo[" << oo << "] >= 0 && !isfinite(r)) ? 1 : 0; }\n";
+    }
+    o << "#pragma unroll\n    for (int off = 1; off < HYX_G; off <<= 1) nf |= __shfl_xor(nf, off);\n";
+    o << "    h_out = h;\n    nf_out = nf != 0;\n}\n";
+    return h.str() + o.str();
 }
 
 } // namespace hyx

@@ -217,7 +217,7 @@ struct hyx_sys {
     uint32_t order = 0;
     uint32_t flags = 0;
     bool compact = false;
-    int form = 0;                 // HYX_FORM_UNROLLED / COMPACT / GROUP
+    int form = 0;                 // HYX_FORM_UNROLLED / COMPACT / GROUP / WAVE
     uint32_t inst_per_block = 64; // instances handled by one 64-lane block
     int sub = 1;                  // lanes per instance (lane forms)
     std::string src;
@@ -340,18 +340,44 @@ int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint3
         else if (flags & HYX_FLAG_FORCE_GROUP) {
             if (!hyx::plan_groups(s.S, gp, why)) return fail(HYX_EINVAL, "the group form does not apply: " + why);
             s.form = HYX_FORM_GROUP;
-        } else if (lane_compact && hyx::plan_groups(s.S, gp, why))
+        } else if (flags & HYX_FLAG_FORCE_WAVE)
+            s.form = HYX_FORM_WAVE;
+        else if (lane_compact && hyx::plan_groups(s.S, gp, why))
             s.form = HYX_FORM_GROUP;
         else
             s.form = lane_compact ? HYX_FORM_COMPACT : HYX_FORM_UNROLLED;
         s.compact = s.form == HYX_FORM_COMPACT;
         const int split = (flags & HYX_FLAG_SPLIT_CHAINS) ? 8 : 0;
-        if (s.form == HYX_FORM_GROUP) {
-            s.inst_per_block = 64u / (uint32_t)gp.G;
-            if (s.has_events && s.ev.dir.size() > 64) return fail(HYX_EINVAL, "at most 64 events are supported");
+        if (s.has_events && s.ev.dir.size() > 64) return fail(HYX_EINVAL, "at most 64 events are supported");
+        std::string wave_src;
+        if (s.form == HYX_FORM_WAVE) {
+            const int G = (flags & HYX_FLAG_WAVE16) ? 16 : (flags & HYX_FLAG_WAVE32) ? 32 : 64;
+            wave_src = hyx::generate_wave_prelude(s.S, (int)order, ha, G, s.has_events ? &s.E : nullptr,
+                                                  (uint32_t)s.ev.dir.size(), why);
+            if (wave_src.empty()) return fail(HYX_EINVAL, "the wave form does not apply: " + why);
+            s.inst_per_block = 64u / (uint32_t)G;
+        } else if (s.form == HYX_FORM_COMPACT && !(flags & HYX_FLAG_FORCE_COMPACT)) {
+            // Systems too large for the register forms: the wave form when its LDS rows fit,
+            // 32 lanes per instance (measured on the J2 + variational ensemble: 3.6x the
+            // compact form's 2-lane tape in device memory; 64 lanes: 2.5x), else 64.
+            for (int G : {32, 64}) {
+                wave_src = hyx::generate_wave_prelude(s.S, (int)order, ha, G, s.has_events ? &s.E : nullptr,
+                                                      (uint32_t)s.ev.dir.size(), why);
+                if (!wave_src.empty()) {
+                    s.form = HYX_FORM_WAVE;
+                    s.compact = false;
+                    s.inst_per_block = 64u / (uint32_t)G;
+                    break;
+                }
+            }
+        }
+        if (s.form == HYX_FORM_GROUP || s.form == HYX_FORM_WAVE) {
+            if (s.form == HYX_FORM_GROUP) s.inst_per_block = 64u / (uint32_t)gp.G;
             s.src = (s.has_events ? hyx::events_forward_decl() : std::string()) +
-                    hyx::generate_group_prelude(s.S, gp, (int)order, ha, split, !(flags & HYX_FLAG_NO_PIPELINE),
-                                                s.has_events ? &s.E : nullptr, (uint32_t)s.ev.dir.size()) +
+                    (s.form == HYX_FORM_WAVE
+                         ? wave_src
+                         : hyx::generate_group_prelude(s.S, gp, (int)order, ha, split, !(flags & HYX_FLAG_NO_PIPELINE),
+                                                       s.has_events ? &s.E : nullptr, (uint32_t)s.ev.dir.size())) +
                     "\n";
             if (s.has_events)
                 s.src += hyx::generate_events_prelude(s.E, s.S.n, (int)order, s.ev, ha) + "\n" + k_hyx_kcommon + "\n" +
@@ -451,6 +477,9 @@ hipError_t clear64(hyx_batch *b, uint64_t *p, unsigned n)
 }
 
 uint64_t blocks_for(uint64_t N) { return (N + 63) / 64; }
+
+// The group and wave forms run the group kernels (G lanes per instance, LDS rows).
+bool group_kernels(const hyx_sys *s) { return s->form == HYX_FORM_GROUP || s->form == HYX_FORM_WAVE; }
 
 // Blocks of the step / plain / persistent kernels: one block holds 64 instances
 // in the lane form and 64 / G in the group form.
@@ -1209,7 +1238,7 @@ int hyx_propagate_grid(hyx_batch *b, const double *grid, uint64_t n_pts, const d
         }
     }
     if (int rc = ensure_grid_module(b->sys)) return rc;
-    if (b->sys->form != HYX_FORM_GROUP && !b->tc) HIPCHK(hipMalloc((void **)&b->tc, (size_t)n * (P + 1) * N * 8));
+    if (!group_kernels(b->sys) && !b->tc) HIPCHK(hipMalloc((void **)&b->tc, (size_t)n * (P + 1) * N * 8));
     if (int rc = reserve(&b->grid, &b->grid_cap, n_pts * N)) return rc;
     if (int rc = reserve(&b->gout, &b->gout_cap, n_pts * n * N)) return rc;
     if (!b->gcur) HIPCHK(hipMalloc((void **)&b->gcur, N * 4));
@@ -1766,7 +1795,7 @@ int hyx_batch_set_clog(hyx_batch *b, uint32_t cap)
 {
     if (!b) return fail(HYX_EINVAL, "null batch");
     if (cap && b->sys->has_events) return fail(HYX_EINVAL, "the step log does not combine with events");
-    if (cap && b->sys->form == HYX_FORM_GROUP) return fail(HYX_EINVAL, "the group form has no step log");
+    if (cap && group_kernels(b->sys)) return fail(HYX_EINVAL, "the group and wave forms have no step log");
     void *ptrs[] = {b->clog, b->clog_t, b->clog_n};
     for (void *p : ptrs)
         if (p) HIPCHK(hipFree(p));

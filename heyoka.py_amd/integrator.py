@@ -153,7 +153,7 @@ class taylor_adaptive_batch:
     def __init__(self, sys, state, time=None, pars=None, tol=0.0, high_accuracy=False, compact_mode=False,
                  t_events=None, nt_events=None, parallel_mode=False, **kwargs):
         gpu_mode = kwargs.pop("gpu_mode", "persistent")
-        code_form = kwargs.pop("code_form", None)  # None (automatic), "compact" or "unrolled"
+        code_form = kwargs.pop("code_form", None)  # None (automatic), "compact", "unrolled", "group", "wave[32|16]"
         # LLVM-specific options of the reference are accepted and ignored.
         for k in list(kwargs):
             if k in ("opt_level", "force_avx512", "slp_vectorize", "fast_math", "code_model", "parjit", "fp_type"):
@@ -238,7 +238,8 @@ class taylor_adaptive_batch:
         if tol == 0.0:
             tol = float(np.finfo(np.float64).eps)
         order = order_from_tol(tol)
-        compact = {"compact": True, "unrolled": False, "group": "group", None: None}[code_form]
+        compact = {"compact": True, "unrolled": False, "group": "group", "wave": "wave", "wave32": "wave32",
+                   "wave16": "wave16", None: None}[code_form]
         tes, ntes = events
         spec = None
         if tes or ntes:
@@ -465,7 +466,7 @@ class taylor_adaptive_batch:
         B = self._hyx_B
         t0 = self._hyx_thi + self._hyx_tlo
         n = self._hyx_state.shape[0]
-        if cb is None and not self.with_events and self._hyx_system.form != "group":
+        if cb is None and not self.with_events and self._hyx_system.form not in ("group", "wave"):
             return self._prop_c_output_device(tf_hi, tf_lo, max_steps, mdt, t0), None
         rec = [([], [], [], []) for _ in range(B)]
 

@@ -66,11 +66,17 @@ typedef struct {
 #define HYX_FLAG_SUB1 512u
 #define HYX_FLAG_SUB4 1024u
 #define HYX_FLAG_SUB8 2048u
+/* Wave form: the nodes spread over the lanes of a group (64 lanes per instance by default), every
+ * coefficient row in LDS. HYX_FLAG_WAVE32 / HYX_FLAG_WAVE16: 32 or 16 lanes per instance. */
+#define HYX_FLAG_FORCE_WAVE 4096u
+#define HYX_FLAG_WAVE32 8192u
+#define HYX_FLAG_WAVE16 16384u
 
 /* Code forms (hyx_sys_code_form). */
 #define HYX_FORM_UNROLLED 0 /* one instance per lane, fully unrolled, histories in registers */
 #define HYX_FORM_COMPACT 1  /* one instance per lane, loop over orders, Taylor tape in device memory */
 #define HYX_FORM_GROUP 2    /* G lanes per instance, one cluster per lane, histories in registers, LDS exchange */
+#define HYX_FORM_WAVE 3     /* G lanes per instance, nodes spread over the lanes, coefficient rows in LDS */
 
 /* Fields for hyx_batch_write / hyx_batch_read / hyx_batch_device_ptr. */
 #define HYX_FIELD_STATE 0   /* [n][N] */
@@ -101,7 +107,7 @@ int hyx_sys_dim(const hyx_sys *s);
 int hyx_sys_npar(const hyx_sys *s);
 /* 1 if the compact code form (tape in device memory) was generated, 0 otherwise. */
 int hyx_sys_is_compact(const hyx_sys *s);
-/* The generated code form: HYX_FORM_UNROLLED, HYX_FORM_COMPACT or HYX_FORM_GROUP. */
+/* The generated code form: HYX_FORM_UNROLLED, HYX_FORM_COMPACT, HYX_FORM_GROUP or HYX_FORM_WAVE. */
 int hyx_sys_code_form(const hyx_sys *s);
 /* The generated HIP source (for inspection); returns its length, copies up to cap-1 bytes. */
 size_t hyx_sys_source(const hyx_sys *s, char *buf, size_t cap);

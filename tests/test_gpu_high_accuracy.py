@@ -12,7 +12,7 @@ from heyoka_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("code_form", ["unrolled", "compact"])
+@pytest.mark.parametrize("code_form", ["unrolled", "compact", "wave"])
 def test_pendulum_high_accuracy(code_form):
     print(f"[gpu] high accuracy pendulum {code_form}", flush=True)
     wl = W.pendulum(64, seed=7)

@@ -181,9 +181,12 @@ def test_group_form_step_tc_and_lane_failure():
     assert all(o == hy.taylor_outcome.time_limit for i, o in enumerate(oc) if i != 2)
 
 
-def test_j2var_short_vs_oracle():
-    print("[gpu] j2var 64 instances t=5", flush=True)
-    _gpu_vs_oracle(W.j2var(64), 64, 5.0, 1e-10)
+@pytest.mark.parametrize("code_form", [None, "compact"])
+def test_j2var_short_vs_oracle(code_form):
+    # Automatic form: the wave form (32 lanes per instance); the compact form's tape in device memory.
+    print(f"[gpu] j2var 64 instances t=5 code_form={code_form}", flush=True)
+    ta, _, _ = _gpu_vs_oracle(W.j2var(64), 64, 5.0, 1e-10, code_form=code_form)
+    assert ta.code_form == (code_form or "wave")
 
 
 def test_modes_bitwise_identical():
