@@ -301,3 +301,59 @@ def test_compact_sublanes_match_oracle(name, sub, tmp_path):
     tco = r["tc"][:, :, 0]
     scale = np.maximum(np.abs(tco), np.max(np.abs(tco), axis=1, keepdims=True) * 1e-3) + 1e-300
     assert np.max(np.abs(tcg - tco) / scale) < 1e-11
+
+
+def _mixed_case():
+    # Every recurrence kind of the wave form: sqrt, div, exp, log, pow, sin / cos, time,
+    # parameters and constants, linear nodes.
+    x, v, y = hy.make_vars("x", "v", "y")
+    sys_ = [(x, v),
+            (v, -hy.sqrt(x * x + 1.5) / (1.0 + y * y) + hy.exp(-x) * hy.par[0] + hy.log(2.0 + y * y) * hy.time),
+            (y, hy.cos(x) * hy.par[1] - (v * v + 0.5) ** 1.5 + hy.time)]
+    return sys_, np.array([[0.3], [0.1], [0.7]]), np.array([[0.4], [1.3]]), 0.5, 1e-15
+
+
+@pytest.mark.parametrize("form", ["wave", "wave32", "wave16"])
+@pytest.mark.parametrize("name", ["pendulum", "cr3bp", "j2var", "mixed"])
+def test_wave_step_matches_oracle(name, form, tmp_path):
+    # The wave form emulated on the CPU with the group harness: one thread per lane of
+    # a group, a barrier at every wavefront-scope fence and __shfl_xor exchange.
+    sys_, st, pars, t0, tol = _mixed_case() if name == "mixed" else _case(name)
+    dec = hy.taylor_decompose(sys_)
+    p = hy.order_from_tol(tol)
+    try:
+        s = _lib.System(dec, p, compact=form, source_only=True)
+    except _lib.HyxError as e:
+        # 16 lanes per instance: 4 instances' rows per wavefront exceed the LDS limit.
+        assert form == "wave16" and name == "j2var" and "64 KB" in str(e)
+        return
+    assert s.form == "wave"
+    prelude = s.source().split("// Hand-written CDNA4 kernels")[0]
+    d = tmp_path / f"{name}_{form}"
+    d.mkdir()
+    (d / "prelude.inc").write_text(prelude)
+    (d / "g.cpp").write_text(GROUP_HARNESS)
+    so = d / "g.so"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-ffp-contract=off", "-shared", "-fPIC", "-pthread", "-o", str(so),
+                    str(d / "g.cpp")], check=True, capture_output=True)
+    L = ct.CDLL(str(so))
+    P = ct.POINTER(ct.c_double)
+    L.gstep_cpu.argtypes = [P, P, ct.c_double, ct.c_double, P, P, ct.POINTER(ct.c_int), P]
+    n = dec.n_state
+    x = np.ascontiguousarray(st[:, 0], dtype=np.float64)
+    pv = np.ascontiguousarray(pars[:, 0] if pars is not None else np.zeros(1), dtype=np.float64)
+    xout = np.zeros(n)
+    h = np.zeros(64)
+    nf = np.zeros(64, dtype=np.int32)
+    tc = np.zeros(n * (p + 1))
+    ptr = lambda a: a.ctypes.data_as(ct.POINTER(ct.c_double))
+    G = L.gstep_cpu(ptr(x), ptr(pv), t0, np.inf, ptr(xout), ptr(h), nf.ctypes.data_as(ct.POINTER(ct.c_int)), ptr(tc))
+    assert G == {"wave": 64, "wave32": 32, "wave16": 16}[form]
+    assert np.all(h[:G] == h[0]) and not nf[:G].any()
+    r = O.step_batch(dec, p, st, pars=pars, t_hi=[t0], want_tc=True)
+    tco = r["tc"][:, :, 0]
+    tcg = tc.reshape(n, p + 1)
+    scale = np.maximum(np.abs(tco), np.max(np.abs(tco), axis=1, keepdims=True) * 1e-3) + 1e-300
+    assert np.max(np.abs(tcg - tco) / scale) < 1e-11
+    assert abs(h[0] - r["h"][0]) <= 1e-13 * abs(r["h"][0])
+    np.testing.assert_allclose(xout, r["state"][:, 0], rtol=1e-13, atol=1e-15)
