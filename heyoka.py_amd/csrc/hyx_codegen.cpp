@@ -1868,9 +1868,10 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
     }
     // Rows with a history: node operands of every recurrence but the linear ones, and
     // the nodes whose recurrence reads their own (or their partner's) history.
+    // A product with a constant or parameter reads its other operand's current order only.
     std::vector<char> hist(NN, 0);
     for (int j = n; j < NN; ++j) {
-        if (kind[j] != WK_LIN)
+        if (kind[j] != WK_LIN && kind[j] != WK_MULS)
             for (auto a : S.args[j])
                 if (is_node(a) && (int)HYX_OPERAND_INDEX(a) >= n) hist[HYX_OPERAND_INDEX(a)] = 1;
         const int k = kind[j];
@@ -1901,8 +1902,8 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
     const int NC = NN + 1 + NS;
     const int flat = CB + NC;
     const int ipb = 64 / G;
-    if ((long)flat * ipb * 8 > 65536) {
-        why = "the coefficient rows exceed 64 KB of LDS per wavefront";
+    if ((long)flat * ipb * 8 > 160 * 1024) {
+        why = "the coefficient rows exceed the 160 KB of LDS of a compute unit";
         return "";
     }
     auto trow = [&](uint32_t w) -> int {
@@ -2002,7 +2003,7 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
     // Readers of each node's cur entry: linear nodes and right-hand sides.
     std::vector<char> cur_read(NN + 1 + NS, 0);
     for (int j = n; j < NN; ++j)
-        if (kind[j] == WK_LIN)
+        if (kind[j] == WK_LIN || kind[j] == WK_MULS)
             for (auto a : S.args[j]) cur_read[cid(a)] = 1;
     for (int i = 0; i < n; ++i) cur_read[cid(S.rhs[i])] = 1;
     for (size_t s = 0; s < slots.size(); ++s) {
@@ -2038,8 +2039,8 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
             break;
         }
         case WK_MULS:
-            // a: the operand with a series (if any); b: the constant or parameter.
-            t.a = addw(lanes([&](uint32_t j) { return trow(is_node(S.args[j][0]) ? S.args[j][0] : S.args[j][1]); }));
+            // a: the cur entry of the operand with a series (if any); b: the constant or parameter.
+            t.a = addw(lanes([&](uint32_t j) { return cid(is_node(S.args[j][0]) ? S.args[j][0] : S.args[j][1]); }));
             t.b = addw(lanes([&](uint32_t j) { return trow(is_node(S.args[j][0]) ? S.args[j][1] : S.args[j][0]); }));
             break;
         case WK_TIME: break;
@@ -2067,18 +2068,93 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
     // Per-slot registers that live across orders (reciprocals of order-0 values).
     auto rv = [](size_t s, const char *nm) { return std::string(nm) + std::to_string(s); };
 
-    auto emit_slot = [&](size_t s, int k) {
+    // Level of each slot; slots of one level are independent of each other.
+    std::vector<int> slvl(slots.size());
+    for (size_t s = 0; s < slots.size(); ++s) slvl[s] = lvl[slots[s].nodes[0]];
+    auto guard = [&](size_t s) {
+        const int w = (int)slots[s].nodes.size();
+        return w < G ? "if (g < " + std::to_string(w) + ") " : std::string();
+    };
+    auto chain = [&](const std::vector<std::pair<std::string, std::string>> &tm, const std::string &nm,
+                     const std::string &ind) {
+        o << ind << nm << " = " << tm[0].first << " * " << tm[0].second << ";\n";
+        for (size_t q = 1; q < tm.size(); ++q)
+            o << ind << nm << " = __builtin_fma(" << tm[q].first << ", " << tm[q].second << ", " << nm << ");\n";
+    };
+    int cur_k = 0; // the order being generated (names of the per-order partial sums)
+    auto ko = [&](size_t s) { return "ko" + std::to_string(s) + "_" + std::to_string(cur_k); };
+    auto kq = [&](size_t s) { return "kq" + std::to_string(s) + "_" + std::to_string(cur_k); };
+    // The part of a recurrence at order k >= 1 that reads orders below k only (no
+    // same-order dependency): emitted one level ahead of the slot, so its LDS reads
+    // overlap the previous level's exchange. Results in ko<s> (and kq<s>).
+    // Returns whether ko<s> holds a value.
+    std::vector<char> has_old(slots.size(), 0);
+    auto emit_old = [&](size_t s, int k) {
         const auto &sl = slots[s];
         const auto &t = st[s];
-        const int w = (int)sl.nodes.size();
         const std::string a = t.a >= 0 ? wv(t.a) : "", b = t.b >= 0 ? wv(t.b) : "";
-        o << "    " << (w < G ? "if (g < " + std::to_string(w) + ") " : "") << "{\n";
-        auto chain = [&](const std::vector<std::pair<std::string, std::string>> &tm, const std::string &nm) {
-            o << "        double " << nm << " = " << tm[0].first << " * " << tm[0].second << ";\n";
-            for (size_t q = 1; q < tm.size(); ++q)
-                o << "        " << nm << " = __builtin_fma(" << tm[q].first << ", " << tm[q].second << ", " << nm << ");\n";
-        };
+        const std::string c = t.orow >= 0 ? wv(t.orow) : "";
         std::vector<std::pair<std::string, std::string>> tm, tm2;
+        has_old[s] = 0;
+        switch (sl.kind) {
+        case WK_MUL:
+            for (int q = 1; q < k; ++q) tm.push_back({Tk(a, q), Tk(b, k - q)});
+            break;
+        case WK_SQ:
+            for (int q = 1; 2 * q < k; ++q) tm.push_back({Tk(a, q), Tk(a, k - q)});
+            break;
+        case WK_POW:
+            for (int q = 1; q < k; ++q) tm.push_back({Tk(a, k - q), Tk(c, q)});
+            break;
+        case WK_DIV:
+            for (int q = 1; q < k; ++q) tm.push_back({Tk(b, q), Tk(c, k - q)});
+            break;
+        case WK_SQRT:
+            for (int q = 1; 2 * q < k; ++q) tm.push_back({Tk(c, q), Tk(c, k - q)});
+            break;
+        case WK_EXP:
+        case WK_TRIG:
+            for (int q = 1; q < k; ++q)
+                tm.push_back({q == 1 ? Tk(a, 1) : "(" + lit((double)q) + " * " + Tk(a, q) + ")",
+                              Tk(sl.kind == WK_EXP ? c : b, k - q)});
+            break;
+        case WK_LOG:
+            for (int q = 1; q < k; ++q) tm.push_back({lit((double)q / k), "(" + Tk(c, q) + " * " + Tk(a, k - q) + ")"});
+            break;
+        default: break;
+        }
+        const bool sqrt_mid = sl.kind == WK_SQRT && k % 2 == 0;
+        if (tm.empty() && !sqrt_mid) return;
+        has_old[s] = 1;
+        o << "    double " << ko(s) << " = 0.0";
+        if (sl.kind == WK_POW && k >= 2) o << ", " << kq(s) << " = 0.0";
+        o << ";\n    " << guard(s) << "{\n";
+        if (!tm.empty()) chain(tm, ko(s), "        ");
+        if (sl.kind == WK_POW && k >= 2) {
+            // S2 = sum_{q=1}^{k-1} a^[k-q] (q c^[q]).
+            for (int q = 1; q < k; ++q) tm2.push_back({Tk(a, k - q), q == 1 ? Tk(c, 1) : "(" + lit((double)q) + " * " + Tk(c, q) + ")"});
+            chain(tm2, kq(s), "        ");
+        }
+        if (sl.kind == WK_SQRT) {
+            // tot = 2 sum c^[q] c^[k-q] (+ c^[k/2]^2 for even k).
+            std::string tot = tm.empty() ? "" : ko(s) + " + " + ko(s);
+            if (sqrt_mid) {
+                const std::string m = Tk(c, k / 2);
+                tot = tot.empty() ? (m + " * " + m) : ("__builtin_fma(" + m + ", " + m + ", " + tot + ")");
+            }
+            o << "        " << ko(s) << " = " << tot << ";\n";
+        }
+        o << "    }\n";
+    };
+    // The rest of the recurrence (the same-order terms) and the stores.
+    auto emit_new = [&](size_t s, int k) {
+        const auto &sl = slots[s];
+        const auto &t = st[s];
+        const std::string a = t.a >= 0 ? wv(t.a) : "", b = t.b >= 0 ? wv(t.b) : "";
+        const std::string c = t.orow >= 0 ? wv(t.orow) : "";
+        const bool old = has_old[s] != 0;
+        const std::string O = ko(s);
+        o << "    " << guard(s) << "{\n";
         switch (sl.kind) {
         case WK_LIN: {
             o << "        const int sm = " << wv(t.sgn) << ";\n";
@@ -2093,128 +2169,84 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
         case WK_MUL:
             if (k == 0)
                 o << "        const double v = " << Tk(a, 0) << " * " << Tk(b, 0) << ";\n";
-            else {
-                for (int q = 0; q <= k; ++q) tm.push_back({Tk(a, q), Tk(b, k - q)});
-                chain(tm, "v");
-            }
+            else
+                o << "        const double v = __builtin_fma(" << Tk(a, k) << ", " << Tk(b, 0) << ", "
+                  << (old ? "__builtin_fma(" + Tk(a, 0) + ", " + Tk(b, k) + ", " + O + ")" : Tk(a, 0) + " * " + Tk(b, k))
+                  << ");\n";
             break;
-        case WK_MULS: o << "        const double v = " << Tk(a, k) << " * " << Tk(b, 0) << ";\n"; break;
+        case WK_MULS: o << "        const double v = " << cur(a) << " * " << Tk(b, 0) << ";\n"; break;
         case WK_SQ:
             if (k == 0)
                 o << "        const double v = " << Tk(a, 0) << " * " << Tk(a, 0) << ";\n";
             else {
-                for (int q = 0; 2 * q < k; ++q) tm.push_back({Tk(a, q), Tk(a, k - q)});
-                chain(tm, "acc");
+                o << "        const double acc = "
+                  << (old ? "__builtin_fma(" + Tk(a, 0) + ", " + Tk(a, k) + ", " + O + ")" : Tk(a, 0) + " * " + Tk(a, k))
+                  << ";\n";
                 if (k % 2)
                     o << "        const double v = acc + acc;\n";
                 else
                     o << "        const double v = __builtin_fma(" << Tk(a, k / 2) << ", " << Tk(a, k / 2) << ", acc + acc);\n";
             }
             break;
-        case WK_POW: {
-            const std::string c = wv(t.orow);
+        case WK_POW:
             if (k == 0) {
                 o << "        const double a0 = " << Tk(a, 0) << ";\n        const double v = pow(a0, " << dv(t.al) << ");\n";
                 o << "        const double ra = 1.0 / a0;\n        " << rv(s, "raa") << " = ra * " << dv(t.al) << ";\n        "
                   << rv(s, "rab") << " = ra * " << dv(t.al1) << ";\n";
             } else {
-                for (int q = 0; q < k; ++q) tm.push_back({Tk(a, k - q), Tk(c, q)});
-                chain(tm, "s1");
+                o << "        const double s1 = "
+                  << (old ? "__builtin_fma(" + Tk(a, k) + ", " + Tk(c, 0) + ", " + O + ")" : Tk(a, k) + " * " + Tk(c, 0))
+                  << ";\n";
                 if (k == 1)
                     o << "        const double v = s1 * " << rv(s, "raa") << ";\n";
-                else {
-                    for (int q = 1; q < k; ++q) {
-                        o << "        const double z" << q << " = " << lit((double)q) << " * " << Tk(c, q) << ";\n";
-                        tm2.push_back({Tk(a, k - q), "z" + std::to_string(q)});
-                    }
-                    chain(tm2, "s2");
-                    o << "        const double v = __builtin_fma(s1, " << rv(s, "raa") << ", -(s2 * " << rv(s, "rab") << ") * "
-                      << lit(1.0 / k) << ");\n";
-                }
+                else
+                    o << "        const double v = __builtin_fma(s1, " << rv(s, "raa") << ", -(" << kq(s) << " * " << rv(s, "rab")
+                      << ") * " << lit(1.0 / k) << ");\n";
             }
             break;
-        }
-        case WK_DIV: {
-            const std::string c = wv(t.orow);
-            if (k == 0) {
+        case WK_DIV:
+            if (k == 0)
                 o << "        const double v = " << Tk(a, 0) << " / " << Tk(b, 0) << ";\n        " << rv(s, "rb") << " = 1.0 / "
                   << Tk(b, 0) << ";\n";
-            } else {
-                for (int q = 1; q <= k; ++q) tm.push_back({Tk(b, q), Tk(c, k - q)});
-                chain(tm, "acc");
-                o << "        const double v = (" << Tk(a, k) << " - acc) * " << rv(s, "rb") << ";\n";
-            }
+            else
+                o << "        const double v = (" << Tk(a, k) << " - "
+                  << (old ? "__builtin_fma(" + Tk(b, k) + ", " + Tk(c, 0) + ", " + O + ")" : Tk(b, k) + " * " + Tk(c, 0))
+                  << ") * " << rv(s, "rb") << ";\n";
             break;
-        }
-        case WK_SQRT: {
-            const std::string c = wv(t.orow);
-            if (k == 0) {
-                o << "        const double v = sqrt(" << Tk(a, 0) << ");\n        " << rv(s, "rs") << " = 0.5 / v;\n";
-            } else {
-                for (int q = 1; 2 * q < k; ++q) tm.push_back({Tk(c, q), Tk(c, k - q)});
-                std::string tot;
-                if (!tm.empty()) {
-                    chain(tm, "acc");
-                    tot = "acc + acc";
-                }
-                if (k % 2 == 0) {
-                    const std::string m = Tk(c, k / 2);
-                    tot = tot.empty() ? (m + " * " + m) : ("__builtin_fma(" + m + ", " + m + ", " + tot + ")");
-                }
-                o << "        const double v = (" << Tk(a, k) << (tot.empty() ? "" : " - (" + tot + ")") << ") * "
-                  << rv(s, "rs") << ";\n";
-            }
-            break;
-        }
-        case WK_EXP: {
-            const std::string c = wv(t.orow);
+        case WK_SQRT:
             if (k == 0)
-                o << "        const double v = exp(" << Tk(a, 0) << ");\n";
-            else {
-                for (int q = 1; q <= k; ++q) {
-                    std::string d = Tk(a, q);
-                    if (q > 1) {
-                        o << "        const double d" << q << " = " << lit((double)q) << " * " << d << ";\n";
-                        d = "d" + std::to_string(q);
-                    }
-                    tm.push_back({d, Tk(c, k - q)});
-                }
-                chain(tm, "acc");
-                o << "        const double v = acc * " << lit(1.0 / k) << ";\n";
+                o << "        const double v = sqrt(" << Tk(a, 0) << ");\n        " << rv(s, "rs") << " = 0.5 / v;\n";
+            else
+                o << "        const double v = (" << Tk(a, k) << (old ? " - (" + O + ")" : "") << ") * " << rv(s, "rs") << ";\n";
+            break;
+        case WK_EXP:
+        case WK_TRIG: {
+            const std::string other = sl.kind == WK_EXP ? c : b;
+            if (k == 0) {
+                if (sl.kind == WK_EXP)
+                    o << "        const double v = exp(" << Tk(a, 0) << ");\n";
+                else
+                    o << "        double sn, cs;\n        sincos(" << Tk(a, 0) << ", &sn, &cs);\n        const double v = "
+                      << wv(t.cosf) << " ? cs : sn;\n";
+                break;
             }
+            const std::string d = k == 1 ? Tk(a, 1) : "(" + lit((double)k) + " * " + Tk(a, k) + ")";
+            o << "        const double acc = "
+              << (old ? "__builtin_fma(" + d + ", " + Tk(other, 0) + ", " + O + ")" : d + " * " + Tk(other, 0)) << ";\n";
+            if (sl.kind == WK_EXP)
+                o << "        const double v = acc * " << lit(1.0 / k) << ";\n";
+            else
+                o << "        const double u = acc * " << lit(1.0 / k) << ";\n        const double v = " << wv(t.cosf)
+                  << " ? -u : u;\n";
             break;
         }
-        case WK_LOG: {
-            const std::string c = wv(t.orow);
+        case WK_LOG:
             if (k == 0)
                 o << "        const double v = log(" << Tk(a, 0) << ");\n        " << rv(s, "rl") << " = 1.0 / " << Tk(a, 0)
                   << ";\n";
-            else {
-                for (int q = 1; q < k; ++q) tm.push_back({lit((double)q / k), "(" + Tk(c, q) + " * " + Tk(a, k - q) + ")"});
-                if (!tm.empty()) chain(tm, "acc");
-                o << "        const double v = (" << Tk(a, k) << (tm.empty() ? "" : " - acc") << ") * " << rv(s, "rl") << ";\n";
-            }
+            else
+                o << "        const double v = (" << Tk(a, k) << (old ? " - " + O : "") << ") * " << rv(s, "rl") << ";\n";
             break;
-        }
-        case WK_TRIG: {
-            if (k == 0) {
-                o << "        double sn, cs;\n        sincos(" << Tk(a, 0) << ", &sn, &cs);\n        const double v = "
-                  << wv(t.cosf) << " ? cs : sn;\n";
-            } else {
-                for (int q = 1; q <= k; ++q) {
-                    std::string d = Tk(a, q);
-                    if (q > 1) {
-                        o << "        const double d" << q << " = " << lit((double)q) << " * " << d << ";\n";
-                        d = "d" + std::to_string(q);
-                    }
-                    tm.push_back({d, Tk(b, k - q)});
-                }
-                chain(tm, "acc");
-                o << "        const double u = acc * " << lit(1.0 / k) << ";\n        const double v = " << wv(t.cosf)
-                  << " ? -u : u;\n";
-            }
-            break;
-        }
         case WK_TIME: o << "        const double v = " << (k == 0 ? "t0" : k == 1 ? "1.0" : "0.0") << ";\n"; break;
         }
         if (t.cur_any) o << "        " << cur(wv(t.oc)) << " = v;\n";
@@ -2222,7 +2254,23 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
             o << "        " << Tk(wv(t.orow), k) << " = v;\n";
         else if (t.hist_any)
             o << "        if (" << wv(t.orow) << " >= 0) " << Tk(wv(t.orow), k) << " = v;\n";
-        o << "    }\n    hyx_wsync();\n";
+        o << "    }\n";
+    };
+    // Order k: per level, the order-k-independent parts of the next level's slots, then
+    // this level's slots, then one fence.
+    auto emit_order_slots = [&](int k) {
+        cur_k = k;
+        size_t s0 = 0;
+        int maxlv = 0;
+        for (size_t s = 0; s < slots.size(); ++s) maxlv = std::max(maxlv, slvl[s]);
+        for (int l = 1; l <= maxlv; ++l) {
+            if (k > 0)
+                for (size_t s = 0; s < slots.size(); ++s)
+                    if (slvl[s] == l + 1) emit_old(s, k);
+            for (size_t s = s0; s < slots.size() && slvl[s] == l; ++s) emit_new(s, k);
+            while (s0 < slots.size() && slvl[s0] == l) ++s0;
+            o << "    hyx_wsync();\n";
+        }
     };
 
     // The per-lane table struct and its loader.
@@ -2287,9 +2335,13 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
         o << "    }\n";
     }
     o << "    hyx_wsync();\n";
-    for (size_t s = 0; s < slots.size(); ++s) emit_slot(s, 0);
+    emit_order_slots(0);
     for (int k = 1; k <= P; ++k) {
         o << "    // order " << k << "\n";
+        cur_k = k;
+        if (k < P)
+            for (size_t s = 0; s < slots.size(); ++s)
+                if (slvl[s] == 1) emit_old(s, k);
         for (int oo = 0; oo < own; ++oo) o << "    const double r" << oo << "_" << k << " = " << cur(wv(f_rhs[oo])) << ";\n";
         o << "    hyx_wsync();\n";
         for (int oo = 0; oo < own; ++oo) {
@@ -2303,8 +2355,7 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
             o << "    }\n";
         }
         o << "    hyx_wsync();\n";
-        if (k < P)
-            for (size_t s = 0; s < slots.size(); ++s) emit_slot(s, k);
+        if (k < P) emit_order_slots(k);
     }
     o << "    if constexpr (WTC) if (tc) {\n";
     for (int oo = 0; oo < own; ++oo) {
