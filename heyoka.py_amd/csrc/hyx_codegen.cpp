@@ -1896,11 +1896,19 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
     int R = n + 1;
     for (int j = n; j < NN; ++j)
         if (hist[j]) hrow[j] = R++;
-    const int srow0 = R;
-    R += NS;
-    const int CB = (P + 1) * R;
+    // Constants and parameters: a T row only where a recurrence reads their higher
+    // (zero) orders; products with them read the value slot sv[q].
+    std::vector<int> srow(NS, -1);
+    for (int j = n; j < NN; ++j)
+        if (kind[j] != WK_LIN && kind[j] != WK_MULS)
+            for (auto a : S.args[j])
+                if (!is_node(a) && srow[scal_index(a)] < 0) srow[scal_index(a)] = R++;
+    // T holds orders 0..P-1 of every row and order P of the state rows (and the zero row):
+    // node coefficients are formed up to order P-1.
+    const int CB = P * R + n + 1;
     const int NC = NN + 1 + NS;
-    const int flat = CB + NC;
+    const int SVB = CB + NC;
+    const int flat = SVB + NS;
     const int ipb = 64 / G;
     if ((long)flat * ipb * 8 > 160 * 1024) {
         why = "the coefficient rows exceed the 160 KB of LDS of a compute unit";
@@ -1908,7 +1916,10 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
     }
     auto trow = [&](uint32_t w) -> int {
         const int idx = (int)HYX_OPERAND_INDEX(w);
-        if (!is_node(w)) return srow0 + scal_index(w);
+        if (!is_node(w)) {
+            if (srow[scal_index(w)] < 0) throw std::logic_error("wave form: constant without a row");
+            return srow[scal_index(w)];
+        }
         if (idx < n) return idx;
         if (hrow[idx] < 0) throw std::logic_error("wave form: operand without a history row");
         return hrow[idx];
@@ -1939,6 +1950,10 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
                 s.kind = kd;
                 s.nodes.assign(ns.begin() + q0, ns.begin() + std::min(ns.size(), q0 + G));
                 for (auto j : s.nodes) s.m = std::max(s.m, (int)S.args[j].size());
+                if (kd == WK_LIN && s.m > 15) {
+                    why = "a sum of more than 15 terms";
+                    return "";
+                }
                 slots.push_back(s);
             }
         }
@@ -2041,7 +2056,7 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
         case WK_MULS:
             // a: the cur entry of the operand with a series (if any); b: the constant or parameter.
             t.a = addw(lanes([&](uint32_t j) { return cid(is_node(S.args[j][0]) ? S.args[j][0] : S.args[j][1]); }));
-            t.b = addw(lanes([&](uint32_t j) { return trow(is_node(S.args[j][0]) ? S.args[j][1] : S.args[j][0]); }));
+            t.b = addw(lanes([&](uint32_t j) { return SVB + scal_index(is_node(S.args[j][0]) ? S.args[j][1] : S.args[j][0]); }));
             break;
         case WK_TIME: break;
         default:
@@ -2063,7 +2078,14 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
             break;
         }
     }
-    auto wv = [](int f) { return "tab.w[" + std::to_string(f) + "]"; };
+    // Table fields are signed 16-bit values, two per 32-bit word (fewer registers).
+    for (auto &f : W)
+        for (int v : f)
+            if (v < -32768 || v > 32767) throw std::logic_error("wave form: table value out of 16-bit range");
+    auto wv = [](int f) {
+        const std::string w = "tab.w[" + std::to_string(f / 2) + "]";
+        return f % 2 ? "(" + w + " >> 16)" : "((int)((unsigned)" + w + " << 16) >> 16)";
+    };
     auto dv = [](int f) { return "tab.d[" + std::to_string(f) + "]"; };
     // Per-slot registers that live across orders (reciprocals of order-0 values).
     auto rv = [](size_t s, const char *nm) { return std::string(nm) + std::to_string(s); };
@@ -2174,7 +2196,7 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
                   << (old ? "__builtin_fma(" + Tk(a, 0) + ", " + Tk(b, k) + ", " + O + ")" : Tk(a, 0) + " * " + Tk(b, k))
                   << ");\n";
             break;
-        case WK_MULS: o << "        const double v = " << cur(a) << " * " << Tk(b, 0) << ";\n"; break;
+        case WK_MULS: o << "        const double v = " << cur(a) << " * fl[" << b << "];\n"; break;
         case WK_SQ:
             if (k == 0)
                 o << "        const double v = " << Tk(a, 0) << " * " << Tk(a, 0) << ";\n";
@@ -2274,11 +2296,17 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
     };
 
     // The per-lane table struct and its loader.
-    h << "struct hyx_gtab {\n    int vo[HYX_OWN];\n    int w[" << std::max<size_t>(W.size(), 1) << "];\n    double d["
+    const size_t NWP = (W.size() + 1) / 2;
+    h << "struct hyx_gtab {\n    int vo[HYX_OWN];\n    int w[" << std::max<size_t>(NWP, 1) << "];\n    double d["
       << std::max<size_t>(D.size(), 1) << "];\n};\n";
     {
         std::vector<int> wf;
-        for (auto &f : W) wf.insert(wf.end(), f.begin(), f.end());
+        for (size_t p2 = 0; p2 < NWP; ++p2)
+            for (int g = 0; g < G; ++g) {
+                const unsigned lo = (unsigned)W[2 * p2][g] & 0xffffu;
+                const unsigned hi = 2 * p2 + 1 < W.size() ? ((unsigned)W[2 * p2 + 1][g] & 0xffffu) : 0u;
+                wf.push_back((int)(lo | (hi << 16)));
+            }
         if (wf.empty()) wf.push_back(0);
         h << "__constant__ int hyx_wtab_w[" << wf.size() << "] = {";
         for (size_t q = 0; q < wf.size(); ++q) h << (q ? "," : "") << wf[q];
@@ -2295,7 +2323,7 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
     }
     h << "__device__ __forceinline__ void hyx_gtab_load(hyx_gtab &t, int g)\n{\n"
          "    for (int q = 0; q < HYX_OWN; ++q) t.vo[q] = hyx_wtab_vo[g * HYX_OWN + q];\n";
-    if (!W.empty()) h << "    for (int f = 0; f < " << W.size() << "; ++f) t.w[f] = hyx_wtab_w[f * HYX_G + g];\n";
+    if (NWP) h << "    for (int f = 0; f < " << NWP << "; ++f) t.w[f] = hyx_wtab_w[f * HYX_G + g];\n";
     if (!D.empty()) h << "    for (int f = 0; f < " << D.size() << "; ++f) t.d[f] = hyx_wtab_d[f * HYX_G + g];\n";
     h << "}\n";
 
@@ -2329,8 +2357,9 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
             const uint32_t w = scal[q];
             const std::string val = HYX_OPERAND_KIND(w) == HYX_KIND_CONST ? lit(S.consts[HYX_OPERAND_INDEX(w)])
                                                                            : "pv[" + std::to_string(HYX_OPERAND_INDEX(w)) + "]";
-            o << "        fl[" << (srow0 + q) << "] = " << val << ";\n";
+            if (srow[q] >= 0) o << "        fl[" << srow[q] << "] = " << val << ";\n";
             o << "        fl[" << (CB + NN + 1 + q) << "] = " << val << ";\n";
+            o << "        fl[" << (SVB + q) << "] = " << val << ";\n";
         }
         o << "    }\n";
     }
