@@ -355,7 +355,10 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
                 }
             }
         }
-        if (!__any(active)) break;
+        // Leave only once the queue is exhausted: a wave whose claimed positions all hold
+        // finished (or unarmed) instances claims again; every claim advances the shared
+        // counter, so this ends after at most N claims.
+        if (!__any(active) && exhausted) break;
         if (active) {
             hyx_i64 res = HYX_TIME_LIMIT;
             const bool done = hyx_prop_iter(s, a, gslot, inst, &res);

@@ -10,7 +10,9 @@ import numpy as np
 import pytest
 
 import heyoka_amd as hy
+from heyoka_amd import _lib
 from heyoka_amd import workloads as W
+from heyoka_amd.integrator import get_system
 from heyoka_amd.serving import Server
 
 pytestmark = pytest.mark.gpu
@@ -110,3 +112,32 @@ def test_eviction_under_hbm_pressure_continues_bitwise():
         assert np.array_equal(ta.state, h.result_state)
         assert [int(p[3]) for p in ta.propagate_res] == h.result["nsteps"].tolist()
         assert [int(p[0]) for p in ta.propagate_res] == h.result["outcome"].tolist()
+
+
+@pytest.mark.parametrize("form", ["wave", "compact", "group"])
+def test_armed_slots_beyond_the_first_claims(form):
+    # A pool whose armed slots all lie beyond the positions the resident waves claim
+    # first (the lower slots are free): one epoch must still run every armed slot to its
+    # final time, as propagate_until does.
+    print(f"[gpu] serving: armed slots high in the pool, form={form}", flush=True)
+    wl = W.j2var(40) if form != "group" else W.nbody6(40)
+    sys_ = wl.sys.sys if hasattr(wl.sys, "n_orig_sv") else wl.sys
+    dec = hy.taylor_decompose(sys_)
+    system = get_system(dec, 19, compact=form if form != "compact" else True)
+    assert system.form == form
+    cap = 16384
+    b = _lib.Batch(system, cap)
+    b.slots_reset()
+    slots = np.arange(cap - 40, cap, dtype=np.uint32)
+    b.slots_write(_lib.FIELD_STATE, slots, wl.state)
+    b.slots_write(_lib.FIELD_TIME_HI, slots, np.zeros((1, 40)))
+    b.slots_write(_lib.FIELD_TIME_LO, slots, np.zeros((1, 40)))
+    b.slots_arm(slots, np.full(40, 3.0))
+    b.slots_launch(100000)
+    assert b.slots_wait() == 0
+    done, res = b.slots_results(slots)
+    assert done.all() and np.all(res["outcome"] == int(hy.taylor_outcome.time_limit))
+    ta = hy.taylor_adaptive_batch(wl.sys, wl.state, tol=1e-15, code_form=form)
+    ta.propagate_until(3.0)
+    assert np.array_equal(b.slots_read(_lib.FIELD_STATE, slots, dec.n_state), ta.state)
+    assert res["nsteps"].tolist() == [int(p[3]) for p in ta.propagate_res]
