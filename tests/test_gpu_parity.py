@@ -248,6 +248,31 @@ def test_resume_after_budget_is_bitwise():
     assert np.array_equal(a.state, c.state) and c.propagate_res == a.propagate_res
 
 
+@pytest.mark.parametrize("code_form", ["unrolled", "wave"])
+def test_budget_relaunch_reaches_instances_past_the_first_claims(code_form):
+    # After the first launch of a budget-limited persistent propagation, only the
+    # instances at the end of the queue are unfinished, beyond every position the
+    # resident waves claim first: the relaunches must still carry them to their final
+    # time (every outcome time_limit at t = tf, states equal to the plain form's).
+    print(f"[gpu] budget relaunch past the first claim round, code_form={code_form}", flush=True)
+    N = 200_000 if code_form == "unrolled" else 40_000
+    rng = np.random.default_rng(2)
+    ic = np.stack([rng.uniform(-1.0, 1.0, N), rng.uniform(-0.5, 0.5, N)])
+    tf = np.full(N, 0.5)
+    tf[-300:] = 40.0
+    runs = []
+    for mode in ("persistent", "plain"):
+        ta = hy.taylor_adaptive_batch(_pend_sys(), ic, code_form=code_form, gpu_mode=mode)
+        if mode == "persistent":
+            ta._hyx_batch.set_iter_budget(64)
+        ta.propagate_until(tf)
+        runs.append(ta)
+    a, b = runs
+    assert np.array_equal(a.time, tf)
+    assert all(p[0] == hy.taylor_outcome.time_limit for p in a.propagate_res)
+    assert np.array_equal(a.state, b.state) and a.propagate_res == b.propagate_res
+
+
 def test_copy_pickle_continuation_bitwise():
     # _test_batch_integrator.py:602-648: a pickled copy continues bit for bit.
     import pickle
