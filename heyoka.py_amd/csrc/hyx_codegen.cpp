@@ -1212,12 +1212,15 @@ bool plan_groups(const Sys &S, GroupPlan &gp, std::string &why)
         for (size_t q = 0; q < B[c].kconsts.size(); ++q) gp.kc[(size_t)c * gp.nkc + q] = S.consts[B[c].kconsts[q]];
         for (size_t q = B[c].kconsts.size(); q < gp.tmpl.nconsts; ++q) gp.kc[(size_t)c * gp.nkc + q] = gp.tmpl.consts[q];
     }
-    // Output q of lane c lives at LDS index q * G + c (the lanes of a group write one
-    // output row with consecutive 8-byte words: no bank conflicts); index G * nout
-    // reads zero.
+    // Output q of lane c lives at LDS index q * (G + 1) + c: the lanes of a group write
+    // one output row with consecutive 8-byte words, and the odd row stride puts the
+    // outputs of one cluster in different banks, so owners summing several outputs of
+    // the same cluster in one load do not collide. Index (G + 1) * nout reads zero.
+    const int OS = G + 1;
+    gp.os = OS;
     std::vector<int> outslot(NN, -1);
     for (int c = 0; c < ncl; ++c)
-        for (size_t q = 0; q < B[c].outs.size(); ++q) outslot[B[c].outs[q]] = (int)q * G + c;
+        for (size_t q = 0; q < B[c].outs.size(); ++q) outslot[B[c].outs[q]] = (int)q * OS + c;
     // Ownership: variables sorted by (output terms, state terms), dealt to lanes slot
     // by slot, so that each slot's term counts are as uniform as possible.
     std::vector<uint32_t> order(n);
@@ -1246,7 +1249,7 @@ bool plan_groups(const Sys &S, GroupPlan &gp, std::string &why)
     gp.tso.assign(own, {});
     gp.tss.assign(own, {});
     for (int o = 0; o < own; ++o) {
-        gp.tio[o].assign((size_t)G * gp.nto[o], G * gp.nout);
+        gp.tio[o].assign((size_t)G * gp.nto[o], OS * gp.nout);
         gp.tso[o].assign((size_t)G * gp.nto[o], 1.0);
         gp.tis[o].assign((size_t)G * gp.nts[o], (int)n);
         gp.tss[o].assign((size_t)G * gp.nts[o], 1.0);
@@ -1281,7 +1284,8 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
 {
     const int G = gp.G, own = gp.own, nin = gp.nin, nout = gp.nout;
     const int n = (int)S.n, nx = n + 1;
-    const int xb = G * std::max(nout, 1) + 1; // LDS: outputs, zero slot, then x[P+1][n+1]
+    const int OS = gp.os; // output row stride
+    const int xb = OS * std::max(nout, 1) + 1; // LDS: outputs, zero slot, then x[P+1][n+1]
     const int flat = xb + (P + 1) * nx;
     std::ostringstream h;
     h << "#define HYX_N " << n << "\n#define HYX_NPAR " << std::max<uint32_t>(S.npar, 1) << "\n#define HYX_P " << P
@@ -1332,7 +1336,7 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
             }
             const int g2 = owner[v].first, oo = owner[v].second;
             for (int t = 0; t < gp.nto[oo]; ++t)
-                if (gp.tio[oo][(size_t)g2 * gp.nto[oo] + t] != G * nout) lagged = false;
+                if (gp.tio[oo][(size_t)g2 * gp.nto[oo] + t] != OS * nout) lagged = false;
             nls = std::max(nls, gp.nts[oo]);
         }
     if (nls == 0) lagged = false;
@@ -1469,7 +1473,7 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
         }
         for (int q = 0; q < nout; ++q) {
             const uint32_t w = HYX_OPERAND(HYX_KIND_NODE, gp.outs[q]);
-            o << "    fl[" << q * G << " + g] = " << (gen.zero(w, k) ? "0.0" : gen.val(w, k)) << ";\n";
+            o << "    fl[" << q * OS << " + g] = " << (gen.zero(w, k) ? "0.0" : gen.val(w, k)) << ";\n";
         }
     };
     if (lagged) {

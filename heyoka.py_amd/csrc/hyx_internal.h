@@ -40,6 +40,7 @@ bool choose_compact(const Sys &S, int order);
 // per instance, with per-lane bindings (see plan_groups in hyx_codegen.cpp).
 struct GroupPlan {
     int G = 0, ncl = 0, nin = 0, nout = 0, nkc = 1, own = 0;
+    int os = 0;                     // LDS row stride of the cluster outputs (G + 1)
     Sys tmpl;                       // cluster template: inputs are its state variables 0..nin-1
     std::vector<char> perlane;      // template constant held per lane
     std::vector<uint32_t> outs;     // template nodes feeding right-hand sides, by output slot
@@ -48,7 +49,7 @@ struct GroupPlan {
     // Owned state variables: lane g, slot o owns vo[g][o] (-1: none). Its right-hand
     // side is rc + sum of output terms (LDS output index, sign) + sum of state terms
     // (variable index, sign); per slot, nto[o] / nts[o] terms (padded: output index
-    // G * nout and variable index n read zeros).
+    // os * nout and variable index n read zeros).
     std::vector<int> vo;            // [G][own]
     std::vector<double> rc;         // [G][own]
     std::vector<int> nto, nts;      // [own]
