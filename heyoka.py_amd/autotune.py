@@ -3,8 +3,10 @@
 Searches, for one system on the current GPU:
 - waves per CU of the persistent launch (occupancy, i.e. instances in flight),
 - the refill threshold (idle lanes that trigger a work-stealing claim),
-- optionally the code form (unrolled / compact / group), which changes the
-  arithmetic order and therefore results at the ulp level.
+- optionally the code form (unrolled / compact / group / wave), which changes the
+  arithmetic order and therefore results at the ulp level; for the wave form also
+  the instances per wavefront ("wave16": 4, "wave32": 2, "wave": 1).
+  `wave_forms(dec, order)` lists the candidates a system admits.
 
 Launch parameters never change an instance's arithmetic, so every candidate of
 a given code form gives bit-identical results (tests/test_gpu_autotune.py).
@@ -36,6 +38,23 @@ def _key(dec, order, form):
     words, consts = _lib.oplist_args(dec)
     h = hashlib.sha1(words.tobytes() + consts.tobytes()).hexdigest()[:16]
     return f"{h}-p{order}-{form}-gfx950"
+
+
+def _label(form, system):
+    """The candidate's name: the wave form's lanes per instance are part of it."""
+    return form if isinstance(form, str) and form.startswith("wave") else system.form
+
+
+def wave_forms(dec, order):
+    """The wave-form variants (instances per wavefront 4, 2, 1) whose LDS rows fit."""
+    out = []
+    for f in ("wave16", "wave32", "wave"):
+        try:
+            get_system(dec, order, compact=f)
+            out.append(f)
+        except _lib.HyxError:
+            pass
+    return out
 
 
 def _load():
@@ -85,9 +104,10 @@ def autotune(dec, order, state, pars=None, t_final=1.0, forms=(None,), waves=(0,
     tf = np.broadcast_to(np.asarray(t_final, dtype=float), (state.shape[1],)).copy()
     for form in forms:
         system = get_system(dec, order, compact=form)
-        key = _key(dec, order, system.form)
+        label = _label(form, system)
+        key = _key(dec, order, label)
         if use_cache and key in db:
-            cand = dict(db[key], form=system.form)
+            cand = dict(db[key], form=label)
             table.append(cand)
             if best is None or cand["rate"] > best["rate"]:
                 best = cand
@@ -96,7 +116,7 @@ def autotune(dec, order, state, pars=None, t_final=1.0, forms=(None,), waves=(0,
         for w in waves:
             for rf in refills:
                 rate, _ = measure(system, state, pars, tf, w, rf)
-                row = {"form": system.form, "waves_per_cu": w, "refill_min": rf, "rate": rate}
+                row = {"form": label, "waves_per_cu": w, "refill_min": rf, "rate": rate}
                 table.append(row)
                 if verbose:
                     print(f"[autotune] {row}", flush=True)
@@ -113,5 +133,5 @@ def autotune(dec, order, state, pars=None, t_final=1.0, forms=(None,), waves=(0,
 def tuned_config(dec, order, form=None):
     """The cached (waves_per_cu, refill_min) for a system, or the defaults (0, 1)."""
     system = get_system(dec, order, compact=form)
-    rec = _load().get(_key(dec, order, system.form))
+    rec = _load().get(_key(dec, order, _label(form, system)))
     return (rec["waves_per_cu"], rec["refill_min"]) if rec else (0, 1)
