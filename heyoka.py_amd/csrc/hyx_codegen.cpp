@@ -1810,19 +1810,24 @@ std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compac
 // gets the level 1 + the highest level of its node operands (state variables are level
 // 0), and the nodes of one level and one recurrence kind form slots of at most G nodes,
 // lane q of a slot evaluating its q-th node. Per order, the state rows come first, then
-// the slots in level order, separated by wavefront-scope fences. Every coefficient
-// lives in the instance's LDS block:
-//   T[k][r] = fl[k * R + r]: every order of the state variables (rows 0..n-1, the rows
-//     the group kernels' state update, grid output and event code read), a zero row
-//     (n), the nodes whose history a recurrence reads, and the constants and
-//     parameters used by the nodes (order 0: their value; higher orders: zero);
-//   cur[c] = fl[(P + 1) * R + c]: the current order of every state variable (c = i),
+// the slots level by level, one wavefront-scope fence per level. Every coefficient lives
+// in the instance's LDS block:
+//   T[k][r] = fl[k * R + r]: orders 0..p-1 of the state variables (rows 0..n-1), a zero
+//     row (n), the nodes whose history a recurrence reads, and the constants and
+//     parameters a recurrence reads at higher (zero) orders; order p of the state rows
+//     and the zero row. The state rows are the layout the group kernels' state update,
+//     grid output and event code read.
+//   cur[c] = fl[P * R + n + 1 + c]: the current order of every state variable (c = i),
 //     the zero entry (c = n), every node (c = node + 1) and every constant or
-//     parameter, read by the linear nodes and the right-hand sides.
+//     parameter (zero above order 0), read by the linear nodes, the products with a
+//     constant or parameter and the right-hand sides;
+//   sv[q]: the value of every constant and parameter.
 // A lane's operand rows come from per-lane tables loaded once into registers. Each
-// recurrence does the arithmetic of the general recurrences of the register forms (the
-// same terms in the same order); terms that those skip because a factor is zero by
-// degree are read here as exact zeros.
+// recurrence at order k is split into the terms that read orders below k only, emitted
+// one level ahead, and the same-order terms on the level's critical path; div, sqrt,
+// exp, log and sin/cos keep the summation order of the register forms, products, squares
+// and pow add their same-order terms last. Terms that the register forms skip because a
+// factor is zero by degree are read here as exact zeros.
 namespace
 {
 enum WaveKind { WK_LIN, WK_MUL, WK_MULS, WK_SQ, WK_POW, WK_DIV, WK_SQRT, WK_EXP, WK_LOG, WK_TRIG, WK_TIME, WK_COUNT };
