@@ -33,9 +33,13 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per instance per persistent launch, from rocprofv3 FETCH_SIZE (x2.000) and
 # WRITE_SIZE (x1.000) counter runs calibrated on a 1 GiB device copy
-# (profiles/hyx_persist_k_nbody6_group.md). A launch loads and stores every instance once and
-# keeps the Taylor coefficients on chip, so the traffic scales with the instances, not the steps.
-MEASURED_TRAFFIC_B_PER_INSTANCE = {("nbody6", "group"): (31690 * 2.0 + 32891 * 1.0) * 1024 / 65536}
+# (profiles/hyx_persist_k_nbody6_group.md, profiles/hyx_persist_k_j2var_wave.md). A launch loads
+# and stores every instance once and keeps the Taylor coefficients on chip, so the traffic scales
+# with the instances, not the steps.
+MEASURED_TRAFFIC_B_PER_INSTANCE = {
+    ("nbody6", "group"): (31690 * 2.0 + 32891 * 1.0) * 1024 / 65536,
+    ("j2var", "wave"): (43603.9 * 2.0 + 40164.4 * 1.0) * 1024 / 32768,
+}
 
 
 def _cpu_baseline(dec, order, state, pars, t0, t1, n_sample, threads):
