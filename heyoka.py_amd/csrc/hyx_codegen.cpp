@@ -1912,6 +1912,9 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
         if (kind[j] != WK_LIN && kind[j] != WK_MULS)
             for (auto a : S.args[j])
                 if (!is_node(a) && srow[scal_index(a)] < 0) srow[scal_index(a)] = R++;
+    // A sink row: the T stores of slot lanes whose node has no history row go there, so
+    // a slot stores without a per-lane branch (nothing reads it).
+    const int sink = R++;
     // T holds orders 0..P-1 of every row and order P of the state rows (and the zero row):
     // node coefficients are formed up to order P-1.
     const int CB = P * R + n + 1;
@@ -2047,7 +2050,7 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
             t.cur_any = t.cur_any || cur_read[j + 1];
         }
         if (t.cur_any) t.oc = addw(lanes([&](uint32_t j) { return (int)j + 1; }));
-        if (t.hist_any) t.orow = addw(lanes([&](uint32_t j) { return hist[j] ? hrow[j] : -1; }));
+        if (t.hist_any) t.orow = addw(lanes([&](uint32_t j) { return hist[j] ? hrow[j] : sink; }));
         switch (sl.kind) {
         case WK_LIN: {
             for (int q = 0; q < sl.m; ++q)
@@ -2076,9 +2079,11 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
                 t.cosf = addw(lanes([&](uint32_t j) { return S.op[j] == HYX_OP_COS ? 1 : 0; }));
             }
             if (sl.kind == WK_POW) {
+                // Lanes past the slot's width repeat the first lane's exponent (they compute
+                // its node again).
                 std::vector<double> al(G, 0.0), al1(G, 0.0);
-                for (int q = 0; q < w; ++q) {
-                    al[q] = S.consts[S.aux[sl.nodes[q]]];
+                for (int q = 0; q < G; ++q) {
+                    al[q] = S.consts[S.aux[sl.nodes[q < w ? q : 0]]];
                     al1[q] = al[q] + 1.0;
                 }
                 t.al = addd(al);
@@ -2102,10 +2107,6 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
     // Level of each slot; slots of one level are independent of each other.
     std::vector<int> slvl(slots.size());
     for (size_t s = 0; s < slots.size(); ++s) slvl[s] = lvl[slots[s].nodes[0]];
-    auto guard = [&](size_t s) {
-        const int w = (int)slots[s].nodes.size();
-        return w < G ? "if (g < " + std::to_string(w) + ") " : std::string();
-    };
     auto chain = [&](const std::vector<std::pair<std::string, std::string>> &tm, const std::string &nm,
                      const std::string &ind) {
         o << ind << nm << " = " << tm[0].first << " * " << tm[0].second << ";\n";
@@ -2159,7 +2160,7 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
         has_old[s] = 1;
         o << "    double " << ko(s) << " = 0.0";
         if (sl.kind == WK_POW && k >= 2) o << ", " << kq(s) << " = 0.0";
-        o << ";\n    " << guard(s) << "{\n";
+        o << ";\n    {\n";
         if (!tm.empty()) chain(tm, ko(s), "        ");
         if (sl.kind == WK_POW && k >= 2) {
             // S2 = sum_{q=1}^{k-1} a^[k-q] (q c^[q]).
@@ -2188,7 +2189,7 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
         const std::string c = t.orow >= 0 ? wv(t.orow) : "";
         const bool old = has_old[s] != 0;
         const std::string O = ko(s);
-        o << "    " << guard(s) << "{\n";
+        o << "    {\n";
         switch (sl.kind) {
         case WK_LIN: {
             o << "        const int sm = " << wv(t.sgn) << ";\n";
@@ -2290,12 +2291,13 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
         case WK_TIME: o << "        const double v = " << (k == 0 ? "t0" : k == 1 ? "1.0" : "0.0") << ";\n"; break;
         }
         if (t.cur_any) o << "        " << cur(wv(t.oc)) << " = v;\n";
-        if (t.hist_all)
-            o << "        " << Tk(wv(t.orow), k) << " = v;\n";
-        else if (t.hist_any)
-            o << "        if (" << wv(t.orow) << " >= 0) " << Tk(wv(t.orow), k) << " = v;\n";
+        if (t.hist_any) o << "        " << Tk(wv(t.orow), k) << " = v;\n";
         o << "    }\n";
     };
+    // Every lane runs every slot: lanes past a slot's width hold the first lane's table
+    // entries, so they compute the same value and store it to the same address (no
+    // branch per slot or per lane). A level is then one straight-line block whose LDS
+    // loads the compiler can issue together behind one wait.
     // Order k: per level, the order-k-independent parts of the next level's slots, then
     // this level's slots, then one fence.
     auto emit_order_slots = [&](int k) {
