@@ -2179,9 +2179,6 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
         o << "    }\n";
     };
     // The rest of the recurrence (the same-order terms) and the stores.
-    // Order-0 operand values a slot reuses at every order (read once per step).
-    auto za = [](size_t s) { return "za" + std::to_string(s); };
-    auto zb = [](size_t s) { return "zb" + std::to_string(s); };
     auto emit_new = [&](size_t s, int k) {
         const auto &sl = slots[s];
         const auto &t = st[s];
@@ -2203,24 +2200,19 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
         }
         case WK_MUL:
             if (k == 0)
-                o << "        " << za(s) << " = " << Tk(a, 0) << ";\n        " << zb(s) << " = " << Tk(b, 0)
-                  << ";\n        const double v = " << za(s) << " * " << zb(s) << ";\n";
+                o << "        const double v = " << Tk(a, 0) << " * " << Tk(b, 0) << ";\n";
             else
-                o << "        const double v = __builtin_fma(" << Tk(a, k) << ", " << zb(s) << ", "
-                  << (old ? "__builtin_fma(" + za(s) + ", " + Tk(b, k) + ", " + O + ")" : za(s) + " * " + Tk(b, k))
+                o << "        const double v = __builtin_fma(" << Tk(a, k) << ", " << Tk(b, 0) << ", "
+                  << (old ? "__builtin_fma(" + Tk(a, 0) + ", " + Tk(b, k) + ", " + O + ")" : Tk(a, 0) + " * " + Tk(b, k))
                   << ");\n";
             break;
-        case WK_MULS:
-            if (k == 0) o << "        " << zb(s) << " = fl[" << b << "];\n";
-            o << "        const double v = " << cur(a) << " * " << zb(s) << ";\n";
-            break;
+        case WK_MULS: o << "        const double v = " << cur(a) << " * fl[" << b << "];\n"; break;
         case WK_SQ:
             if (k == 0)
-                o << "        " << za(s) << " = " << Tk(a, 0) << ";\n        const double v = " << za(s) << " * " << za(s)
-                  << ";\n";
+                o << "        const double v = " << Tk(a, 0) << " * " << Tk(a, 0) << ";\n";
             else {
                 o << "        const double acc = "
-                  << (old ? "__builtin_fma(" + za(s) + ", " + Tk(a, k) + ", " + O + ")" : za(s) + " * " + Tk(a, k))
+                  << (old ? "__builtin_fma(" + Tk(a, 0) + ", " + Tk(a, k) + ", " + O + ")" : Tk(a, 0) + " * " + Tk(a, k))
                   << ";\n";
                 if (k % 2)
                     o << "        const double v = acc + acc;\n";
@@ -2232,10 +2224,10 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
             if (k == 0) {
                 o << "        const double a0 = " << Tk(a, 0) << ";\n        const double v = pow(a0, " << dv(t.al) << ");\n";
                 o << "        const double ra = 1.0 / a0;\n        " << rv(s, "raa") << " = ra * " << dv(t.al) << ";\n        "
-                  << rv(s, "rab") << " = ra * " << dv(t.al1) << ";\n        " << zb(s) << " = v;\n";
+                  << rv(s, "rab") << " = ra * " << dv(t.al1) << ";\n";
             } else {
                 o << "        const double s1 = "
-                  << (old ? "__builtin_fma(" + Tk(a, k) + ", " + zb(s) + ", " + O + ")" : Tk(a, k) + " * " + zb(s))
+                  << (old ? "__builtin_fma(" + Tk(a, k) + ", " + Tk(c, 0) + ", " + O + ")" : Tk(a, k) + " * " + Tk(c, 0))
                   << ";\n";
                 if (k == 1)
                     o << "        const double v = s1 * " << rv(s, "raa") << ";\n";
@@ -2247,10 +2239,10 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
         case WK_DIV:
             if (k == 0)
                 o << "        const double v = " << Tk(a, 0) << " / " << Tk(b, 0) << ";\n        " << rv(s, "rb") << " = 1.0 / "
-                  << Tk(b, 0) << ";\n        " << zb(s) << " = v;\n";
+                  << Tk(b, 0) << ";\n";
             else
                 o << "        const double v = (" << Tk(a, k) << " - "
-                  << (old ? "__builtin_fma(" + Tk(b, k) + ", " + zb(s) + ", " + O + ")" : Tk(b, k) + " * " + zb(s))
+                  << (old ? "__builtin_fma(" + Tk(b, k) + ", " + Tk(c, 0) + ", " + O + ")" : Tk(b, k) + " * " + Tk(c, 0))
                   << ") * " << rv(s, "rb") << ";\n";
             break;
         case WK_SQRT:
@@ -2264,16 +2256,15 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
             const std::string other = sl.kind == WK_EXP ? c : b;
             if (k == 0) {
                 if (sl.kind == WK_EXP)
-                    o << "        const double v = exp(" << Tk(a, 0) << ");\n        " << zb(s) << " = v;\n";
+                    o << "        const double v = exp(" << Tk(a, 0) << ");\n";
                 else
-                    // The partner's order 0 is the other half of the same sincos.
                     o << "        double sn, cs;\n        sincos(" << Tk(a, 0) << ", &sn, &cs);\n        const double v = "
-                      << wv(t.cosf) << " ? cs : sn;\n        " << zb(s) << " = " << wv(t.cosf) << " ? sn : cs;\n";
+                      << wv(t.cosf) << " ? cs : sn;\n";
                 break;
             }
             const std::string d = k == 1 ? Tk(a, 1) : "(" + lit((double)k) + " * " + Tk(a, k) + ")";
             o << "        const double acc = "
-              << (old ? "__builtin_fma(" + d + ", " + zb(s) + ", " + O + ")" : d + " * " + zb(s)) << ";\n";
+              << (old ? "__builtin_fma(" + d + ", " + Tk(other, 0) + ", " + O + ")" : d + " * " + Tk(other, 0)) << ";\n";
             if (sl.kind == WK_EXP)
                 o << "        const double v = acc * " << lit(1.0 / k) << ";\n";
             else
@@ -2361,9 +2352,6 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
         if (kd == WK_DIV) o << "    double " << rv(s, "rb") << " = 0.0;\n";
         if (kd == WK_SQRT) o << "    double " << rv(s, "rs") << " = 0.0;\n";
         if (kd == WK_LOG) o << "    double " << rv(s, "rl") << " = 0.0;\n";
-        if (kd == WK_MUL || kd == WK_SQ) o << "    double za" << s << " = 0.0;\n";
-        if (kd == WK_MUL || kd == WK_MULS || kd == WK_POW || kd == WK_DIV || kd == WK_EXP || kd == WK_TRIG)
-            o << "    double zb" << s << " = 0.0;\n";
     }
     for (int oo = 0; oo < own; ++oo)
         o << "    const int " << vcol(oo) << " = (tab.vo[" << oo << "] >= 0) ? tab.vo[" << oo << "] : HYX_N;\n";
