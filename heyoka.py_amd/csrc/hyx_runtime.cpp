@@ -358,8 +358,8 @@ int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint3
             s.inst_per_block = 64u / (uint32_t)G;
         } else if (s.form == HYX_FORM_COMPACT && !(flags & HYX_FLAG_FORCE_COMPACT)) {
             // Systems too large for the register forms: the wave form when its LDS rows fit,
-            // 32 lanes per instance (measured on the J2 + variational ensemble: 4.6x the
-            // compact form's 2-lane tape in device memory; 16 lanes: 4.4x; 64 lanes: 2.5x),
+            // 32 lanes per instance (measured on the J2 + variational ensemble: 5.9x the
+            // compact form's 2-lane tape in device memory; 16 lanes: 5.1x; 64 lanes: 3.2x),
             // else 64.
             for (int G : {32, 64}) {
                 wave_src = hyx::generate_wave_prelude(s.S, (int)order, ha, G, s.has_events ? &s.E : nullptr,
