@@ -2068,7 +2068,7 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
         case WK_MULS:
             // a: the cur entry of the operand with a series (if any); b: the constant or parameter.
             t.a = addw(lanes([&](uint32_t j) { return cid(is_node(S.args[j][0]) ? S.args[j][0] : S.args[j][1]); }));
-            t.b = addw(lanes([&](uint32_t j) { return SVB + scal_index(is_node(S.args[j][0]) ? S.args[j][1] : S.args[j][0]); }));
+            t.b = addw(lanes([&](uint32_t j) { return scal_index(is_node(S.args[j][0]) ? S.args[j][1] : S.args[j][0]); }));
             break;
         case WK_TIME: break;
         default:
@@ -2092,13 +2092,21 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
             break;
         }
     }
-    // Table fields are signed 16-bit values, two per 32-bit word (fewer registers).
+    // Table fields are unsigned 8-bit values, four per 32-bit word, when every value fits
+    // (rows, entries and masks of systems up to ~250 nodes), else 16-bit, two per word:
+    // fewer registers for the tables leave the scheduler room to batch a level's loads.
+    int tb_bits = 8;
     for (auto &f : W)
-        for (int v : f)
-            if (v < -32768 || v > 32767) throw std::logic_error("wave form: table value out of 16-bit range");
-    auto wv = [](int f) {
-        const std::string w = "tab.w[" + std::to_string(f / 2) + "]";
-        return f % 2 ? "(" + w + " >> 16)" : "((int)((unsigned)" + w + " << 16) >> 16)";
+        for (int v : f) {
+            if (v < 0 || v > 65535) throw std::logic_error("wave form: table value out of 16-bit range");
+            if (v > 255) tb_bits = 16;
+        }
+    const int tb_per = 32 / tb_bits;
+    auto wv = [tb_bits, tb_per](int f) {
+        const std::string w = "tab.w[" + std::to_string(f / tb_per) + "]";
+        const int sh = tb_bits * (f % tb_per);
+        if (sh + tb_bits == 32) return "(int)((unsigned)" + w + " >> " + std::to_string(sh) + ")";
+        return "(int)(((unsigned)" + w + " >> " + std::to_string(sh) + ") & " + std::to_string((1u << tb_bits) - 1) + "u)";
     };
     auto dv = [](int f) { return "tab.d[" + std::to_string(f) + "]"; };
     // Per-slot registers that live across orders (reciprocals of order-0 values).
@@ -2206,7 +2214,7 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
                   << (old ? "__builtin_fma(" + Tk(a, 0) + ", " + Tk(b, k) + ", " + O + ")" : Tk(a, 0) + " * " + Tk(b, k))
                   << ");\n";
             break;
-        case WK_MULS: o << "        const double v = " << cur(a) << " * fl[" << b << "];\n"; break;
+        case WK_MULS: o << "        const double v = " << cur(a) << " * fl[" << SVB << " + " << b << "];\n"; break;
         case WK_SQ:
             if (k == 0)
                 o << "        const double v = " << Tk(a, 0) << " * " << Tk(a, 0) << ";\n";
@@ -2307,16 +2315,19 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
     };
 
     // The per-lane table struct and its loader.
-    const size_t NWP = (W.size() + 1) / 2;
+    const size_t NWP = (W.size() + tb_per - 1) / tb_per;
     h << "struct hyx_gtab {\n    int vo[HYX_OWN];\n    int w[" << std::max<size_t>(NWP, 1) << "];\n    double d["
       << std::max<size_t>(D.size(), 1) << "];\n};\n";
     {
         std::vector<int> wf;
         for (size_t p2 = 0; p2 < NWP; ++p2)
             for (int g = 0; g < G; ++g) {
-                const unsigned lo = (unsigned)W[2 * p2][g] & 0xffffu;
-                const unsigned hi = 2 * p2 + 1 < W.size() ? ((unsigned)W[2 * p2 + 1][g] & 0xffffu) : 0u;
-                wf.push_back((int)(lo | (hi << 16)));
+                unsigned word = 0;
+                for (int h2 = 0; h2 < tb_per; ++h2) {
+                    const size_t f = p2 * tb_per + h2;
+                    if (f < W.size()) word |= ((unsigned)W[f][g]) << (tb_bits * h2);
+                }
+                wf.push_back((int)word);
             }
         if (wf.empty()) wf.push_back(0);
         h << "__constant__ int hyx_wtab_w[" << wf.size() << "] = {";
