@@ -2007,7 +2007,7 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
     h << "typedef unsigned long long hyx_u64;\n";
     h << "#ifndef HYX_CPU_EMU\n"
          "// A block is one wavefront, and the LDS operations of a wavefront complete in issue\n"
-         "// order: the exchanges between slots only need the compiler to keep memory accesses\n"
+         "// order: the exchanges between levels only need the compiler to keep memory accesses\n"
          "// on their side of the fence (no s_waitcnt, no s_barrier).\n"
          "__device__ __forceinline__ void hyx_wsync()\n{\n"
          "    __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"wavefront\");\n"
@@ -2016,7 +2016,6 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
          "#endif\n";
 
     // Slot code. Operand rows are per-lane registers tab.w[f]; T[k][row] is fl[row + k * R].
-    const std::string Rs = std::to_string(R);
     auto Tk = [&](const std::string &row, int k) {
         return k == 0 ? "fl[" + row + "]" : "fl[" + row + " + " + std::to_string(k * R) + "]";
     };
@@ -2024,7 +2023,7 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
     struct SlotTab {
         int a = -1, b = -1, oc = -1, orow = -1, sgn = -1, al = -1, al1 = -1, cosf = -1;
         std::vector<int> lin;
-        bool cur_any = false, hist_all = false, hist_any = false;
+        bool cur_any = false, hist_any = false;
     };
     std::vector<SlotTab> st(slots.size());
     // Readers of each node's cur entry: linear nodes and right-hand sides.
@@ -2043,9 +2042,7 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
             for (int q = w; q < G; ++q) v[q] = v[0];
             return v;
         };
-        t.hist_all = true;
         for (auto j : sl.nodes) {
-            t.hist_all = t.hist_all && hist[j];
             t.hist_any = t.hist_any || hist[j];
             t.cur_any = t.cur_any || cur_read[j + 1];
         }
