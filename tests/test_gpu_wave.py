@@ -120,3 +120,21 @@ def test_wave_events_vs_oracle():
         got = sorted(t for bb, t, _ in hits if bb == b)
         want = [t for _, t, _ in ref["nt"][b]]
         np.testing.assert_allclose(got, want, rtol=1e-12)
+
+
+def test_wave_backward_and_continuous_output():
+    # Backward propagation and c_output=True (host-driven steps for the group kernels) in the
+    # wave form, against the compact form.
+    print("[gpu] wave form backward propagation and continuous output", flush=True)
+    wl = W.j2var(20)
+    res = {}
+    for form in ("wave", "compact"):
+        ta = hy.taylor_adaptive_batch(wl.sys, wl.state, tol=1e-15, code_form=form)
+        ta.propagate_until(np.full(20, -6.0))
+        assert all(p[0] == hy.taylor_outcome.time_limit for p in ta.propagate_res)
+        back = ta.state.copy()
+        tb = hy.taylor_adaptive_batch(wl.sys, wl.state, tol=1e-15, code_form=form)
+        co, _ = tb.propagate_for(np.full(20, 4.0), c_output=True)
+        res[form] = (back, tb.state.copy(), co(np.full(20, 2.5)).copy())
+    for a, b in zip(res["wave"], res["compact"]):
+        np.testing.assert_allclose(a, b, rtol=1e-11, atol=1e-12)
