@@ -31,15 +31,61 @@ METRIC = "FP64 instance-steps/sec at tol=1e-15 vs plain GPU & CPU; p50/p99 reque
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per instance per persistent launch, from rocprofv3 FETCH_SIZE (x2.000) and
-# WRITE_SIZE (x1.000) counter runs calibrated on a 1 GiB device copy
-# (profiles/hyx_persist_k_nbody6_group.md, profiles/hyx_persist_k_j2var_wave.md). A launch loads
-# and stores every instance once and keeps the Taylor coefficients on chip, so the traffic scales
-# with the instances, not the steps.
-MEASURED_TRAFFIC_B_PER_INSTANCE = {
-    ("nbody6", "group"): (31690 * 2.0 + 32891 * 1.0) * 1024 / 65536,
-    ("j2var", "wave"): (43603.9 * 2.0 + 40164.4 * 1.0) * 1024 / 32768,
-}
+# HBM bytes per instance per persistent launch of the default workload's kernel, from
+# rocprofv3 FETCH_SIZE (x2.000) and WRITE_SIZE (x1.000) counter runs calibrated on a 1 GiB
+# device copy, keyed by (workload, code form, the first 16 hex digits of the sha1 of the
+# generated kernel source). A launch loads and stores every instance once and keeps the
+# Taylor coefficients on chip, so the traffic scales with the instances, not the steps. A
+# kernel whose source changed has no measured figure: the line then says null.
+MEASURED_TRAFFIC_B_PER_INSTANCE = {}
+
+
+def _host_info(threads):
+    """CPU model and the cores this process may use (sched_getaffinity)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count()
+    return {"cpu_model": model, "host_cores_available": avail, "threads_used": threads}
+
+
+def _default_threads():
+    """One worker per available core, at most 16 (the host share of one GPU)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def _cold_compile_s(sys_name, form):
+    """Seconds to generate and compile a system's kernels with an empty code-object cache
+    (a fresh process, so no in-memory cache either): the analogue of heyoka's LLVM JIT
+    construction time."""
+    import subprocess
+    import tempfile
+
+    code = ("import time, sys; sys.path.insert(0, %r)\n"
+            "import heyoka_amd as hy\nfrom heyoka_amd import workloads as W, _lib\n"
+            "wl = getattr(W, %r)(8)\nsy = wl.sys.sys if hasattr(wl.sys, 'n_orig_sv') else wl.sys\n"
+            "dec = hy.taylor_decompose(sy)\nt = time.perf_counter()\n"
+            "_lib.System(dec, hy.order_from_tol(wl.tol), compile_only=True, compact=%r)\n"
+            "print(time.perf_counter() - t)\n") % (ROOT, sys_name, form)
+    with tempfile.TemporaryDirectory() as d:
+        env = dict(os.environ, HYX_CACHE_DIR=d)
+        out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=600)
+    if out.returncode != 0:
+        return None
+    return round(float(out.stdout.strip().splitlines()[-1]), 3)
 
 
 def _cpu_baseline(dec, order, state, pars, t0, t1, n_sample, threads):
@@ -110,24 +156,29 @@ def main():
     ap.add_argument("--config", default="nbody6", choices=["nbody6", "cr3bp", "j2var", "serving", "events"])
     ap.add_argument("--rates", default="1,4", help="serving: Poisson arrival rates (requests/s)")
     ap.add_argument("--duration", type=float, default=4.0, help="serving: seconds of arrivals per rate")
+    ap.add_argument("--requests", type=int, default=None, help="serving: requests per rate (default rate x duration)")
     ap.add_argument("--n", type=int, default=None, help="instances (default: the config's)")
     ap.add_argument("--dt", type=float, default=25.0, help="nbody6: time units per bench step")
-    ap.add_argument("--code-form", default=None, choices=[None, "compact", "unrolled", "group", "wave", "wave32",
-                                                           "wave16"])
+    ap.add_argument("--code-form", default=None, choices=[None, "compact", "unrolled", "unrolled_lds",
+                                                           "unrolled_regs", "group", "wave", "wave32", "wave16"])
     ap.add_argument("--split-chains", action="store_true",
                     help="two accumulators per long convolution sum instead of one (A/B of the code variant)")
     ap.add_argument("--no-pipeline", action="store_true",
                     help="group form: two LDS exchanges per order instead of one (A/B of the code variant)")
     ap.add_argument("--sub", type=int, default=2, choices=[1, 2, 4, 8],
                     help="compact form: lanes per instance (A/B of the code variant)")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=None, help="default: available cores, at most 16")
     ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="nbody6: only the headline slices (no plain / CR3BP / J2V / ensemble / serving records)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.cpu_threads is None:
+        args.cpu_threads = _default_threads()
     if args.copy_calibration:
         import torch
 
@@ -214,6 +265,10 @@ def main():
         print(f"[bench] step {i}: {r['nsteps'].astype(np.float64).sum():.3e} instance-steps in {ms:.1f} ms "
               f"({nl} launches), outcomes not time_limit: {bad}", file=sys.stderr, flush=True)
     value = total_steps / wall
+    import hashlib
+
+    src_sha = hashlib.sha1(system.source().encode()).hexdigest()[:16]
+    traffic_key = (args.config, system.label, src_sha)
     avg_launch_s = kernel_ms / 1e3 / max(launches, 1)
     flops_per_launch = flops * total_steps / max(launches, 1)
     achieved_tf = flops_per_launch / avg_launch_s / 1e12
@@ -243,19 +298,22 @@ def main():
             "code_form": system.form,
             "sum_chains": "split" if args.split_chains else "serial",
             "group_exchanges_per_order": ((2 if args.no_pipeline else 1) if system.form == "group" else None),
-            "lanes_per_instance": {"group": None, "wave": None, "compact": args.sub, "unrolled": 1}[system.form],
+            "lanes_per_instance": system.lanes_per_instance,
             "form": "persistent",
+            "launch_shape": ({"waves_per_cu": batch.tuned[0], "refill_min": batch.tuned[1], "source": "autotuned"}
+                             if batch.tuned else {"waves_per_cu": "occupancy", "refill_min": 1, "source": "default"}),
+            "kernel_source_sha16": src_sha,
             "algorithmic_flops_per_instance_step": flops,
             "hiprtc_compile_s": round(t_compile, 3),
         },
         "roofline": {
-            "bound": "mfma",
+            "bound": "fp64_valu",
             "achieved": achieved_tf,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tf / FP64_PEAK_TFLOPS,
-            "traffic": (MEASURED_TRAFFIC_B_PER_INSTANCE[(args.config, system.form)] * N
-                        if (args.config, system.form) in MEASURED_TRAFFIC_B_PER_INSTANCE else None),
+            "traffic": (MEASURED_TRAFFIC_B_PER_INSTANCE[traffic_key] * N
+                        if traffic_key in MEASURED_TRAFFIC_B_PER_INSTANCE else None),
             "kernel": "hyx_persist_k",
             "avg_launch_ms": avg_launch_s * 1e3,
         },
@@ -303,8 +361,152 @@ def main():
             sample = (f"first {ns} instances of the same ensemble to their own horizons, {args.cpu_threads} threads, "
                       "compiled port of the recurrences")
         out["cpu_baseline"] = {"value": v, "unit": "instance-steps/s", "cores": args.cpu_threads, "kind": "port",
-                               "sample": sample + f" ({secs:.1f} s)"}
+                               "sample": sample + f" ({secs:.1f} s)", **_host_info(args.cpu_threads)}
+    if args.config == "nbody6" and not args.no_extras:
+        out["plain_gpu"] = _plain_slice(batch, load, N, args.dt, value)
+        out["cold_compile_s"] = _cold_compile_s("nbody6", None)
+        out["cr3bp_ragged"] = _cr3bp_record()
+        out["j2var"] = _j2var_record(args.dt)
+        out["ensemble_api"] = _ensemble_record(wl)
+        out["serving"] = _serving_record()
     print(json.dumps(out), flush=True)
+
+
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _timed(batch, tf, mode):
+    from heyoka_amd import _lib
+
+    batch.sync()
+    t = time.perf_counter()
+    r = batch.propagate_until(tf, None, None, 0, mode)
+    batch.sync()
+    wall = time.perf_counter() - t
+    ms, nl = batch.last_timing()
+    return r, wall, ms, nl
+
+
+def _plain_slice(batch, load, N, dt, persistent_value):
+    """The plain one-launch-per-step form on one slice of the headline workload."""
+    from heyoka_amd import _lib
+
+    load()
+    r, wall, ms, nl = _timed(batch, np.full(N, dt), _lib.MODE_PLAIN)
+    v = float(r["nsteps"].astype(np.float64).sum()) / wall
+    _log(f"nbody6 plain slice: {v:.3e} instance-steps/s ({nl} launches)")
+    return {"value": v, "unit": "instance-steps/s", "launches": int(nl), "kernel_ms": ms,
+            "workload": f"the first slice (t in [0, {dt}]) of the same ensemble, one launch per step",
+            "persistent_over_plain": persistent_value / v}
+
+
+def _cr3bp_record():
+    """The ragged CR3BP ensemble (131,072 instances, horizons 10**U(1, 4)): persistent
+    work-stealing launch against the plain one-launch-per-step form, same code object."""
+    import heyoka_amd as hy
+    from heyoka_amd import _lib, workloads as W
+    from heyoka_amd.integrator import get_system
+
+    wl = W.cr3bp(131072)
+    dec = hy.taylor_decompose(wl.sys)
+    system = get_system(dec, hy.order_from_tol(wl.tol))
+    N = wl.state.shape[1]
+    b = _lib.Batch(system, N)
+
+    def load():
+        b.write(_lib.FIELD_STATE, wl.state)
+        b.write(_lib.FIELD_PARS, wl.pars)
+        b.write(_lib.FIELD_TIME_HI, np.zeros(N))
+        b.write(_lib.FIELD_TIME_LO, np.zeros(N))
+
+    out = {"workload": f"cr3bp ragged ensemble: {N} instances, 4 states + par[0], tol 1e-15, horizons 10**U(1,4), "
+                       "whole propagation from the initial conditions",
+           "code_form": system.label,
+           "launch_shape": ({"waves_per_cu": b.tuned[0], "refill_min": b.tuned[1], "source": "autotuned"}
+                            if b.tuned else {"waves_per_cu": "occupancy", "refill_min": 1, "source": "default"})}
+    for label, mode in (("persistent", _lib.MODE_PERSISTENT), ("plain", _lib.MODE_PLAIN)):
+        load()
+        r, wall, ms, nl = _timed(b, wl.t_final, mode)
+        ns = r["nsteps"].astype(np.int64)
+        out[label] = {"value": float(ns.sum()) / wall, "wall_s": wall, "kernel_ms": ms, "launches": int(nl),
+                      "instance_steps": int(ns.sum()), "nsteps_max": int(ns.max()),
+                      "err_nf_state": int(np.sum(r["outcome"] == -4294967300))}
+        _log(f"cr3bp {label}: {out[label]['value']:.3e} instance-steps/s, {wall:.2f} s, {nl} launches")
+    out["persistent_over_plain"] = out["persistent"]["value"] / out["plain"]["value"]
+    b.close()
+    return out
+
+
+def _j2var_record(dt):
+    """J2 + variational ensemble (32,768 instances): one persistent slice of dt."""
+    import heyoka_amd as hy
+    from heyoka_amd import _lib, workloads as W
+    from heyoka_amd.integrator import get_system
+
+    wl = W.j2var(32768)
+    dec = hy.taylor_decompose(wl.sys.sys)
+    system = get_system(dec, hy.order_from_tol(wl.tol))
+    N = wl.state.shape[1]
+    b = _lib.Batch(system, N)
+    b.write(_lib.FIELD_STATE, wl.state)
+    b.write(_lib.FIELD_TIME_HI, np.zeros(N))
+    b.write(_lib.FIELD_TIME_LO, np.zeros(N))
+    _timed(b, np.full(N, 1.0), _lib.MODE_PERSISTENT)  # warm-up
+    r, wall, ms, nl = _timed(b, np.full(N, 1.0 + dt), _lib.MODE_PERSISTENT)
+    ns = float(r["nsteps"].astype(np.float64).sum())
+    flops = dec.algorithmic_flops(hy.order_from_tol(wl.tol))["total"]
+    tf = flops * ns / (ms / 1e3) / 1e12
+    _log(f"j2var slice: {ns / wall:.3e} instance-steps/s ({system.label})")
+    b.close()
+    return {"workload": f"J2 + first-order variational ensemble: {N} instances, 42 states, tol 1e-15, one slice "
+                        f"t in [1, {1.0 + dt}]",
+            "code_form": system.label, "lanes_per_instance": system.lanes_per_instance,
+            "value": ns / wall, "kernel_ms": ms, "algorithmic_flops_per_instance_step": flops,
+            "achieved_tflops": tf, "frac_of_fp64_peak": tf / FP64_PEAK_TFLOPS}
+
+
+def _ensemble_record(wl):
+    """ensemble_propagate_until through the drop-in API on the 6-body ensemble (65,536
+    instances to t = 100): 16 members of a 4,096-instance template with a Python
+    generator (gen(deepcopy(ta), i)), and 65,536 one-instance members from arrays
+    (ensemble_arrays). Host overhead = wall time - kernel time."""
+    import heyoka_amd as hy
+
+    T = 100.0
+    out = {"workload": f"nbody6 ensemble, {wl.state.shape[1]} instances, propagate_until t={T}"}
+    B = 4096
+    tmpl = hy.taylor_adaptive_batch(wl.sys, wl.state[:, :B].copy(), tol=1e-15)
+
+    def gen(ta, i):
+        ta.state[:] = wl.state[:, i * B:(i + 1) * B]
+        return ta
+
+    for label, fn in (("generator_16x4096", lambda: hy.ensemble_propagate_until_batch(tmpl, T, 16, gen)),
+                      ("arrays_65536x1", lambda: hy.ensemble_propagate_until_batch(
+                          hy.taylor_adaptive_batch(wl.sys, wl.state[:, :1].copy(), tol=1e-15), T, wl.state.shape[1],
+                          hy.ensemble_arrays(state=wl.state)))):
+        from heyoka_amd import ensemble as E
+
+        fn()  # warm-up (first launch of the packed batch size)
+        t = time.perf_counter()
+        res = fn()
+        wall = time.perf_counter() - t
+        ms = E.last_run["kernel_ms"]
+        ns = E.last_run["instance_steps"]
+        out[label] = {"wall_s": wall, "kernel_s": ms / 1e3, "host_overhead_s": wall - ms / 1e3,
+                      "host_overhead_frac_of_kernel": (wall - ms / 1e3) / (ms / 1e3), "members": len(res),
+                      "instance_steps": ns, "instance_steps_per_s": ns / wall}
+        _log(f"ensemble {label}: wall {wall:.2f} s, kernel {ms / 1e3:.2f} s")
+    return out
+
+
+def _serving_record():
+    """A short Poisson stream through the continuous-batching server (all four systems)."""
+    import argparse as _ap
+
+    a = _ap.Namespace(rates="4", duration=4.0, requests=None)
+    return serving_bench(a, emit=False)
 
 
 def events_bench(args):
@@ -408,72 +610,77 @@ def events_bench(args):
     print(json.dumps(out), flush=True)
 
 
-def serving_bench(args):
-    """Poisson stream of propagation requests through the continuous-batching server.
+def serving_bench(args, emit=True):
+    """Poisson streams of propagation requests through the continuous-batching server.
 
     Each request: a system drawn uniformly from {pendulum, cr3bp, nbody6, j2var},
     floor(10**U(0, log10 4096)) instances with seeded initial conditions,
-    horizons 10**U(1, 3), tol 1e-15. Reports p50 / p99 latency (completion -
-    arrival) and aggregate instance-steps/s per arrival rate.
+    horizons 10**U(1, 3), tol 1e-15. Per arrival rate: `--requests` requests (else
+    rate x duration), p50 / p99 latency (completion - arrival), the completed requests
+    per second and the aggregate instance-steps/s. Requests of one kind share one
+    system object, as a client of a server would.
     """
-    import heyoka_amd as hy
     from heyoka_amd import workloads as W
     from heyoka_amd.serving import Server
 
     rng = np.random.default_rng(5)
     kinds = ["pendulum", "cr3bp", "nbody6", "j2var"]
+    gens = {"pendulum": W.pendulum, "cr3bp": W.cr3bp, "nbody6": W.nbody6, "j2var": W.j2var}
+    shared = {k: gens[k](1, seed=0).sys for k in kinds}
     results = []
     for rate in [float(r) for r in args.rates.split(",")]:
-        srv = Server(capacity=16384, epoch_iters=256)
-        t_start = time.perf_counter()
-        t_arr = 0.0
+        n_req = getattr(args, "requests", None) or max(1, int(round(rate * args.duration)))
         reqs = []
-        k = 0
-        while True:
+        t_arr = 0.0
+        for k in range(n_req):
             t_arr += rng.exponential(1.0 / rate)
-            if t_arr > args.duration:
-                break
             kind = kinds[int(rng.integers(0, 4))]
             m = int(10 ** rng.uniform(0, np.log10(4096)))
-            seed = 1000 + k
-            k += 1
-            hor = 10 ** rng.uniform(1, 3, m)
-            if kind == "pendulum":
-                wl = W.pendulum(m, seed=seed)
-            elif kind == "cr3bp":
-                wl = W.cr3bp(m, seed=seed)
-            elif kind == "nbody6":
-                wl = W.nbody6(m, seed=seed)
-            else:
-                wl = W.j2var(m, seed=seed)
-            reqs.append((t_start + t_arr, wl, hor))
-        print(f"[bench] serving rate {rate}/s: {len(reqs)} requests", file=sys.stderr, flush=True)
-        handles = [srv.submit(wl.sys, wl.state, hor, pars=wl.pars, tol=1e-15, arrival=ta) for ta, wl, hor in reqs]
+            wl = gens[kind](m, seed=1000 + k)
+            reqs.append((t_arr, kind, wl, 10 ** rng.uniform(1, 3, m)))
+        srv = Server(capacity=16384, epoch_iters=256)
+        # Compile and create each pool before the stream starts.
+        for kind in kinds:
+            wl = gens[kind](1, seed=0)
+            h = srv.submit(shared[kind], wl.state, np.full(1, 0.5), pars=wl.pars, tol=1e-15)
+            srv.run_until_idle()
+            assert h.finished is not None
+        srv.done.clear()
+        srv.instance_steps = 0
+        _log(f"serving rate {rate}/s: {len(reqs)} requests")
+        t_start = time.perf_counter()
+        handles = [srv.submit(shared[kind], wl.state, hor, pars=wl.pars, tol=1e-15, arrival=t_start + ta)
+                   for ta, kind, wl, hor in reqs]
         last = time.perf_counter()
-        while True:
-            if srv.step() == 0 and all(h.finished is not None for h in handles):
-                break
-            if all(h.finished is not None for h in handles):
-                break
+        while not all(h.finished is not None for h in handles):
+            srv.step()
             now = time.perf_counter()
             if now - last > 5.0:
                 last = now
-                ndone = sum(h.finished is not None for h in handles)
-                print(f"[bench] serving rate {rate}/s: {ndone}/{len(handles)} requests done after "
-                      f"{now - t_start:.1f} s", file=sys.stderr, flush=True)
-            pending = [h.arrival for h in handles if h.finished is None]
-            if pending and min(pending) > now and not any(
-                    getattr(p, "running", False) for p in srv.pools.values()):
+                _log(f"serving rate {rate}/s: {sum(h.finished is not None for h in handles)}/{len(handles)} "
+                     f"done after {now - t_start:.1f} s")
+            pending = [h.arrival for h in handles if h.finished is None and h.admitted is None]
+            busy = any(getattr(p, "running", None) for p in srv.pools.values())
+            if pending and min(pending) > now and not busy:
                 time.sleep(min(0.01, min(pending) - now))
         t_end = max(h.finished for h in handles)
-        lat = srv.latency_percentiles()
-        results.append({"rate": rate, "requests": len(handles),
-                        "instances": int(sum(h.n for h in handles)),
-                        "p50_s": lat.get("p50"), "p99_s": lat.get("p99"),
-                        "instance_steps_per_s": srv.instance_steps / (t_end - t_start),
-                        "epochs": srv.epochs})
-        print(f"[bench] rate {rate}: p50 {lat.get('p50'):.3f} s p99 {lat.get('p99'):.3f} s, "
-              f"{srv.instance_steps / (t_end - t_start):.3e} instance-steps/s", file=sys.stderr, flush=True)
+        lat = np.array([h.latency for h in handles])
+        span = t_end - t_start
+        row = {"rate": rate, "requests": len(handles), "instances": int(sum(h.n for h in handles)),
+               "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+               "completed_per_s": len(handles) / span, "instance_steps_per_s": srv.instance_steps / span,
+               "span_s": span, "epochs": srv.epochs}
+        results.append(row)
+        _log(f"rate {rate}: p50 {row['p50_s']:.3f} s p99 {row['p99_s']:.3f} s, {row['completed_per_s']:.1f} req/s, "
+             f"{row['instance_steps_per_s']:.3e} instance-steps/s")
+    # The knee: the first rate whose p99 exceeds twice the lowest rate's.
+    knee = next((r["rate"] for r in results if r["p99_s"] > 2.0 * results[0]["p99_s"]), None)
+    summary = {"workload": "serving: Poisson request stream over pendulum / cr3bp / nbody6 / j2var, sizes "
+                           "10**U(0, log10 4096), horizons 10**U(1, 3), tol 1e-15, continuous batching",
+               "latency": results, "knee_rate": knee,
+               "saturation_instance_steps_per_s": max(r["instance_steps_per_s"] for r in results)}
+    if not emit:
+        return summary
     best = max(results, key=lambda r: r["instance_steps_per_s"])
     out = {
         "metric": METRIC,
@@ -488,10 +695,8 @@ def serving_bench(args):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "serving: Poisson request stream over pendulum / cr3bp / nbody6 / j2var, sizes "
-                               "10**U(0, log10 4096), horizons 10**U(1, 3), tol 1e-15, continuous batching",
-                   "duration_s": args.duration},
-        "latency": results,
+        "config": {"workload": summary["workload"], "duration_s": args.duration},
+        **{k: v for k, v in summary.items() if k != "workload"},
     }
     print(json.dumps(out), flush=True)
 

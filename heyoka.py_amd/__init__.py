@@ -25,6 +25,7 @@ from .ensemble import (  # noqa: F401
     ensemble_propagate_until_batch,
     ensemble_propagate_for_batch,
     ensemble_propagate_grid_batch,
+    ensemble_arrays,
 )
 
 

@@ -25,6 +25,8 @@ FLAG_FORCE_GROUP = 32
 FLAG_FORCE_WAVE = 4096
 FLAG_WAVE32 = 8192
 FLAG_WAVE16 = 16384
+FLAG_LDS_ROWS = 32768
+FLAG_NO_LDS_ROWS = 65536
 FORMS = {0: "unrolled", 1: "compact", 2: "group", 3: "wave"}
 FLAG_GRID = 64
 FLAG_SPLIT_CHAINS = 128
@@ -84,6 +86,7 @@ def lib():
             "hyx_sys_npar": (ct.c_int, [vp]),
             "hyx_sys_is_compact": (ct.c_int, [vp]),
             "hyx_sys_code_form": (ct.c_int, [vp]),
+            "hyx_sys_lanes_per_instance": (ct.c_int, [vp]),
             "hyx_sys_source": (ct.c_size_t, [vp, ct.c_char_p, ct.c_size_t]),
             "hyx_sys_kernel_info": (ct.c_int, [vp, P(ct.c_int), P(ct.c_int)]),
             "hyx_sys_destroy": (None, [vp]),
@@ -144,7 +147,7 @@ def lib():
 
 EXPORTED = (
     "hyx_sys_create", "hyx_sys_precompile", "hyx_sys_order", "hyx_sys_dim", "hyx_sys_npar", "hyx_sys_is_compact",
-    "hyx_sys_code_form",
+    "hyx_sys_code_form", "hyx_sys_lanes_per_instance",
     "hyx_sys_source",
     "hyx_sys_kernel_info", "hyx_sys_destroy", "hyx_batch_create", "hyx_batch_destroy", "hyx_batch_size",
     "hyx_batch_write", "hyx_batch_read", "hyx_batch_device_ptr", "hyx_step", "hyx_read_tc", "hyx_propagate_until",
@@ -178,13 +181,18 @@ def oplist_args(dec):
 
 
 def _flags(high_accuracy=False, compact=None):
-    """compact: None (automatic), True / "compact", False / "unrolled", "group", or "wave" (64 lanes per
-    instance; "wave32" / "wave16": 32 or 16)."""
+    """compact: None (automatic), True / "compact", False / "unrolled" ("unrolled_lds" / "unrolled_regs":
+    state rows in LDS / in registers), "group", or "wave" (64 lanes per instance; "wave32" / "wave16": 32
+    or 16)."""
     f = FLAG_HIGH_ACCURACY if high_accuracy else 0
     if compact is True or compact == "compact":
         f |= FLAG_FORCE_COMPACT
     elif compact is False or compact == "unrolled":
         f |= FLAG_FORCE_UNROLLED
+    elif compact == "unrolled_lds":
+        f |= FLAG_FORCE_UNROLLED | FLAG_LDS_ROWS
+    elif compact == "unrolled_regs":
+        f |= FLAG_FORCE_UNROLLED | FLAG_NO_LDS_ROWS
     elif compact == "group":
         f |= FLAG_FORCE_GROUP
     elif compact in ("wave", "wave32", "wave16"):
@@ -199,6 +207,56 @@ def precompile(dec, order: int, high_accuracy: bool = False, arch: str = "gfx950
     flags = _flags(high_accuracy, compact) | (FLAG_GRID if grid else 0)
     check(lib().hyx_sys_precompile(dptr(words, ct.c_uint32), len(words), dptr(consts), len(dec.consts), order, flags,
                                    arch.encode()))
+
+
+# ---- tuned launch shapes ---------------------------------------------------------------
+# The autotuner (autotune.py) records per system and code form the fastest persistent
+# launch shape (waves per CU, refill threshold) in the user cache
+# ($HYX_CACHE_DIR or code_cache/autotune.json); tuned_gfx950.json beside this file holds
+# the shapes measured for the benchmark workloads and ships with the package. Every Batch
+# applies the shape recorded for its system. Launch shapes never change results.
+TUNED_PATH = os.path.join(PKG_DIR, "tuned_gfx950.json")
+_tuned_cache = {}
+
+
+def tune_cache_path() -> str:
+    d = os.environ.get("HYX_CACHE_DIR") or os.path.join(PKG_DIR, "code_cache")
+    return os.path.join(d, "autotune.json")
+
+
+def tune_key(words, consts, order: int, label: str) -> str:
+    import hashlib
+
+    h = hashlib.sha1(np.ascontiguousarray(words).tobytes() + np.ascontiguousarray(consts).tobytes()).hexdigest()[:16]
+    return f"{h}-p{int(order)}-{label}-gfx950"
+
+
+def _tuned_db(path):
+    import json
+
+    try:
+        mt = os.stat(path).st_mtime_ns
+    except OSError:
+        return {}
+    hit = _tuned_cache.get(path)
+    if hit is None or hit[0] != mt:
+        try:
+            with open(path) as f:
+                db = json.load(f)
+        except (OSError, ValueError):
+            db = {}
+        _tuned_cache[path] = hit = (mt, db)
+    return hit[1]
+
+
+def tuned_launch(key: str):
+    """The recorded (waves_per_cu, refill_min, rate) for a system key, or None: the user
+    cache first, then the shipped table."""
+    for path in (tune_cache_path(), TUNED_PATH):
+        rec = _tuned_db(path).get(key)
+        if rec:
+            return int(rec["waves_per_cu"]), int(rec["refill_min"]), float(rec.get("rate", 0.0))
+    return None
 
 
 def device_count() -> int:
@@ -254,6 +312,16 @@ class System:
         self._L = L
         self.compact = bool(L.hyx_sys_is_compact(self.h))
         self.form = FORMS[int(L.hyx_sys_code_form(self.h))]
+        self.lanes_per_instance = int(L.hyx_sys_lanes_per_instance(self.h))
+        # The autotuner's name for this code object: the wave variants carry their lanes per
+        # instance, the unrolled form where its state rows live when forced.
+        if self.form == "wave":
+            self.label = {64: "wave", 32: "wave32", 16: "wave16"}[self.lanes_per_instance]
+        elif isinstance(compact, str) and compact.startswith("unrolled_"):
+            self.label = compact
+        else:
+            self.label = self.form
+        self.tune_key = tune_key(words, consts, order, self.label) if events is None and not high_accuracy else None
 
     def source(self) -> str:
         n = self._L.hyx_sys_source(self.h, None, 0)
@@ -283,6 +351,10 @@ class Batch:
         h = ct.c_void_p()
         check(self._L.hyx_batch_create(system.h, self.N, ct.byref(h)))
         self.h = h
+        # The tuned persistent launch shape of this system, if one is recorded.
+        self.tuned = tuned_launch(system.tune_key) if getattr(system, "tune_key", None) else None
+        if self.tuned is not None:
+            self.set_persistent_config(self.tuned[0], self.tuned[1])
 
     def write(self, field: int, arr: np.ndarray):
         a = np.ascontiguousarray(arr, dtype=np.float64)

@@ -17,7 +17,6 @@ op-list, the order and the GPU architecture, like the code objects.
 
 from __future__ import annotations
 
-import hashlib
 import json
 import os
 import time
@@ -29,20 +28,20 @@ from .integrator import get_system
 
 
 def _cache_path():
-    d = os.environ.get("HYX_CACHE_DIR") or os.path.join(_lib.PKG_DIR, "code_cache")
-    os.makedirs(d, exist_ok=True)
-    return os.path.join(d, "autotune.json")
+    p = _lib.tune_cache_path()
+    os.makedirs(os.path.dirname(p), exist_ok=True)
+    return p
 
 
 def _key(dec, order, form):
     words, consts = _lib.oplist_args(dec)
-    h = hashlib.sha1(words.tobytes() + consts.tobytes()).hexdigest()[:16]
-    return f"{h}-p{order}-{form}-gfx950"
+    return _lib.tune_key(words, consts, order, form)
 
 
 def _label(form, system):
-    """The candidate's name: the wave form's lanes per instance are part of it."""
-    return form if isinstance(form, str) and form.startswith("wave") else system.form
+    """The candidate's name: the wave form's lanes per instance and the unrolled form's
+    state-row placement are part of it."""
+    return system.label
 
 
 def wave_forms(dec, order):
@@ -135,3 +134,41 @@ def tuned_config(dec, order, form=None):
     system = get_system(dec, order, compact=form)
     rec = _load().get(_key(dec, order, _label(form, system)))
     return (rec["waves_per_cu"], rec["refill_min"]) if rec else (0, 1)
+
+
+def tune_workloads(out_path=None, verbose=True):
+    """Tune the benchmark workloads (6-body group form; ragged CR3BP over the unrolled
+    form with state rows in registers or in LDS and the compact form; J2 + variational
+    over the wave variants) on samples of their seeded inputs, into the user cache and,
+    with out_path, into a table of the shipped format."""
+    from . import workloads as W
+    from .decompose import taylor_decompose, order_from_tol
+
+    jobs = []
+    nb = W.nbody6(16384)
+    jobs.append((nb.sys, nb.tol, nb.state, None, 2.0, (None,)))
+    cr = W.cr3bp(16384, log_t=(1.0, 2.5))
+    jobs.append((cr.sys, cr.tol, cr.state, cr.pars, cr.t_final, ("unrolled_regs", "unrolled_lds", True)))
+    jv = W.j2var(8192)
+    jobs.append((jv.sys.sys, jv.tol, jv.state, None, 2.0, ("wave16", "wave32", "wave")))
+    table = {}
+    for sys_, tol, st, pa, tf, forms in jobs:
+        dec = taylor_decompose(sys_)
+        order = order_from_tol(tol)
+        best, rows = autotune(dec, order, st, pars=pa, t_final=tf, forms=forms, use_cache=False, verbose=verbose)
+        db = _load()
+        for f in forms:
+            system = get_system(dec, order, compact=f)
+            lab = _label(f, system)
+            cand = [r for r in rows if r["form"] == lab]
+            win = max(cand, key=lambda r: r["rate"])
+            db[_key(dec, order, lab)] = table[_key(dec, order, lab)] = {
+                "waves_per_cu": win["waves_per_cu"], "refill_min": win["refill_min"], "rate": win["rate"],
+                "workload": f"{len(st[0])} instances, form {lab}"}
+        _store(db)
+        if verbose:
+            print(f"[autotune] best {best}", flush=True)
+    if out_path:
+        with open(out_path, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+    return table

@@ -1781,7 +1781,7 @@ bool choose_compact(const Sys &S, int order)
 }
 
 std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compact, int split_min, int sub,
-                             const Sys *E, uint32_t nev)
+                             const Sys *E, uint32_t nev, int lds_rows)
 {
     if (order < 2) throw std::invalid_argument("the Taylor order must be at least 2");
     if (compact) {
@@ -1798,8 +1798,10 @@ std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compac
     // VGPRs (two waves per SIMD) and the rows of a 64-lane block fit in 64 KB. When the
     // node histories alone exceed that, the extra LDS traffic buys no occupancy.
     const long hist = history_nodes(S), nst = S.n;
-    g.lds_state = hist * (order + 1) > 160 && (hist - nst) * (order + 1) <= 96
-                  && (long)(order + 1) * S.n * 64 * 8 <= 65536;
+    const bool rows_fit = (long)(order + 1) * S.n * 64 * 8 <= 65536;
+    if (lds_rows == 1 && !rows_fit) throw std::invalid_argument("the state rows of a 64-lane block exceed 64 KB of LDS");
+    g.lds_state = lds_rows >= 0 ? lds_rows == 1
+                                : hist * (order + 1) > 160 && (hist - nst) * (order + 1) <= 96 && rows_fit;
     return g.run();
 }
 

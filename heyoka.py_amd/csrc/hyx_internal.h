@@ -29,8 +29,10 @@ int parse_oplist(const uint32_t *w, uint32_t n_words, const double *consts, uint
 // sub: lanes per instance of the compact form (1, 2, 4 or 8).
 // With events (E: the event op-list, nev events) the step takes an extra
 // `unsigned long long &hyx_evm`: per event, 1 when its root scan has to run this step.
+// lds_rows (unrolled form): 1 = state rows in LDS, 0 = in registers, -1 = chosen from the
+// register budget.
 std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compact, int split_min = 0, int sub = 1,
-                             const Sys *E = nullptr, uint32_t nev = 0);
+                             const Sys *E = nullptr, uint32_t nev = 0, int lds_rows = -1);
 // Number of nodes whose coefficient history the recurrences read.
 int history_nodes(const Sys &S);
 // Default choice of code form: compact when the histories exceed the register file.

@@ -143,7 +143,9 @@ __device__ __forceinline__ bool hyx_prop_iter(hyx_inst &s, const hyx_kargs &a, h
     s.iter += 1;
     s.nst += (h != 0.0) ? 1 : 0;
     const hyx_i64 oc = nf ? HYX_ERR_NF_STATE : ((te < 0 && h == lim) ? HYX_TIME_LIMIT : HYX_SUCCESS);
-    if (oc == HYX_SUCCESS) {
+    // min_h / max_h over steps the rule chose alone: not those clamped by the time limit
+    // or shortened by a terminal event.
+    if (oc == HYX_SUCCESS && te < 0) {
         const double ah = fabs(h);
         s.minh = (ah < s.minh) ? ah : s.minh;
         s.maxh = (ah > s.maxh) ? ah : s.maxh;

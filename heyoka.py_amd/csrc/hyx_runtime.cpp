@@ -398,7 +398,8 @@ int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint3
             if (s.has_events && s.ev.dir.size() > 64) return fail(HYX_EINVAL, "at most 64 events are supported");
             s.src = (s.has_events ? hyx::events_forward_decl() : std::string()) +
                     hyx::generate_prelude(s.S, (int)order, ha, s.compact, split, sub, s.has_events ? &s.E : nullptr,
-                                          (uint32_t)s.ev.dir.size()) +
+                                          (uint32_t)s.ev.dir.size(),
+                                          (flags & HYX_FLAG_LDS_ROWS) ? 1 : (flags & HYX_FLAG_NO_LDS_ROWS) ? 0 : -1) +
                     "\n";
             if (s.has_events)
                 s.src += hyx::generate_events_prelude(s.E, s.S.n, (int)order, s.ev, ha) + "\n" + k_hyx_kcommon + "\n" +
@@ -783,6 +784,7 @@ int hyx_sys_dim(const hyx_sys *s) { return s ? (int)s->S.n : HYX_EINVAL; }
 int hyx_sys_npar(const hyx_sys *s) { return s ? (int)s->S.npar : HYX_EINVAL; }
 int hyx_sys_is_compact(const hyx_sys *s) { return s ? (int)s->compact : HYX_EINVAL; }
 int hyx_sys_code_form(const hyx_sys *s) { return s ? s->form : HYX_EINVAL; }
+int hyx_sys_lanes_per_instance(const hyx_sys *s) { return s ? (int)(64u / s->inst_per_block) : HYX_EINVAL; }
 
 size_t hyx_sys_source(const hyx_sys *s, char *buf, size_t cap)
 {

@@ -211,6 +211,154 @@ class Decomposition:
         out.extend(ex(w) for w in self.rhs)
         return out
 
+    def as_heyoka(self) -> list:
+        """The decomposition as heyoka's ``taylor_dc_t``: ``(expression, hidden_deps)``
+        pairs, the hidden dependency of a sin node being its cos partner and vice versa
+        (the shape of the reference's ``decomposition`` property,
+        ``heyoka/expose_batch_integrators.cpp:231``)."""
+        deps = [[] for _ in range(self.n_nodes + self.n_state)]
+        for j, nd in enumerate(self.nodes):
+            if nd.op in (OP_SIN, OP_COS):
+                deps[self.n_state + j] = [int(nd.aux)]
+        return [(e, d) for e, d in zip(self.as_expressions(), deps)]
+
+    @classmethod
+    def from_heyoka(cls, dc, n_state=None) -> "Decomposition":
+        """Lower a heyoka-shaped Taylor decomposition to an op-list.
+
+        ``dc`` is the list heyoka's ``taylor_decompose`` / ``decomposition`` returns
+        (``heyoka/expose_batch_integrators.cpp:231``): the n state variables, then m
+        elementary operations over the variables ``u_0 .. u_{n+m-1}`` (each referring
+        only to earlier u's), numbers, ``par[i]`` and ``time``, then the n right-hand
+        sides (a u variable, a number or a parameter). Entries may be expressions or
+        ``(expression, hidden_deps)`` pairs; hidden dependencies are not needed, since a
+        sin and a cos of the same argument are paired here. n is the number of leading
+        entries that are variables unless ``n_state`` is given.
+
+        Each operation becomes one node (n-ary ``prod`` a chain of binary products), so the
+        op-list keeps heyoka's node structure: summation and product orders are those of
+        the decomposition. A sin or a cos without its partner gets one added.
+        """
+        entries = [e[0] if isinstance(e, (tuple, list)) else e for e in dc]
+        entries = [_E._ex(e) for e in entries]
+        if n_state is None:
+            n_state = 0
+            while n_state < len(entries) and entries[n_state].is_var():
+                n_state += 1
+            n_state = min(n_state, len(entries) // 2)
+        n = int(n_state)
+        m = len(entries) - 2 * n
+        if n <= 0 or m < 0:
+            raise ValueError(f"A Taylor decomposition of {len(entries)} entries cannot hold {n} state variables "
+                             "(n variables, m operations and n right-hand sides are expected)")
+        names = []
+        for i in range(n):
+            if not entries[i].is_var():
+                raise ValueError(f"Entry {i} of the decomposition must be a state variable, not '{entries[i]}'")
+            names.append(entries[i].name)
+        lw = _Lowerer(names)
+        w = {i: operand(KIND_NODE, i) for i in range(n)}
+
+        def arg(a, i):
+            if a.is_var():
+                nm = a.name
+                if not (nm.startswith("u_") and nm[2:].isdigit()):
+                    raise ValueError(f"Entry {i} of the decomposition refers to '{nm}', which is not a u variable")
+                j = int(nm[2:])
+                if j >= i or j not in w:
+                    raise ValueError(f"Entry {i} of the decomposition refers to u_{j}, which is not defined before it")
+                return w[j]
+            if a.is_number():
+                return lw.const(a.value)
+            if a.is_par():
+                lw.n_par = max(lw.n_par, a.idx + 1)
+                return operand(KIND_PAR, a.idx)
+            if a.is_time():
+                return lw.node(OP_TIME, ())
+            raise ValueError(f"Entry {i} of the decomposition has a nested operand '{a}': every operand of an "
+                             "elementary operation must be a u variable, a number, a parameter or time")
+
+        binary = {"add": OP_ADD, "sub": OP_SUB, "div": OP_DIV}
+        unary = {"neg": OP_NEG, "square": OP_SQUARE, "sqrt": OP_SQRT, "exp": OP_EXP, "log": OP_LOG}
+        for i in range(n, n + m):
+            e = entries[i]
+            if e.is_time():
+                w[i] = lw.node(OP_TIME, ())
+                continue
+            if not e.is_func():
+                raise ValueError(f"Entry {i} of the decomposition must be an elementary operation, not '{e}'")
+            name = e.name
+            # A power's exponent is a number folded into the node, not an operand.
+            a = [arg(x, i) for x in (e.args[:1] if name == "pow" else e.args)]
+            if name == "sum" or (name == "add" and len(a) != 2):
+                if len(a) < 2:
+                    raise ValueError(f"Entry {i} of the decomposition is a sum of {len(a)} term(s)")
+                w[i] = lw.node(OP_ADD, a) if len(a) == 2 else lw.node(OP_SUM, a)
+            elif name in ("mul", "prod"):
+                if len(a) < 2:
+                    raise ValueError(f"Entry {i} of the decomposition is a product of {len(a)} factor(s)")
+                acc = lw.node(OP_MUL, a[:2])
+                for x in a[2:]:
+                    acc = lw.node(OP_MUL, [acc, x])
+                w[i] = acc
+            elif name in binary and len(a) == 2:
+                w[i] = lw.node(binary[name], a)
+            elif name in unary and len(a) == 1:
+                w[i] = lw.node(unary[name], a)
+            elif name in ("sin", "cos") and len(a) == 1:
+                s, c = lw.sincos(a[0])
+                w[i] = s if name == "sin" else c
+            elif name == "pow" and len(e.args) == 2:
+                ex = e.args[1]
+                if not ex.is_number():
+                    raise NotImplementedError(f"Entry {i} of the decomposition is a power with a non-constant "
+                                              "exponent, which heyoka decomposes further")
+                v = ex.value
+                if v == 2.0:
+                    w[i] = lw.node(OP_SQUARE, a[:1])
+                elif v == 0.5:
+                    w[i] = lw.node(OP_SQRT, a[:1])
+                else:
+                    w[i] = lw.node(OP_POW, a[:1], operand_index(lw.const(v)))
+            else:
+                raise NotImplementedError(f"Unsupported elementary operation '{name}' with {len(a)} argument(s) "
+                                          f"in entry {i} of the decomposition")
+        rhs = [arg(entries[n + m + j], n + m + j) if not entries[n + m + j].is_time() else lw.node(OP_TIME, ())
+               for j in range(n)]
+        nodes, rhs, consts = _canonical_consts(lw.nodes, rhs, lw.consts)
+        return cls(state_names=names, n_par=lw.n_par, nodes=nodes, rhs=rhs, consts=consts, sys=[])
+
+
+def _canonical_consts(nodes, rhs, consts):
+    """Renumber the constants in order of first use (node operands and pow exponents in
+    node order, then the right-hand sides), so that equal node lists give equal op-list
+    words whatever order a lowering met the constants in."""
+    order, seen = [], {}
+
+    def use(i):
+        if i not in seen:
+            seen[i] = len(order)
+            order.append(i)
+
+    for nd in nodes:
+        for a in nd.args:
+            if operand_kind(a) == KIND_CONST:
+                use(operand_index(a))
+        if nd.op == OP_POW:
+            use(nd.aux)
+    for a in rhs:
+        if operand_kind(a) == KIND_CONST:
+            use(operand_index(a))
+    for i in range(len(consts)):
+        use(i)
+
+    def mp(a):
+        return operand(KIND_CONST, seen[operand_index(a)]) if operand_kind(a) == KIND_CONST else a
+
+    new_nodes = [Node(nd.op, tuple(mp(a) for a in nd.args), seen[nd.aux] if nd.op == OP_POW else nd.aux)
+                 for nd in nodes]
+    return new_nodes, [mp(a) for a in rhs], [consts[i] for i in order]
+
 
 class _Lowerer:
     def __init__(self, state_names):
@@ -302,6 +450,12 @@ class _Lowerer:
             if len(ws) == 2:
                 return self.node(OP_ADD, ws)
             return self.node(OP_SUM, ws)
+        if name == "prod":
+            # heyoka's n-ary product: binary products from the left.
+            acc = args[0]
+            for a in args[1:]:
+                acc = expression._func("mul", acc, a)
+            return self.lower(acc)
         if name == "sub":
             return self.node(OP_SUB, [self.lower(args[0]), self.lower(args[1])])
         if name == "neg":
@@ -369,7 +523,8 @@ def taylor_decompose(sys) -> Decomposition:
         raise ValueError("Cannot decompose an empty ODE system")
     lw = _Lowerer(names)
     rhs = [lw.lower(r) for _, r in pairs]
-    return Decomposition(state_names=names, n_par=lw.n_par, nodes=lw.nodes, rhs=rhs, consts=lw.consts, sys=pairs)
+    nodes, rhs, consts = _canonical_consts(lw.nodes, rhs, lw.consts)
+    return Decomposition(state_names=names, n_par=lw.n_par, nodes=nodes, rhs=rhs, consts=consts, sys=pairs)
 
 
 def order_from_tol(tol: float) -> int:

@@ -40,7 +40,7 @@ __all__ = [
 _VAR, _NUM, _PAR, _TIME, _FUNC = range(5)
 
 # Function names understood by the decomposer.
-_FUNCS = {"add", "sub", "mul", "div", "neg", "pow", "sqrt", "exp", "log", "sin", "cos", "square", "sum"}
+_FUNCS = {"add", "sub", "mul", "div", "neg", "pow", "sqrt", "exp", "log", "sin", "cos", "square", "sum", "prod"}
 
 
 def _fmt_num(v: float) -> str:

@@ -118,7 +118,19 @@ def get_system(dec, order, high_accuracy=False, compact=None, events=None):
         if s is None:
             s = _lib.System(dec, order, high_accuracy=high_accuracy, compact=compact, events=events)
             _sys_cache[key] = s
-        return s
+    if compact is None and s.form == "wave" and s.tune_key is not None:
+        # The wave form's variants (16, 32, 64 lanes per instance) compute bit-identical
+        # results: the automatic choice takes the one the autotuner recorded as fastest.
+        rates = {}
+        for lab in ("wave16", "wave32", "wave"):
+            rec = _lib.tuned_launch(_lib.tune_key(words, consts, order, lab))
+            if rec is not None:
+                rates[lab] = rec[2]
+        if rates:
+            best = max(rates, key=rates.get)
+            if best != s.label:
+                return get_system(dec, order, high_accuracy, best, events)
+    return s
 
 
 # ---- double-length arithmetic on the host (same algorithm as the kernels) ----------------
@@ -238,7 +250,8 @@ class taylor_adaptive_batch:
         if tol == 0.0:
             tol = float(np.finfo(np.float64).eps)
         order = order_from_tol(tol)
-        compact = {"compact": True, "unrolled": False, "group": "group", "wave": "wave", "wave32": "wave32",
+        compact = {"compact": True, "unrolled": False, "unrolled_lds": "unrolled_lds",
+                   "unrolled_regs": "unrolled_regs", "group": "group", "wave": "wave", "wave32": "wave32",
                    "wave16": "wave16", None: None}[code_form]
         tes, ntes = events
         spec = None
@@ -261,7 +274,10 @@ class taylor_adaptive_batch:
         d["_hyx_system"] = system
         d["_hyx_B"] = B
         d["_hyx_npar"] = npar
-        d["_hyx_batch"] = _lib.Batch(system, B)
+        # The device batch is allocated on first use (_hyx_batch): copies made for an
+        # ensemble are packed into one device batch and never need their own.
+        d["_hyx_batch_obj"] = None
+        d["_hyx_cd_pending"] = None
         d["_hyx_state"] = np.ascontiguousarray(st, dtype=np.float64).copy()
         d["_hyx_pars"] = np.ascontiguousarray(pa, dtype=np.float64).copy()
         d["_hyx_thi"] = tm.copy()
@@ -276,6 +292,36 @@ class taylor_adaptive_batch:
                 d[k] = v
 
     # ---- host <-> device ------------------------------------------------------------------
+    @property
+    def _hyx_batch(self):
+        """The device batch of this integrator, created on first use (with the cooldowns a
+        copy inherited)."""
+        d = self.__dict__
+        b = d["_hyx_batch_obj"]
+        if b is None:
+            b = _lib.Batch(self._hyx_system, self._hyx_B)
+            if d["_hyx_cd_pending"] is not None:
+                b.set_cooldowns(d["_hyx_cd_pending"])
+                d["_hyx_cd_pending"] = None
+            d["_hyx_batch_obj"] = b
+        return b
+
+    def _hyx_cooldowns(self):
+        """[n_te, 3, B] cooldowns (trigger time hi, lo, length; length < 0: none)."""
+        d = self.__dict__
+        if d["_hyx_batch_obj"] is None:
+            if d["_hyx_cd_pending"] is not None:
+                return d["_hyx_cd_pending"].copy()
+            return np.full((len(self._hyx_tes), 3, self._hyx_B), -1.0)
+        return d["_hyx_batch_obj"].cooldowns()
+
+    def _hyx_set_cooldowns(self, cd):
+        d = self.__dict__
+        if d["_hyx_batch_obj"] is None:
+            d["_hyx_cd_pending"] = np.array(cd, dtype=np.float64)
+        else:
+            d["_hyx_batch_obj"].set_cooldowns(cd)
+
     def _push(self):
         b = self._hyx_batch
         b.write(_lib.FIELD_STATE, self._hyx_state)
@@ -556,9 +602,10 @@ class taylor_adaptive_batch:
             it += 1
             act = ~done
             nst += (act & (h != 0)).astype(np.int64)
-            # Steps that ended normally or at a terminal event (codes above success) count
-            # towards min_h / max_h; a stopping terminal event (-(index + 1)) ends the lane.
-            succ = act & (oc >= int(taylor_outcome.success))
+            # Only steps that ended normally count towards min_h / max_h: steps clamped by
+            # the time limit or shortened by a terminal event (an event index, or
+            # -(index + 1) when it stops the lane) do not.
+            succ = act & (oc == int(taylor_outcome.success))
             min_h = np.where(succ, np.minimum(min_h, np.abs(h)), min_h)
             max_h = np.where(succ, np.maximum(max_h, np.abs(h)), max_h)
             nf = act & (oc == int(taylor_outcome.err_nf_state))
@@ -766,7 +813,9 @@ class taylor_adaptive_batch:
 
     @property
     def decomposition(self):
-        return self._hyx_dec.as_expressions()
+        """heyoka's taylor_dc_t: (expression, hidden_deps) pairs (n variables, m operations,
+        n right-hand sides)."""
+        return self._hyx_dec.as_heyoka()
 
     @property
     def sys(self):
@@ -798,7 +847,7 @@ class taylor_adaptive_batch:
         B = self._hyx_B
         if not self._hyx_tes:
             return [[] for _ in range(B)]
-        cd = self._hyx_batch.cooldowns()
+        cd = self._hyx_cooldowns()
         return [[(float(cd[e, 0, b] + cd[e, 1, b]), float(cd[e, 2, b])) if cd[e, 2, b] >= 0 else None
                  for e in range(len(self._hyx_tes))] for b in range(B)]
 
@@ -806,7 +855,7 @@ class taylor_adaptive_batch:
         """Clear the cooldowns of every batch element, or of batch element i."""
         if not self._hyx_tes:
             return
-        cd = self._hyx_batch.cooldowns()
+        cd = self._hyx_cooldowns()
         if i is None:
             cd[:, 2, :] = -1.0
         else:
@@ -814,7 +863,7 @@ class taylor_adaptive_batch:
                 raise ValueError(f"Cannot reset the cooldowns at batch index {i}: the batch size for this integrator "
                                  f"is only {self._hyx_B}")
             cd[:, 2, int(i)] = -1.0
-        self._hyx_batch.set_cooldowns(cd)
+        self._hyx_set_cooldowns(cd)
 
     # ---- variational --------------------------------------------------------------------
     @property
@@ -908,16 +957,25 @@ class taylor_adaptive_batch:
     def _events_copy(self):
         return ([_copy.copy(e) for e in self._hyx_tes], [_copy.copy(e) for e in self._hyx_ntes])
 
+    # Shared by copies: the system, its decomposition and the compiled code do not change
+    # after construction.
+    _SHARED_KEYS = ("_hyx_pairs", "_hyx_vsys", "_hyx_dec", "_hyx_order", "_hyx_tol", "_hyx_flags", "_hyx_system",
+                    "_hyx_B", "_hyx_npar")
+
     def _clone(self, user):
+        """A copy that shares the decomposition and the compiled system, with its own host
+        state and (on first use) its own device batch."""
         new = taylor_adaptive_batch.__new__(taylor_adaptive_batch)
-        f = self._hyx_flags
-        restore = {k: _copy.deepcopy(self.__dict__[k]) for k in self._STATE_KEYS}
-        sysarg = self._hyx_vsys if self._hyx_vsys is not None else self._hyx_pairs
-        new._hyx_init(sysarg, self._hyx_state, self._hyx_thi, self._hyx_pars, self._hyx_tol, f[0], f[1], f[2], f[3],
-                      f[4], restore=restore, events=self._events_copy())
-        if self._hyx_tes:
-            new._hyx_batch.set_cooldowns(self._hyx_batch.cooldowns())
-        new.__dict__.update(user)
+        d, src = new.__dict__, self.__dict__
+        for k in self._SHARED_KEYS:
+            d[k] = src[k]
+        for k in self._STATE_KEYS:
+            v = src[k]
+            d[k] = v.copy() if isinstance(v, np.ndarray) else list(v)
+        d["_hyx_tes"], d["_hyx_ntes"] = self._events_copy()
+        d["_hyx_batch_obj"] = None
+        d["_hyx_cd_pending"] = self._hyx_cooldowns() if self._hyx_tes else None
+        d.update(user)
         return new
 
     def __copy__(self):
@@ -932,7 +990,7 @@ class taylor_adaptive_batch:
                            self._hyx_flags)
         st["_hyx_user"] = self._user_attrs()
         st["_hyx_events"] = (self._hyx_tes, self._hyx_ntes)
-        st["_hyx_cooldowns"] = self._hyx_batch.cooldowns() if self._hyx_tes else None
+        st["_hyx_cooldowns"] = self._hyx_cooldowns() if self._hyx_tes else None
         return st
 
     def __setstate__(self, st):
@@ -941,7 +999,7 @@ class taylor_adaptive_batch:
         self._hyx_init(sysarg, restore["_hyx_state"], restore["_hyx_thi"], restore["_hyx_pars"], tol, f[0], f[1], f[2],
                        f[3], f[4], restore=restore, events=st.get("_hyx_events", ([], [])))
         if st.get("_hyx_cooldowns") is not None:
-            self._hyx_batch.set_cooldowns(st["_hyx_cooldowns"])
+            self._hyx_set_cooldowns(st["_hyx_cooldowns"])
         self.__dict__.update(st["_hyx_user"])
 
     def __repr__(self):

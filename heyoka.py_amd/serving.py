@@ -108,6 +108,7 @@ class Server:
         self.instance_steps = 0
         self.busy_time = 0.0
         self.epochs = 0
+        self._systems: dict = {}
 
     # ---- submission ---------------------------------------------------------------------
     def submit(self, sys, state, t_final, pars=None, tol=1e-15, t0=0.0, arrival=None) -> Request:
@@ -117,12 +118,17 @@ class Server:
         m = st.shape[1]
         if m == 0 or m > self.capacity:
             raise ValueError(f"A request must hold between 1 and {self.capacity} instances")
-        pairs = sys.sys if hasattr(sys, "n_orig_sv") else [(_ex(a), _ex(b)) for a, b in sys]
-        dec = taylor_decompose(pairs)
+        order = order_from_tol(tol)
+        # Requests usually repeat a system object: its decomposition and compiled code are
+        # looked up by identity (the cache holds the object, so its id stays valid).
+        hit = self._systems.get((id(sys), order))
+        if hit is None:
+            pairs = sys.sys if hasattr(sys, "n_orig_sv") else [(_ex(a), _ex(b)) for a, b in sys]
+            dec = taylor_decompose(pairs)
+            hit = self._systems[(id(sys), order)] = (sys, dec, get_system(dec, order))
+        _, dec, system = hit
         if st.shape[0] != dec.n_state:
             raise ValueError(f"The state has {st.shape[0]} rows but the system has {dec.n_state} variables")
-        order = order_from_tol(tol)
-        system = get_system(dec, order)
         key = (id(system),)
         pa = None
         if dec.n_par:

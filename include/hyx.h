@@ -71,6 +71,10 @@ typedef struct {
 #define HYX_FLAG_FORCE_WAVE 4096u
 #define HYX_FLAG_WAVE32 8192u
 #define HYX_FLAG_WAVE16 16384u
+/* Unrolled form: keep every order of the state variables in LDS rows (HYX_FLAG_LDS_ROWS) or
+ * in registers (HYX_FLAG_NO_LDS_ROWS); neither: chosen from the register budget. */
+#define HYX_FLAG_LDS_ROWS 32768u
+#define HYX_FLAG_NO_LDS_ROWS 65536u
 
 /* Code forms (hyx_sys_code_form). */
 #define HYX_FORM_UNROLLED 0 /* one instance per lane, fully unrolled, histories in registers */
@@ -109,6 +113,9 @@ int hyx_sys_npar(const hyx_sys *s);
 int hyx_sys_is_compact(const hyx_sys *s);
 /* The generated code form: HYX_FORM_UNROLLED, HYX_FORM_COMPACT, HYX_FORM_GROUP or HYX_FORM_WAVE. */
 int hyx_sys_code_form(const hyx_sys *s);
+/* Lanes of a wavefront that one instance occupies (64 / instances per wavefront). No
+ * counterpart in the reference, whose instances are SIMD lanes of a CPU batch. */
+int hyx_sys_lanes_per_instance(const hyx_sys *s);
 /* The generated HIP source (for inspection); returns its length, copies up to cap-1 bytes. */
 size_t hyx_sys_source(const hyx_sys *s, char *buf, size_t cap);
 /* Register usage of the compiled Taylor-step kernels: VGPRs, scratch bytes per lane. */

@@ -276,7 +276,8 @@ def propagate(hy, sys_pairs, state, tf, t_events=(), nt_events=(), pars=None, to
                 res[b], done[b] = ERR_NF_STATE, True
                 continue
             step_oc = (-(te[0] + 1)) if te is not None else oc
-            if te is not None or oc == SUCCESS:
+            # heyoka counts only unclamped steps: a step shortened by a terminal event does not count.
+            if te is None and oc == SUCCESS:
                 mn[b] = min(mn[b], abs(hend))
                 mx[b] = max(mx[b], abs(hend))
             if te is not None and tes[te[0]][3]:
