@@ -99,6 +99,9 @@ def lib():
             "hyx_step": (ct.c_int, [vp, P(d), ct.c_int, P(i64), P(d)]),
             "hyx_read_tc": (ct.c_int, [vp, P(d)]),
             "hyx_propagate_until": (ct.c_int, [vp, P(d), P(d), P(d), u64, ct.c_int, P(prop_res)]),
+            # Device pointers throughout (caller-owned arrays), then a hipStream_t.
+            "hyx_propagate_until_noalloc": (ct.c_int, [vp, u64, vp, vp, vp, vp, vp, vp, vp, u64, vp, vp, vp, vp,
+                                                       vp]),
             "hyx_propagate_grid": (ct.c_int, [vp, P(d), u64, P(d), u64, ct.c_int, P(prop_res), P(d)]),
             "hyx_batch_last_timing": (ct.c_int, [vp, P(d), P(u64)]),
             "hyx_batch_set_persistent_config": (ct.c_int, [vp, u32, u32]),
@@ -151,6 +154,7 @@ EXPORTED = (
     "hyx_sys_source",
     "hyx_sys_kernel_info", "hyx_sys_destroy", "hyx_batch_create", "hyx_batch_destroy", "hyx_batch_size",
     "hyx_batch_write", "hyx_batch_read", "hyx_batch_device_ptr", "hyx_step", "hyx_read_tc", "hyx_propagate_until",
+    "hyx_propagate_until_noalloc",
     "hyx_propagate_grid",
     "hyx_batch_last_timing", "hyx_batch_set_persistent_config", "hyx_batch_set_queue", "hyx_batch_set_iter_budget",
     "hyx_batch_sync", "hyx_slots_reset", "hyx_slots_write", "hyx_slots_read", "hyx_slots_arm", "hyx_slots_launch",

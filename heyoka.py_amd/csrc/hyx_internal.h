@@ -9,6 +9,22 @@
 namespace hyx
 {
 
+// propagate_until over caller-owned device arrays with every scratch buffer carved out of
+// `scratch` (cap bytes): the body of hyx_propagate_until_noalloc (hyx_noalloc.hip), which
+// owns the fixed device array. Enqueues its kernels on `stream` and allocates nothing.
+struct NoAllocArgs {
+    uint64_t n_inst;
+    double *state;
+    const double *pars;
+    double *t_hi, *t_lo;
+    const double *tf_hi, *tf_lo, *max_dt;
+    uint64_t max_steps;
+    int64_t *outcome;
+    double *min_h, *max_h;
+    uint64_t *nsteps;
+    void *stream;
+};
+
 // A parsed op-list (include/hyx_oplist.h).
 struct Sys {
     uint32_t n = 0, npar = 0, nnodes = 0, nconsts = 0;
@@ -97,5 +113,12 @@ std::string events_forward_decl();
 std::string lit(double v);
 
 } // namespace hyx
+
+struct hyx_sys;
+// Enqueues the propagation of NoAllocArgs on its stream with every scratch buffer inside
+// [scratch, scratch + cap) (a device address); returns 0 or a negative HYX_E* code.
+int hyx_noalloc_propagate(hyx_sys *s, const hyx::NoAllocArgs &na, unsigned char *scratch, uint64_t cap);
+// Sets the thread's hyx_last_error() message and returns code.
+int hyx_internal_fail(int code, const std::string &msg);
 
 #endif

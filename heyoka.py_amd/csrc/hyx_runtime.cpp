@@ -1213,6 +1213,122 @@ int hyx_propagate_until(hyx_batch *b, const double *tf_hi, const double *tf_lo, 
     return run_propagation(b, max_steps, mode, res, false);
 }
 
+} // extern "C"
+
+int hyx_internal_fail(int code, const std::string &msg) { return fail(code, msg); }
+
+// The body of hyx_propagate_until_noalloc (hyx_noalloc.hip): the persistent form over
+// caller-owned device arrays. Scratch layout inside [scratch, scratch + cap), 256-byte
+// aligned pieces: the work counter and the unfinished count, then per instance the
+// iteration count, the status word and the last step size, a zero parameter row when the
+// system has no parameters, tf_lo / max_delta_t when the caller passes none, and the
+// compact form's Taylor tape, whose columns are the persistent grid's slots (the grid is
+// narrowed until the tape fits). One launch with an unbounded iteration budget runs every
+// instance to its end, so nothing is read back: the kernels are only enqueued on the
+// caller's stream, and the host allocates nothing.
+int hyx_noalloc_propagate(hyx_sys *s, const hyx::NoAllocArgs &na, unsigned char *scratch, uint64_t cap)
+{
+    if (!s || !s->mod) return fail(HYX_EINVAL, "the system was not created for a device");
+    if (s->has_events) return fail(HYX_EINVAL, "systems with events are not supported without allocation");
+    const uint64_t N = na.n_inst;
+    if (N == 0 || N > (1ull << 32)) return fail(HYX_EINVAL, "the number of instances must be in [1, 2^32]");
+    if (!na.state || !na.t_hi || !na.t_lo || !na.tf_hi || !na.outcome || !na.min_h || !na.max_h || !na.nsteps)
+        return fail(HYX_EINVAL, "null device array");
+    if (s->S.npar > 0 && !na.pars) return fail(HYX_EINVAL, "the system has parameters: pars must be given");
+    int cur = -1;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != s->device) return fail(HYX_EINVAL, "the current device is not the one the system was created on");
+    hipStream_t st = (hipStream_t)na.stream;
+    auto up = [](uint64_t x) { return (x + 255) & ~255ull; };
+    uint64_t off = 256;
+    auto take = [&](uint64_t bytes) {
+        const uint64_t o = off;
+        off = up(off + bytes);
+        return o;
+    };
+    const uint64_t o_iters = take(N * 8), o_status = take(N * 4), o_hout = take(N * 8);
+    const uint64_t o_pars = na.pars ? 0 : take(N * 8);
+    const uint64_t o_tflo = na.tf_lo ? 0 : take(N * 8);
+    const uint64_t o_mdt = na.max_dt ? 0 : take(N * 8);
+    if (off > cap)
+        return fail(HYX_EINVAL, "hyx_propagate_until_noalloc: " + std::to_string(N) + " instances need " +
+                                    std::to_string(off) + " bytes of scratch, beyond the fixed " + std::to_string(cap) +
+                                    " bytes (HYX_NOALLOC_SCRATCH_BYTES)");
+    int occ = 0, n_cu = 0;
+    HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, s->k_persist, 64, 0));
+    HIPCHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, s->device));
+    const uint64_t blocks = (N + s->inst_per_block - 1) / s->inst_per_block;
+    uint64_t pgrid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)std::max(occ, 1) * (uint64_t)n_cu, blocks));
+    uint64_t o_tape = 0, stride = 0;
+    if (s->compact) {
+        const uint64_t rows = (uint64_t)s->S.nnodes * (s->order + 1);
+        const uint64_t room = cap > off ? cap - off : 0;
+        const uint64_t per_block = rows * s->inst_per_block * 8;
+        pgrid = std::min<uint64_t>(pgrid, room / per_block);
+        if (pgrid == 0)
+            return fail(HYX_EINVAL, "hyx_propagate_until_noalloc: the Taylor tape of one block (" +
+                                        std::to_string(per_block) + " bytes) does not fit the scratch left (" +
+                                        std::to_string(room) + " bytes)");
+        stride = pgrid * s->inst_per_block;
+        o_tape = take(rows * stride * 8);
+    }
+    unsigned char *const sc = scratch;
+    KArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.state = na.state;
+    a.pars = na.pars ? na.pars : (const double *)(sc + o_pars);
+    a.t_hi = na.t_hi;
+    a.t_lo = na.t_lo;
+    a.tf_hi = na.tf_hi;
+    a.tf_lo = na.tf_lo ? na.tf_lo : (const double *)(sc + o_tflo);
+    a.max_dt = na.max_dt ? na.max_dt : (const double *)(sc + o_mdt);
+    a.h_out = (double *)(sc + o_hout);
+    a.outcome = na.outcome;
+    a.min_h = na.min_h;
+    a.max_h = na.max_h;
+    a.nsteps = na.nsteps;
+    a.iters = (uint64_t *)(sc + o_iters);
+    a.status = (unsigned *)(sc + o_status);
+    a.tape = s->compact ? (double *)(sc + o_tape) : nullptr;
+    a.tape_stride = stride;
+    a.work_counter = (uint64_t *)sc;
+    a.unfinished = (unsigned *)(sc + 8);
+    a.N = N;
+    a.ld = N;
+    a.max_steps = na.max_steps;
+    a.iter_budget = ~0ull;
+    a.refill_min = 1;
+    auto fill64 = [&](const void *p, double v) -> hipError_t {
+        uint64_t bits;
+        std::memcpy(&bits, &v, 8);
+        void *pp = const_cast<void *>(p);
+        uint64_t n = N;
+        void *params[] = {&pp, &bits, &n};
+        const unsigned g = (unsigned)std::min<uint64_t>((N + 255) / 256, 4096);
+        return hipModuleLaunchKernel(s->k_fill64, g, 1, 1, 256, 1, 1, 0, st, params, nullptr);
+    };
+    // The scratch keeps the previous call's contents: everything read before it is
+    // written is set here (the counters, the default arrays; the init kernel sets the
+    // per-instance bookkeeping).
+    if (!na.pars) HIPCHK(fill64(a.pars, 0.0));
+    if (!na.tf_lo) HIPCHK(fill64(a.tf_lo, 0.0));
+    if (!na.max_dt) HIPCHK(fill64(a.max_dt, std::numeric_limits<double>::infinity()));
+    {
+        uint64_t *w = a.work_counter;
+        unsigned one = 1;
+        void *p64[] = {&w, &one};
+        HIPCHK(hipModuleLaunchKernel(s->k_clr64, 1, 1, 1, 64, 1, 1, 0, st, p64, nullptr));
+        unsigned *u = a.unfinished;
+        void *p32[] = {&u, &one};
+        HIPCHK(hipModuleLaunchKernel(s->k_clr32, 1, 1, 1, 64, 1, 1, 0, st, p32, nullptr));
+    }
+    HIPCHK(launch(s->k_init, blocks_for(N), st, &a));
+    HIPCHK(launch(s->k_persist, pgrid, st, &a));
+    return 0;
+}
+
+extern "C" {
+
 int hyx_propagate_grid(hyx_batch *b, const double *grid, uint64_t n_pts, const double *max_delta_t,
                        uint64_t max_steps, int mode, hyx_prop_res *res, double *out)
 {

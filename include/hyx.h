@@ -149,6 +149,27 @@ int hyx_read_tc(hyx_batch *b, double *dst);
 int hyx_propagate_until(hyx_batch *b, const double *tf_hi, const double *tf_lo, const double *max_delta_t,
                         uint64_t max_steps, int mode, hyx_prop_res *res);
 
+/* No counterpart in the reference: propagate_until for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). It runs the kernels of hyx_propagate_until's persistent form over the
+ * caller's device arrays: state [n][N], pars [n_par][N] (NULL when the system has no
+ * parameters), t_hi / t_lo [N] (read, then written with the final times), tf_hi [N],
+ * tf_lo [N] or NULL (0), max_delta_t [N] or NULL (no limit); outputs outcome, min_h,
+ * max_h, nsteps [N] (propagate_res). It allocates nothing: its scratch (per-instance
+ * bookkeeping, work-queue counters, default arrays, the compact form's Taylor tape) is a
+ * fixed device array of HYX_NOALLOC_SCRATCH_BYTES per GPU, held from the first call on a
+ * device until the process ends; a call whose scratch would not fit fails with HYX_EINVAL
+ * and a message giving the bytes needed (about 20-44 bytes per instance, plus the tape).
+ * The kernels are enqueued on `stream` (a hipStream_t; NULL = the null stream) and the call
+ * returns without waiting: results are valid once the stream has completed. Every call on
+ * one GPU shares that GPU's scratch, so calls that overlap on one GPU race. Systems with
+ * events are not supported. The current device must be the system's. */
+#define HYX_NOALLOC_SCRATCH_BYTES (64u << 20)
+int hyx_propagate_until_noalloc(hyx_sys *s, uint64_t n_inst, double *state, const double *pars, double *t_hi,
+                                double *t_lo, const double *tf_hi, const double *tf_lo, const double *max_delta_t,
+                                uint64_t max_steps, int64_t *outcome, double *min_h, double *max_h,
+                                uint64_t *nsteps, void *stream);
+
 /* propagate_grid (heyoka/expose_batch_integrators.cpp:315-392): propagates every
  * instance to its last grid time and writes the state at each grid time by dense
  * output of the step that contains it. grid: [n_pts][N] host times; per instance
