@@ -27,6 +27,7 @@ FLAG_WAVE32 = 8192
 FLAG_WAVE16 = 16384
 FLAG_LDS_ROWS = 32768
 FLAG_NO_LDS_ROWS = 65536
+FLAG_NO_TWIN = 131072
 FORMS = {0: "unrolled", 1: "compact", 2: "group", 3: "wave"}
 FLAG_GRID = 64
 FLAG_SPLIT_CHAINS = 128
@@ -87,6 +88,9 @@ def lib():
             "hyx_sys_is_compact": (ct.c_int, [vp]),
             "hyx_sys_code_form": (ct.c_int, [vp]),
             "hyx_sys_lanes_per_instance": (ct.c_int, [vp]),
+            "hyx_sys_twin_source": (ct.c_size_t, [vp, ct.c_char_p, ct.c_size_t]),
+            "hyx_batch_set_twin_tail": (ct.c_int, [vp, ct.c_uint32]),
+            "hyx_batch_twin_launches": (u64, [vp]),
             "hyx_sys_source": (ct.c_size_t, [vp, ct.c_char_p, ct.c_size_t]),
             "hyx_sys_kernel_info": (ct.c_int, [vp, P(ct.c_int), P(ct.c_int)]),
             "hyx_sys_destroy": (None, [vp]),
@@ -150,7 +154,8 @@ def lib():
 
 EXPORTED = (
     "hyx_sys_create", "hyx_sys_precompile", "hyx_sys_order", "hyx_sys_dim", "hyx_sys_npar", "hyx_sys_is_compact",
-    "hyx_sys_code_form", "hyx_sys_lanes_per_instance",
+    "hyx_sys_code_form", "hyx_sys_lanes_per_instance", "hyx_sys_twin_source", "hyx_batch_set_twin_tail",
+    "hyx_batch_twin_launches",
     "hyx_sys_source",
     "hyx_sys_kernel_info", "hyx_sys_destroy", "hyx_batch_create", "hyx_batch_destroy", "hyx_batch_size",
     "hyx_batch_write", "hyx_batch_read", "hyx_batch_device_ptr", "hyx_step", "hyx_read_tc", "hyx_propagate_until",
@@ -289,12 +294,13 @@ class System:
     """Owner of a hyx_sys handle."""
 
     def __init__(self, dec, order: int, high_accuracy: bool = False, compile_only: bool = False, compact=None,
-                 source_only: bool = False, split_chains: bool = False,
+                 source_only: bool = False, split_chains: bool = False, twin: bool = True,
                  pipeline: bool = True, sub: int = 2, events: EventSpec | None = None):
         L = lib()
         words, consts = oplist_args(dec)
         flags = (_flags(high_accuracy, compact) | (FLAG_COMPILE_ONLY if compile_only else 0)
                  | (FLAG_SOURCE_ONLY if source_only else 0) | (FLAG_SPLIT_CHAINS if split_chains else 0)
+                 | (0 if twin else FLAG_NO_TWIN)
                  | (0 if pipeline else FLAG_NO_PIPELINE) | FLAG_SUB[int(sub)])
         h = ct.c_void_p()
         if events is None:
@@ -331,6 +337,15 @@ class System:
         n = self._L.hyx_sys_source(self.h, None, 0)
         buf = ct.create_string_buffer(n + 1)
         self._L.hyx_sys_source(self.h, buf, n + 1)
+        return buf.value.decode()
+
+    def twin_source(self) -> str:
+        """The twin form's source ("" when the system has none)."""
+        n = self._L.hyx_sys_twin_source(self.h, None, 0)
+        if n == 0:
+            return ""
+        buf = ct.create_string_buffer(n + 1)
+        self._L.hyx_sys_twin_source(self.h, buf, n + 1)
         return buf.value.decode()
 
     def kernel_info(self):
@@ -517,6 +532,13 @@ class Batch:
         else:
             o = np.ascontiguousarray(order, dtype=np.uint32)
             check(self._L.hyx_batch_set_queue(self.h, dptr(o, ct.c_uint32)))
+
+    def set_twin_tail(self, tail_min: int):
+        """Hand-off threshold of persistent propagations to the twin form (0: never)."""
+        check(self._L.hyx_batch_set_twin_tail(self.h, int(tail_min)))
+
+    def twin_launches(self) -> int:
+        return int(self._L.hyx_batch_twin_launches(self.h))
 
     def set_iter_budget(self, iters: int):
         check(self._L.hyx_batch_set_iter_budget(self.h, int(iters)))

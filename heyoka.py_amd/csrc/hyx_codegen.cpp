@@ -21,10 +21,13 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <functional>
+#include <map>
 #include <set>
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -62,6 +65,8 @@ void compute_degrees(Sys &S)
         case HYX_OP_MUL: d = std::min(INF_DEG, od(A[0]) + od(A[1])); break;
         case HYX_OP_SQUARE: d = std::min(INF_DEG, 2 * od(A[0])); break;
         case HYX_OP_DIV: d = (od(A[1]) == 0) ? od(A[0]) : INF_DEG; break;
+        case HYX_OP_TWIN_BIND: d = 0; break;
+        case HYX_OP_TWIN_XCH: d = od(A[0]); break;
         default: d = (od(A[0]) == 0) ? 0 : INF_DEG;
         }
         S.deg[j] = d;
@@ -198,6 +203,14 @@ struct Gen {
     const Sys *E = nullptr;
     uint32_t nev = 0;
     const Gen *base = nullptr;
+    // Twin form (HYX_OP_TWIN_BIND / HYX_OP_TWIN_XCH nodes, see plan_twin): two lanes per
+    // instance, the lane flag hyx_twin_b and the partner exchange hyx_twin_swap().
+    bool twin = false;
+    // No implicit fusion of a product into a following add inside the step: the lane
+    // form of a system that has a twin form must round exactly as the twin does, and
+    // the exchange changes which products the compiler would fuse.
+    bool no_contract = false;
+
     // Group form events: the state variables' coefficients are read from the LDS rows
     // fl[lds_xb + k * lds_nx + i] (lds_xb < 0: not used).
     int lds_xb = -1, lds_nx = 0;
@@ -371,7 +384,34 @@ struct Gen {
             break;
         }
         case HYX_OP_TIME: o << "    const double " << c << " = t0;\n"; break;
+        case HYX_OP_TWIN_BIND: {
+            // The lane's own degree-0 value: lane A the first operand, lane B the second,
+            // each negated when its aux bit says so.
+            const std::string a = ((S.aux[j] & 1u) ? "-" : "") + v(0), b = ((S.aux[j] & 2u) ? "-" : "") + v(1);
+            o << "    const double " << c << " = hyx_twin_b ? (" << b << ") : (" << a << ");\n";
+            break;
         }
+        case HYX_OP_TWIN_XCH: xch(j, 0); break;
+        }
+    }
+
+    // Exchange node j at order k: aux 0 gives lane A's value of the slot on both lanes,
+    // aux 1 lane B's. The partner's value is swapped in once per slot and order.
+    void xch(uint32_t j, int k)
+    {
+        const uint32_t w = S.args[j][0];
+        if (zero(w, k)) {
+            set_zero(j, k);
+            return;
+        }
+        const uint32_t sl = HYX_OPERAND_INDEX(w);
+        const std::string own = val(w, k), nm = "p" + std::to_string(sl) + "_" + std::to_string(k);
+        if (!dvars.count({sl | 0x40000000u, k})) {
+            dvars.insert({sl | 0x40000000u, k});
+            o << "    const double " << nm << " = hyx_twin_swap(" << own << ");\n";
+        }
+        o << "    const double " << cv(j, k) << " = hyx_twin_b ? " << (S.aux[j] ? own : nm) << " : "
+          << (S.aux[j] ? nm : own) << ";\n";
     }
 
     void orderk(uint32_t j, int k)
@@ -546,6 +586,7 @@ struct Gen {
             else
                 set_zero(j, k);
             break;
+        case HYX_OP_TWIN_XCH: xch(j, k); break;
         }
     }
 
@@ -556,6 +597,20 @@ struct Gen {
         o << "#define HYX_NPAR " << std::max<uint32_t>(S.npar, 1) << "\n";
         o << "#define HYX_P " << P << "\n";
         o << "typedef unsigned long long hyx_u64;\n";
+        if (twin) {
+            // Two lanes per instance (adjacent, lane B odd): identical state and stores, the
+            // twin nodes split between them; the partner's value by a DPP quad permutation
+            // [1, 0, 3, 2], which swaps the two lanes of every pair.
+            o << "#define HYX_SUB 2\n#define HYX_TWIN 1\n";
+            o << "#ifndef HYX_CPU_EMU\n";
+            o << "__device__ __forceinline__ bool hyx_twin_lane_b() { return (__lane_id() & 1u) != 0u; }\n";
+            o << "__device__ __forceinline__ double hyx_twin_swap(double v)\n{\n"
+                 "    const hyx_u64 b = __builtin_bit_cast(hyx_u64, v);\n"
+                 "    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, 0xB1, 0xF, 0xF, true);\n"
+                 "    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), 0xB1, 0xF, 0xF, true);\n"
+                 "    return __builtin_bit_cast(double, ((hyx_u64)(unsigned)hi << 32) | (hyx_u64)(unsigned)lo);\n}\n";
+            o << "#endif\n";
+        }
         if (lds_state) {
             o << "#define HYX_LDS_STATE 1\n";
             o << "#ifndef HYX_CPU_EMU\n";
@@ -568,7 +623,9 @@ struct Gen {
              "double t0, double max_dt, double &h_out, bool &nf_out, double *__restrict__ tc, hyx_u64 tcs, "
              "double *__restrict__ tape, hyx_u64 tps"
           << (E ? ", unsigned long long &hyx_evm" : "") << ")\n{\n";
+        if (no_contract) o << "#pragma clang fp contract(off)\n";
         o << "    (void)pv; (void)t0; (void)tape; (void)tps; (void)tc; (void)tcs;\n";
+        if (twin) o << "    const bool hyx_twin_b = hyx_twin_lane_b();\n";
         if (lds_state) o << "    double *const xs = hyx_xs_lane();\n";
         // The rows of order k go to LDS as soon as they exist; the empty asm at the
         // end of each order keeps the compiler from forwarding them in registers.
@@ -1794,6 +1851,7 @@ std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compac
     g.split_min = split_min;
     g.E = E;
     g.nev = nev;
+    g.no_contract = E == nullptr && has_twin(S);
     // State rows go to LDS when that brings the remaining histories within about 200
     // VGPRs (two waves per SIMD) and the rows of a 64-lane block fit in 64 KB. When the
     // node histories alone exceed that, the extra LDS traffic buys no occupancy.
@@ -1802,6 +1860,346 @@ std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compac
     if (lds_rows == 1 && !rows_fit) throw std::invalid_argument("the state rows of a 64-lane block exceed 64 KB of LDS");
     g.lds_state = lds_rows >= 0 ? lds_rows == 1
                                 : hist * (order + 1) > 160 && (hist - nst) * (order + 1) <= 96 && rows_fit;
+    return g.run();
+}
+
+// ---- Twin form ----------------------------------------------------------------------------
+//
+// A latency form for small systems whose right-hand sides add two isomorphic
+// subexpressions (the planar CR3BP: the pull of each primary). Two adjacent lanes hold
+// one instance. The isomorphic pairs of nodes become twin slots: lane A evaluates the
+// first member of each pair, lane B the second, with the same instructions; where the
+// members differ only in a degree-0 operand (a parameter, a number, a node of constant
+// series) the slot reads a per-lane binding (HYX_OP_TWIN_BIND). Every other node (shared
+// operands, the linear sums that combine the pairs, the state variables, the step size
+// and the update) is evaluated identically by both lanes; a consumer of a twin member
+// reads it through HYX_OP_TWIN_XCH (its own value or the partner's, swapped in by DPP).
+// Each node keeps the operands, operand order and recurrence of the lane form, so with
+// implicit fusion off in both forms the twin form's arithmetic is the lane form's bit for
+// bit: the persistent kernel can hand its last, longest-running instances to the twin
+// form, which then runs each step on two lanes in about half the instructions.
+
+namespace
+{
+
+struct TwinPlanner {
+    const Sys &S;
+    std::vector<int> partner; // node -> partner (itself for a shared node), -1 unpaired
+    std::vector<char> side;   // 1: lane A member, 2: lane B member
+    std::vector<uint64_t> hs;
+    std::string why;
+
+    explicit TwinPlanner(const Sys &s) : S(s), partner(s.nnodes, -1), side(s.nnodes, 0), hs(s.nnodes, 0) {}
+
+    bool node(uint32_t w) const { return HYX_OPERAND_KIND(w) == HYX_KIND_NODE; }
+    bool k0(uint32_t w) const { return !node(w) || (HYX_OPERAND_INDEX(w) >= S.n && S.deg[HYX_OPERAND_INDEX(w)] == 0); }
+
+    // The operand position holding the degree-0 value of an add / sub / mul with exactly one
+    // such operand (-1: none; sub only as x - K).
+    int kpos(uint32_t j) const
+    {
+        const uint32_t op = S.op[j];
+        if (op != HYX_OP_ADD && op != HYX_OP_SUB && op != HYX_OP_MUL) return -1;
+        const auto &A = S.args[j];
+        const bool a0 = k0(A[0]), a1 = k0(A[1]);
+        if (a0 == a1) return -1;
+        if (op == HYX_OP_SUB && a0) return -1;
+        return a0 ? 0 : 1;
+    }
+
+    // Shape class of a node for pairing: add and sub with one degree-0 operand are one
+    // class (x + K), products with one degree-0 operand another (K * x).
+    uint64_t cls(uint32_t j) const
+    {
+        const int q = kpos(j);
+        if (q >= 0) return S.op[j] == HYX_OP_MUL ? 1000 : 1001;
+        return S.op[j];
+    }
+
+    uint64_t mix(uint64_t h, uint64_t v) const { return (h ^ v) * 1099511628211ull + 0x9e3779b97f4a7c15ull; }
+
+    // Structural hash: states by index, degree-0 operands as one wildcard.
+    uint64_t hash(uint32_t w)
+    {
+        if (k0(w)) return 7;
+        const uint32_t i = HYX_OPERAND_INDEX(w);
+        if (i < S.n) return mix(11, i);
+        if (hs[i]) return hs[i];
+        uint64_t h = mix(13, cls(i));
+        const int q = kpos(i);
+        const auto &A = S.args[i];
+        for (size_t a = 0; a < A.size(); ++a)
+            if ((int)a != q) h = mix(h, hash(A[a]));
+        if (S.op[i] == HYX_OP_POW) {
+            uint64_t b;
+            std::memcpy(&b, &S.consts[S.aux[i]], 8);
+            h = mix(h, b);
+        }
+        if (S.op[i] == HYX_OP_SIN || S.op[i] == HYX_OP_COS) h = mix(h, S.op[i]);
+        return hs[i] = h | 1;
+    }
+
+    bool fail(const std::string &m)
+    {
+        if (why.empty()) why = m;
+        return false;
+    }
+
+    bool pair_operand(uint32_t wa, uint32_t wb)
+    {
+        if (k0(wa) || k0(wb)) return (k0(wa) && k0(wb)) ? true : fail("a degree-0 operand pairs with a series");
+        const uint32_t a = HYX_OPERAND_INDEX(wa), b = HYX_OPERAND_INDEX(wb);
+        if (a < S.n || b < S.n) return a == b ? true : fail("twins read different state variables");
+        return pair(a, b);
+    }
+
+    bool pair(uint32_t a, uint32_t b)
+    {
+        if (a == b) {
+            if (side[a]) return fail("a node is both shared and a twin");
+            partner[a] = (int)a;
+            return true;
+        }
+        if (partner[a] >= 0 || partner[b] >= 0) {
+            return (partner[a] == (int)b && side[a] == 1 && side[b] == 2) ? true : fail("inconsistent twin pairing");
+        }
+        if (cls(a) != cls(b) || S.args[a].size() != S.args[b].size()) return fail("twins differ in shape");
+        if (S.op[a] == HYX_OP_POW && S.consts[S.aux[a]] != S.consts[S.aux[b]]) return fail("different exponents");
+        if (S.op[a] == HYX_OP_TIME) return fail("time in a twin");
+        partner[a] = (int)b;
+        partner[b] = (int)a;
+        side[a] = 1;
+        side[b] = 2;
+        const int q = kpos(a), qb = kpos(b);
+        const auto &A = S.args[a], &B = S.args[b];
+        if (q >= 0) {
+            // x + K / K * x: the series operands pair, the degree-0 ones become a binding.
+            if (!pair_operand(A[1 - q], B[1 - qb])) return false;
+        } else {
+            for (size_t i = 0; i < A.size(); ++i)
+                if (!pair_operand(A[i], B[i])) return false;
+        }
+        if (S.op[a] == HYX_OP_SIN || S.op[a] == HYX_OP_COS) return pair(S.aux[a], S.aux[b]);
+        return true;
+    }
+
+    // Pair the isomorphic terms of each right-hand side's linear sum.
+    bool plan()
+    {
+        for (uint32_t i = 0; i < S.n; ++i) {
+            std::vector<uint32_t> terms;
+            std::vector<uint32_t> st{S.rhs[i]};
+            while (!st.empty()) {
+                const uint32_t w = st.back();
+                st.pop_back();
+                if (!node(w)) continue;
+                const uint32_t j = HYX_OPERAND_INDEX(w);
+                if (j < S.n || S.deg[j] == 0) continue;
+                const uint32_t op = S.op[j];
+                if ((op == HYX_OP_ADD || op == HYX_OP_SUB || op == HYX_OP_SUM || op == HYX_OP_NEG) && kpos(j) < 0) {
+                    for (auto a : S.args[j]) st.push_back(a);
+                    continue;
+                }
+                terms.push_back(j);
+            }
+            std::reverse(terms.begin(), terms.end());
+            std::vector<char> used(terms.size(), 0);
+            for (size_t x = 0; x < terms.size(); ++x) {
+                if (used[x] || side[terms[x]]) continue;
+                for (size_t y = x + 1; y < terms.size(); ++y) {
+                    if (used[y] || side[terms[y]] || terms[y] == terms[x]) continue;
+                    if (hash(HYX_OPERAND(HYX_KIND_NODE, terms[x])) != hash(HYX_OPERAND(HYX_KIND_NODE, terms[y]))) continue;
+                    // Lane A takes the first term unless its subtrees are already lane B's.
+                    const auto p0 = partner;
+                    const auto s0 = side;
+                    if (!pair(terms[x], terms[y])) {
+                        partner = p0;
+                        side = s0;
+                        why.clear();
+                        if (!pair(terms[y], terms[x])) return false;
+                    }
+                    used[x] = used[y] = 1;
+                    break;
+                }
+            }
+        }
+        // A shared node may not read a twin member directly (it would differ per lane).
+        for (uint32_t j = S.n; j < S.nnodes; ++j) {
+            if (partner[j] != (int)j) continue;
+            for (auto a : S.args[j])
+                if (node(a) && side[HYX_OPERAND_INDEX(a)]) return fail("a shared node reads a twin member");
+        }
+        // Worth it: the twins hold at least a third of the convolution work.
+        long twin = 0, all = 0;
+        for (uint32_t j = S.n; j < S.nnodes; ++j) {
+            const uint32_t op = S.op[j];
+            long w = 1;
+            if (S.deg[j] > 0 && kpos(j) < 0 && op != HYX_OP_ADD && op != HYX_OP_SUB && op != HYX_OP_SUM &&
+                op != HYX_OP_NEG && op != HYX_OP_TIME)
+                w = 3;
+            if (op == HYX_OP_POW) w = 4;
+            all += w;
+            if (side[j]) twin += w;
+        }
+        if (twin == 0) return fail("no isomorphic right-hand-side terms");
+        if (3 * twin < all) return fail("the twin nodes hold too little of the work");
+        return true;
+    }
+};
+
+// The op-list of the twin form: twin pairs become slots, per-lane bindings and exchanges
+// become nodes, in an order where every node follows its operands.
+Sys build_twin(const Sys &S, const TwinPlanner &tp)
+{
+    // An operand of a proto node: a raw operand word of S (state variable, number,
+    // parameter) or a proto node id.
+    struct Ref {
+        bool proto;
+        uint32_t v;
+        bool operator<(const Ref &o) const { return proto != o.proto ? proto < o.proto : v < o.v; }
+    };
+    struct Proto {
+        uint32_t op, aux;
+        std::vector<Ref> args;
+        int partner = -1; // sin / cos: the partner's proto id
+    };
+    std::vector<Proto> pr;
+    std::vector<int> pid(S.nnodes, -1); // S node -> proto (both members of a twin pair map to the slot)
+    std::map<std::pair<uint32_t, uint32_t>, uint32_t> xch;
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>, uint32_t> bind;
+    auto addp = [&](Proto p) {
+        pr.push_back(std::move(p));
+        return (uint32_t)(pr.size() - 1);
+    };
+    std::function<uint32_t(uint32_t)> mk;
+    // An operand as a non-twin node reads it: twin members through an exchange.
+    auto opnd = [&](uint32_t w) -> Ref {
+        if (HYX_OPERAND_KIND(w) != HYX_KIND_NODE || HYX_OPERAND_INDEX(w) < S.n) return Ref{false, w};
+        const uint32_t i = HYX_OPERAND_INDEX(w);
+        const uint32_t p = mk(i);
+        if (!tp.side[i]) return Ref{true, p};
+        const uint32_t aux = tp.side[i] == 1 ? 0u : 1u;
+        auto it = xch.find({p, aux});
+        if (it == xch.end()) it = xch.emplace(std::make_pair(p, aux), addp(Proto{HYX_OP_TWIN_XCH, aux, {Ref{true, p}}})).first;
+        return Ref{true, it->second};
+    };
+    auto binding = [&](uint32_t wa, bool na, uint32_t wb, bool nb) -> Ref {
+        const Ref a = opnd(wa), b = opnd(wb);
+        const uint32_t aux = (na ? 1u : 0u) | (nb ? 2u : 0u);
+        const auto key = std::make_tuple((uint32_t)a.proto, a.v, (uint32_t)b.proto, b.v, aux);
+        auto it = bind.find(key);
+        if (it == bind.end()) it = bind.emplace(key, addp(Proto{HYX_OP_TWIN_BIND, aux, {a, b}})).first;
+        return Ref{true, it->second};
+    };
+    mk = [&](uint32_t j) -> uint32_t {
+        if (pid[j] >= 0) return (uint32_t)pid[j];
+        if (!tp.side[j]) {
+            Proto p{S.op[j], S.aux[j], {}};
+            for (auto w : S.args[j]) p.args.push_back(opnd(w));
+            const uint32_t id = addp(p);
+            pid[j] = (int)id;
+            return id;
+        }
+        const uint32_t a = tp.side[j] == 1 ? j : (uint32_t)tp.partner[j], b = (uint32_t)tp.partner[a];
+        const int q = tp.kpos(a), qb = tp.kpos(b);
+        // A series operand of a slot: a slot, a shared node or a state variable.
+        auto sop = [&](uint32_t w) -> Ref {
+            if (HYX_OPERAND_KIND(w) != HYX_KIND_NODE || HYX_OPERAND_INDEX(w) < S.n) return Ref{false, w};
+            return Ref{true, mk(HYX_OPERAND_INDEX(w))};
+        };
+        const auto &A = S.args[a], &B = S.args[b];
+        Proto p{S.op[a], S.aux[a], {}};
+        if (q >= 0) {
+            // x + K (lane A add or sub x - K) or K * x: add / mul of the series slot and the
+            // binding, in lane A's operand order (x - K as x + (-K), which rounds the same).
+            const Ref x = sop(A[1 - q]);
+            const Ref k = binding(A[q], S.op[a] == HYX_OP_SUB, B[qb], S.op[b] == HYX_OP_SUB);
+            p.op = S.op[a] == HYX_OP_MUL ? (uint32_t)HYX_OP_MUL : (uint32_t)HYX_OP_ADD;
+            p.aux = 0;
+            p.args = (q == 0) ? std::vector<Ref>{k, x} : std::vector<Ref>{x, k};
+        } else {
+            for (size_t i = 0; i < A.size(); ++i) {
+                if (tp.k0(A[i]))
+                    p.args.push_back(A[i] == B[i] ? opnd(A[i]) : binding(A[i], false, B[i], false));
+                else
+                    p.args.push_back(sop(A[i]));
+            }
+        }
+        const uint32_t id = addp(p);
+        pid[a] = pid[b] = (int)id;
+        return id;
+    };
+    for (uint32_t j = S.n; j < S.nnodes; ++j) mk(j);
+    for (uint32_t j = S.n; j < S.nnodes; ++j)
+        if (S.op[j] == HYX_OP_SIN || S.op[j] == HYX_OP_COS) pr[pid[j]].partner = pid[S.aux[j]];
+    std::vector<Ref> rhs;
+    for (uint32_t i = 0; i < S.n; ++i) rhs.push_back(opnd(S.rhs[i]));
+    // Topological order, operands first.
+    std::vector<int> state(pr.size(), 0), slot(pr.size(), -1);
+    std::vector<uint32_t> order;
+    std::function<void(uint32_t)> visit = [&](uint32_t id) {
+        if (state[id] == 2) return;
+        state[id] = 1;
+        for (const Ref &r : pr[id].args)
+            if (r.proto) visit(r.v);
+        state[id] = 2;
+        slot[id] = (int)order.size();
+        order.push_back(id);
+    };
+    for (uint32_t id = 0; id < pr.size(); ++id) visit(id);
+    Sys T;
+    T.n = S.n;
+    T.npar = S.npar;
+    T.consts = S.consts;
+    T.nconsts = S.nconsts;
+    T.nnodes = S.n + (uint32_t)order.size();
+    T.op.assign(T.nnodes, 0);
+    T.aux.assign(T.nnodes, 0);
+    T.args.assign(T.nnodes, {});
+    auto word = [&](const Ref &r) { return r.proto ? HYX_OPERAND(HYX_KIND_NODE, S.n + (uint32_t)slot[r.v]) : r.v; };
+    for (uint32_t r = 0; r < order.size(); ++r) {
+        const Proto &p = pr[order[r]];
+        const uint32_t j = S.n + r;
+        T.op[j] = p.op;
+        T.aux[j] = (p.op == HYX_OP_SIN || p.op == HYX_OP_COS) ? S.n + (uint32_t)slot[p.partner] : p.aux;
+        for (const Ref &a : p.args) T.args[j].push_back(word(a));
+    }
+    for (const Ref &r : rhs) T.rhs.push_back(word(r));
+    compute_degrees(T);
+    return T;
+}
+
+} // namespace
+
+bool has_twin(const Sys &S)
+{
+    TwinPlanner tp(S);
+    return tp.plan();
+}
+
+Sys twin_oplist(const Sys &S, std::string &why)
+{
+    TwinPlanner tp(S);
+    if (!tp.plan()) {
+        why = tp.why;
+        return Sys{};
+    }
+    return build_twin(S, tp);
+}
+
+std::string generate_twin_prelude(const Sys &S, int order, bool high_acc, int split_min, std::string &why)
+{
+    if (order < 2) throw std::invalid_argument("the Taylor order must be at least 2");
+    TwinPlanner tp(S);
+    if (!tp.plan()) {
+        why = tp.why;
+        return "";
+    }
+    const Sys T = build_twin(S, tp);
+    Gen g(T, order, high_acc);
+    g.split_min = split_min;
+    g.twin = true;
+    g.no_contract = true;
     return g.run();
 }
 

@@ -25,6 +25,11 @@ struct NoAllocArgs {
     void *stream;
 };
 
+// Internal opcodes of the twin form (never in a user op-list): a per-lane degree-0 binding
+// (lane A: operand 0, lane B: operand 1; aux bit 0 / 1 negates it) and a partner exchange
+// of a twin slot (aux 0: lane A's value on both lanes, 1: lane B's).
+enum { HYX_OP_TWIN_BIND = 100, HYX_OP_TWIN_XCH = 101 };
+
 // A parsed op-list (include/hyx_oplist.h).
 struct Sys {
     uint32_t n = 0, npar = 0, nnodes = 0, nconsts = 0;
@@ -49,6 +54,14 @@ int parse_oplist(const uint32_t *w, uint32_t n_words, const double *consts, uint
 // register budget.
 std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compact, int split_min = 0, int sub = 1,
                              const Sys *E = nullptr, uint32_t nev = 0, int lds_rows = -1);
+// Twin form (hyx_codegen.cpp): two lanes per instance for systems whose right-hand sides
+// add isomorphic subexpressions; the same arithmetic as the unrolled form, bit for bit.
+// has_twin: whether the system admits it (the unrolled form then compiles without implicit
+// fusion); twin_oplist: its op-list (internal opcodes); generate_twin_prelude: its step
+// source ("" with the reason in why when it does not apply).
+bool has_twin(const Sys &S);
+Sys twin_oplist(const Sys &S, std::string &why);
+std::string generate_twin_prelude(const Sys &S, int order, bool high_acc, int split_min, std::string &why);
 // Number of nodes whose coefficient history the recurrences read.
 int history_nodes(const Sys &S);
 // Default choice of code form: compact when the histories exceed the register file.

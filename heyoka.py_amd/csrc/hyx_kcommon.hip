@@ -72,6 +72,10 @@ struct hyx_kargs {
     unsigned *clog_n;       // continuous output: [N] steps logged since the host last drained
     unsigned clog_cap;      // continuous output: steps per instance before it parks
     unsigned pad3_;
+    unsigned *inflight;     // persistent lane form with a twin form: instances in flight (null: off)
+    unsigned *tailq;        // persistent form: [N] instances written back unfinished, by slot of *unfinished
+    unsigned tail_min;      // hand-off: once drained and at most this many in flight, write back (0: never)
+    unsigned pad4_;
 };
 
 // Propagation status of an instance: 0 / 2 running forward / backward, 1 done,

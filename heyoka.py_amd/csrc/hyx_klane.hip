@@ -321,8 +321,16 @@ extern "C" __global__ void hyx_clear_u64_k(hyx_u64 *p, unsigned n)
 // claim new queue positions with one atomic per wave (ballot + prefix count),
 // so a ragged ensemble keeps every lane busy until the queue drains. An
 // instance that reaches iter_budget iterations in this launch is written back
-// unfinished (counted in *unfinished) and the host launches again; so every
-// lane leaves the loop after a bounded number of iterations.
+// unfinished (counted in *unfinished, listed in tailq when given) and the host
+// launches again; so every lane leaves the loop after a bounded number of iterations.
+//
+// Tail hand-off (a.inflight given, tail_min > 0: the system has a twin form): the
+// kernel counts the instances in flight; once the queue is drained and at most
+// tail_min remain, every wave writes its instances back (listed in tailq) and leaves,
+// and the host continues them in the twin form, two lanes per instance and about
+// half the instructions per step, bit for bit the same arithmetic. The last
+// instances of a ragged ensemble, which run alone on their SIMDs, then take steps
+// of lower latency.
 extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
 {
     const int lane = (int)__lane_id();
@@ -333,7 +341,9 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
     hyx_u64 inst = 0, budget = 0;
     bool active = false;
     bool exhausted = false;
+    unsigned tick = 0;
     for (;;) {
+        bool claimed = false;
         const hyx_u64 idle = __ballot(!active && sub == 0);
         const unsigned nidle = (unsigned)__popcll(idle);
         if (!exhausted && nidle != 0 && (nidle * HYX_SUB >= a.refill_min || nidle == 64u / HYX_SUB)) {
@@ -353,22 +363,50 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
                         hyx_resume(s, a, inst, st);
                         budget = 0;
                         active = true;
+                        claimed = true;
                     }
                 }
             }
+        }
+        if (a.inflight) {
+            const hyx_u64 got = __ballot(claimed && sub == 0);
+            if (lane == 0 && got != 0) atomicAdd(a.inflight, (unsigned)__popcll(got));
         }
         // Leave only once the queue is exhausted: a wave whose claimed positions all hold
         // finished (or unarmed) instances claims again; every claim advances the shared
         // counter, so this ends after at most N claims.
         if (!__any(active) && exhausted) break;
+        bool ended = false;
         if (active) {
             hyx_i64 res = HYX_TIME_LIMIT;
             const bool done = hyx_prop_iter(s, a, gslot, inst, &res);
             budget += 1;
             if (done || budget >= a.iter_budget) {
                 hyx_suspend(s, a, inst, done, res);
-                if (!done && sub == 0) atomicAdd(a.unfinished, 1u);
+                if (!done && sub == 0) {
+                    const unsigned q = atomicAdd(a.unfinished, 1u);
+                    if (a.tailq) a.tailq[q] = (unsigned)inst;
+                }
                 active = false;
+                ended = true;
+            }
+        }
+        if (a.inflight) {
+            const hyx_u64 fin = __ballot(ended && sub == 0);
+            if (lane == 0 && fin != 0) atomicSub(a.inflight, (unsigned)__popcll(fin));
+            if (a.tail_min != 0u && exhausted && ((++tick & 3u) == 0u)) {
+                unsigned infl = 0;
+                if (lane == 0) infl = __hip_atomic_load(a.inflight, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                infl = (unsigned)__shfl((int)infl, 0);
+                if (infl <= a.tail_min && active) {
+                    // Hand-off: written back unfinished, continued by the twin form.
+                    hyx_suspend(s, a, inst, false, HYX_TIME_LIMIT);
+                    if (sub == 0) {
+                        const unsigned q = atomicAdd(a.unfinished, 1u);
+                        a.tailq[q] = (unsigned)inst;
+                    }
+                    active = false;
+                }
             }
         }
     }

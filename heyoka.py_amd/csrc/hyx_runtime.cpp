@@ -96,6 +96,10 @@ struct KArgs {
     unsigned *clog_n;
     unsigned clog_cap;
     unsigned pad3_;
+    unsigned *inflight;
+    unsigned *tailq;
+    unsigned tail_min;
+    unsigned pad4_;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -236,6 +240,12 @@ struct hyx_sys {
     // Continuous output: the same source built with HYX_CLOG, loaded on first use.
     hipModule_t cmod = nullptr;
     hipFunction_t kc_init = nullptr, kc_iter = nullptr, kc_persist = nullptr;
+    // Twin form (unrolled systems whose right-hand sides add isomorphic terms): its own
+    // source and module, whose persistent kernel continues the last instances of a
+    // persistent propagation two lanes per instance (empty source: no twin form).
+    std::string tsrc;
+    hipModule_t tmod = nullptr;
+    hipFunction_t kt_persist = nullptr;
     // Events: the event op-list (rows = state variables, right-hand sides = event
     // functions, terminal ones first) and its specification.
     bool has_events = false;
@@ -286,6 +296,11 @@ struct hyx_batch {
     double *clog = nullptr, *clog_t = nullptr;
     unsigned *clog_n = nullptr;
     unsigned clog_cap = 0;
+    // Twin hand-off of persistent propagations: the in-flight counter, two lists of
+    // instances written back unfinished ([N] each) and the hand-off threshold (0: off).
+    unsigned *inflight = nullptr, *tailq = nullptr;
+    uint32_t tail_min = 0;
+    uint64_t twin_handoffs = 0; // twin launches of the last propagations (diagnostics)
 };
 
 namespace
@@ -396,6 +411,11 @@ int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint3
             s.inst_per_block = 64u / (uint32_t)sub;
             s.sub = sub;
             if (s.has_events && s.ev.dir.size() > 64) return fail(HYX_EINVAL, "at most 64 events are supported");
+            if (s.form == HYX_FORM_UNROLLED && !s.has_events && !(flags & HYX_FLAG_NO_TWIN)) {
+                std::string twhy;
+                const std::string tp = hyx::generate_twin_prelude(s.S, (int)order, ha, split, twhy);
+                if (!tp.empty()) s.tsrc = tp + "\n" + k_hyx_kcommon + "\n" + k_hyx_klane;
+            }
             s.src = (s.has_events ? hyx::events_forward_decl() : std::string()) +
                     hyx::generate_prelude(s.S, (int)order, ha, s.compact, split, sub, s.has_events ? &s.E : nullptr,
                                           (uint32_t)s.ev.dir.size(),
@@ -686,6 +706,8 @@ int hyx_sys_precompile(const uint32_t *ops, uint32_t n_words, const double *cons
     int rc = build_sys(ops, n_words, consts, n_consts, order, flags, s);
     if (rc) return rc;
     std::vector<char> code;
+    if (!(flags & HYX_FLAG_GRID) && !s.tsrc.empty())
+        if (int rc2 = get_code(s.tsrc, arch ? arch : "gfx950", code)) return rc2;
     return get_code((flags & HYX_FLAG_GRID) ? "#define HYX_GRID 1\n" + s.src : s.src, arch ? arch : "gfx950", code);
 }
 
@@ -706,6 +728,10 @@ static int create_sys(const uint32_t *ops, uint32_t n_words, const double *const
     }
     if (flags & HYX_FLAG_COMPILE_ONLY) {
         rc = get_code(s->src, "gfx950", s->code);
+        if (!rc && !s->tsrc.empty()) {
+            std::vector<char> tcode;
+            rc = get_code(s->tsrc, "gfx950", tcode);
+        }
         if (rc) {
             delete s;
             return rc;
@@ -750,6 +776,24 @@ static int create_sys(const uint32_t *ops, uint32_t n_words, const double *const
             return fail(HYX_EHIP, std::string("hipModuleGetFunction(") + f.name + "): " + hipGetErrorString(e));
         }
     }
+    if (!s->tsrc.empty()) {
+        std::vector<char> tcode;
+        rc = get_code(s->tsrc, device_arch(dev), tcode);
+        if (!rc && hipModuleLoadData(&s->tmod, tcode.data()) == hipSuccess &&
+            hipModuleGetFunction(&s->kt_persist, s->tmod, "hyx_persist_k") == hipSuccess) {
+            // loaded
+        } else {
+            if (s->tmod) hipModuleUnload(s->tmod);
+            s->tmod = nullptr;
+            s->kt_persist = nullptr;
+            s->tsrc.clear();
+            if (rc) {
+                hipModuleUnload(s->mod);
+                delete s;
+                return rc;
+            }
+        }
+    }
     int v = -1, sc = -1;
     if (hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_NUM_REGS, s->k_persist) == hipSuccess) s->vgprs = v;
     if (hipFuncGetAttribute(&sc, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, s->k_persist) == hipSuccess) s->scratch = sc;
@@ -786,6 +830,27 @@ int hyx_sys_is_compact(const hyx_sys *s) { return s ? (int)s->compact : HYX_EINV
 int hyx_sys_code_form(const hyx_sys *s) { return s ? s->form : HYX_EINVAL; }
 int hyx_sys_lanes_per_instance(const hyx_sys *s) { return s ? (int)(64u / s->inst_per_block) : HYX_EINVAL; }
 
+size_t hyx_sys_twin_source(const hyx_sys *s, char *buf, size_t cap)
+{
+    if (!s) return 0;
+    if (buf && cap) {
+        const size_t n = std::min(cap - 1, s->tsrc.size());
+        std::memcpy(buf, s->tsrc.data(), n);
+        buf[n] = '\0';
+    }
+    return s->tsrc.size();
+}
+
+int hyx_batch_set_twin_tail(hyx_batch *b, uint32_t tail_min)
+{
+    if (!b) return fail(HYX_EINVAL, "null batch");
+    if (tail_min > 0 && !b->sys->kt_persist) return fail(HYX_EINVAL, "the system has no twin form");
+    b->tail_min = tail_min;
+    return 0;
+}
+
+uint64_t hyx_batch_twin_launches(const hyx_batch *b) { return b ? b->twin_handoffs : 0; }
+
 size_t hyx_sys_source(const hyx_sys *s, char *buf, size_t cap)
 {
     if (!s) return 0;
@@ -810,6 +875,7 @@ void hyx_sys_destroy(hyx_sys *s)
     if (!s) return;
     if (s->cmod) hipModuleUnload(s->cmod);
     if (s->gmod) hipModuleUnload(s->gmod);
+    if (s->tmod) hipModuleUnload(s->tmod);
     if (s->mod) hipModuleUnload(s->mod);
     delete s;
 }
@@ -878,6 +944,13 @@ int hyx_batch_create(hyx_sys *s, uint64_t n_inst, hyx_batch **out)
             return cleanup(HYX_EHIP, "hipMemcpy failed");
         if (hipStreamSynchronize(b->stream) != hipSuccess) return cleanup(HYX_EHIP, "hipStreamSynchronize failed");
     }
+    if (s->kt_persist) {
+        // Twin hand-off bookkeeping; hand-off once the drained queue leaves at most two
+        // instances in flight per CU.
+        if (hipMalloc((void **)&b->inflight, 64) != hipSuccess || hipMalloc((void **)&b->tailq, 2 * N * 4) != hipSuccess)
+            return cleanup(HYX_ENOMEM, "hipMalloc failed (device memory)");
+        b->tail_min = 2u * (uint32_t)std::max(b->n_cu, 1);
+    }
     if (hipStreamSynchronize(b->stream) != hipSuccess) return cleanup(HYX_EHIP, "hipStreamSynchronize failed");
     *out = b;
     return 0;
@@ -893,7 +966,7 @@ void hyx_batch_destroy(hyx_batch *b)
                     b->hout,  b->minh, b->maxh,   b->tc,    b->oc,     b->nsteps, b->iters,
                     b->work,  b->status, b->active, b->unfinished, b->queue, b->tape, b->stage, b->stage_idx,
                     b->grid,  b->gout, b->gcur, b->evcd, b->te, b->ntc, b->nt_time, b->nt_code,
-                    b->clog, b->clog_t, b->clog_n};
+                    b->clog, b->clog_t, b->clog_n, b->inflight, b->tailq};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (b->ev0) hipEventDestroy(b->ev0);
@@ -1116,16 +1189,46 @@ int run_propagation(hyx_batch *b, uint64_t max_steps, int mode, hyx_prop_res *re
     uint64_t launches = 0;
     HIPCHK(hipEventRecord(b->ev0, b->stream));
     if (mode == HYX_MODE_PERSISTENT) {
-        const uint64_t grid = pgrid;
+        // With a twin form, the last instances are handed off to it: the lane kernel
+        // writes them back once few remain in flight, and the twin kernel continues the
+        // listed ones (and any an iteration budget writes back) until none is left.
+        const bool twin = s->kt_persist && b->tail_min > 0 && !grid_run && !clog && !b->ev_pause;
+        uint64_t grid = pgrid;
+        hipFunction_t kp = k_persist;
+        int cur = 0;
+        if (twin) {
+            HIPCHK(clear32(b, b->inflight, 1));
+            a.inflight = b->inflight;
+            a.tail_min = b->tail_min;
+            a.tailq = b->tailq;
+        }
+        uint64_t twin_grid = 0;
+        if (twin) {
+            int occ = 0;
+            HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, s->kt_persist, 64, 0));
+            twin_grid = (uint64_t)std::max(occ, 1) * (uint64_t)b->n_cu;
+        }
         for (;;) {
             HIPCHK(clear64(b, b->work, 1));
             HIPCHK(clear32(b, b->unfinished, 1));
-            HIPCHK(launch(k_persist, grid, b->stream, &a));
+            HIPCHK(launch(kp, grid, b->stream, &a));
             ++launches;
             unsigned unf = 0;
             HIPCHK(hipMemcpyAsync(&unf, b->unfinished, 4, hipMemcpyDeviceToHost, b->stream));
             HIPCHK(hipStreamSynchronize(b->stream));
             if (unf == 0) break;
+            if (twin) {
+                // Continue the listed instances in the twin form, 32 per wave.
+                kp = s->kt_persist;
+                a.queue = b->tailq + (uint64_t)cur * N;
+                a.N = unf;
+                cur ^= 1;
+                a.tailq = b->tailq + (uint64_t)cur * N;
+                a.inflight = nullptr;
+                a.tail_min = 0;
+                grid = std::max<uint64_t>(1, std::min<uint64_t>(twin_grid, ((uint64_t)unf * 2 + 63) / 64));
+                ++b->twin_handoffs;
+            }
         }
     } else {
         const uint64_t grid = blocks_of(b);

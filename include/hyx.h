@@ -75,6 +75,8 @@ typedef struct {
  * in registers (HYX_FLAG_NO_LDS_ROWS); neither: chosen from the register budget. */
 #define HYX_FLAG_LDS_ROWS 32768u
 #define HYX_FLAG_NO_LDS_ROWS 65536u
+/* Unrolled form: no twin form (see hyx_batch_set_twin_tail). */
+#define HYX_FLAG_NO_TWIN 131072u
 
 /* Code forms (hyx_sys_code_form). */
 #define HYX_FORM_UNROLLED 0 /* one instance per lane, fully unrolled, histories in registers */
@@ -271,6 +273,20 @@ typedef struct {
     int from_cache;
 } hyx_tune_res;
 int hyx_autotune(hyx_batch *b, const double *tf_hi, const char *cache_path, hyx_tune_res *best);
+
+/* Twin form (no counterpart in the reference, whose instances are CPU SIMD lanes): an
+ * unrolled system whose right-hand sides add isomorphic subexpressions (the planar CR3BP:
+ * the pull of each primary) also gets a form with two lanes per instance that splits the
+ * isomorphic nodes between the lanes with the unrolled form's arithmetic, bit for bit.
+ * A persistent propagation hands its last instances to it once its queue is drained and
+ * at most tail_min instances are in flight (default: two per CU; 0: never), so the
+ * longest-running instances of a ragged ensemble take lower-latency steps. Results do
+ * not depend on the hand-off. hyx_sys_twin_source: the twin form's HIP source (length 0:
+ * the system has none); hyx_batch_twin_launches: twin-form launches of the batch's
+ * propagations so far. */
+int hyx_batch_set_twin_tail(hyx_batch *b, uint32_t tail_min);
+size_t hyx_sys_twin_source(const hyx_sys *s, char *buf, size_t cap);
+uint64_t hyx_batch_twin_launches(const hyx_batch *b);
 
 /* Persistent-form instance order (a permutation of 0..N-1), or NULL for identity. */
 int hyx_batch_set_queue(hyx_batch *b, const uint32_t *order);

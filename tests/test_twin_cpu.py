@@ -1,0 +1,148 @@
+"""The twin form on the CPU: the same step as the unrolled form, bit for bit.
+
+The twin form (two lanes per instance, the isomorphic right-hand-side terms split
+between them, partner values exchanged) is compiled here with g++, one thread per lane
+and a barrier exchange standing in for the DPP swap; the unrolled form's step is
+compiled the same way with one thread. Both are built without implicit fusion, as on
+the GPU. For the planar CR3BP (the pull of each primary) and a pendulum pulled by two
+phase-shifted sines (sin / cos pairs, parameter bindings), every lane of the twin must
+end with exactly the lane form's state, step size and Taylor coefficients.
+"""
+
+import ctypes as ct
+import subprocess
+
+import numpy as np
+import pytest
+
+import heyoka_amd as hy
+from heyoka_amd import _lib
+from heyoka_amd import workloads as W
+
+COMMON = r"""
+#include <cmath>
+#include <cstdint>
+#include <math.h>
+#include <pthread.h>
+#define __device__
+#define __forceinline__ inline
+#define __constant__ static const
+#define HYX_CPU_EMU 1
+using std::sqrt; using std::pow; using std::exp; using std::log; using std::fabs; using std::isfinite;
+"""
+
+TWIN = COMMON + r"""
+static pthread_barrier_t bar;
+static thread_local int tl_lane;
+static double xch[2];
+static bool hyx_twin_lane_b() { return (tl_lane & 1) != 0; }
+static double hyx_twin_swap(double v) { xch[tl_lane] = v; pthread_barrier_wait(&bar); double r = xch[tl_lane ^ 1]; pthread_barrier_wait(&bar); return r; }
+#include "prelude.inc"
+struct Arg { int lane; const double *x; const double *pv; double t0, mdt; double *xout; double *h; int *nf; double *tc; };
+static void *run(void *p)
+{
+    Arg *a = (Arg *)p;
+    tl_lane = a->lane;
+    double x[HYX_N], pv[HYX_NPAR];
+    for (int i = 0; i < HYX_N; ++i) x[i] = a->x[i];
+    for (int i = 0; i < HYX_NPAR; ++i) pv[i] = a->pv[i];
+    double h; bool nf;
+    hyx_taylor_step<true>(x, pv, a->t0, a->mdt, h, nf, a->tc + (size_t)a->lane * HYX_N * (HYX_P + 1), 1, nullptr, 1);
+    for (int i = 0; i < HYX_N; ++i) a->xout[a->lane * HYX_N + i] = x[i];
+    a->h[a->lane] = h; a->nf[a->lane] = nf;
+    return nullptr;
+}
+extern "C" int step2(const double *x, const double *pv, double t0, double mdt, double *xout, double *h, int *nf, double *tc)
+{
+    pthread_barrier_init(&bar, nullptr, 2);
+    pthread_t th[2]; Arg args[2];
+    for (int l = 0; l < 2; ++l) { args[l] = Arg{l, x, pv, t0, mdt, xout, h, nf, tc}; pthread_create(&th[l], nullptr, run, &args[l]); }
+    for (int l = 0; l < 2; ++l) pthread_join(th[l], nullptr);
+    pthread_barrier_destroy(&bar);
+    return 0;
+}
+"""
+
+LANE = COMMON + r"""
+static double hyx_xs_emu[64 * 64 * 64];
+static inline double *hyx_xs_lane() { return hyx_xs_emu; }
+#include "prelude.inc"
+extern "C" int step1(const double *xi, const double *pvi, double t0, double mdt, double *xout, double *h, int *nf, double *tc)
+{
+    double x[HYX_N], pv[HYX_NPAR];
+    for (int i = 0; i < HYX_N; ++i) x[i] = xi[i];
+    for (int i = 0; i < HYX_NPAR; ++i) pv[i] = pvi[i];
+    bool nfb;
+    hyx_taylor_step<true>(x, pv, t0, mdt, *h, nfb, tc, 1, nullptr, 1);
+    for (int i = 0; i < HYX_N; ++i) xout[i] = x[i];
+    *nf = nfb;
+    return 0;
+}
+"""
+
+
+def _build(src, harness, d, name, fn):
+    prelude = src.split("// Hand-written CDNA4 kernels")[0]
+    d.mkdir(exist_ok=True)
+    (d / "prelude.inc").write_text(prelude)
+    (d / f"{name}.cpp").write_text(harness)
+    so = d / f"{name}.so"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-ffp-contract=off", "-shared", "-fPIC", "-pthread", "-o", str(so),
+                    str(d / f"{name}.cpp")], check=True, capture_output=True)
+    L = ct.CDLL(str(so))
+    P = ct.POINTER(ct.c_double)
+    getattr(L, fn).argtypes = [P, P, ct.c_double, ct.c_double, P, P, ct.POINTER(ct.c_int), P]
+    return getattr(L, fn)
+
+
+def _two_sines():
+    x, v = hy.make_vars("x", "v")
+    return [(x, v), (v, -(hy.par[0] * hy.sin(x + hy.par[1])) - hy.par[2] * hy.sin(x - hy.par[3]) - 0.1 * v)]
+
+
+def _cases(name):
+    rng = np.random.default_rng(17)
+    if name == "cr3bp":
+        wl = W.cr3bp(24)
+        st, pa = wl.state, wl.pars
+        # a few close to the primaries too
+        st = np.concatenate([st, np.array([[-0.02, 0.97, 0.9], [0.05, 0.02, -0.03], [0.3, 0.1, 0.0],
+                                           [-0.4, 0.2, 0.5]])], axis=1)
+        pa = np.concatenate([pa, np.array([[0.01, 0.04, 0.001]])], axis=1)
+        return wl.sys, st, pa
+    st = np.stack([rng.uniform(-2, 2, 16), rng.uniform(-1, 1, 16)])
+    pa = np.stack([rng.uniform(0.5, 2, 16), rng.uniform(-1, 1, 16), rng.uniform(0.5, 2, 16), rng.uniform(-1, 1, 16)])
+    return _two_sines(), st, pa
+
+
+@pytest.mark.parametrize("name", ["cr3bp", "two_sines"])
+def test_twin_step_is_the_lane_step_bitwise(name, tmp_path):
+    sys_, st, pa = _cases(name)
+    dec = hy.taylor_decompose(sys_)
+    p = 19
+    s = _lib.System(dec, p, compact=False, source_only=True)
+    tsrc = s.twin_source()
+    assert tsrc and "#define HYX_SUB 2" in tsrc and "hyx_twin_swap" in tsrc
+    assert "#pragma clang fp contract(off)" in s.source()
+    f2 = _build(tsrc, TWIN, tmp_path / "twin", "twin", "step2")
+    f1 = _build(s.source(), LANE, tmp_path / "lane", "lane", "step1")
+    n = dec.n_state
+    ptr = lambda a: a.ctypes.data_as(ct.POINTER(ct.c_double))
+    for b in range(st.shape[1]):
+        x = np.ascontiguousarray(st[:, b])
+        pv = np.ascontiguousarray(pa[:, b])
+        for mdt in (np.inf, 0.01):
+            x1, h1, nf1, tc1 = np.zeros(n), np.zeros(1), ct.c_int(), np.zeros(n * (p + 1))
+            f1(ptr(x), ptr(pv), 0.25, mdt, ptr(x1), ptr(h1), ct.byref(nf1), ptr(tc1))
+            x2, h2, nf2, tc2 = np.zeros(2 * n), np.zeros(2), np.zeros(2, dtype=np.int32), np.zeros(2 * n * (p + 1))
+            f2(ptr(x), ptr(pv), 0.25, mdt, ptr(x2), ptr(h2), nf2.ctypes.data_as(ct.POINTER(ct.c_int)), ptr(tc2))
+            for lane in range(2):
+                assert np.array_equal(x2[lane * n:(lane + 1) * n], x1), (b, lane)
+                assert h2[lane] == h1[0] and nf2[lane] == nf1.value
+                assert np.array_equal(tc2[lane * n * (p + 1):(lane + 1) * n * (p + 1)], tc1), (b, lane)
+
+
+def test_no_twin_without_isomorphic_terms():
+    for sys_ in (W.pendulum().sys, hy.model.j2()):
+        s = _lib.System(hy.taylor_decompose(sys_), 19, compact=False, source_only=True)
+        assert s.twin_source() == "" and "fp contract(off)" not in s.source()
