@@ -324,11 +324,11 @@ class System:
         self.form = FORMS[int(L.hyx_sys_code_form(self.h))]
         self.lanes_per_instance = int(L.hyx_sys_lanes_per_instance(self.h))
         # The autotuner's name for this code object: the wave variants carry their lanes per
-        # instance, the unrolled form where its state rows live when forced.
+        # instance, the unrolled form where its state rows live.
         if self.form == "wave":
             self.label = {64: "wave", 32: "wave32", 16: "wave16"}[self.lanes_per_instance]
-        elif isinstance(compact, str) and compact.startswith("unrolled_"):
-            self.label = compact
+        elif self.form == "unrolled":
+            self.label = "unrolled_lds" if "#define HYX_LDS_STATE 1" in self.source() else "unrolled"
         else:
             self.label = self.form
         self.tune_key = tune_key(words, consts, order, self.label) if events is None and not high_accuracy else None

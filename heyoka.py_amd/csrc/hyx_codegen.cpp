@@ -204,7 +204,7 @@ struct Gen {
     uint32_t nev = 0;
     const Gen *base = nullptr;
     // Twin form (HYX_OP_TWIN_BIND / HYX_OP_TWIN_XCH nodes, see plan_twin): two lanes per
-    // instance, the lane flag hyx_twin_b and the partner exchange hyx_twin_swap().
+    // instance, the lane flag hyx_twin_b and the pair broadcasts hyx_twin_from_a / _b().
     bool twin = false;
     // No implicit fusion of a product into a following add inside the step: the lane
     // form of a system that has a twin form must round exactly as the twin does, and
@@ -396,7 +396,7 @@ struct Gen {
     }
 
     // Exchange node j at order k: aux 0 gives lane A's value of the slot on both lanes,
-    // aux 1 lane B's. The partner's value is swapped in once per slot and order.
+    // aux 1 lane B's: a DPP broadcast from that lane of the pair.
     void xch(uint32_t j, int k)
     {
         const uint32_t w = S.args[j][0];
@@ -404,14 +404,8 @@ struct Gen {
             set_zero(j, k);
             return;
         }
-        const uint32_t sl = HYX_OPERAND_INDEX(w);
-        const std::string own = val(w, k), nm = "p" + std::to_string(sl) + "_" + std::to_string(k);
-        if (!dvars.count({sl | 0x40000000u, k})) {
-            dvars.insert({sl | 0x40000000u, k});
-            o << "    const double " << nm << " = hyx_twin_swap(" << own << ");\n";
-        }
-        o << "    const double " << cv(j, k) << " = hyx_twin_b ? " << (S.aux[j] ? own : nm) << " : "
-          << (S.aux[j] ? nm : own) << ";\n";
+        o << "    const double " << cv(j, k) << " = " << (S.aux[j] ? "hyx_twin_from_b(" : "hyx_twin_from_a(")
+          << val(w, k) << ");\n";
     }
 
     void orderk(uint32_t j, int k)
@@ -599,16 +593,19 @@ struct Gen {
         o << "typedef unsigned long long hyx_u64;\n";
         if (twin) {
             // Two lanes per instance (adjacent, lane B odd): identical state and stores, the
-            // twin nodes split between them; the partner's value by a DPP quad permutation
-            // [1, 0, 3, 2], which swaps the two lanes of every pair.
+            // twin nodes split between them.
             o << "#define HYX_SUB 2\n#define HYX_TWIN 1\n";
             o << "#ifndef HYX_CPU_EMU\n";
             o << "__device__ __forceinline__ bool hyx_twin_lane_b() { return (__lane_id() & 1u) != 0u; }\n";
-            o << "__device__ __forceinline__ double hyx_twin_swap(double v)\n{\n"
+            // A lane's value to both lanes of its pair: DPP quad permutations [0, 0, 2, 2]
+            // (from lane A) and [1, 1, 3, 3] (from lane B).
+            o << "template <int CTL>\n__device__ __forceinline__ double hyx_twin_bcast(double v)\n{\n"
                  "    const hyx_u64 b = __builtin_bit_cast(hyx_u64, v);\n"
-                 "    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, 0xB1, 0xF, 0xF, true);\n"
-                 "    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), 0xB1, 0xF, 0xF, true);\n"
-                 "    return __builtin_bit_cast(double, ((hyx_u64)(unsigned)hi << 32) | (hyx_u64)(unsigned)lo);\n}\n";
+                 "    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTL, 0xF, 0xF, true);\n"
+                 "    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTL, 0xF, 0xF, true);\n"
+                 "    return __builtin_bit_cast(double, ((hyx_u64)(unsigned)hi << 32) | (hyx_u64)(unsigned)lo);\n}\n"
+                 "__device__ __forceinline__ double hyx_twin_from_a(double v) { return hyx_twin_bcast<0xA0>(v); }\n"
+                 "__device__ __forceinline__ double hyx_twin_from_b(double v) { return hyx_twin_bcast<0xF5>(v); }\n";
             o << "#endif\n";
         }
         if (lds_state) {
@@ -1873,7 +1870,7 @@ std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compac
 // series) the slot reads a per-lane binding (HYX_OP_TWIN_BIND). Every other node (shared
 // operands, the linear sums that combine the pairs, the state variables, the step size
 // and the update) is evaluated identically by both lanes; a consumer of a twin member
-// reads it through HYX_OP_TWIN_XCH (its own value or the partner's, swapped in by DPP).
+// reads it through HYX_OP_TWIN_XCH (lane A's or lane B's value, broadcast over the pair by DPP).
 // Each node keeps the operands, operand order and recurrence of the lane form, so with
 // implicit fusion off in both forms the twin form's arithmetic is the lane form's bit for
 // bit: the persistent kernel can hand its last, longest-running instances to the twin

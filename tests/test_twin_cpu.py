@@ -2,7 +2,7 @@
 
 The twin form (two lanes per instance, the isomorphic right-hand-side terms split
 between them, partner values exchanged) is compiled here with g++, one thread per lane
-and a barrier exchange standing in for the DPP swap; the unrolled form's step is
+and a barrier exchange standing in for the DPP broadcasts; the unrolled form's step is
 compiled the same way with one thread. Both are built without implicit fusion, as on
 the GPU. For the planar CR3BP (the pull of each primary) and a pendulum pulled by two
 phase-shifted sines (sin / cos pairs, parameter bindings), every lane of the twin must
@@ -36,7 +36,9 @@ static pthread_barrier_t bar;
 static thread_local int tl_lane;
 static double xch[2];
 static bool hyx_twin_lane_b() { return (tl_lane & 1) != 0; }
-static double hyx_twin_swap(double v) { xch[tl_lane] = v; pthread_barrier_wait(&bar); double r = xch[tl_lane ^ 1]; pthread_barrier_wait(&bar); return r; }
+static double bcast(double v, int from) { xch[tl_lane] = v; pthread_barrier_wait(&bar); double r = xch[from]; pthread_barrier_wait(&bar); return r; }
+static double hyx_twin_from_a(double v) { return bcast(v, 0); }
+static double hyx_twin_from_b(double v) { return bcast(v, 1); }
 #include "prelude.inc"
 struct Arg { int lane; const double *x; const double *pv; double t0, mdt; double *xout; double *h; int *nf; double *tc; };
 static void *run(void *p)
@@ -122,7 +124,7 @@ def test_twin_step_is_the_lane_step_bitwise(name, tmp_path):
     p = 19
     s = _lib.System(dec, p, compact=False, source_only=True)
     tsrc = s.twin_source()
-    assert tsrc and "#define HYX_SUB 2" in tsrc and "hyx_twin_swap" in tsrc
+    assert tsrc and "#define HYX_SUB 2" in tsrc and "hyx_twin_from_b" in tsrc
     assert "#pragma clang fp contract(off)" in s.source()
     f2 = _build(tsrc, TWIN, tmp_path / "twin", "twin", "step2")
     f1 = _build(s.source(), LANE, tmp_path / "lane", "lane", "step1")
