@@ -37,7 +37,10 @@ HBM_PEAK_GBS = 8000.0
 # generated kernel source). A launch loads and stores every instance once and keeps the
 # Taylor coefficients on chip, so the traffic scales with the instances, not the steps. A
 # kernel whose source changed has no measured figure: the line then says null.
-MEASURED_TRAFFIC_B_PER_INSTANCE = {}
+MEASURED_TRAFFIC_B_PER_INSTANCE = {
+    # 65536 instances, dt=25 slices: FETCH_SIZE 30049.9 KiB, WRITE_SIZE 22999.1 KiB per launch.
+    ("nbody6", "group", "1e52c82ea6171bab"): 1298.4,
+}
 
 
 def _host_info(threads):

@@ -268,6 +268,18 @@ def tuned_launch(key: str):
     return None
 
 
+def tuned_rates(keys: dict) -> dict:
+    """{label: recorded rate} for the labelled keys, all from one table: the user cache if it
+    records any of them, else the shipped table (rates measured on different samples do not
+    compare)."""
+    for path in (tune_cache_path(), TUNED_PATH):
+        db = _tuned_db(path)
+        rates = {lab: float(db[k].get("rate", 0.0)) for lab, k in keys.items() if k in db}
+        if rates:
+            return rates
+    return {}
+
+
 def device_count() -> int:
     return int(lib().hyx_device_count())
 

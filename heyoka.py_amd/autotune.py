@@ -130,10 +130,11 @@ def autotune(dec, order, state, pars=None, t_final=1.0, forms=(None,), waves=(0,
 
 
 def tuned_config(dec, order, form=None):
-    """The cached (waves_per_cu, refill_min) for a system, or the defaults (0, 1)."""
+    """The (waves_per_cu, refill_min) a Batch of the system applies: the user cache's, else
+    the shipped table's, else the defaults (0, 1)."""
     system = get_system(dec, order, compact=form)
-    rec = _load().get(_key(dec, order, _label(form, system)))
-    return (rec["waves_per_cu"], rec["refill_min"]) if rec else (0, 1)
+    rec = _lib.tuned_launch(_key(dec, order, _label(form, system)))
+    return (rec[0], rec[1]) if rec else (0, 1)
 
 
 def tune_workloads(out_path=None, verbose=True):

@@ -2204,8 +2204,9 @@ std::string generate_twin_prelude(const Sys &S, int order, bool high_acc, int sp
 //
 // G lanes per instance (64: one instance per wavefront, or 32 / 16). The nodes of the
 // decomposition are spread over the lanes of the group by a static schedule: each node
-// gets the level 1 + the highest level of its node operands (state variables are level
-// 0), and the nodes of one level and one recurrence kind form slots of at most G nodes,
+// gets a level above the levels of its node operands (state variables are level 0),
+// chosen so that nodes of one kind share levels, and the nodes of one level and one
+// recurrence kind form slots of at most G nodes,
 // lane q of a slot evaluating its q-th node. Per order, the state rows come first, then
 // the slots level by level, one wavefront-scope fence per level. Every coefficient lives
 // in the instance's LDS block:
@@ -2237,7 +2238,7 @@ struct WaveSlot {
 } // namespace
 
 std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, const Sys *E, uint32_t nev,
-                                  std::string &why)
+                                  int split_min, std::string &why)
 {
     const int n = (int)S.n, NN = (int)S.nnodes;
     if (NN == n) {
@@ -2338,16 +2339,51 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
         if (!is_node(w)) return NN + 1 + scal_index(w);
         return idx < n ? idx : idx + 1;
     };
-    // Levels and slots.
-    std::vector<int> lvl(NN, 0);
+    // Levels. A node's level lies between its earliest, 1 + the highest earliest level of
+    // its node operands, and its latest, 1 below the lowest latest level of the nodes that
+    // read it (the last level for the nodes only right-hand sides read). Levels are
+    // assigned in increasing order: at each level, a recurrence kind with ready nodes at
+    // their latest level opens slots for them, and the spare lanes of those slots take the
+    // kind's other ready nodes, the nearest latest level first; the ready nodes of a kind
+    // that opens no slot wait. Fewer, fuller slots mean fewer instructions per order, and
+    // the level of a node changes none of its arithmetic.
+    std::vector<int> asap(NN, 0), alap(NN, 0), lvl(NN, 0);
     int maxl = 0;
     for (int j = n; j < NN; ++j) {
         int l = 1;
         for (auto a : S.args[j])
-            if (is_node(a) && (int)HYX_OPERAND_INDEX(a) >= n) l = std::max(l, lvl[HYX_OPERAND_INDEX(a)] + 1);
-        lvl[j] = l;
+            if (is_node(a) && (int)HYX_OPERAND_INDEX(a) >= n) l = std::max(l, asap[HYX_OPERAND_INDEX(a)] + 1);
+        asap[j] = l;
         maxl = std::max(maxl, l);
     }
+    for (int j = n; j < NN; ++j) alap[j] = maxl;
+    for (int j = NN - 1; j >= n; --j)
+        for (auto a : S.args[j])
+            if (is_node(a) && (int)HYX_OPERAND_INDEX(a) >= n)
+                alap[HYX_OPERAND_INDEX(a)] = std::min(alap[HYX_OPERAND_INDEX(a)], alap[j] - 1);
+    for (int l = 1; l <= maxl; ++l) {
+        std::vector<std::vector<int>> must(WK_COUNT), spare(WK_COUNT);
+        for (int j = n; j < NN; ++j) {
+            if (lvl[j]) continue;
+            bool ready = true;
+            for (auto a : S.args[j])
+                if (is_node(a) && (int)HYX_OPERAND_INDEX(a) >= n) {
+                    const int la = lvl[HYX_OPERAND_INDEX(a)];
+                    ready = ready && la >= 1 && la < l;
+                }
+            if (ready) (alap[j] == l ? must : spare)[kind[j]].push_back(j);
+        }
+        for (int kd = 0; kd < WK_COUNT; ++kd) {
+            if (must[kd].empty()) continue;
+            auto &sp = spare[kd];
+            std::stable_sort(sp.begin(), sp.end(), [&](int x, int y) { return alap[x] < alap[y]; });
+            const size_t cap = (must[kd].size() + G - 1) / G * G;
+            for (int j : must[kd]) lvl[j] = l;
+            for (size_t q = 0; q < sp.size() && must[kd].size() + q < cap; ++q) lvl[sp[q]] = l;
+        }
+    }
+    for (int j = n; j < NN; ++j)
+        if (lvl[j] < asap[j] || lvl[j] > alap[j]) throw std::logic_error("wave form: a node left without a level");
     std::vector<WaveSlot> slots;
     for (int l = 1; l <= maxl; ++l)
         for (int kd = 0; kd < WK_COUNT; ++kd) {
@@ -2509,8 +2545,20 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
     // Level of each slot; slots of one level are independent of each other.
     std::vector<int> slvl(slots.size());
     for (size_t s = 0; s < slots.size(); ++s) slvl[s] = lvl[slots[s].nodes[0]];
+    // nm = the sum of the products tm, as one fma chain or, from split_min terms on, as two
+    // interleaved chains (even and odd terms) added at the end: half the dependent FP64
+    // latency per sum, at the price of a different summation order.
     auto chain = [&](const std::vector<std::pair<std::string, std::string>> &tm, const std::string &nm,
                      const std::string &ind) {
+        if (split_min > 0 && (int)tm.size() >= split_min) {
+            o << ind << "{\n" << ind << "    double e = " << tm[0].first << " * " << tm[0].second << ";\n" << ind
+              << "    double f = " << tm[1].first << " * " << tm[1].second << ";\n";
+            for (size_t q = 2; q < tm.size(); ++q)
+                o << ind << "    " << (q % 2 ? "f" : "e") << " = __builtin_fma(" << tm[q].first << ", " << tm[q].second
+                  << ", " << (q % 2 ? "f" : "e") << ");\n";
+            o << ind << "    " << nm << " = e + f;\n" << ind << "}\n";
+            return;
+        }
         o << ind << nm << " = " << tm[0].first << " * " << tm[0].second << ";\n";
         for (size_t q = 1; q < tm.size(); ++q)
             o << ind << nm << " = __builtin_fma(" << tm[q].first << ", " << tm[q].second << ", " << nm << ");\n";

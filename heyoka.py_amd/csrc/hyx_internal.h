@@ -99,7 +99,7 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int order,
 // every coefficient row in LDS; the same hyx_gstep contract as the group form, so the
 // group kernels run it. Returns "" with `why` set when the form does not apply.
 std::string generate_wave_prelude(const Sys &S, int order, bool high_acc, int G, const Sys *E, uint32_t nev,
-                                  std::string &why);
+                                  int split_min, std::string &why);
 
 // Compiled function: the order-0 values of the first n_out right-hand sides (hyx_cfunc_eval).
 std::string generate_cfunc_prelude(const Sys &S, uint32_t n_out);

@@ -368,7 +368,7 @@ int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint3
         if (s.form == HYX_FORM_WAVE) {
             const int G = (flags & HYX_FLAG_WAVE16) ? 16 : (flags & HYX_FLAG_WAVE32) ? 32 : 64;
             wave_src = hyx::generate_wave_prelude(s.S, (int)order, ha, G, s.has_events ? &s.E : nullptr,
-                                                  (uint32_t)s.ev.dir.size(), why);
+                                                  (uint32_t)s.ev.dir.size(), split, why);
             if (wave_src.empty()) return fail(HYX_EINVAL, "the wave form does not apply: " + why);
             s.inst_per_block = 64u / (uint32_t)G;
         } else if (s.form == HYX_FORM_COMPACT && !(flags & HYX_FLAG_FORCE_COMPACT)) {
@@ -378,7 +378,7 @@ int build_sys(const uint32_t *ops, uint32_t n_words, const double *consts, uint3
             // else 64.
             for (int G : {32, 64}) {
                 wave_src = hyx::generate_wave_prelude(s.S, (int)order, ha, G, s.has_events ? &s.E : nullptr,
-                                                      (uint32_t)s.ev.dir.size(), why);
+                                                      (uint32_t)s.ev.dir.size(), split, why);
                 if (!wave_src.empty()) {
                     s.form = HYX_FORM_WAVE;
                     s.compact = false;

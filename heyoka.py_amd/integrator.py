@@ -121,11 +121,7 @@ def get_system(dec, order, high_accuracy=False, compact=None, events=None):
     if compact is None and s.form == "wave" and s.tune_key is not None:
         # The wave form's variants (16, 32, 64 lanes per instance) compute bit-identical
         # results: the automatic choice takes the one the autotuner recorded as fastest.
-        rates = {}
-        for lab in ("wave16", "wave32", "wave"):
-            rec = _lib.tuned_launch(_lib.tune_key(words, consts, order, lab))
-            if rec is not None:
-                rates[lab] = rec[2]
+        rates = _lib.tuned_rates({lab: _lib.tune_key(words, consts, order, lab) for lab in ("wave16", "wave32", "wave")})
         if rates:
             best = max(rates, key=rates.get)
             if best != s.label:

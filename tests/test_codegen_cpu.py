@@ -313,18 +313,21 @@ def _mixed_case():
     return sys_, np.array([[0.3], [0.1], [0.7]]), np.array([[0.4], [1.3]]), 0.5, 1e-15
 
 
-@pytest.mark.parametrize("form", ["wave", "wave32", "wave16"])
+@pytest.mark.parametrize("form,split", [("wave", False), ("wave32", False), ("wave16", False), ("wave32", True)])
 @pytest.mark.parametrize("name", ["pendulum", "cr3bp", "j2var", "mixed"])
-def test_wave_step_matches_oracle(name, form, tmp_path):
+def test_wave_step_matches_oracle(name, form, split, tmp_path):
     # The wave form emulated on the CPU with the group harness: one thread per lane of
-    # a group, a barrier at every wavefront-scope fence and __shfl_xor exchange.
+    # a group, a barrier at every wavefront-scope fence and __shfl_xor exchange. With
+    # split, the long convolution sums run as two interleaved accumulators.
     sys_, st, pars, t0, tol = _mixed_case() if name == "mixed" else _case(name)
     dec = hy.taylor_decompose(sys_)
     p = hy.order_from_tol(tol)
-    s = _lib.System(dec, p, compact=form, source_only=True)
+    s = _lib.System(dec, p, compact=form, source_only=True, split_chains=split)
+    if split:
+        assert "double e = " in s.source()
     assert s.form == "wave"
     prelude = s.source().split("// Hand-written CDNA4 kernels")[0]
-    d = tmp_path / f"{name}_{form}"
+    d = tmp_path / f"{name}_{form}_{int(split)}"
     d.mkdir()
     (d / "prelude.inc").write_text(prelude)
     (d / "g.cpp").write_text(GROUP_HARNESS)
