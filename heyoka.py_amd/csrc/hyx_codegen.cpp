@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -2273,6 +2274,48 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
                 return "";
             }
     }
+    // Derived nodes: a linear node or a product with a constant whose only node operand is
+    // another node, and whose other operands are constants. The lane that computes that
+    // operand (its root: the first node of the chain that is not derived) computes it too,
+    // right after it, from the operand's value in a register and with the arithmetic of its
+    // own recurrence: no level of its own, and no LDS round trip on the critical path.
+    // dlist[r] lists the nodes derived from root r in index (dependency) order.
+    std::vector<int> dsrc(NN, -1), root(NN, -1);
+    std::vector<std::vector<int>> dlist(NN);
+    for (int j = n; j < NN; ++j) {
+        root[j] = j;
+        if (kind[j] != WK_LIN && kind[j] != WK_MULS) continue;
+        int cnt = 0, src = -1;
+        bool consts = true;
+        for (auto a : S.args[j]) {
+            if (is_node(a)) {
+                ++cnt;
+                src = (int)HYX_OPERAND_INDEX(a);
+            } else if (HYX_OPERAND_KIND(a) != HYX_KIND_CONST)
+                consts = false;
+        }
+        if (cnt == 1 && src >= n && consts && S.args[j].size() <= 15) {
+            dsrc[j] = src;
+            root[j] = root[src];
+            dlist[root[j]].push_back(j);
+        }
+    }
+    // Position of a derived node's source in its root's chain: 0 = the root, d + 1 = step d.
+    auto dpos = [&](int x) {
+        const int r = root[x];
+        if (dsrc[x] == r) return 0;
+        for (size_t e = 0; e < dlist[r].size(); ++e)
+            if (dlist[r][e] == dsrc[x]) return (int)e + 1;
+        throw std::logic_error("wave form: a derived node without its source");
+    };
+    // The shape of a derived step: kind, operation and which operands are nodes (the
+    // constants may differ between lanes); its source is matched separately.
+    auto dshape = [&](int x) {
+        std::string sg = kind[x] == WK_MULS ? "M" : "L" + std::to_string(S.op[x]) + ":";
+        if (kind[x] == WK_LIN)
+            for (auto a : S.args[x]) sg += is_node(a) ? "n" : "c";
+        return sg;
+    };
     // Rows with a history: node operands of every recurrence but the linear ones, and
     // the nodes whose recurrence reads their own (or their partner's) history.
     // A product with a constant or parameter reads its other operand's current order only.
@@ -2316,7 +2359,8 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
     // T holds orders 0..P-1 of every row and order P of the state rows (and the zero row):
     // node coefficients are formed up to order P-1.
     const int CB = P * R + n + 1;
-    const int NC = NN + 1 + NS;
+    const int NC = NN + 1 + NS + 1; // the last entry is a sink, like the sink row
+    const int csink = NN + 1 + NS;
     const int SVB = CB + NC;
     const int flat = SVB + NS;
     const int ipb = 64 / G;
@@ -2347,59 +2391,145 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
     // kind's other ready nodes, the nearest latest level first; the ready nodes of a kind
     // that opens no slot wait. Fewer, fuller slots mean fewer instructions per order, and
     // the level of a node changes none of its arithmetic.
+    // Derived nodes take their root's level; a node that reads a derived node depends on
+    // its root.
+    std::vector<std::vector<int>> eops(NN);
+    for (int j = n; j < NN; ++j)
+        if (dsrc[j] < 0)
+            for (auto a : S.args[j])
+                if (is_node(a) && (int)HYX_OPERAND_INDEX(a) >= n) eops[j].push_back(root[HYX_OPERAND_INDEX(a)]);
     std::vector<int> asap(NN, 0), alap(NN, 0), lvl(NN, 0);
     int maxl = 0;
     for (int j = n; j < NN; ++j) {
+        if (dsrc[j] >= 0) continue;
         int l = 1;
-        for (auto a : S.args[j])
-            if (is_node(a) && (int)HYX_OPERAND_INDEX(a) >= n) l = std::max(l, asap[HYX_OPERAND_INDEX(a)] + 1);
+        for (int a : eops[j]) l = std::max(l, asap[a] + 1);
         asap[j] = l;
         maxl = std::max(maxl, l);
     }
     for (int j = n; j < NN; ++j) alap[j] = maxl;
     for (int j = NN - 1; j >= n; --j)
-        for (auto a : S.args[j])
-            if (is_node(a) && (int)HYX_OPERAND_INDEX(a) >= n)
-                alap[HYX_OPERAND_INDEX(a)] = std::min(alap[HYX_OPERAND_INDEX(a)], alap[j] - 1);
+        for (int a : eops[j]) alap[a] = std::min(alap[a], alap[j] - 1);
     for (int l = 1; l <= maxl; ++l) {
-        std::vector<std::vector<int>> must(WK_COUNT), spare(WK_COUNT);
+        std::map<int, std::vector<int>> must, spare;
         for (int j = n; j < NN; ++j) {
-            if (lvl[j]) continue;
+            if (lvl[j] || dsrc[j] >= 0) continue;
             bool ready = true;
-            for (auto a : S.args[j])
-                if (is_node(a) && (int)HYX_OPERAND_INDEX(a) >= n) {
-                    const int la = lvl[HYX_OPERAND_INDEX(a)];
-                    ready = ready && la >= 1 && la < l;
-                }
+            for (int a : eops[j]) ready = ready && lvl[a] >= 1 && lvl[a] < l;
             if (ready) (alap[j] == l ? must : spare)[kind[j]].push_back(j);
         }
-        for (int kd = 0; kd < WK_COUNT; ++kd) {
-            if (must[kd].empty()) continue;
-            auto &sp = spare[kd];
+        for (auto &mk : must) {
+            auto &sp = spare[mk.first];
             std::stable_sort(sp.begin(), sp.end(), [&](int x, int y) { return alap[x] < alap[y]; });
-            const size_t cap = (must[kd].size() + G - 1) / G * G;
-            for (int j : must[kd]) lvl[j] = l;
-            for (size_t q = 0; q < sp.size() && must[kd].size() + q < cap; ++q) lvl[sp[q]] = l;
+            const size_t cap = (mk.second.size() + G - 1) / G * G;
+            for (int j : mk.second) lvl[j] = l;
+            for (size_t q = 0; q < sp.size() && mk.second.size() + q < cap; ++q) lvl[sp[q]] = l;
         }
     }
-    for (int j = n; j < NN; ++j)
+    for (int j = n; j < NN; ++j) {
+        if (dsrc[j] >= 0) {
+            lvl[j] = lvl[root[j]];
+            continue;
+        }
         if (lvl[j] < asap[j] || lvl[j] > alap[j]) throw std::logic_error("wave form: a node left without a level");
+    }
+    // Slots: per level and kind, the nodes' derived chains are merged into templates (a
+    // sequence of step shapes, each with its source position); a chain fits a template
+    // when its steps map in order onto template steps of the same shape whose sources map
+    // onto its sources' steps. Lanes run every template step; a step that is not one of
+    // their chain's computes on and stores to the sinks. A template grows by appending the
+    // steps a chain does not find, up to 6 steps.
     std::vector<WaveSlot> slots;
+    std::vector<std::vector<std::pair<std::string, int>>> tmpls; // per slot: (shape, source position)
+    std::vector<std::vector<std::vector<int>>> tmap;              // per slot, lane, template step: node or -1
+    auto embed = [&](const std::vector<int> &ch, std::vector<std::pair<std::string, int>> &t, bool grow,
+                     std::vector<int> *pos) {
+        std::vector<int> ps;
+        size_t i = 0;
+        auto tt = t;
+        for (int x : ch) {
+            const int sp = dpos(x);
+            const std::pair<std::string, int> want(dshape(x), sp == 0 ? 0 : ps[sp - 1] + 1);
+            while (i < tt.size() && tt[i] != want) ++i;
+            if (i >= tt.size()) {
+                if (!grow) return false;
+                tt.push_back(want);
+                i = tt.size() - 1;
+            }
+            ps.push_back((int)i++);
+        }
+        if (grow) {
+            if (tt.size() > 6 && tt.size() > t.size()) return false;
+            t = tt;
+        }
+        if (pos) *pos = ps;
+        return true;
+    };
     for (int l = 1; l <= maxl; ++l)
         for (int kd = 0; kd < WK_COUNT; ++kd) {
-            std::vector<uint32_t> ns;
+            std::vector<int> ns;
             for (int j = n; j < NN; ++j)
-                if (lvl[j] == l && kind[j] == kd) ns.push_back((uint32_t)j);
-            for (size_t q0 = 0; q0 < ns.size(); q0 += G) {
-                WaveSlot s;
-                s.kind = kd;
-                s.nodes.assign(ns.begin() + q0, ns.begin() + std::min(ns.size(), q0 + G));
-                for (auto j : s.nodes) s.m = std::max(s.m, (int)S.args[j].size());
-                if (kd == WK_LIN && s.m > 15) {
-                    why = "a sum of more than 15 terms";
-                    return "";
+                if (lvl[j] == l && kind[j] == kd && dsrc[j] < 0) ns.push_back(j);
+            if (ns.empty()) continue;
+            // Templates from the longest chains down; nodes without derived nodes fill the
+            // spare lanes of the templates' last slots, then slots of their own.
+            std::vector<int> byl, bare;
+            for (int j : ns) (dlist[j].empty() ? bare : byl).push_back(j);
+            std::stable_sort(byl.begin(), byl.end(), [&](int x, int y) { return dlist[x].size() > dlist[y].size(); });
+            std::vector<std::vector<std::pair<std::string, int>>> temps;
+            std::vector<std::vector<int>> members;
+            for (int j : byl) {
+                int best = -1;
+                size_t best_len = 0;
+                for (size_t ti = 0; ti < temps.size(); ++ti) {
+                    auto t2 = temps[ti];
+                    if (embed(dlist[j], t2, true, nullptr) && (best < 0 || t2.size() < best_len)) {
+                        best = (int)ti;
+                        best_len = t2.size();
+                    }
                 }
-                slots.push_back(s);
+                if (best < 0) {
+                    temps.emplace_back();
+                    members.emplace_back();
+                    best = (int)temps.size() - 1;
+                }
+                embed(dlist[j], temps[best], true, nullptr);
+                members[best].push_back(j);
+            }
+            for (size_t ti = 0; ti < temps.size(); ++ti) {
+                auto &mb = members[ti];
+                std::sort(mb.begin(), mb.end());
+                while (!bare.empty() && mb.size() % G) {
+                    mb.push_back(bare.front());
+                    bare.erase(bare.begin());
+                }
+            }
+            if (!bare.empty()) {
+                temps.emplace_back();
+                members.push_back(bare);
+            }
+            for (size_t ti = 0; ti < temps.size(); ++ti) {
+                const auto &mb = members[ti];
+                for (size_t q0 = 0; q0 < mb.size(); q0 += G) {
+                    WaveSlot sl;
+                    sl.kind = kd;
+                    for (size_t q = q0; q < std::min(mb.size(), q0 + G); ++q) sl.nodes.push_back((uint32_t)mb[q]);
+                    for (auto j : sl.nodes) sl.m = std::max(sl.m, (int)S.args[j].size());
+                    if (kd == WK_LIN && sl.m > 15) {
+                        why = "a sum of more than 15 terms";
+                        return "";
+                    }
+                    std::vector<std::vector<int>> mp;
+                    for (auto j : sl.nodes) {
+                        std::vector<int> m(temps[ti].size(), -1), ps;
+                        if (!embed(dlist[j], temps[ti], false, &ps)) throw std::logic_error("wave form: a chain outside its template");
+                        for (size_t e = 0; e < ps.size(); ++e) m[ps[e]] = dlist[j][e];
+                        mp.push_back(m);
+                    }
+                    slots.push_back(sl);
+                    tmpls.push_back(temps[ti]);
+                    tmap.push_back(mp);
+                }
             }
         }
     const int own = (n + G - 1) / G;
@@ -2459,12 +2589,24 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
         bool cur_any = false, hist_any = false;
     };
     std::vector<SlotTab> st(slots.size());
-    // Readers of each node's cur entry: linear nodes and right-hand sides.
-    std::vector<char> cur_read(NN + 1 + NS, 0);
+    // Readers of each node's cur entry: linear nodes and right-hand sides (a derived node
+    // reads its source from a register).
+    std::vector<char> cur_read(NC, 0);
     for (int j = n; j < NN; ++j)
         if (kind[j] == WK_LIN || kind[j] == WK_MULS)
-            for (auto a : S.args[j]) cur_read[cid(a)] = 1;
+            for (auto a : S.args[j])
+                if (!(dsrc[j] >= 0 && is_node(a))) cur_read[cid(a)] = 1;
     for (int i = 0; i < n; ++i) cur_read[cid(S.rhs[i])] = 1;
+    // Derived steps of each slot (the shape is the slot's; lanes past its width repeat the
+    // first lane's, like its node): a product's constant per lane (d-field), a linear
+    // node's constant operands per lane (their cur entries, read at order 0 only: they are
+    // zero above), and the stores.
+    struct DStep {
+        int kd = 0, src = 0, op = 0, cval = -1, oc = -1, orow = -1;
+        bool cur_any = false, hist_any = false;
+        std::vector<int> cterm; // linear: cur entry field of each constant operand (-1: the source)
+    };
+    std::vector<std::vector<DStep>> dst(slots.size());
     for (size_t s = 0; s < slots.size(); ++s) {
         const auto &sl = slots[s];
         auto &t = st[s];
@@ -2520,6 +2662,44 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
                 t.al1 = addd(al1);
             }
             break;
+        }
+        // Derived steps from the slot's template; lanes past the width repeat the first lane.
+        const auto &T = tmpls[s];
+        for (size_t d = 0; d < T.size(); ++d) {
+            DStep ds;
+            const std::string &shp = T[d].first;
+            ds.kd = shp[0] == 'M' ? WK_MULS : WK_LIN;
+            ds.src = T[d].second;
+            auto at = [&](int q) { return tmap[s][q < w ? q : 0][d]; };
+            auto dl = [&](auto f) {
+                std::vector<int> v(G, 0);
+                for (int q = 0; q < G; ++q) v[q] = f(at(q));
+                return v;
+            };
+            for (int q = 0; q < w; ++q)
+                if (at(q) >= 0) {
+                    ds.cur_any = ds.cur_any || cur_read[at(q) + 1];
+                    ds.hist_any = ds.hist_any || hist[at(q)];
+                }
+            if (ds.kd == WK_MULS) {
+                std::vector<double> cv(G, 0.0);
+                for (int q = 0; q < G; ++q) {
+                    const int x = at(q);
+                    if (x < 0) continue;
+                    const uint32_t c = is_node(S.args[x][0]) ? S.args[x][1] : S.args[x][0];
+                    cv[q] = S.consts[HYX_OPERAND_INDEX(c)];
+                }
+                ds.cval = addd(cv);
+            } else {
+                // "L<op>:<n|c per operand>"
+                ds.op = std::atoi(shp.c_str() + 1);
+                const std::string pat = shp.substr(shp.find(':') + 1);
+                for (size_t q = 0; q < pat.size(); ++q)
+                    ds.cterm.push_back(pat[q] == 'n' ? -1 : addw(dl([&](int x) { return x < 0 ? n : cid(S.args[x][q]); })));
+            }
+            if (ds.cur_any) ds.oc = addw(dl([&](int x) { return x >= 0 && cur_read[x + 1] ? x + 1 : csink; }));
+            if (ds.hist_any) ds.orow = addw(dl([&](int x) { return x >= 0 && hist[x] ? hrow[x] : sink; }));
+            dst[s].push_back(ds);
         }
     }
     // Table fields are unsigned 8-bit values, four per 32-bit word, when every value fits
@@ -2733,6 +2913,26 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
         }
         if (t.cur_any) o << "        " << cur(wv(t.oc)) << " = v;\n";
         if (t.hist_any) o << "        " << Tk(wv(t.orow), k) << " = v;\n";
+        // Derived steps: the recurrences of the nodes they replace (a product with a
+        // constant at every order; a linear node's constant operands are zero above order 0).
+        for (size_t d = 0; d < dst[s].size(); ++d) {
+            const auto &ds = dst[s][d];
+            const std::string dn = "dv" + std::to_string(d);
+            const std::string src = ds.src == 0 ? "v" : "dv" + std::to_string(ds.src - 1);
+            if (ds.kd == WK_MULS)
+                o << "        const double " << dn << " = " << src << " * " << dv(ds.cval) << ";\n";
+            else {
+                for (size_t q = 0; q < ds.cterm.size(); ++q) {
+                    const std::string y = ds.cterm[q] < 0 ? src : k == 0 ? cur(wv(ds.cterm[q])) : std::string("0.0");
+                    const bool neg = ds.op == HYX_OP_NEG || (ds.op == HYX_OP_SUB && q == 1);
+                    o << "        const double " << dn << "_" << q << " = " << (neg ? "-" : "") << y << ";\n";
+                }
+                o << "        double " << dn << " = " << dn << "_0;\n";
+                for (size_t q = 1; q < ds.cterm.size(); ++q) o << "        " << dn << " = " << dn << " + " << dn << "_" << q << ";\n";
+            }
+            if (ds.cur_any) o << "        " << cur(wv(ds.oc)) << " = " << dn << ";\n";
+            if (ds.hist_any) o << "        " << Tk(wv(ds.orow), k) << " = " << dn << ";\n";
+        }
         o << "    }\n";
     };
     // Every lane runs every slot: lanes past a slot's width hold the first lane's table

@@ -355,3 +355,20 @@ def test_wave_step_matches_oracle(name, form, split, tmp_path):
     assert np.max(np.abs(tcg - tco) / scale) < 1e-11
     assert abs(h[0] - r["h"][0]) <= 1e-13 * abs(r["h"][0])
     np.testing.assert_allclose(xout, r["state"][:, 0], rtol=1e-13, atol=1e-15)
+
+
+def test_wave_schedule_j2var():
+    # The wave form's static schedule of the J2 + variational system at 32 lanes per
+    # instance. A linear node or a product with a constant whose only node operand is
+    # another node is computed by that node's lane right after it (a derived step, no level
+    # of its own), and nodes move within their slack so that a level holds few slots: the
+    # 20 dependency levels per order become 14, in 15 slots. The arithmetic is checked
+    # against the oracle by test_wave_step_matches_oracle.
+    import re
+
+    dec = hy.taylor_decompose(W.j2var(8).sys.sys)
+    src = _lib.System(dec, 19, compact="wave32", source_only=True).source()
+    assert int(re.search(r"#define HYX_WAVE_SLOTS (\d+)", src).group(1)) == 15
+    assert "const double dv0 = " in src
+    order10 = src.split("// order 10\n")[1].split("// order 11\n")[0]
+    assert order10.count("hyx_wsync();") == 14 + 2  # levels, plus the state rows' two exchanges
