@@ -28,7 +28,7 @@ from copy import copy, deepcopy
 import numpy as np
 
 from . import _lib
-from .integrator import taylor_adaptive_batch, dfl_add, _outcome, _MODES
+from .integrator import taylor_adaptive_batch, dfl_add, _MODES, _PropRes
 
 
 def _ensemble_generic(tp, ta, arg, n_iter, gen, **kwargs):
@@ -190,8 +190,7 @@ def _unpack_member(m, sl, st, thi, tlo, r, cds):
     m._hyx_state[...] = st[:, sl]
     m._hyx_thi[...] = thi[sl]
     m._hyx_tlo[...] = tlo[sl]
-    m._hyx_prop_res = [(_outcome(o), float(a), float(b), int(c))
-                       for o, a, b, c in zip(r["outcome"][sl], r["min_h"][sl], r["max_h"][sl], r["nsteps"][sl])]
+    m._hyx_prop_res = _PropRes.of(r, sl)
 
 
 def _ensemble_packed(tp, ta, arg, n_iter, gen, kwargs):

@@ -52,6 +52,30 @@ def _outcome(v):
         return int(v)
 
 
+class _PropRes:
+    """A batch's propagate_res kept as its result arrays until it is read: building the
+    reference's list of (outcome, min_h, max_h, nsteps) tuples costs about a microsecond
+    per lane, which ensembles of many members would otherwise pay for every member."""
+
+    __slots__ = ("oc", "mn", "mx", "ns")
+
+    def __init__(self, oc, mn, mx, ns):
+        self.oc, self.mn, self.mx, self.ns = (np.array(oc, dtype=np.int64), np.array(mn, dtype=np.float64),
+                                              np.array(mx, dtype=np.float64), np.array(ns, dtype=np.uint64))
+
+    @classmethod
+    def of(cls, r, sl=slice(None)):
+        return cls(r["outcome"][sl], r["min_h"][sl], r["max_h"][sl], r["nsteps"][sl])
+
+    def copy(self):
+        return _PropRes(self.oc, self.mn, self.mx, self.ns)
+
+    def tolist(self):
+        ocs = {v: _outcome(v) for v in set(self.oc.tolist())}
+        return [(ocs[o], a, b, c) for o, a, b, c in zip(self.oc.tolist(), self.mn.tolist(), self.mx.tolist(),
+                                                          self.ns.tolist())]
+
+
 _MODES = {"persistent": _lib.MODE_PERSISTENT, "plain": _lib.MODE_PLAIN, "graph": _lib.MODE_GRAPH}
 
 _PAD = "__hyx_event_pad_"
@@ -436,8 +460,7 @@ class taylor_adaptive_batch:
         finally:
             b.set_event_pause(False)
         self._pull()
-        self._hyx_prop_res = [(_outcome(o), float(a), float(c), int(d))
-                              for o, a, c, d in zip(r["outcome"], r["min_h"], r["max_h"], r["nsteps"])]
+        self._hyx_prop_res = _PropRes.of(r)
         return rounds
 
     def step(self, max_delta_t=None, write_tc=False):
@@ -495,8 +518,7 @@ class taylor_adaptive_batch:
             r = self._hyx_batch.propagate_until(tf_hi, tf_lo, mdt, int(max_steps),
                                                 _MODES[self._hyx_flags[3]])
             self._pull()
-            self._hyx_prop_res = [(_outcome(o), float(a), float(b), int(c))
-                                  for o, a, b, c in zip(r["outcome"], r["min_h"], r["max_h"], r["nsteps"])]
+            self._hyx_prop_res = _PropRes.of(r)
             return None, None
         return self._prop_stepwise(tf_hi, tf_lo, int(max_steps), mdt, cb, write_tc)
 
@@ -557,8 +579,7 @@ class taylor_adaptive_batch:
         finally:
             b.set_clog(0)
         self._pull()
-        self._hyx_prop_res = [(_outcome(o), float(a), float(c), int(d))
-                              for o, a, c, d in zip(r["outcome"], r["min_h"], r["max_h"], r["nsteps"])]
+        self._hyx_prop_res = _PropRes.of(r)
         steps = []
         for lane in range(B):
             ps = parts[lane]
@@ -628,7 +649,7 @@ class taylor_adaptive_batch:
                     stop = True
             if stop or done.all():
                 break
-        self._hyx_prop_res = [(_outcome(o), float(a), float(b), int(c)) for o, a, b, c in zip(res_oc, min_h, max_h, nst)]
+        self._hyx_prop_res = _PropRes(res_oc, min_h, max_h, nst)
         return None, cb
 
     def propagate_until(self, t, max_steps=0, max_delta_t=None, callback=None, write_tc=False, c_output=False):
@@ -693,8 +714,7 @@ class taylor_adaptive_batch:
             self._push()
             r, out = self._hyx_batch.propagate_grid(g, mdt, int(max_steps), _MODES[self._hyx_flags[3]])
             self._pull()
-            self._hyx_prop_res = [(_outcome(o), float(a), float(b), int(c))
-                                  for o, a, b, c in zip(r["outcome"], r["min_h"], r["max_h"], r["nsteps"])]
+            self._hyx_prop_res = _PropRes.of(r)
             return None, out
         # With a step callback: host-driven steps with write_tc, dense output on the host.
         n, B = self._hyx_state.shape[0], self._hyx_B
@@ -785,6 +805,8 @@ class taylor_adaptive_batch:
 
     @property
     def propagate_res(self):
+        if isinstance(self._hyx_prop_res, _PropRes):
+            self._hyx_prop_res = self._hyx_prop_res.tolist()
         return list(self._hyx_prop_res)
 
     @property
@@ -967,7 +989,7 @@ class taylor_adaptive_batch:
             d[k] = src[k]
         for k in self._STATE_KEYS:
             v = src[k]
-            d[k] = v.copy() if isinstance(v, np.ndarray) else list(v)
+            d[k] = v.copy() if isinstance(v, (np.ndarray, _PropRes)) else list(v)
         d["_hyx_tes"], d["_hyx_ntes"] = self._events_copy()
         d["_hyx_batch_obj"] = None
         d["_hyx_cd_pending"] = self._hyx_cooldowns() if self._hyx_tes else None
@@ -982,6 +1004,8 @@ class taylor_adaptive_batch:
 
     def __getstate__(self):
         st = {k: self.__dict__[k] for k in self._STATE_KEYS}
+        if isinstance(st["_hyx_prop_res"], _PropRes):
+            st["_hyx_prop_res"] = st["_hyx_prop_res"].tolist()
         st["_hyx_ctor"] = (self._hyx_vsys if self._hyx_vsys is not None else self._hyx_pairs, self._hyx_tol,
                            self._hyx_flags)
         st["_hyx_user"] = self._user_attrs()
