@@ -125,3 +125,22 @@ def test_decomposition_and_flop_model():
     assert fl["transcendentals"] == 2
     w = dec.to_words()
     assert w[0] == 0x31585948 and w[1] == 2 and w[3] == 5
+
+
+def test_lazy_propagate_res_matches_the_list():
+    # A batch keeps propagate_res as its result arrays until it is read; the list it then
+    # builds is the reference's: (taylor_outcome or int, min_h, max_h, nsteps) per lane,
+    # with Python floats and ints, and copies and pickles keep it.
+    from heyoka_amd.integrator import _PropRes, _outcome
+
+    r = {"outcome": np.array([-4294967299, -4294967297, -1, 7], dtype=np.int64),
+         "min_h": np.array([0.5, 0.25, np.inf, 0.0]), "max_h": np.array([1.0, 2.0, 0.0, 0.0]),
+         "nsteps": np.array([3, 4, 0, 9], dtype=np.uint64)}
+    eager = [(_outcome(o), float(a), float(b), int(c))
+             for o, a, b, c in zip(r["outcome"], r["min_h"], r["max_h"], r["nsteps"])]
+    lazy = _PropRes.of(r)
+    assert lazy.tolist() == eager
+    assert [type(x) for x in lazy.tolist()[0]] == [type(x) for x in eager[0]]
+    assert lazy.tolist()[0][0] is hy.taylor_outcome.time_limit and lazy.tolist()[2][0] == -1
+    assert _PropRes.of(r, slice(1, 3)).tolist() == eager[1:3]
+    assert copy.deepcopy(lazy).tolist() == eager and pickle.loads(pickle.dumps(lazy)).tolist() == eager
