@@ -313,16 +313,34 @@ def _mixed_case():
     return sys_, np.array([[0.3], [0.1], [0.7]]), np.array([[0.4], [1.3]]), 0.5, 1e-15
 
 
+def _chains_case():
+    # Derived chains of every shape the wave form fuses: products with constants of
+    # products with constants, negations, constant + node, sums of a node and constants,
+    # chains of different shapes from nodes of one kind and one level (merged templates
+    # with steps some lanes skip), and a chain whose end is a right-hand side.
+    x, y, z = hy.make_vars("x", "y", "z")
+    a = x * y
+    b = y * z
+    c = x * z
+    sys_ = [(x, -(5.0 * (2.0 * a)) + 3.0 * c),
+            (y, (1.0 + 0.5 * b) * x - (-(3.0 * c)) + 2.0),
+            (z, (2.0 * (-1.5 * (0.25 * a))) * hy.sin(y) - 0.5 * b)]
+    return sys_, np.array([[0.3], [-0.2], [0.4]]), None, 0.1, 1e-15
+
+
 @pytest.mark.parametrize("form,split", [("wave", False), ("wave32", False), ("wave16", False), ("wave32", True)])
-@pytest.mark.parametrize("name", ["pendulum", "cr3bp", "j2var", "mixed"])
+@pytest.mark.parametrize("name", ["pendulum", "cr3bp", "j2var", "mixed", "chains"])
 def test_wave_step_matches_oracle(name, form, split, tmp_path):
     # The wave form emulated on the CPU with the group harness: one thread per lane of
     # a group, a barrier at every wavefront-scope fence and __shfl_xor exchange. With
     # split, the long convolution sums run as two interleaved accumulators.
-    sys_, st, pars, t0, tol = _mixed_case() if name == "mixed" else _case(name)
+    sys_, st, pars, t0, tol = (_mixed_case() if name == "mixed" else _chains_case() if name == "chains"
+                               else _case(name))
     dec = hy.taylor_decompose(sys_)
     p = hy.order_from_tol(tol)
     s = _lib.System(dec, p, compact=form, source_only=True, split_chains=split)
+    if name == "chains":
+        assert "const double dv1 = " in s.source()  # chains of at least two derived steps
     if split:
         assert "double e = " in s.source()
     assert s.form == "wave"
