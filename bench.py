@@ -560,7 +560,10 @@ def events_bench(args):
     T = args.dt if args.dt != 25.0 else 100.0
     x, y, vx, vy = hy.make_vars("x", "y", "vx", "vy")
     results = {}
-    for label in ("A_no_events", "B0_terminal_never_triggers", "B_terminal_device", "C_terminal_and_section_parked"):
+    # A0: A with the twin-form tail hand-off off, the lane form throughout like the runs with
+    # events (the twin form has no event detection): the baseline of the per-step event cost.
+    for label in ("A_no_events", "A0_no_events_lane_form_only", "B0_terminal_never_triggers", "B_terminal_device",
+                  "C_terminal_and_section_parked"):
         crossings = [0]
 
         def cb(ta, t, d_sgn, bidx):
@@ -570,11 +573,13 @@ def events_bench(args):
         if label.startswith("B0"):
             # Same detection work per step, no trigger: the per-step cost of event detection.
             kw["t_events"] = [hy.t_event_batch(x * x + y * y - 1e4, direction=hy.event_direction.positive)]
-        elif label != "A_no_events":
+        elif not label.startswith("A"):
             kw["t_events"] = [hy.t_event_batch(x * x + y * y - 4.0, direction=hy.event_direction.positive)]
         if label.startswith("C"):
             kw["nt_events"] = [hy.nt_event_batch(x - 0.5 + hy.par[0], cb, direction=hy.event_direction.positive)]
         ta = hy.taylor_adaptive_batch(wl.sys, wl.state, pars=wl.pars, tol=1e-15, code_form="unrolled", **kw)
+        if label.startswith("A0") and ta._hyx_system.twin_source():
+            ta._hyx_batch.set_twin_tail(0)
         ta.propagate_until(np.full(N, 1.0))  # warm-up (compiles, first launch)
         ta.set_time(np.zeros(N))
         ta.state[:] = wl.state
@@ -592,7 +597,7 @@ def events_bench(args):
                           "crossings_per_s": crossings[0] / wall if crossings[0] else None}
         print(f"[bench] events {label}: {ns / wall:.3e} instance-steps/s, {crossings[0]} crossings, "
               f"{stopped} stopped, {wall:.2f} s", file=sys.stderr, flush=True)
-    base = results["A_no_events"]["instance_steps_per_s"]
+    base = results["A0_no_events_lane_form_only"]["instance_steps_per_s"]
     # 6-body ensemble, group form: the same comparison for a terminal close-approach
     # event between bodies 1 and 2 (|r12| = 0.1, never reached: the per-step cost) and
     # for a stopping event that does trigger (body 1 drifting past x = x0 - 2).
@@ -636,7 +641,7 @@ def events_bench(args):
                                "form, persistent; terminal escape event r^2 = 4 and a Python-logged section "
                                "x = 1/2 - mu (direction positive)"},
         "events": results,
-        "relative_to_no_events": {k: v["instance_steps_per_s"] / base for k, v in results.items()},
+        "relative_to_no_events_lane_form": {k: v["instance_steps_per_s"] / base for k, v in results.items()},
         "nbody6_group_events": nb,
         "nbody6_kernel_ms_relative_to_no_events": {k: v["kernel_ms"] / nb["A_no_events"]["kernel_ms"]
                                                    for k, v in nb.items()},
