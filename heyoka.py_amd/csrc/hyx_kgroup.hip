@@ -394,14 +394,24 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
         // finished (or unarmed) instances claims again; every claim advances the shared
         // counter, so this ends after at most N claims.
         if (!__any(active) && exhausted) break;
-        hyx_i64 res = HYX_TIME_LIMIT;
-        const bool done = hyx_gprop_iter(s, a, g, fl, tab, inst, active, &res);
-        if (active) {
-            budget += 1;
-            if (done || budget >= a.iter_budget) {
-                hyx_gsuspend(s, a, inst, g, tab, done, res);
-                if (!done && g == 0) atomicAdd(a.unfinished, 1u);
-                active = false;
+        // Step until an instance of the wave ends: the claim code above, and the registers
+        // it needs, stay out of the loop that the steps run in. (A wave with no instance
+        // claimed none this time: it claims again.)
+        if (__any(active)) {
+            for (;;) {
+                bool ended = false;
+                hyx_i64 res = HYX_TIME_LIMIT;
+                const bool done = hyx_gprop_iter(s, a, g, fl, tab, inst, active, &res);
+                if (active) {
+                    budget += 1;
+                    if (done || budget >= a.iter_budget) {
+                        hyx_gsuspend(s, a, inst, g, tab, done, res);
+                        if (!done && g == 0) atomicAdd(a.unfinished, 1u);
+                        active = false;
+                        ended = true;
+                    }
+                }
+                if (__any(ended)) break;
             }
         }
     }

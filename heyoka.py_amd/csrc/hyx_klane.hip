@@ -341,7 +341,6 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
     hyx_u64 inst = 0, budget = 0;
     bool active = false;
     bool exhausted = false;
-    unsigned tick = 0;
     for (;;) {
         bool claimed = false;
         const hyx_u64 idle = __ballot(!active && sub == 0);
@@ -372,29 +371,58 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
             const hyx_u64 got = __ballot(claimed && sub == 0);
             if (lane == 0 && got != 0) atomicAdd(a.inflight, (unsigned)__popcll(got));
         }
+        // Queue drained and no hand-off to watch: nothing is claimed or counted any more, so
+        // the remaining instances step in a loop of their own, free of the claim code and of
+        // what it keeps live (the straggler of a ragged ensemble runs this loop alone).
+        if (exhausted && !a.inflight) {
+            while (active) {
+                hyx_i64 res = HYX_TIME_LIMIT;
+                const bool done = hyx_prop_iter(s, a, gslot, inst, &res);
+                budget += 1;
+                if (done || budget >= a.iter_budget) {
+                    hyx_suspend(s, a, inst, done, res);
+                    if (!done && sub == 0) {
+                        const unsigned q = atomicAdd(a.unfinished, 1u);
+                        if (a.tailq) a.tailq[q] = (unsigned)inst;
+                    }
+                    active = false;
+                }
+            }
+            break;
+        }
         // Leave only once the queue is exhausted: a wave whose claimed positions all hold
         // finished (or unarmed) instances claims again; every claim advances the shared
         // counter, so this ends after at most N claims.
         if (!__any(active) && exhausted) break;
+        // Step until an instance of the wave ends (or, once the queue is drained, until
+        // the next hand-off check, every 16 iterations): the claim and count code above,
+        // and the registers it needs, stay out of the loop that the steps run in.
+        // (A wave with no instance claimed none this time: it claims again.)
         bool ended = false;
-        if (active) {
-            hyx_i64 res = HYX_TIME_LIMIT;
-            const bool done = hyx_prop_iter(s, a, gslot, inst, &res);
-            budget += 1;
-            if (done || budget >= a.iter_budget) {
-                hyx_suspend(s, a, inst, done, res);
-                if (!done && sub == 0) {
-                    const unsigned q = atomicAdd(a.unfinished, 1u);
-                    if (a.tailq) a.tailq[q] = (unsigned)inst;
+        if (__any(active)) {
+            for (unsigned it = 1;; ++it) {
+                if (active) {
+                    hyx_i64 res = HYX_TIME_LIMIT;
+                    const bool done = hyx_prop_iter(s, a, gslot, inst, &res);
+                    budget += 1;
+                    if (done || budget >= a.iter_budget) {
+                        hyx_suspend(s, a, inst, done, res);
+                        if (!done && sub == 0) {
+                            const unsigned q = atomicAdd(a.unfinished, 1u);
+                            if (a.tailq) a.tailq[q] = (unsigned)inst;
+                        }
+                        active = false;
+                        ended = true;
+                    }
                 }
-                active = false;
-                ended = true;
+                if (__any(ended)) break;
+                if (exhausted && (it & 15u) == 0u) break;
             }
         }
         if (a.inflight) {
             const hyx_u64 fin = __ballot(ended && sub == 0);
             if (lane == 0 && fin != 0) atomicSub(a.inflight, (unsigned)__popcll(fin));
-            if (a.tail_min != 0u && exhausted && ((++tick & 3u) == 0u)) {
+            if (a.tail_min != 0u && exhausted) {
                 unsigned infl = 0;
                 if (lane == 0) infl = __hip_atomic_load(a.inflight, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 infl = (unsigned)__shfl((int)infl, 0);
