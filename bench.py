@@ -437,16 +437,23 @@ def _cr3bp_record():
                       "err_nf_state": int(np.sum(r["outcome"] == -4294967300))}
         _log(f"cr3bp {label}: {out[label]['value']:.3e} instance-steps/s, {wall:.2f} s, {nl} launches")
     out["persistent_over_plain"] = out["persistent"]["value"] / out["plain"]["value"]
-    out["twin_launches"] = b.twin_launches()
+    tsrc = system.twin_source()
+    out["tail_form"] = ("quad: 4 lanes per instance" if "#define HYX_QUAD 1" in tsrc
+                        else "twin: 2 lanes per instance" if tsrc else None)
+    out["tail_launches"] = b.twin_launches()
     b.close()
-    # Step latency of one instance running alone (the makespan's tail) in each form.
-    one = _lib.Batch(system, 1)
+    # Step latency of one instance running alone (the makespan's tail) in each form; the
+    # twin beside the quad from a system built without the quad form.
     lone = {}
-    for label, mode, tail in (("lane_persistent", _lib.MODE_PERSISTENT, 0),
-                              ("twin_persistent", _lib.MODE_PERSISTENT, 1 << 30), ("plain", _lib.MODE_PLAIN, 0)):
-        if tail and not system.twin_source():
-            continue
-        if system.twin_source():
+    forms = [(system, "lane_persistent", _lib.MODE_PERSISTENT, 0), (system, "plain", _lib.MODE_PLAIN, 0)]
+    if tsrc:
+        forms.insert(1, (system, "tail_form_persistent", _lib.MODE_PERSISTENT, 1 << 30))
+    if "#define HYX_QUAD 1" in tsrc:
+        forms.insert(2, (_lib.System(dec, hy.order_from_tol(wl.tol), quad=False), "twin_persistent",
+                         _lib.MODE_PERSISTENT, 1 << 30))
+    for sysm, label, mode, tail in forms:
+        one = _lib.Batch(sysm, 1)
+        if sysm.twin_source():
             one.set_twin_tail(tail)
         one.write(_lib.FIELD_STATE, np.ascontiguousarray(wl.state[:, 5:6]))
         one.write(_lib.FIELD_PARS, np.ascontiguousarray(wl.pars[:, 5:6]))
@@ -454,8 +461,8 @@ def _cr3bp_record():
         one.write(_lib.FIELD_TIME_LO, np.zeros(1))
         r, wall, ms, nl = _timed(one, np.full(1, 2000.0), mode)
         lone[label] = ms * 1e3 / float(r["nsteps"][0])
+        one.close()
     out["lone_instance_us_per_step"] = lone
-    one.close()
     return out
 
 

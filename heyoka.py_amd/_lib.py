@@ -28,6 +28,7 @@ FLAG_WAVE16 = 16384
 FLAG_LDS_ROWS = 32768
 FLAG_NO_LDS_ROWS = 65536
 FLAG_NO_TWIN = 131072
+FLAG_NO_QUAD = 262144
 FORMS = {0: "unrolled", 1: "compact", 2: "group", 3: "wave"}
 FLAG_GRID = 64
 FLAG_SPLIT_CHAINS = 128
@@ -307,12 +308,12 @@ class System:
 
     def __init__(self, dec, order: int, high_accuracy: bool = False, compile_only: bool = False, compact=None,
                  source_only: bool = False, split_chains: bool = False, twin: bool = True,
-                 pipeline: bool = True, sub: int = 2, events: EventSpec | None = None):
+                 pipeline: bool = True, sub: int = 2, events: EventSpec | None = None, quad: bool = True):
         L = lib()
         words, consts = oplist_args(dec)
         flags = (_flags(high_accuracy, compact) | (FLAG_COMPILE_ONLY if compile_only else 0)
                  | (FLAG_SOURCE_ONLY if source_only else 0) | (FLAG_SPLIT_CHAINS if split_chains else 0)
-                 | (0 if twin else FLAG_NO_TWIN)
+                 | (0 if twin else FLAG_NO_TWIN) | (0 if quad else FLAG_NO_QUAD)
                  | (0 if pipeline else FLAG_NO_PIPELINE) | FLAG_SUB[int(sub)])
         h = ct.c_void_p()
         if events is None:

@@ -68,6 +68,8 @@ void compute_degrees(Sys &S)
         case HYX_OP_DIV: d = (od(A[1]) == 0) ? od(A[0]) : INF_DEG; break;
         case HYX_OP_TWIN_BIND: d = 0; break;
         case HYX_OP_TWIN_XCH: d = od(A[0]); break;
+        case HYX_OP_QUAD_BIND: d = 0; break;
+        case HYX_OP_QUAD_SWAP: d = od(A[0]); break;
         default: d = (od(A[0]) == 0) ? 0 : INF_DEG;
         }
         S.deg[j] = d;
@@ -207,6 +209,14 @@ struct Gen {
     // Twin form (HYX_OP_TWIN_BIND / HYX_OP_TWIN_XCH nodes, see plan_twin): two lanes per
     // instance, the lane flag hyx_twin_b and the pair broadcasts hyx_twin_from_a / _b().
     bool twin = false;
+    // Quad form (with twin set; HYX_OP_QUAD_BIND / HYX_OP_QUAD_SWAP nodes, see QuadPlanner):
+    // four lanes per instance, q = 2 * primary + component. Lane flags hyx_twin_b (primary B)
+    // and hyx_quad_y (component y); hyx_twin_from_a / _b broadcast the same component's lane
+    // of primary A / B, hyx_quad_swap the other component's lane of the same primary. The
+    // state is held in lane-relative order: variable i is x[i] on x lanes and x[qsig[i]] on
+    // y lanes (qsig: the pairing of the state variables, an involution).
+    bool quad = false;
+    std::vector<uint32_t> qsig;
     // No implicit fusion of a product into a following add inside the step: the lane
     // form of a system that has a twin form must round exactly as the twin does, and
     // the exchange changes which products the compiler would fuse.
@@ -393,7 +403,23 @@ struct Gen {
             break;
         }
         case HYX_OP_TWIN_XCH: xch(j, 0); break;
+        case HYX_OP_QUAD_BIND:
+            // The component's own degree-0 value: x lanes operand 0, y lanes operand 1.
+            o << "    const double " << c << " = hyx_quad_y ? (" << v(1) << ") : (" << v(0) << ");\n";
+            break;
+        case HYX_OP_QUAD_SWAP: qswap(j, 0); break;
         }
+    }
+
+    // The other component's value of a quad node at order k (lanes x <-> y of a primary).
+    void qswap(uint32_t j, int k)
+    {
+        const uint32_t w = S.args[j][0];
+        if (zero(w, k)) {
+            set_zero(j, k);
+            return;
+        }
+        o << "    const double " << cv(j, k) << " = hyx_quad_swap(" << val(w, k) << ");\n";
     }
 
     // Exchange node j at order k: aux 0 gives lane A's value of the slot on both lanes,
@@ -582,6 +608,7 @@ struct Gen {
                 set_zero(j, k);
             break;
         case HYX_OP_TWIN_XCH: xch(j, k); break;
+        case HYX_OP_QUAD_SWAP: qswap(j, k); break;
         }
     }
 
@@ -592,7 +619,24 @@ struct Gen {
         o << "#define HYX_NPAR " << std::max<uint32_t>(S.npar, 1) << "\n";
         o << "#define HYX_P " << P << "\n";
         o << "typedef unsigned long long hyx_u64;\n";
-        if (twin) {
+        if (quad) {
+            // Four lanes per instance (lane q = 2 * primary + component): identical stores,
+            // the quad nodes split between them.
+            o << "#define HYX_SUB 4\n#define HYX_TWIN 1\n#define HYX_QUAD 1\n";
+            o << "#ifndef HYX_CPU_EMU\n";
+            o << "__device__ __forceinline__ unsigned hyx_quad_lane() { return __lane_id() & 3u; }\n";
+            // DPP quad permutations [0, 1, 0, 1] (primary A, same component), [2, 3, 2, 3]
+            // (primary B, same component) and [1, 0, 3, 2] (same primary, other component).
+            o << "template <int CTL>\n__device__ __forceinline__ double hyx_twin_bcast(double v)\n{\n"
+                 "    const hyx_u64 b = __builtin_bit_cast(hyx_u64, v);\n"
+                 "    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTL, 0xF, 0xF, true);\n"
+                 "    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTL, 0xF, 0xF, true);\n"
+                 "    return __builtin_bit_cast(double, ((hyx_u64)(unsigned)hi << 32) | (hyx_u64)(unsigned)lo);\n}\n"
+                 "__device__ __forceinline__ double hyx_twin_from_a(double v) { return hyx_twin_bcast<0x44>(v); }\n"
+                 "__device__ __forceinline__ double hyx_twin_from_b(double v) { return hyx_twin_bcast<0xEE>(v); }\n"
+                 "__device__ __forceinline__ double hyx_quad_swap(double v) { return hyx_twin_bcast<0xB1>(v); }\n";
+            o << "#endif\n";
+        } else if (twin) {
             // Two lanes per instance (adjacent, lane B odd): identical state and stores, the
             // twin nodes split between them.
             o << "#define HYX_SUB 2\n#define HYX_TWIN 1\n";
@@ -623,7 +667,11 @@ struct Gen {
           << (E ? ", unsigned long long &hyx_evm" : "") << ")\n{\n";
         if (no_contract) o << "#pragma clang fp contract(off)\n";
         o << "    (void)pv; (void)t0; (void)tape; (void)tps; (void)tc; (void)tcs;\n";
-        if (twin) o << "    const bool hyx_twin_b = hyx_twin_lane_b();\n";
+        if (quad)
+            o << "    const unsigned hyx_q = hyx_quad_lane();\n    const bool hyx_twin_b = (hyx_q & 2u) != 0u;\n"
+                 "    const bool hyx_quad_y = (hyx_q & 1u) != 0u;\n";
+        else if (twin)
+            o << "    const bool hyx_twin_b = hyx_twin_lane_b();\n";
         if (lds_state) o << "    double *const xs = hyx_xs_lane();\n";
         // The rows of order k go to LDS as soon as they exist; the empty asm at the
         // end of each order keeps the compiler from forwarding them in registers.
@@ -634,7 +682,12 @@ struct Gen {
         };
         // order 0
         cur_k = 0;
-        for (uint32_t i = 0; i < n; ++i) o << "    const double " << cv(i, 0) << " = x[" << i << "];\n";
+        for (uint32_t i = 0; i < n; ++i) {
+            if (quad && qsig[i] != i)
+                o << "    const double " << cv(i, 0) << " = hyx_quad_y ? x[" << qsig[i] << "] : x[" << i << "];\n";
+            else
+                o << "    const double " << cv(i, 0) << " = x[" << i << "];\n";
+        }
         store_rows(0);
         for (uint32_t j = n; j < S.nnodes; ++j) order0(j);
         for (int k = 1; k <= P; ++k) {
@@ -660,8 +713,13 @@ struct Gen {
         // Taylor coefficients of the state, when asked for.
         o << "    if constexpr (WTC) {\n";
         for (uint32_t i = 0; i < n; ++i)
-            for (int k = 0; k <= P; ++k)
-                o << "        tc[(hyx_u64)" << (i * (P + 1) + k) << " * tcs] = " << nv(i, k) << ";\n";
+            for (int k = 0; k <= P; ++k) {
+                if (quad && qsig[i] != i)
+                    o << "        tc[(hyx_u64)(hyx_quad_y ? " << (qsig[i] * (P + 1) + k) << " : " << (i * (P + 1) + k)
+                      << ") * tcs] = " << nv(i, k) << ";\n";
+                else
+                    o << "        tc[(hyx_u64)" << (i * (P + 1) + k) << " * tcs] = " << nv(i, k) << ";\n";
+            }
         o << "    }\n";
         if (E) {
             // The event functions' coefficients, from the state's registers: a nested scope
@@ -726,20 +784,26 @@ struct Gen {
             o << "        }\n    }\n";
         }
         o << "    bool nf = false;\n";
+        // Quad form: the new state in lane-relative order first, then back to x's order.
+        const std::string xo = quad ? "hyx_xr" : "x";
+        if (quad) o << "    double hyx_xr[HYX_N];\n";
         for (uint32_t i = 0; i < n; ++i) {
             if (!high_acc) {
                 o << "    { double r = " << nv(i, P) << ";\n";
                 for (int k = P - 1; k >= 0; --k) o << "      r = __builtin_fma(r, h, " << nv(i, k) << ");\n";
-                o << "      x[" << i << "] = r; nf = nf || !isfinite(r); }\n";
+                o << "      " << xo << "[" << i << "] = r; nf = nf || !isfinite(r); }\n";
             } else {
                 // Kahan-compensated sum of x^[k] h^k, k = 0..p.
                 o << "    { double sm = " << nv(i, 0) << ", cp = 0.0, hk = 1.0;\n";
                 for (int k = 1; k <= P; ++k)
                     o << "      { hk *= h; const double y = " << nv(i, k)
                       << " * hk - cp; const double tt = sm + y; cp = (tt - sm) - y; sm = tt; }\n";
-                o << "      x[" << i << "] = sm; nf = nf || !isfinite(sm); }\n";
+                o << "      " << xo << "[" << i << "] = sm; nf = nf || !isfinite(sm); }\n";
             }
         }
+        if (quad)
+            for (uint32_t i = 0; i < n; ++i)
+                o << "    x[" << i << "] = hyx_quad_y ? hyx_xr[" << qsig[i] << "] : hyx_xr[" << i << "];\n";
         o << "    h_out = h;\n    nf_out = nf;\n}\n";
         return o.str();
     }
@@ -2197,6 +2261,354 @@ std::string generate_twin_prelude(const Sys &S, int order, bool high_acc, int sp
     Gen g(T, order, high_acc);
     g.split_min = split_min;
     g.twin = true;
+    g.no_contract = true;
+    return g.run();
+}
+
+// ---- Quad form ----------------------------------------------------------------------------
+//
+// A latency form one step past the twin: four lanes per instance, lane q = 2 * primary +
+// component. It applies to a twin op-list T whose nodes split once more under a pairing
+// sigma of the state variables (an involution without fixed points; the planar CR3BP's
+// x <-> y, vx <-> vy), such that the right-hand sides of i and sigma(i) are isomorphic when
+// every state variable s read by one is read as sigma(s) by the other. The lanes of
+// component x evaluate the first member of each pair of nodes, the lanes of component y the
+// second, with the same instructions. Each lane holds the state in its own order (variable
+// i of the op-list is x[i] on the x lanes, x[sigma(i)] on the y lanes), so a node that reads
+// s on the x lanes reads sigma(s) on the y lanes. Members that differ in a degree-0 operand
+// read a per-component binding (HYX_OP_QUAD_BIND); a node s + K whose partner is the bare
+// state variable sigma(s) adds K on the x lanes and -0.0 on the y lanes (v + -0.0 == v for
+// every v, -0.0 included). Nodes common to both components (the twin's shared nodes, and
+// + or * of the two members of a pair, such as x^2 + y^2: own value op the other
+// component's) are evaluated by all four lanes; the right-hand side of sigma(i) is the
+// other component's value of the right-hand side of i (HYX_OP_QUAD_SWAP). Operands, their
+// order (up to swapping the operands of + and *, which round the same) and every recurrence
+// stay the lane form's, so the quad form's arithmetic is the lane form's bit for bit, and
+// each lane runs about the instructions of one component of one primary per order.
+
+namespace
+{
+
+struct QuadPlanner {
+    const Sys &T;
+    std::vector<uint32_t> sg; // the pairing of the state variables
+    std::vector<int> partner; // node -> partner node (itself: common), -1 unset
+    std::vector<char> side;   // 1: x member, 2: y member
+    std::vector<char> sym;    // common + / * of a pair: 1 operand 0 is the x member, 2 operand 1 is
+    std::vector<char> addk;   // s + K paired with a bare state variable: 1 evaluated on x lanes, 2 on y
+    std::string why;
+
+    explicit QuadPlanner(const Sys &t) : T(t) {}
+
+    bool node(uint32_t w) const { return HYX_OPERAND_KIND(w) == HYX_KIND_NODE; }
+    uint32_t ix(uint32_t w) const { return HYX_OPERAND_INDEX(w); }
+    bool k0(uint32_t w) const { return !node(w) || (ix(w) >= T.n && T.deg[ix(w)] == 0); }
+    bool fail(const std::string &m)
+    {
+        if (why.empty()) why = m;
+        return false;
+    }
+
+    // s + K or K + s (a state variable plus a degree-0 operand): the position of K, else -1.
+    int kpos_state_add(uint32_t j) const
+    {
+        if (T.op[j] != HYX_OP_ADD || T.args[j].size() != 2) return -1;
+        const uint32_t a = T.args[j][0], b = T.args[j][1];
+        if (k0(a) && node(b) && ix(b) < T.n) return 0;
+        if (k0(b) && node(a) && ix(a) < T.n) return 1;
+        return -1;
+    }
+
+    // Operand wx on the x lanes pairs with wy on the y lanes.
+    bool pair2(uint32_t wx, uint32_t wy)
+    {
+        const bool kx = k0(wx), ky = k0(wy);
+        if (kx || ky) return (kx && ky) ? true : fail("a degree-0 operand pairs with a series");
+        const uint32_t a = ix(wx), b = ix(wy);
+        const bool sa = a < T.n, sb = b < T.n;
+        if (sa && sb) return sg[a] == b ? true : fail("paired nodes read unpaired state variables");
+        if (sa || sb) {
+            const uint32_t nd = sa ? b : a, st = sa ? a : b;
+            const int q = kpos_state_add(nd);
+            if (q < 0) return fail("a state variable pairs with a node other than s + K");
+            if (sg[ix(T.args[nd][1 - q])] != st) return fail("s + K pairs with an unpaired state variable");
+            const char sd = sa ? 2 : 1;
+            if (partner[nd] >= 0) return fail("s + K is also paired with a node");
+            if (addk[nd] && addk[nd] != sd) return fail("s + K pairs on both components");
+            addk[nd] = sd;
+            return true;
+        }
+        return pairn(a, b);
+    }
+
+    bool pairn(uint32_t a, uint32_t b)
+    {
+        if (a == b) return common(a);
+        if (addk[a] || addk[b]) return fail("s + K is also paired with a node");
+        if (partner[a] >= 0 || partner[b] >= 0)
+            return (partner[a] == (int)b && side[a] == 1 && side[b] == 2) ? true
+                                                                        : fail("inconsistent component pairing");
+        const uint32_t op = T.op[a];
+        if (op != T.op[b] || T.args[a].size() != T.args[b].size()) return fail("paired nodes differ in shape");
+        if (op == HYX_OP_TIME) return fail("time in a component pair");
+        if (op == HYX_OP_POW && T.consts[T.aux[a]] != T.consts[T.aux[b]]) return fail("different exponents");
+        if (op != HYX_OP_POW && op != HYX_OP_SIN && op != HYX_OP_COS && T.aux[a] != T.aux[b])
+            return fail("paired nodes differ in shape");
+        partner[a] = (int)b;
+        partner[b] = (int)a;
+        side[a] = 1;
+        side[b] = 2;
+        const auto &A = T.args[a], &B = T.args[b];
+        for (size_t i = 0; i < A.size(); ++i)
+            if (!pair2(A[i], B[i])) return false;
+        if (op == HYX_OP_SIN || op == HYX_OP_COS) return pairn(T.aux[a], T.aux[b]);
+        return true;
+    }
+
+    // Node j is evaluated alike by both components.
+    bool common(uint32_t j)
+    {
+        if (partner[j] == (int)j) return true;
+        if (partner[j] >= 0 || addk[j]) return fail("a node is both common and paired");
+        partner[j] = (int)j;
+        const uint32_t op = T.op[j];
+        const auto &A = T.args[j];
+        if ((op == HYX_OP_ADD || op == HYX_OP_MUL) && A.size() == 2 && !k0(A[0]) && !k0(A[1]) && A[0] != A[1]) {
+            // The two members of a pair combined by a commutative op (x^2 + y^2): either
+            // operand may be the x member.
+            for (int o = 0; o < 2; ++o) {
+                const auto p0 = partner;
+                const auto s0 = side, y0 = sym, k0v = addk;
+                const std::string w0 = why;
+                if (pair2(A[o], A[1 - o])) {
+                    sym[j] = (char)(1 + o);
+                    return true;
+                }
+                partner = p0;
+                side = s0;
+                sym = y0;
+                addk = k0v;
+                why = w0;
+            }
+        }
+        for (auto w : A) {
+            if (k0(w)) continue;
+            if (ix(w) < T.n) return fail("a common node reads a state variable");
+            if (!common(ix(w))) return false;
+        }
+        if (op == HYX_OP_SIN || op == HYX_OP_COS) return common(T.aux[j]);
+        return true;
+    }
+
+    bool attempt(const std::vector<uint32_t> &s)
+    {
+        sg = s;
+        partner.assign(T.nnodes, -1);
+        side.assign(T.nnodes, 0);
+        sym.assign(T.nnodes, 0);
+        addk.assign(T.nnodes, 0);
+        for (uint32_t f = 0; f < T.n; ++f)
+            if (f < sg[f] && !pair2(T.rhs[f], T.rhs[sg[f]])) return false;
+        // Worth it: the component pairs hold convolution work.
+        long split = 0;
+        for (uint32_t j = T.n; j < T.nnodes; ++j) {
+            const uint32_t op = T.op[j];
+            if (side[j] && T.deg[j] > 0 && (op == HYX_OP_MUL || op == HYX_OP_SQUARE || op == HYX_OP_DIV ||
+                                            op == HYX_OP_POW || op == HYX_OP_EXP || op == HYX_OP_SIN))
+                ++split;
+        }
+        return split > 0 ? true : fail("no convolution splits between the components");
+    }
+
+    bool plan()
+    {
+        const uint32_t n = T.n;
+        if (n % 2 != 0) return fail("an odd number of state variables");
+        if (n > 12) return fail("more than 12 state variables");
+        std::vector<uint32_t> cur(n, UINT32_MAX);
+        std::function<bool(uint32_t)> rec = [&](uint32_t i) -> bool {
+            while (i < n && cur[i] != UINT32_MAX) ++i;
+            if (i == n) return attempt(cur);
+            for (uint32_t j = i + 1; j < n; ++j) {
+                if (cur[j] != UINT32_MAX) continue;
+                cur[i] = j;
+                cur[j] = i;
+                if (rec(i + 1)) return true;
+                cur[i] = cur[j] = UINT32_MAX;
+            }
+            return false;
+        };
+        if (rec(0)) {
+            why.clear();
+            return true;
+        }
+        if (why.empty()) why = "no pairing of the state variables";
+        return false;
+    }
+};
+
+// The op-list of the quad form: x members (and the twin's nodes common to both components)
+// as nodes, per-component bindings and component swaps as nodes, operands first.
+Sys build_quad(const Sys &T, const QuadPlanner &qp)
+{
+    struct Ref {
+        bool proto;
+        uint32_t v;
+        bool operator<(const Ref &o) const { return proto != o.proto ? proto < o.proto : v < o.v; }
+    };
+    struct Proto {
+        uint32_t op, aux;
+        std::vector<Ref> args;
+        int partner = -1; // sin / cos: the partner's T node
+    };
+    std::vector<Proto> pr;
+    std::vector<int> pid(T.nnodes, -1), pk(T.nnodes, -1);
+    std::map<uint32_t, uint32_t> swp;
+    std::map<std::pair<Ref, Ref>, uint32_t> qb;
+    Sys Q;
+    Q.n = T.n;
+    Q.npar = T.npar;
+    Q.consts = T.consts;
+    uint32_t negz = UINT32_MAX;
+    auto addp = [&](Proto p) {
+        pr.push_back(std::move(p));
+        return (uint32_t)(pr.size() - 1);
+    };
+    auto swap_of = [&](uint32_t p) {
+        auto it = swp.find(p);
+        if (it == swp.end()) it = swp.emplace(p, addp(Proto{HYX_OP_QUAD_SWAP, 0, {Ref{true, p}}})).first;
+        return it->second;
+    };
+    auto qbind = [&](Ref x, Ref y) -> Ref {
+        auto it = qb.find({x, y});
+        if (it == qb.end()) it = qb.emplace(std::make_pair(x, y), addp(Proto{HYX_OP_QUAD_BIND, 0, {x, y}})).first;
+        return Ref{true, it->second};
+    };
+    std::function<uint32_t(uint32_t)> mk;
+    auto k0ref = [&](uint32_t w) -> Ref { return qp.node(w) ? Ref{true, mk(qp.ix(w))} : Ref{false, w}; };
+    auto state = [&](uint32_t i) { return Ref{false, HYX_OPERAND(HYX_KIND_NODE, i)}; };
+    // s + K paired with a bare state variable.
+    auto mkk = [&](uint32_t nd) -> uint32_t {
+        if (pk[nd] >= 0) return (uint32_t)pk[nd];
+        const int q = qp.kpos_state_add(nd);
+        const uint32_t s = qp.ix(T.args[nd][1 - q]);
+        if (negz == UINT32_MAX) {
+            negz = (uint32_t)Q.consts.size();
+            Q.consts.push_back(-0.0);
+        }
+        const Ref kr = k0ref(T.args[nd][q]), zr{false, HYX_OPERAND(HYX_KIND_CONST, negz)};
+        const bool onx = qp.addk[nd] == 1;
+        const Ref b = onx ? qbind(kr, zr) : qbind(zr, kr);
+        // Evaluated on the x lanes it reads s; on the y lanes it reads s there, which is
+        // relative variable sigma(s).
+        const Ref st = state(onx ? s : qp.sg[s]);
+        Proto p{HYX_OP_ADD, 0, {}};
+        p.args = (q == 0) ? std::vector<Ref>{b, st} : std::vector<Ref>{st, b};
+        pk[nd] = (int)addp(p);
+        return (uint32_t)pk[nd];
+    };
+    // The operand pair (x lanes wx, y lanes wy).
+    auto pref = [&](uint32_t wx, uint32_t wy) -> Ref {
+        if (qp.k0(wx)) return wx == wy ? k0ref(wx) : qbind(k0ref(wx), k0ref(wy));
+        const uint32_t a = qp.ix(wx), b = qp.ix(wy);
+        if (a < T.n && b < T.n) return state(a);
+        if (a < T.n || b < T.n) return Ref{true, mkk(a < T.n ? b : a)};
+        return Ref{true, mk(a)};
+    };
+    mk = [&](uint32_t j) -> uint32_t {
+        if (pid[j] >= 0) return (uint32_t)pid[j];
+        if (qp.side[j] == 2) return mk((uint32_t)qp.partner[j]);
+        Proto p{T.op[j], T.aux[j], {}};
+        const auto &A = T.args[j];
+        if (T.op[j] == HYX_OP_SIN || T.op[j] == HYX_OP_COS) p.partner = (int)T.aux[j];
+        if (qp.side[j] == 1) {
+            const auto &B = T.args[qp.partner[j]];
+            for (size_t i = 0; i < A.size(); ++i) p.args.push_back(pref(A[i], B[i]));
+        } else if (qp.sym[j]) {
+            const int o = qp.sym[j] - 1; // operand o is the x member
+            const Ref own = pref(A[o], A[1 - o]);
+            const Ref oth = own.proto ? Ref{true, swap_of(own.v)} : state(qp.sg[own.v & 0x3fffffffu]);
+            p.args = (o == 0) ? std::vector<Ref>{own, oth} : std::vector<Ref>{oth, own};
+        } else {
+            for (auto w : A) p.args.push_back(k0ref(w));
+        }
+        const uint32_t id = addp(p);
+        pid[j] = (int)id;
+        if (qp.side[j] == 1) pid[qp.partner[j]] = (int)id;
+        return id;
+    };
+    std::vector<Ref> rhs(T.n);
+    for (uint32_t f = 0; f < T.n; ++f) {
+        const uint32_t g = qp.sg[f];
+        if (f > g) continue;
+        const uint32_t wx = T.rhs[f], wy = T.rhs[g];
+        const Ref r = pref(wx, wy);
+        rhs[f] = r;
+        if (qp.k0(wx))
+            rhs[g] = (wx == wy) ? r : qbind(k0ref(wy), k0ref(wx));
+        else if (!r.proto)
+            rhs[g] = state(qp.ix(wy));
+        else if (qp.ix(wx) == qp.ix(wy))
+            rhs[g] = r;
+        else
+            rhs[g] = Ref{true, swap_of(r.v)};
+    }
+    // Topological order, operands first.
+    for (uint32_t j = T.n; j < T.nnodes; ++j)
+        if (T.op[j] == HYX_OP_SIN || T.op[j] == HYX_OP_COS)
+            if (pid[j] >= 0 && pr[pid[j]].partner >= 0) mk((uint32_t)pr[pid[j]].partner);
+    std::vector<int> st(pr.size(), 0), slot(pr.size(), -1);
+    std::vector<uint32_t> order;
+    std::function<void(uint32_t)> visit = [&](uint32_t id) {
+        if (st[id] == 2) return;
+        st[id] = 1;
+        for (const Ref &r : pr[id].args)
+            if (r.proto) visit(r.v);
+        st[id] = 2;
+        slot[id] = (int)order.size();
+        order.push_back(id);
+    };
+    for (uint32_t id = 0; id < pr.size(); ++id) visit(id);
+    Q.nconsts = (uint32_t)Q.consts.size();
+    Q.nnodes = Q.n + (uint32_t)order.size();
+    Q.op.assign(Q.nnodes, 0);
+    Q.aux.assign(Q.nnodes, 0);
+    Q.args.assign(Q.nnodes, {});
+    auto word = [&](const Ref &r) { return r.proto ? HYX_OPERAND(HYX_KIND_NODE, Q.n + (uint32_t)slot[r.v]) : r.v; };
+    for (uint32_t r = 0; r < order.size(); ++r) {
+        const Proto &p = pr[order[r]];
+        const uint32_t j = Q.n + r;
+        Q.op[j] = p.op;
+        Q.aux[j] = (p.op == HYX_OP_SIN || p.op == HYX_OP_COS) ? Q.n + (uint32_t)slot[pid[p.partner]] : p.aux;
+        for (const Ref &a : p.args) Q.args[j].push_back(word(a));
+    }
+    for (const Ref &r : rhs) Q.rhs.push_back(word(r));
+    compute_degrees(Q);
+    return Q;
+}
+
+} // namespace
+
+std::string generate_quad_prelude(const Sys &S, int order, bool high_acc, int split_min, std::string &why)
+{
+    if (order < 2) throw std::invalid_argument("the Taylor order must be at least 2");
+    TwinPlanner tp(S);
+    if (!tp.plan()) {
+        why = tp.why;
+        return "";
+    }
+    const Sys T = build_twin(S, tp);
+    QuadPlanner qp(T);
+    if (!qp.plan()) {
+        why = qp.why;
+        return "";
+    }
+    const Sys Q = build_quad(T, qp);
+    Gen g(Q, order, high_acc);
+    g.split_min = split_min;
+    g.twin = true;
+    g.quad = true;
+    g.qsig = qp.sg;
     g.no_contract = true;
     return g.run();
 }

@@ -29,6 +29,9 @@ struct NoAllocArgs {
 // (lane A: operand 0, lane B: operand 1; aux bit 0 / 1 negates it) and a partner exchange
 // of a twin slot (aux 0: lane A's value on both lanes, 1: lane B's).
 enum { HYX_OP_TWIN_BIND = 100, HYX_OP_TWIN_XCH = 101 };
+// Internal opcodes of the quad form: a per-component degree-0 binding (x lanes: operand 0,
+// y lanes: operand 1) and the partner component's value of a node (lanes x <-> y).
+enum { HYX_OP_QUAD_BIND = 102, HYX_OP_QUAD_SWAP = 103 };
 
 // A parsed op-list (include/hyx_oplist.h).
 struct Sys {
@@ -62,6 +65,11 @@ std::string generate_prelude(const Sys &S, int order, bool high_acc, bool compac
 bool has_twin(const Sys &S);
 Sys twin_oplist(const Sys &S, std::string &why);
 std::string generate_twin_prelude(const Sys &S, int order, bool high_acc, int split_min, std::string &why);
+// Quad form (hyx_codegen.cpp): four lanes per instance for twin systems whose twin slots
+// split once more under a pairing of the state variables (the planar CR3BP: x <-> y,
+// vx <-> vy); the same arithmetic as the unrolled form, bit for bit. "" (reason in why)
+// when it does not apply.
+std::string generate_quad_prelude(const Sys &S, int order, bool high_acc, int split_min, std::string &why);
 // Number of nodes whose coefficient history the recurrences read.
 int history_nodes(const Sys &S);
 // Default choice of code form: compact when the histories exceed the register file.

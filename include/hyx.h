@@ -77,6 +77,9 @@ typedef struct {
 #define HYX_FLAG_NO_LDS_ROWS 65536u
 /* Unrolled form: no twin form (see hyx_batch_set_twin_tail). */
 #define HYX_FLAG_NO_TWIN 131072u
+/* Unrolled form: the twin form (two lanes per instance) continues the last instances even
+ * where the quad form (four lanes) applies. */
+#define HYX_FLAG_NO_QUAD 262144u
 
 /* Code forms (hyx_sys_code_form). */
 #define HYX_FORM_UNROLLED 0 /* one instance per lane, fully unrolled, histories in registers */
@@ -277,13 +280,15 @@ int hyx_autotune(hyx_batch *b, const double *tf_hi, const char *cache_path, hyx_
 /* Twin form (no counterpart in the reference, whose instances are CPU SIMD lanes): an
  * unrolled system whose right-hand sides add isomorphic subexpressions (the planar CR3BP:
  * the pull of each primary) also gets a form with two lanes per instance that splits the
- * isomorphic nodes between the lanes with the unrolled form's arithmetic, bit for bit.
- * A persistent propagation hands its last instances to it once its queue is drained and
- * at most tail_min instances are in flight (default: two per CU; 0: never), so the
- * longest-running instances of a ragged ensemble take lower-latency steps. Results do
- * not depend on the hand-off. hyx_sys_twin_source: the twin form's HIP source (length 0:
- * the system has none); hyx_batch_twin_launches: twin-form launches of the batch's
- * propagations so far. */
+ * isomorphic nodes between the lanes with the unrolled form's arithmetic, bit for bit;
+ * where the twin's nodes split again under a pairing of the state variables (the planar
+ * CR3BP: x <-> y, vx <-> vy), the quad form takes its place, four lanes per instance, the
+ * same arithmetic (HYX_FLAG_NO_QUAD: keep the twin). A persistent propagation hands its
+ * last instances to it once its queue is drained and at most tail_min instances are in
+ * flight (default: eight per CU; 0: never), so the longest-running instances of a ragged
+ * ensemble take lower-latency steps. Results do not depend on the hand-off.
+ * hyx_sys_twin_source: that form's HIP source (length 0: the system has none);
+ * hyx_batch_twin_launches: its launches in the batch's propagations so far. */
 int hyx_batch_set_twin_tail(hyx_batch *b, uint32_t tail_min);
 size_t hyx_sys_twin_source(const hyx_sys *s, char *buf, size_t cap);
 uint64_t hyx_batch_twin_launches(const hyx_batch *b);

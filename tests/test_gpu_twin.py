@@ -1,8 +1,9 @@
-"""The twin form on the GPU: persistent propagations that hand their last instances to
-the twin form (two lanes per instance) end bit for bit where the plain one-launch-per-step
-lane form ends, for the ragged CR3BP ensemble and a pendulum pulled by two phase-shifted
-sines (sin / cos pairs, parameter bindings); with the hand-off forced at the start, the
-whole propagation runs in the twin form.
+"""The twin and quad forms on the GPU: persistent propagations that hand their last
+instances to the twin form (two lanes per instance) or to the quad form (four lanes, the
+CR3BP's default) end bit for bit where the plain one-launch-per-step lane form ends, for
+the ragged CR3BP ensemble and a pendulum pulled by two phase-shifted sines (twin only:
+sin / cos pairs, parameter bindings); with the hand-off forced at the start, the whole
+propagation runs in the tail form.
 """
 
 import numpy as np
@@ -47,14 +48,16 @@ def _run(system, st, pa, tf, mode, tail=None):
     return out
 
 
-@pytest.mark.parametrize("name", ["cr3bp", "two_sines"])
+@pytest.mark.parametrize("name", ["cr3bp", "two_sines", "cr3bp_twin"])
 @pytest.mark.parametrize("tail", [None, 1 << 30, 0])
 def test_twin_handoff_bitwise_equal_to_plain(name, tail):
     print(f"[gpu] twin hand-off: {name}, tail_min={tail}", flush=True)
     n = 6000
-    sys_, st, pa, tf = _case(name, n)
-    system = get_system(hy.taylor_decompose(sys_), 19)
-    assert system.twin_source() != ""
+    sys_, st, pa, tf = _case(name.replace("_twin", ""), n)
+    dec = hy.taylor_decompose(sys_)
+    system = _lib.System(dec, 19, quad=False) if name == "cr3bp_twin" else get_system(dec, 19)
+    tsrc = system.twin_source()
+    assert tsrc != "" and ("#define HYX_QUAD 1" in tsrc) == (name == "cr3bp")
     ps, pt, pr, _ = _run(system, st, pa, tf, _lib.MODE_PLAIN)
     qs, qt, qr, launches = _run(system, st, pa, tf, _lib.MODE_PERSISTENT, tail)
     if tail == 0:

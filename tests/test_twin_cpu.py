@@ -1,12 +1,14 @@
-"""The twin form on the CPU: the same step as the unrolled form, bit for bit.
+"""The twin and quad forms on the CPU: the same step as the unrolled form, bit for bit.
 
 The twin form (two lanes per instance, the isomorphic right-hand-side terms split
-between them, partner values exchanged) is compiled here with g++, one thread per lane
-and a barrier exchange standing in for the DPP broadcasts; the unrolled form's step is
-compiled the same way with one thread. Both are built without implicit fusion, as on
-the GPU. For the planar CR3BP (the pull of each primary) and a pendulum pulled by two
-phase-shifted sines (sin / cos pairs, parameter bindings), every lane of the twin must
-end with exactly the lane form's state, step size and Taylor coefficients.
+between them, partner values exchanged) and the quad form (four lanes: each primary's
+pull split again into its x and y components, the state held in lane-relative order)
+are compiled here with g++, one thread per lane and a barrier exchange standing in for
+the DPP broadcasts; the unrolled form's step is compiled the same way with one thread.
+All are built without implicit fusion, as on the GPU. For the planar CR3BP (twin and
+quad) and a pendulum pulled by two phase-shifted sines (twin: sin / cos pairs,
+parameter bindings), every lane must end with exactly the lane form's state, step size
+and Taylor coefficients.
 """
 
 import ctypes as ct
@@ -32,11 +34,12 @@ using std::sqrt; using std::pow; using std::exp; using std::log; using std::fabs
 """
 
 TWIN = COMMON + r"""
+#define NL 2
 static pthread_barrier_t bar;
 static thread_local int tl_lane;
-static double xch[2];
-static bool hyx_twin_lane_b() { return (tl_lane & 1) != 0; }
+static double xch[NL];
 static double bcast(double v, int from) { xch[tl_lane] = v; pthread_barrier_wait(&bar); double r = xch[from]; pthread_barrier_wait(&bar); return r; }
+static bool hyx_twin_lane_b() { return (tl_lane & 1) != 0; }
 static double hyx_twin_from_a(double v) { return bcast(v, 0); }
 static double hyx_twin_from_b(double v) { return bcast(v, 1); }
 #include "prelude.inc"
@@ -56,14 +59,25 @@ static void *run(void *p)
 }
 extern "C" int step2(const double *x, const double *pv, double t0, double mdt, double *xout, double *h, int *nf, double *tc)
 {
-    pthread_barrier_init(&bar, nullptr, 2);
-    pthread_t th[2]; Arg args[2];
-    for (int l = 0; l < 2; ++l) { args[l] = Arg{l, x, pv, t0, mdt, xout, h, nf, tc}; pthread_create(&th[l], nullptr, run, &args[l]); }
-    for (int l = 0; l < 2; ++l) pthread_join(th[l], nullptr);
+    pthread_barrier_init(&bar, nullptr, NL);
+    pthread_t th[NL]; Arg args[NL];
+    for (int l = 0; l < NL; ++l) { args[l] = Arg{l, x, pv, t0, mdt, xout, h, nf, tc}; pthread_create(&th[l], nullptr, run, &args[l]); }
+    for (int l = 0; l < NL; ++l) pthread_join(th[l], nullptr);
     pthread_barrier_destroy(&bar);
     return 0;
 }
 """
+
+# Quad lanes q = 2 * primary + component: from_a / from_b read primary A's / B's lane of
+# the same component, hyx_quad_swap the other component's lane of the same primary.
+QUAD = TWIN.replace("#define NL 2", "#define NL 4").replace(
+    """static bool hyx_twin_lane_b() { return (tl_lane & 1) != 0; }
+static double hyx_twin_from_a(double v) { return bcast(v, 0); }
+static double hyx_twin_from_b(double v) { return bcast(v, 1); }""",
+    """static unsigned hyx_quad_lane() { return (unsigned)tl_lane; }
+static double hyx_twin_from_a(double v) { return bcast(v, tl_lane & 1); }
+static double hyx_twin_from_b(double v) { return bcast(v, 2 + (tl_lane & 1)); }
+static double hyx_quad_swap(double v) { return bcast(v, tl_lane ^ 1); }""")
 
 LANE = COMMON + r"""
 static double hyx_xs_emu[64 * 64 * 64];
@@ -117,16 +131,17 @@ def _cases(name):
     return _two_sines(), st, pa
 
 
-@pytest.mark.parametrize("name", ["cr3bp", "two_sines"])
-def test_twin_step_is_the_lane_step_bitwise(name, tmp_path):
+@pytest.mark.parametrize("name,lanes", [("cr3bp", 2), ("two_sines", 2), ("cr3bp", 4)])
+def test_twin_step_is_the_lane_step_bitwise(name, lanes, tmp_path):
     sys_, st, pa = _cases(name)
     dec = hy.taylor_decompose(sys_)
     p = 19
-    s = _lib.System(dec, p, compact=False, source_only=True)
+    s = _lib.System(dec, p, compact=False, source_only=True, quad=lanes == 4)
     tsrc = s.twin_source()
-    assert tsrc and "#define HYX_SUB 2" in tsrc and "hyx_twin_from_b" in tsrc
+    assert tsrc and f"#define HYX_SUB {lanes}" in tsrc and "hyx_twin_from_b" in tsrc
+    assert ("hyx_quad_swap" in tsrc) == (lanes == 4)
     assert "#pragma clang fp contract(off)" in s.source()
-    f2 = _build(tsrc, TWIN, tmp_path / "twin", "twin", "step2")
+    f2 = _build(tsrc, TWIN if lanes == 2 else QUAD, tmp_path / "twin", "twin", "step2")
     f1 = _build(s.source(), LANE, tmp_path / "lane", "lane", "step1")
     n = dec.n_state
     ptr = lambda a: a.ctypes.data_as(ct.POINTER(ct.c_double))
@@ -136,12 +151,22 @@ def test_twin_step_is_the_lane_step_bitwise(name, tmp_path):
         for mdt in (np.inf, 0.01):
             x1, h1, nf1, tc1 = np.zeros(n), np.zeros(1), ct.c_int(), np.zeros(n * (p + 1))
             f1(ptr(x), ptr(pv), 0.25, mdt, ptr(x1), ptr(h1), ct.byref(nf1), ptr(tc1))
-            x2, h2, nf2, tc2 = np.zeros(2 * n), np.zeros(2), np.zeros(2, dtype=np.int32), np.zeros(2 * n * (p + 1))
+            L = lanes
+            x2, h2, nf2, tc2 = np.zeros(L * n), np.zeros(L), np.zeros(L, dtype=np.int32), np.zeros(L * n * (p + 1))
             f2(ptr(x), ptr(pv), 0.25, mdt, ptr(x2), ptr(h2), nf2.ctypes.data_as(ct.POINTER(ct.c_int)), ptr(tc2))
-            for lane in range(2):
+            for lane in range(L):
                 assert np.array_equal(x2[lane * n:(lane + 1) * n], x1), (b, lane)
                 assert h2[lane] == h1[0] and nf2[lane] == nf1.value
                 assert np.array_equal(tc2[lane * n * (p + 1):(lane + 1) * n * (p + 1)], tc1), (b, lane)
+
+
+def test_quad_form_applies_where_the_state_pairs():
+    # The two-sines pendulum has a twin form but no pairing of its state variables (x' = v
+    # is not isomorphic to v' = ...): its tail form stays the twin.
+    s = _lib.System(hy.taylor_decompose(_two_sines()), 19, compact=False, source_only=True)
+    assert "#define HYX_SUB 2" in s.twin_source()
+    s = _lib.System(hy.taylor_decompose(W.cr3bp(4).sys), 19, compact=False, source_only=True)
+    assert "#define HYX_SUB 4" in s.twin_source()
 
 
 def test_no_twin_without_isomorphic_terms():
