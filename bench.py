@@ -338,6 +338,9 @@ def main():
         ns = r["nsteps"].astype(np.int64)
         out["ragged"] = {"nsteps_median": float(np.median(ns)), "nsteps_p99": float(np.percentile(ns, 99)),
                          "nsteps_max": int(ns.max()), "persistent_launches": int(nl)}
+        if system.twin_source() and not args.no_extras:
+            k = int(np.argmax(ns))
+            out["straggler_alone"] = _straggler_alone(system, wl, k, final[:, k].copy(), out["ms_per_step"] / 1e3)
     if not args.no_accuracy:
         ns = {"nbody6": 16, "cr3bp": 64, "j2var": 8}[args.config]
         ta = {"nbody6": 2.0, "cr3bp": 5.0, "j2var": 2.0}[args.config]
@@ -436,8 +439,13 @@ def _cr3bp_record():
                       "instance_steps": int(ns.sum()), "nsteps_max": int(ns.max()),
                       "err_nf_state": int(np.sum(r["outcome"] == -4294967300))}
         _log(f"cr3bp {label}: {out[label]['value']:.3e} instance-steps/s, {wall:.2f} s, {nl} launches")
+        if label == "persistent":
+            k = int(np.argmax(ns))
+            end_k = b.read(_lib.FIELD_STATE, wl.state.shape)[:, k].copy()
     out["persistent_over_plain"] = out["persistent"]["value"] / out["plain"]["value"]
     tsrc = system.twin_source()
+    if tsrc:
+        out["straggler_alone"] = _straggler_alone(system, wl, k, end_k, out["persistent"]["wall_s"])
     out["tail_form"] = ("quad: 4 lanes per instance" if "#define HYX_QUAD 1" in tsrc
                         else "twin: 2 lanes per instance" if tsrc else None)
     out["tail_launches"] = b.twin_launches()
@@ -463,6 +471,29 @@ def _cr3bp_record():
         lone[label] = ms * 1e3 / float(r["nsteps"][0])
         one.close()
     out["lone_instance_us_per_step"] = lone
+    return out
+
+
+def _straggler_alone(system, wl, k, end_k, ensemble_wall):
+    """Ensemble instance k (its longest-running one) propagated alone in the tail form from
+    the start: the floor of the ensemble's persistent makespan. It must end where it ended
+    inside the ensemble (end_k), bit for bit."""
+    from heyoka_amd import _lib
+
+    n = wl.state.shape[0]
+    one = _lib.Batch(system, 1)
+    one.set_twin_tail(1 << 30)
+    one.write(_lib.FIELD_STATE, np.ascontiguousarray(wl.state[:, k:k + 1]))
+    one.write(_lib.FIELD_PARS, np.ascontiguousarray(wl.pars[:, k:k + 1]))
+    one.write(_lib.FIELD_TIME_HI, np.zeros(1))
+    one.write(_lib.FIELD_TIME_LO, np.zeros(1))
+    r, wall, ms, nl = _timed(one, wl.t_final[k:k + 1].copy(), _lib.MODE_PERSISTENT)
+    same = bool(np.array_equal(one.read(_lib.FIELD_STATE, (n, 1))[:, 0], end_k))
+    one.close()
+    out = {"instance": k, "nsteps": int(r["nsteps"][0]), "wall_s": wall, "kernel_ms": ms, "launches": int(nl),
+           "us_per_step": ms * 1e3 / float(r["nsteps"][0]), "end_state_equals_ensemble": same,
+           "share_of_persistent_wall": wall / ensemble_wall}
+    _log(f"cr3bp straggler {k} alone: {ms:.0f} ms, {out['us_per_step']:.2f} us/step, same end state: {same}")
     return out
 
 
