@@ -26,10 +26,29 @@ def _two_sines(n, seed=5):
     return sys_, st, pa, 10 ** rng.uniform(0.5, 2.0, n)
 
 
+def _planar_drag(n, seed=9):
+    # A planar two-primary problem with smooth Gaussian pulls and a speed-dependent drag
+    # (quad form; |v| is common to both components); no singularity, so no instance can
+    # spiral into a primary with ever smaller steps.
+    x, y, vx, vy = hy.make_vars("x", "y", "vx", "vy")
+    mu = hy.par[0]
+    d1, d2 = x + mu, x - (1.0 - mu)
+    f1, f2 = hy.exp(-(d1 ** 2 + y ** 2)), hy.exp(-(d2 ** 2 + y ** 2))
+    sp = hy.sqrt(vx ** 2 + vy ** 2)
+    sys_ = [(x, vx), (y, vy), (vx, 2.0 * vy + x - (f1 * d1 * (1.0 - mu) + f2 * d2 * mu) - 0.01 * sp * vx),
+            (vy, -2.0 * vx + y - (f1 * y * (1.0 - mu) + f2 * y * mu) - 0.01 * sp * vy)]
+    rng = np.random.default_rng(seed)
+    st = np.stack([rng.uniform(-1.5, 1.5, n), rng.uniform(0.4, 1.5, n) * rng.choice([-1, 1], n),
+                   rng.uniform(-0.3, 0.3, n), rng.uniform(-0.3, 0.3, n)])
+    return sys_, st, rng.uniform(0.001, 0.05, (1, n)), 10 ** rng.uniform(0.5, 1.8, n)
+
+
 def _case(name, n):
     if name == "cr3bp":
         wl = W.cr3bp(n, log_t=(1.0, 2.6))
         return wl.sys, wl.state, wl.pars, wl.t_final
+    if name == "planar_drag":
+        return _planar_drag(n)
     return _two_sines(n)
 
 
@@ -48,7 +67,7 @@ def _run(system, st, pa, tf, mode, tail=None):
     return out
 
 
-@pytest.mark.parametrize("name", ["cr3bp", "two_sines", "cr3bp_twin"])
+@pytest.mark.parametrize("name", ["cr3bp", "two_sines", "cr3bp_twin", "planar_drag"])
 @pytest.mark.parametrize("tail", [None, 1 << 30, 0])
 def test_twin_handoff_bitwise_equal_to_plain(name, tail):
     print(f"[gpu] twin hand-off: {name}, tail_min={tail}", flush=True)
@@ -57,15 +76,18 @@ def test_twin_handoff_bitwise_equal_to_plain(name, tail):
     dec = hy.taylor_decompose(sys_)
     system = _lib.System(dec, 19, quad=False) if name == "cr3bp_twin" else get_system(dec, 19)
     tsrc = system.twin_source()
-    assert tsrc != "" and ("#define HYX_QUAD 1" in tsrc) == (name == "cr3bp")
+    assert tsrc != "" and ("#define HYX_QUAD 1" in tsrc) == (name in ("cr3bp", "planar_drag"))
     ps, pt, pr, _ = _run(system, st, pa, tf, _lib.MODE_PLAIN)
     qs, qt, qr, launches = _run(system, st, pa, tf, _lib.MODE_PERSISTENT, tail)
     if tail == 0:
         assert launches == 0
     else:
         assert launches > 0
-    assert np.array_equal(pt, qt) and np.array_equal(pt, tf)
-    assert np.array_equal(ps, qs)
+    assert np.array_equal(pt, qt)
+    # Instances that collide with a primary stop early (err_nf_state) in both runs alike.
+    done = pr["outcome"] == int(hy.taylor_outcome.time_limit)
+    assert done.mean() > 0.5 and np.array_equal(pt[done], tf[done])
+    assert np.array_equal(ps[:, done], qs[:, done])
     for k in ("outcome", "min_h", "max_h", "nsteps"):
         assert np.array_equal(pr[k], qr[k]), k
     print(f"[gpu] twin hand-off: {name}: {launches} twin launches, max nsteps {int(qr['nsteps'].max())}", flush=True)

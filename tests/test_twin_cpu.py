@@ -116,8 +116,34 @@ def _two_sines():
     return [(x, v), (v, -(hy.par[0] * hy.sin(x + hy.par[1])) - hy.par[2] * hy.sin(x - hy.par[3]) - 0.1 * v)]
 
 
+def _planar(force, drag=False):
+    # A planar two-primary problem in the rotating frame with another central force law
+    # (and optionally a speed-dependent drag, whose |v| is common to both components).
+    x, y, vx, vy = hy.make_vars("x", "y", "vx", "vy")
+    mu = hy.par[0]
+    d1, d2 = x + mu, x - (1.0 - mu)
+    f1, f2 = force(d1 ** 2 + y ** 2), force(d2 ** 2 + y ** 2)
+    ax = 2.0 * vy + x - (f1 * d1 * (1.0 - mu) + f2 * d2 * mu)
+    ay = -2.0 * vx + y - (f1 * y * (1.0 - mu) + f2 * y * mu)
+    if drag:
+        sp = hy.sqrt(vx ** 2 + vy ** 2)
+        ax, ay = ax - 0.01 * sp * vx, ay - 0.01 * sp * vy
+    return [(x, vx), (y, vy), (vx, ax), (vy, ay)]
+
+
+PLANAR = {"planar_r5": lambda: _planar(lambda r2: r2 ** -2.5),
+          "planar_exp": lambda: _planar(lambda r2: hy.exp(-r2)),
+          "planar_drag": lambda: _planar(lambda r2: r2 ** -1.5, drag=True)}
+
+
 def _cases(name):
     rng = np.random.default_rng(17)
+    if name in PLANAR:
+        n = 12
+        st = np.stack([rng.uniform(-1.5, 1.5, n), rng.uniform(0.3, 1.5, n) * rng.choice([-1, 1], n),
+                       rng.uniform(-0.5, 0.5, n), rng.uniform(-0.5, 0.5, n)])
+        st[1, 0] = -0.0  # a signed zero on the y lanes' bare coordinate
+        return PLANAR[name](), st, rng.uniform(0.001, 0.05, (1, n))
     if name == "cr3bp":
         wl = W.cr3bp(24)
         st, pa = wl.state, wl.pars
@@ -131,7 +157,8 @@ def _cases(name):
     return _two_sines(), st, pa
 
 
-@pytest.mark.parametrize("name,lanes", [("cr3bp", 2), ("two_sines", 2), ("cr3bp", 4)])
+@pytest.mark.parametrize("name,lanes", [("cr3bp", 2), ("two_sines", 2), ("cr3bp", 4), ("planar_r5", 4),
+                                        ("planar_exp", 4), ("planar_drag", 4)])
 def test_twin_step_is_the_lane_step_bitwise(name, lanes, tmp_path):
     sys_, st, pa = _cases(name)
     dec = hy.taylor_decompose(sys_)
