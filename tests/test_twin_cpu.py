@@ -196,6 +196,21 @@ def test_quad_form_applies_where_the_state_pairs():
     assert "#define HYX_SUB 4" in s.twin_source()
 
 
+def test_no_quad_without_a_state_pairing():
+    # A perturbation on the x component only (or a different pull law per component) breaks
+    # the x <-> y pairing: the tail form stays the twin.
+    x, y, vx, vy = hy.make_vars("x", "y", "vx", "vy")
+    mu = hy.par[0]
+    d1, d2 = x + mu, x - (1.0 - mu)
+    f1, f2 = (d1 ** 2 + y ** 2) ** -1.5, (d2 ** 2 + y ** 2) ** -1.5
+    ax = 2.0 * vy + x - (f1 * d1 * (1.0 - mu) + f2 * d2 * mu)
+    ay = -2.0 * vx + y - (f1 * y * (1.0 - mu) + f2 * y * mu)
+    for sys_ in ([(x, vx), (y, vy), (vx, ax + 0.01 * x * x), (vy, ay)],
+                 [(x, vx), (y, vy), (vx, ax), (vy, -2.0 * vx + y - (f1 * y * (1.0 - mu) + f2 * y * y * mu))]):
+        s = _lib.System(hy.taylor_decompose(sys_), 19, compact=False, source_only=True)
+        assert "#define HYX_SUB 2" in s.twin_source()
+
+
 def test_no_twin_without_isomorphic_terms():
     for sys_ in (W.pendulum().sys, hy.model.j2()):
         s = _lib.System(hy.taylor_decompose(sys_), 19, compact=False, source_only=True)
