@@ -353,7 +353,8 @@ class System:
         return buf.value.decode()
 
     def twin_source(self) -> str:
-        """The twin form's source ("" when the system has none)."""
+        """The tail form's source: the quad form (four lanes per instance) where the system's
+        state variables pair, else the twin form (two); "" when the system has neither."""
         n = self._L.hyx_sys_twin_source(self.h, None, 0)
         if n == 0:
             return ""
@@ -547,7 +548,8 @@ class Batch:
             check(self._L.hyx_batch_set_queue(self.h, dptr(o, ct.c_uint32)))
 
     def set_twin_tail(self, tail_min: int):
-        """Hand-off threshold of persistent propagations to the twin form (0: never)."""
+        """Hand-off threshold of persistent propagations to the tail form (instances in flight;
+        default eight per CU, 0: never)."""
         check(self._L.hyx_batch_set_twin_tail(self.h, int(tail_min)))
 
     def twin_launches(self) -> int:

@@ -324,11 +324,12 @@ extern "C" __global__ void hyx_clear_u64_k(hyx_u64 *p, unsigned n)
 // unfinished (counted in *unfinished, listed in tailq when given) and the host
 // launches again; so every lane leaves the loop after a bounded number of iterations.
 //
-// Tail hand-off (a.inflight given, tail_min > 0: the system has a twin form): the
+// Tail hand-off (a.inflight given, tail_min > 0: the system has a tail form): the
 // kernel counts the instances in flight; once the queue is drained and at most
 // tail_min remain, every wave writes its instances back (listed in tailq) and leaves,
-// and the host continues them in the twin form, two lanes per instance and about
-// half the instructions per step, bit for bit the same arithmetic. The last
+// and the host continues them in the tail form (this same kernel compiled with the
+// quad prelude, four lanes per instance, or the twin prelude, two), with fewer
+// instructions per step per lane and bit for bit the same arithmetic. The last
 // instances of a ragged ensemble, which run alone on their SIMDs, then take steps
 // of lower latency.
 extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
@@ -427,7 +428,7 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
                 if (lane == 0) infl = __hip_atomic_load(a.inflight, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 infl = (unsigned)__shfl((int)infl, 0);
                 if (infl <= a.tail_min && active) {
-                    // Hand-off: written back unfinished, continued by the twin form.
+                    // Hand-off: written back unfinished, continued by the tail form.
                     hyx_suspend(s, a, inst, false, HYX_TIME_LIMIT);
                     if (sub == 0) {
                         const unsigned q = atomicAdd(a.unfinished, 1u);

@@ -302,7 +302,7 @@ struct hyx_batch {
     // instances written back unfinished ([N] each) and the hand-off threshold (0: off).
     unsigned *inflight = nullptr, *tailq = nullptr;
     uint32_t tail_min = 0;
-    uint64_t twin_handoffs = 0; // twin launches of the last propagations (diagnostics)
+    uint64_t twin_handoffs = 0; // tail-form launches of the last propagations (diagnostics)
 };
 
 namespace
@@ -1196,8 +1196,8 @@ int run_propagation(hyx_batch *b, uint64_t max_steps, int mode, hyx_prop_res *re
     uint64_t launches = 0;
     HIPCHK(hipEventRecord(b->ev0, b->stream));
     if (mode == HYX_MODE_PERSISTENT) {
-        // With a twin form, the last instances are handed off to it: the lane kernel
-        // writes them back once few remain in flight, and the twin kernel continues the
+        // With a tail form (quad or twin), the last instances are handed off to it: the lane
+        // kernel writes them back once few remain in flight, and the tail kernel continues the
         // listed ones (and any an iteration budget writes back) until none is left.
         const bool twin = s->kt_persist && b->tail_min > 0 && !grid_run && !clog && !b->ev_pause;
         uint64_t grid = pgrid;
@@ -1225,7 +1225,7 @@ int run_propagation(hyx_batch *b, uint64_t max_steps, int mode, hyx_prop_res *re
             HIPCHK(hipStreamSynchronize(b->stream));
             if (unf == 0) break;
             if (twin) {
-                // Continue the listed instances in the twin form, 32 per wave (quad: 16).
+                // Continue the listed instances in the tail form, 16 per wave (twin: 32).
                 kp = s->kt_persist;
                 a.queue = b->tailq + (uint64_t)cur * N;
                 a.N = unf;
