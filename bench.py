@@ -443,6 +443,8 @@ def _cr3bp_record():
         if label == "persistent":
             k = int(np.argmax(ns))
             end_k = b.read(_lib.FIELD_STATE, wl.state.shape)[:, k].copy()
+            # From the first tail-form launch to the end: the stretch only the last instances run.
+            out[label]["tail_s"] = b.last_tail_ms() / 1e3
     out["persistent_over_plain"] = out["persistent"]["value"] / out["plain"]["value"]
     tsrc = system.twin_source()
     if tsrc:

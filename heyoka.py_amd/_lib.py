@@ -109,6 +109,7 @@ def lib():
                                                        vp]),
             "hyx_propagate_grid": (ct.c_int, [vp, P(d), u64, P(d), u64, ct.c_int, P(prop_res), P(d)]),
             "hyx_batch_last_timing": (ct.c_int, [vp, P(d), P(u64)]),
+            "hyx_batch_last_tail_ms": (ct.c_int, [vp, P(d)]),
             "hyx_batch_set_persistent_config": (ct.c_int, [vp, u32, u32]),
             "hyx_batch_set_queue": (ct.c_int, [vp, P(u32)]),
             "hyx_batch_set_iter_budget": (ct.c_int, [vp, u64]),
@@ -162,7 +163,7 @@ EXPORTED = (
     "hyx_batch_write", "hyx_batch_read", "hyx_batch_device_ptr", "hyx_step", "hyx_read_tc", "hyx_propagate_until",
     "hyx_propagate_until_noalloc",
     "hyx_propagate_grid",
-    "hyx_batch_last_timing", "hyx_batch_set_persistent_config", "hyx_batch_set_queue", "hyx_batch_set_iter_budget",
+    "hyx_batch_last_timing", "hyx_batch_last_tail_ms", "hyx_batch_set_persistent_config", "hyx_batch_set_queue", "hyx_batch_set_iter_budget",
     "hyx_batch_sync", "hyx_slots_reset", "hyx_slots_write", "hyx_slots_read", "hyx_slots_arm", "hyx_slots_launch",
     "hyx_slots_wait", "hyx_slots_poll", "hyx_slots_results", "hyx_autotune",
     "hyx_slots_snapshot_words", "hyx_slots_snapshot", "hyx_slots_restore",
@@ -536,6 +537,12 @@ class Batch:
         ms, n = ct.c_double(), ct.c_uint64()
         check(self._L.hyx_batch_last_timing(self.h, ct.byref(ms), ct.byref(n)))
         return ms.value, n.value
+
+    def last_tail_ms(self) -> float:
+        """Of the last persistent propagation: ms from the first tail-form launch to the end."""
+        ms = ct.c_double()
+        check(self._L.hyx_batch_last_tail_ms(self.h, ct.byref(ms)))
+        return ms.value
 
     def set_persistent_config(self, waves_per_cu: int = 0, refill_min: int = 1):
         check(self._L.hyx_batch_set_persistent_config(self.h, int(waves_per_cu), int(refill_min)))

@@ -272,7 +272,9 @@ struct hyx_batch {
     uint32_t waves_per_cu = 0, refill_min = 1;
     uint64_t iter_budget = 1ull << 16;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t evt = nullptr; // the first tail-form launch of a persistent propagation
     double last_ms = 0.0;
+    double last_tail_ms = 0.0; // from that launch to the end (0: no hand-off)
     uint64_t last_launches = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -905,6 +907,7 @@ int hyx_batch_create(hyx_sys *s, uint64_t n_inst, hyx_batch **out)
         return cleanup(HYX_EHIP, "hipStreamCreate failed");
     hipEventCreate(&b->ev0);
     hipEventCreate(&b->ev1);
+    hipEventCreate(&b->evt);
     const uint64_t N = n_inst, n = s->S.n, np = std::max<uint32_t>(s->S.npar, 1);
     struct {
         void **p;
@@ -978,6 +981,7 @@ void hyx_batch_destroy(hyx_batch *b)
         if (p) hipFree(p);
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->evt) hipEventDestroy(b->evt);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
 }
@@ -1194,6 +1198,7 @@ int run_propagation(hyx_batch *b, uint64_t max_steps, int mode, hyx_prop_res *re
     if (init) HIPCHK(launch(k_init, blocks_for(N), b->stream, &a));
     b->last_grid = grid_run;
     uint64_t launches = 0;
+    bool tail_started = false;
     HIPCHK(hipEventRecord(b->ev0, b->stream));
     if (mode == HYX_MODE_PERSISTENT) {
         // With a tail form (quad or twin), the last instances are handed off to it: the lane
@@ -1226,6 +1231,10 @@ int run_propagation(hyx_batch *b, uint64_t max_steps, int mode, hyx_prop_res *re
             if (unf == 0) break;
             if (twin) {
                 // Continue the listed instances in the tail form, 16 per wave (twin: 32).
+                if (kp != s->kt_persist) {
+                    HIPCHK(hipEventRecord(b->evt, b->stream));
+                    tail_started = true;
+                }
                 kp = s->kt_persist;
                 a.queue = b->tailq + (uint64_t)cur * N;
                 a.N = unf;
@@ -1279,6 +1288,12 @@ int run_propagation(hyx_batch *b, uint64_t max_steps, int mode, hyx_prop_res *re
     hipEventElapsedTime(&ms, b->ev0, b->ev1);
     b->last_ms = ms;
     b->last_launches = launches;
+    b->last_tail_ms = 0.0;
+    if (tail_started) {
+        float tms = 0.f;
+        hipEventElapsedTime(&tms, b->evt, b->ev1);
+        b->last_tail_ms = tms;
+    }
     if (res) {
         std::vector<int64_t> oc(N);
         std::vector<double> mn(N), mx(N);
@@ -1567,6 +1582,13 @@ int hyx_batch_last_timing(const hyx_batch *b, double *kernel_ms, uint64_t *launc
     if (!b) return fail(HYX_EINVAL, "null batch");
     if (kernel_ms) *kernel_ms = b->last_ms;
     if (launches) *launches = b->last_launches;
+    return 0;
+}
+
+int hyx_batch_last_tail_ms(const hyx_batch *b, double *tail_ms)
+{
+    if (!b || !tail_ms) return fail(HYX_EINVAL, "null argument");
+    *tail_ms = b->last_tail_ms;
     return 0;
 }
 

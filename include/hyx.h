@@ -255,6 +255,10 @@ int hyx_cooldowns_write(hyx_batch *b, const double *src);
 /* Kernel time (ms, HIP events on the batch's stream) and launch count of the
  * last hyx_propagate_until / hyx_step call. */
 int hyx_batch_last_timing(const hyx_batch *b, double *kernel_ms, uint64_t *launches);
+/* (No counterpart in the reference: diagnostics of the tail hand-off.) Of the last
+ * persistent hyx_propagate_until, the time (ms) from the first tail-form launch to the
+ * end; 0 when no instance was handed off. */
+int hyx_batch_last_tail_ms(const hyx_batch *b, double *tail_ms);
 /* Persistent-form tuning: waves per CU (0 = occupancy limit) and the idle-lane
  * count that triggers a refill (1..64). */
 int hyx_batch_set_persistent_config(hyx_batch *b, uint32_t waves_per_cu, uint32_t refill_min);
