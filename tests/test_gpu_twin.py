@@ -62,6 +62,10 @@ def _run(system, st, pa, tf, mode, tail=None):
     b.write(_lib.FIELD_TIME_HI, np.zeros(N))
     b.write(_lib.FIELD_TIME_LO, np.zeros(N))
     r = b.propagate_until(tf, None, None, 0, mode)
+    ms, _ = b.last_timing()
+    tail_ms = b.last_tail_ms()
+    # The tail phase is part of the propagation's kernel time, and absent without a hand-off.
+    assert 0.0 <= tail_ms <= ms and (tail_ms > 0.0) == (b.twin_launches() > 0)
     out = (b.read(_lib.FIELD_STATE, st.shape), b.read(_lib.FIELD_TIME_HI, (N,)), r, b.twin_launches())
     b.close()
     return out
