@@ -111,3 +111,20 @@ def test_twin_integrator_api_and_oracle():
     r = O.propagate_until(hy.taylor_decompose(sys_), 19, st, tf, pars=pa, nthreads=8, lane_batch=1)
     err = np.max(np.abs(ta.state - r["state"]) / np.maximum(np.abs(r["state"]), 1.0))
     assert err < 1e-11, err
+
+
+def test_quad_integrator_api_and_oracle():
+    # The CR3BP through the drop-in API: a batch small enough to run wholly in the quad
+    # form, against the oracle on horizons short enough to stay well conditioned.
+    import pyoracle as O
+
+    print("[gpu] quad via taylor_adaptive_batch vs oracle", flush=True)
+    wl = W.cr3bp(64, log_t=(0.3, 1.0))
+    ta = hy.taylor_adaptive_batch(wl.sys, wl.state, pars=wl.pars, tol=1e-15)
+    assert "#define HYX_QUAD 1" in ta._hyx_batch.sys.twin_source()
+    ta.propagate_until(wl.t_final)
+    assert ta._hyx_batch.twin_launches() > 0
+    r = O.propagate_until(hy.taylor_decompose(wl.sys), 19, wl.state, wl.t_final, pars=wl.pars, nthreads=8,
+                          lane_batch=1)
+    err = np.max(np.abs(ta.state - r["state"]) / np.maximum(np.abs(r["state"]), 1.0))
+    assert err < 1e-10, err
