@@ -1,9 +1,10 @@
 """The twin and quad forms on the GPU: persistent propagations that hand their last
 instances to the twin form (two lanes per instance) or to the quad form (four lanes, the
 CR3BP's default) end bit for bit where the plain one-launch-per-step lane form ends, for
-the ragged CR3BP ensemble and a pendulum pulled by two phase-shifted sines (twin only:
-sin / cos pairs, parameter bindings); with the hand-off forced at the start, the whole
-propagation runs in the tail form.
+the ragged CR3BP ensemble (forward and backward in time), a planar two-primary problem with
+drag, and a pendulum pulled by two phase-shifted sines (twin only: sin / cos pairs,
+parameter bindings); with the hand-off forced at the start, the whole propagation runs in
+the tail form.
 """
 
 import numpy as np
@@ -49,6 +50,9 @@ def _case(name, n):
         return wl.sys, wl.state, wl.pars, wl.t_final
     if name == "planar_drag":
         return _planar_drag(n)
+    if name == "cr3bp_back":
+        wl = W.cr3bp(n, log_t=(1.0, 2.6), seed=11)
+        return wl.sys, wl.state, wl.pars, -wl.t_final
     return _two_sines(n)
 
 
@@ -71,7 +75,7 @@ def _run(system, st, pa, tf, mode, tail=None):
     return out
 
 
-@pytest.mark.parametrize("name", ["cr3bp", "two_sines", "cr3bp_twin", "planar_drag"])
+@pytest.mark.parametrize("name", ["cr3bp", "two_sines", "cr3bp_twin", "planar_drag", "cr3bp_back"])
 @pytest.mark.parametrize("tail", [None, 1 << 30, 0])
 def test_twin_handoff_bitwise_equal_to_plain(name, tail):
     print(f"[gpu] twin hand-off: {name}, tail_min={tail}", flush=True)
@@ -80,7 +84,7 @@ def test_twin_handoff_bitwise_equal_to_plain(name, tail):
     dec = hy.taylor_decompose(sys_)
     system = _lib.System(dec, 19, quad=False) if name == "cr3bp_twin" else get_system(dec, 19)
     tsrc = system.twin_source()
-    assert tsrc != "" and ("#define HYX_QUAD 1" in tsrc) == (name in ("cr3bp", "planar_drag"))
+    assert tsrc != "" and ("#define HYX_QUAD 1" in tsrc) == (name in ("cr3bp", "planar_drag", "cr3bp_back"))
     ps, pt, pr, _ = _run(system, st, pa, tf, _lib.MODE_PLAIN)
     qs, qt, qr, launches = _run(system, st, pa, tf, _lib.MODE_PERSISTENT, tail)
     if tail == 0:

@@ -157,13 +157,14 @@ def _cases(name):
     return _two_sines(), st, pa
 
 
-@pytest.mark.parametrize("name,lanes", [("cr3bp", 2), ("two_sines", 2), ("cr3bp", 4), ("planar_r5", 4),
-                                        ("planar_exp", 4), ("planar_drag", 4)])
-def test_twin_step_is_the_lane_step_bitwise(name, lanes, tmp_path):
+@pytest.mark.parametrize("name,lanes,ha", [("cr3bp", 2, False), ("two_sines", 2, False), ("cr3bp", 4, False),
+                                           ("planar_r5", 4, False), ("planar_exp", 4, False),
+                                           ("planar_drag", 4, False), ("cr3bp", 4, True)])
+def test_twin_step_is_the_lane_step_bitwise(name, lanes, ha, tmp_path):
     sys_, st, pa = _cases(name)
     dec = hy.taylor_decompose(sys_)
     p = 19
-    s = _lib.System(dec, p, compact=False, source_only=True, quad=lanes == 4)
+    s = _lib.System(dec, p, compact=False, source_only=True, quad=lanes == 4, high_accuracy=ha)
     tsrc = s.twin_source()
     assert tsrc and f"#define HYX_SUB {lanes}" in tsrc and "hyx_twin_from_b" in tsrc
     assert ("hyx_quad_swap" in tsrc) == (lanes == 4)
@@ -175,7 +176,7 @@ def test_twin_step_is_the_lane_step_bitwise(name, lanes, tmp_path):
     for b in range(st.shape[1]):
         x = np.ascontiguousarray(st[:, b])
         pv = np.ascontiguousarray(pa[:, b])
-        for mdt in (np.inf, 0.01):
+        for mdt in (np.inf, 0.01, -np.inf):
             x1, h1, nf1, tc1 = np.zeros(n), np.zeros(1), ct.c_int(), np.zeros(n * (p + 1))
             f1(ptr(x), ptr(pv), 0.25, mdt, ptr(x1), ptr(h1), ct.byref(nf1), ptr(tc1))
             L = lanes
