@@ -211,11 +211,12 @@ def _flags(high_accuracy=False, compact=None):
     return f
 
 
-def precompile(dec, order: int, high_accuracy: bool = False, arch: str = "gfx950", compact=None, grid=False):
-    """Compile the system's kernels (grid=True: its propagate_grid kernels) into the code-object
-    cache (no GPU needed)."""
+def precompile(dec, order: int, high_accuracy: bool = False, arch: str = "gfx950", compact=None, grid=False,
+               quad: bool = True):
+    """Compile the system's kernels (grid=True: its propagate_grid kernels; quad=False: the twin
+    rather than the quad tail form) into the code-object cache (no GPU needed)."""
     words, consts = oplist_args(dec)
-    flags = _flags(high_accuracy, compact) | (FLAG_GRID if grid else 0)
+    flags = _flags(high_accuracy, compact) | (FLAG_GRID if grid else 0) | (0 if quad else FLAG_NO_QUAD)
     check(lib().hyx_sys_precompile(dptr(words, ct.c_uint32), len(words), dptr(consts), len(dec.consts), order, flags,
                                    arch.encode()))
 
