@@ -394,9 +394,14 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
         // finished (or unarmed) instances claims again; every claim advances the shared
         // counter, so this ends after at most N claims.
         if (!__any(active) && exhausted) break;
-        // Step until an instance of the wave ends: the claim code above, and the registers
-        // it needs, stay out of the loop that the steps run in. (A wave with no instance
-        // claimed none this time: it claims again.)
+        // Step until an instance of the wave ends, or (while the queue lasts) once when idle
+        // groups, claims that found no runnable instance, are enough to claim again: the claim
+        // code above, and the registers it needs, stay out of the loop the steps run in, and
+        // claims happen after the same steps as with a claim check on every iteration (idle
+        // groups only change when an instance ends, which leaves the loop). A wave with no
+        // instance claimed none this time: it claims again.
+        const unsigned nid = (unsigned)__popcll(__ballot(g == 0 && !active));
+        const bool reclaim = !exhausted && nid != 0 && (nid * HYX_G >= a.refill_min || nid == HYX_I);
         if (__any(active)) {
             for (;;) {
                 bool ended = false;
@@ -411,7 +416,7 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
                         ended = true;
                     }
                 }
-                if (__any(ended)) break;
+                if (__any(ended) || reclaim) break;
             }
         }
     }

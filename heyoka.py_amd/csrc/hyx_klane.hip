@@ -395,11 +395,16 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
         // finished (or unarmed) instances claims again; every claim advances the shared
         // counter, so this ends after at most N claims.
         if (!__any(active) && exhausted) break;
-        // Step until an instance of the wave ends (or, once the queue is drained, until
-        // the next hand-off check, every 16 iterations): the claim and count code above,
-        // and the registers it needs, stay out of the loop that the steps run in.
-        // (A wave with no instance claimed none this time: it claims again.)
+        // Step until an instance of the wave ends, or (while the queue lasts) once when idle
+        // slots, claims that found no runnable instance, are enough to claim again, or (once
+        // it is drained) until the next hand-off check, every 16 iterations: the claim and
+        // count code above, and the registers it needs, stay out of the loop that the steps
+        // run in, and claims happen after the same steps as with a claim check on every
+        // iteration (idle slots only change when an instance ends, which leaves the loop).
+        // A wave with no instance claimed none this time: it claims again.
         bool ended = false;
+        const unsigned nid = (unsigned)__popcll(__ballot(!active && sub == 0));
+        const bool reclaim = !exhausted && nid != 0 && (nid * HYX_SUB >= a.refill_min || nid == 64u / HYX_SUB);
         if (__any(active)) {
             for (unsigned it = 1;; ++it) {
                 if (active) {
@@ -416,7 +421,7 @@ extern "C" __global__ void __launch_bounds__(64) hyx_persist_k(hyx_kargs a)
                         ended = true;
                     }
                 }
-                if (__any(ended)) break;
+                if (__any(ended) || reclaim) break;
                 if (exhausted && (it & 15u) == 0u) break;
             }
         }
