@@ -223,8 +223,9 @@ struct Gen {
     bool no_contract = false;
 
     // Group form events: the state variables' coefficients are read from the LDS rows
-    // fl[lds_xb + k * lds_nx + i] (lds_xb < 0: not used).
+    // fl[lds_xb + k * lds_nx + lds_col[i]] (lds_xb < 0: not used).
     int lds_xb = -1, lds_nx = 0;
+    std::vector<int> lds_col; // column of each state variable (empty: the variable's index)
 
     Gen(const Sys &s, int p, bool ha) : S(s), P(p), high_acc(ha), perlane(s.nconsts, 0) {}
 
@@ -235,7 +236,8 @@ struct Gen {
     // Coefficient k of node idx: a register, or for a state variable in LDS an older row.
     std::string nv(uint32_t idx, int k) const
     {
-        if (lds_xb >= 0 && idx < S.n) return "fl[" + std::to_string(lds_xb + k * lds_nx + (int)idx) + "]";
+        if (lds_xb >= 0 && idx < S.n)
+            return "fl[" + std::to_string(lds_xb + k * lds_nx + (lds_col.empty() ? (int)idx : lds_col[idx])) + "]";
         if (base && idx < base->S.n) return base->nv(idx, k);
         if (lds_state && idx < S.n && k != cur_k) return xs(idx, k);
         return cv(idx, k);
@@ -1398,13 +1400,166 @@ bool plan_groups(const Sys &S, GroupPlan &gp, std::string &why)
     return true;
 }
 
+// LDS column of each state variable in the group form's x rows (column n is the zero
+// column). A double at LDS word address a occupies the 4-byte banks 2a and 2a + 1 of 32,
+// and a 64-bit read is served 16 lanes (one group) per pass: lanes whose addresses differ
+// but agree mod 16 serialise. Every column read of the step is the same table column for
+// the 16 lanes of a group (reads[r], with its count per step), so the map is chosen to
+// spread those columns over the 16 double banks, by a deterministic local search (pair
+// swaps kept when they do not raise the serialised passes). Measured on the 6-body group
+// kernel (65,536 instances, one dt = 25 slice), with the output slots of lds_output_map:
+// SQ_LDS_BANK_CONFLICT 1.50e10 -> 7.33e9, the same instructions, 147.0 -> 144.0 ms.
+static std::vector<int> lds_column_map(int n, int G, const std::vector<std::pair<std::vector<int>, int>> &reads)
+{
+    std::vector<int> col((size_t)n + 1);
+    for (int v = 0; v <= n; ++v) col[(size_t)v] = v;
+    if (G != 16 || reads.empty()) return col;
+    auto cost = [&](const std::vector<int> &c) {
+        long tot = 0;
+        for (const auto &r : reads) {
+            int cnt[16] = {0};
+            int seen[16][16];
+            int worst = 1;
+            for (int v : r.first) {
+                const int a = c[(size_t)v], b = a & 15;
+                bool dup = false;
+                for (int j = 0; j < cnt[b]; ++j) dup = dup || seen[b][j] == a;
+                if (!dup) {
+                    seen[b][cnt[b]++] = a;
+                    worst = std::max(worst, cnt[b]);
+                }
+            }
+            tot += (long)(worst - 1) * r.second;
+        }
+        return tot;
+    };
+    long best = cost(col);
+    uint64_t s = 0x9e3779b97f4a7c15ull;
+    for (int it = 0; it < 20000 && best > 0; ++it) {
+        s = s * 6364136223846793005ull + 1442695040888963407ull;
+        const int i = (int)((s >> 33) % (uint64_t)(n + 1));
+        s = s * 6364136223846793005ull + 1442695040888963407ull;
+        const int j = (int)((s >> 33) % (uint64_t)(n + 1));
+        if (i == j) continue;
+        std::swap(col[(size_t)i], col[(size_t)j]);
+        const long c = cost(col);
+        if (c <= best)
+            best = c;
+        else
+            std::swap(col[(size_t)i], col[(size_t)j]);
+    }
+    return col;
+}
+
+// Slots of the cluster outputs in the group form's LDS block: output q of lane c at
+// row[q] + perm[c], the zero slot at zero. One lane permutation for every row keeps a
+// lane's output addresses at constant distances (the compiler pairs its writes into
+// ds_write2_b64); the rows start at chosen residues mod 16, 16 doubles apart or more. The
+// owners' loads read one output each per lane (reads[r]: the 16 addresses of the
+// identity layout q * OS + c, OS * nout the zero slot, with the count per step); perm and
+// the row residues are chosen, like the column map, to spread each load over the 16
+// double banks. With G != 16 the identity layout is kept.
+struct OutputSlots {
+    std::vector<int> perm, row;
+    int zero = 0;
+    int map(int a, int OS, int nout) const
+    {
+        if (a >= OS * nout) return zero;
+        return row[(size_t)(a / OS)] + perm[(size_t)(a % OS)];
+    }
+};
+
+static void place_rows(OutputSlots &m, const std::vector<int> &res, int G)
+{
+    int next = 0;
+    for (size_t q = 0; q < res.size(); ++q) {
+        int r = next + ((res[q] - next) % 16 + 16) % 16;
+        m.row[q] = r;
+        next = r + G;
+    }
+    m.zero = next;
+}
+
+static OutputSlots lds_output_map(int G, int OS, int nout, const std::vector<std::pair<std::vector<int>, int>> &reads)
+{
+    OutputSlots m;
+    m.perm.resize((size_t)G);
+    m.row.resize((size_t)nout);
+    for (int c = 0; c < G; ++c) m.perm[(size_t)c] = c;
+    for (int q = 0; q < nout; ++q) m.row[(size_t)q] = q * OS;
+    m.zero = OS * nout;
+    if (G != 16 || nout == 0 || reads.empty()) return m;
+    std::vector<int> res((size_t)nout);
+    for (int q = 0; q < nout; ++q) res[(size_t)q] = (q * OS) % 16;
+    auto cost = [&](const OutputSlots &t) {
+        long tot = 0;
+        for (const auto &r : reads) {
+            int cnt[16] = {0};
+            int seen[16][16];
+            int worst = 1;
+            for (int a0 : r.first) {
+                const int a = t.map(a0, OS, nout), b = a & 15;
+                bool dup = false;
+                for (int j = 0; j < cnt[b]; ++j) dup = dup || seen[b][j] == a;
+                if (!dup) {
+                    seen[b][cnt[b]++] = a;
+                    worst = std::max(worst, cnt[b]);
+                }
+            }
+            tot += (long)(worst - 1) * r.second;
+        }
+        return tot;
+    };
+    OutputSlots cur = m;
+    place_rows(cur, res, G);
+    const long base = cost(m);
+    long best = cost(cur);
+    uint64_t s = 0x243f6a8885a308d3ull;
+    auto rnd = [&](int k) {
+        s = s * 6364136223846793005ull + 1442695040888963407ull;
+        return (int)((s >> 33) % (uint64_t)k);
+    };
+    for (int it = 0; it < 60000 && best > 0; ++it) {
+        if (rnd(2) == 0) {
+            const int i = rnd(G), j = rnd(G);
+            if (i == j) continue;
+            std::swap(cur.perm[(size_t)i], cur.perm[(size_t)j]);
+            const long c = cost(cur);
+            if (c <= best)
+                best = c;
+            else
+                std::swap(cur.perm[(size_t)i], cur.perm[(size_t)j]);
+        } else {
+            const int q = rnd(nout), old = res[(size_t)q];
+            res[(size_t)q] = rnd(16);
+            place_rows(cur, res, G);
+            const long c = cost(cur);
+            if (c <= best) {
+                best = c;
+            } else {
+                res[(size_t)q] = old;
+                place_rows(cur, res, G);
+            }
+        }
+    }
+    return best < base ? cur : m;
+}
+
 std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, bool high_acc, int split_min,
                                    bool pipeline, const Sys *E, uint32_t nev)
 {
     const int G = gp.G, own = gp.own, nin = gp.nin, nout = gp.nout;
     const int n = (int)S.n, nx = n + 1;
-    const int OS = gp.os; // output row stride
-    const int xb = OS * std::max(nout, 1) + 1; // LDS: outputs, zero slot, then x[P+1][n+1]
+    const int OS = gp.os; // output row stride of the identity layout (the tio tables)
+    std::vector<std::pair<std::vector<int>, int>> oreads;
+    for (int o = 0; o < own; ++o)
+        for (int t = 0; t < gp.nto[o]; ++t) {
+            std::vector<int> as;
+            for (int g = 0; g < G; ++g) as.push_back(gp.tio[o][(size_t)g * gp.nto[o] + t]);
+            oreads.push_back({as, P});
+        }
+    const OutputSlots oslot = lds_output_map(G, OS, nout, oreads);
+    const int xb = std::max(oslot.zero, OS) + 1; // LDS: outputs, zero slot, then x[P+1][n+1]
     const int flat = xb + (P + 1) * nx;
     std::ostringstream h;
     h << "#define HYX_N " << n << "\n#define HYX_NPAR " << std::max<uint32_t>(S.npar, 1) << "\n#define HYX_P " << P
@@ -1487,20 +1642,53 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
         }
     }
     h << "#define HYX_PIPELINED " << (lagged ? 1 : 0) << "\n";
-    itab("hyx_tab_iv", gp.iv);
+    // Column reads of one step, per group: the owners' rows (initial write, Horner, the
+    // coefficient log), the cluster inputs (lagged: their state terms), the owners' state
+    // terms.
+    std::vector<std::pair<std::vector<int>, int>> creads;
+    auto add_reads = [&](const std::vector<int> &tabv, int cnt, int weight) {
+        for (int q = 0; q < cnt; ++q) {
+            std::vector<int> vs;
+            for (int g = 0; g < G; ++g) {
+                const int v = tabv[(size_t)g * cnt + q];
+                vs.push_back(v >= 0 && v < n ? v : n);
+            }
+            creads.push_back({vs, weight});
+        }
+    };
+    add_reads(gp.vo, own, P + 3);
+    if (lagged)
+        add_reads(lis, nin * nls, P);
+    else
+        add_reads(gp.iv, nin, P + 1);
+    for (int o = 0; o < own; ++o) add_reads(gp.tis[o], gp.nts[o], P + 1);
+    const std::vector<int> col = lds_column_map(n, G, creads);
+    auto cols = [&](const std::vector<int> &tabv) {
+        std::vector<int> r(tabv.size());
+        for (size_t q = 0; q < tabv.size(); ++q) r[q] = col[(size_t)(tabv[q] >= 0 && tabv[q] < n ? tabv[q] : n)];
+        return r;
+    };
+    itab("hyx_tab_iv", cols(gp.iv));
+    auto oslots = [&](const std::vector<int> &tabv) {
+        std::vector<int> r(tabv.size());
+        for (size_t q = 0; q < tabv.size(); ++q) r[q] = oslot.map(tabv[q], OS, nout);
+        return r;
+    };
+    itab("hyx_tab_wp", oslot.perm);
     dtab("hyx_tab_kc", gp.kc);
     if (lagged) {
-        itab("hyx_tab_lis", lis);
+        itab("hyx_tab_lis", cols(lis));
         if (lss_table) dtab("hyx_tab_lss", lss);
         if (lrc_nonzero) dtab("hyx_tab_lrc", lrc);
     }
     itab("hyx_tab_vo", gp.vo);
+    itab("hyx_tab_vc", cols(gp.vo));
     dtab("hyx_tab_rc", gp.rc);
     // Per-slot term tables; a sign uniform over the lanes becomes a literal.
     std::vector<std::vector<std::string>> sgn_o(own), sgn_s(own);
     for (int o = 0; o < own; ++o) {
-        itab("hyx_tab_tio" + std::to_string(o), gp.tio[o]);
-        itab("hyx_tab_tis" + std::to_string(o), gp.tis[o]);
+        itab("hyx_tab_tio" + std::to_string(o), oslots(gp.tio[o]));
+        itab("hyx_tab_tis" + std::to_string(o), cols(gp.tis[o]));
         dtab("hyx_tab_tso" + std::to_string(o), gp.tso[o]);
         dtab("hyx_tab_tss" + std::to_string(o), gp.tss[o]);
         auto uni = [&](const std::vector<double> &v, int cnt, int t, const std::string &arr) {
@@ -1513,7 +1701,8 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
         for (int t = 0; t < gp.nts[o]; ++t) sgn_s[o].push_back(uni(gp.tss[o], gp.nts[o], t, "tss" + std::to_string(o)));
     }
     // The per-lane table struct and its loader.
-    h << "struct hyx_gtab {\n    int iv[HYX_NIN];\n    double kc[HYX_NKC];\n    int vo[HYX_OWN];\n    double rc[HYX_OWN];\n";
+    h << "struct hyx_gtab {\n    int iv[HYX_NIN];\n    double kc[HYX_NKC];\n    int vo[HYX_OWN];\n    int vc[HYX_OWN];\n"
+         "    double rc[HYX_OWN];\n    int wp;\n";
     if (lagged) {
         h << "    int lis[" << nin * nls << "];\n";
         if (lss_table) h << "    double lss[" << nin * nls << "];\n";
@@ -1525,8 +1714,10 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
           << o << "[" << std::max(gp.nts[o], 1) << "];\n";
     h << "};\n__device__ __forceinline__ void hyx_gtab_load(hyx_gtab &t, int g)\n{\n";
     h << "    for (int q = 0; q < HYX_NIN; ++q) t.iv[q] = hyx_tab_iv[g * HYX_NIN + q];\n";
+    h << "    t.wp = hyx_tab_wp[g];\n";
     h << "    for (int q = 0; q < HYX_NKC; ++q) t.kc[q] = hyx_tab_kc[g * HYX_NKC + q];\n";
-    h << "    for (int q = 0; q < HYX_OWN; ++q) { t.vo[q] = hyx_tab_vo[g * HYX_OWN + q]; t.rc[q] = hyx_tab_rc[g * HYX_OWN + q]; }\n";
+    h << "    for (int q = 0; q < HYX_OWN; ++q) { t.vo[q] = hyx_tab_vo[g * HYX_OWN + q]; t.vc[q] = hyx_tab_vc[g * HYX_OWN + q]; "
+         "t.rc[q] = hyx_tab_rc[g * HYX_OWN + q]; }\n";
     if (lagged) {
         h << "    for (int q = 0; q < " << nin * nls << "; ++q) t.lis[q] = hyx_tab_lis[g * " << nin * nls << " + q];\n";
         if (lss_table) h << "    for (int q = 0; q < " << nin * nls << "; ++q) t.lss[q] = hyx_tab_lss[g * " << nin * nls << " + q];\n";
@@ -1559,7 +1750,7 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
     o << "    (void)pv; (void)t0; (void)tc; (void)tcs; (void)g;\n";
     // Column of each owned variable in the x rows (column n reads zero for empty slots).
     for (int oo = 0; oo < own; ++oo)
-        o << "    const int " << vcol(oo) << " = (tab.vo[" << oo << "] >= 0) ? tab.vo[" << oo << "] : HYX_N;\n";
+        o << "    const int " << vcol(oo) << " = tab.vc[" << oo << "];\n";
     o << "    hyx_wsync();\n";
     for (int oo = 0; oo < own; ++oo) {
         o << "    const double " << xv(oo, 0) << " = xo[" << oo << "];\n";
@@ -1592,7 +1783,7 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
         }
         for (int q = 0; q < nout; ++q) {
             const uint32_t w = HYX_OPERAND(HYX_KIND_NODE, gp.outs[q]);
-            o << "    fl[" << q * OS << " + g] = " << (gen.zero(w, k) ? "0.0" : gen.val(w, k)) << ";\n";
+            o << "    fl[" << oslot.row[(size_t)q] << " + tab.wp] = " << (gen.zero(w, k) ? "0.0" : gen.val(w, k)) << ";\n";
         }
     };
     if (lagged) {
@@ -1684,6 +1875,7 @@ std::string generate_group_prelude(const Sys &S, const GroupPlan &gp, int P, boo
         Gen eg(*E, P, high_acc);
         eg.lds_xb = xb;
         eg.lds_nx = nx;
+        eg.lds_col = col;
         o << "    double hyx_evg[" << nev << "][" << (P + 1) << "];\n    {\n";
         for (uint32_t j = E->n; j < E->nnodes; ++j) eg.order0(j);
         for (int k = 1; k <= P; ++k)
@@ -3370,7 +3562,7 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
 
     // The per-lane table struct and its loader.
     const size_t NWP = (W.size() + tb_per - 1) / tb_per;
-    h << "struct hyx_gtab {\n    int vo[HYX_OWN];\n    int w[" << std::max<size_t>(NWP, 1) << "];\n    double d["
+    h << "struct hyx_gtab {\n    int vo[HYX_OWN];\n    int vc[HYX_OWN];\n    int w[" << std::max<size_t>(NWP, 1) << "];\n    double d["
       << std::max<size_t>(D.size(), 1) << "];\n};\n";
     {
         std::vector<int> wf;
@@ -3398,7 +3590,7 @@ std::string generate_wave_prelude(const Sys &S, int P, bool high_acc, int G, con
         h << "};\n";
     }
     h << "__device__ __forceinline__ void hyx_gtab_load(hyx_gtab &t, int g)\n{\n"
-         "    for (int q = 0; q < HYX_OWN; ++q) t.vo[q] = hyx_wtab_vo[g * HYX_OWN + q];\n";
+         "    for (int q = 0; q < HYX_OWN; ++q) { t.vo[q] = hyx_wtab_vo[g * HYX_OWN + q]; t.vc[q] = t.vo[q] >= 0 ? t.vo[q] : HYX_N; }\n";
     if (NWP) h << "    for (int f = 0; f < " << NWP << "; ++f) t.w[f] = hyx_wtab_w[f * HYX_G + g];\n";
     if (!D.empty()) h << "    for (int f = 0; f < " << D.size() << "; ++f) t.d[f] = hyx_wtab_d[f * HYX_G + g];\n";
     h << "}\n";

@@ -13,7 +13,8 @@
 // tab.vo[o] (o < HYX_OWN). Per order, coefficients are exchanged through this
 // instance's LDS block `fl` of HYX_FLAT doubles: the cluster outputs of the
 // current order, a zero slot, and every order of every state variable
-// (rows of HYX_NX = n + 1, the last column zero), which the state update reads.
+// (rows of HYX_NX = n + 1 columns, one of them zero; variable tab.vo[o] sits in
+// column tab.vc[o]), which the state update reads.
 //
 // Every lane of the wave executes every step, also lanes whose group has no
 // instance (their results are not stored): hyx_gstep exchanges data between the
@@ -59,7 +60,7 @@ __device__ __forceinline__ double hyx_gevents(hyx_ginst &s, const hyx_kargs &a, 
 #pragma unroll
     for (int o = 0; o < HYX_OWN; ++o) {
         if (t.vo[o] < 0) continue;
-        const double *row = fl + HYX_XB + t.vo[o];
+        const double *row = fl + HYX_XB + t.vc[o];
 #ifdef HYX_HIGH_ACC
         double r = row[0], cp = 0.0, hk = 1.0;
         for (int k = 1; k <= HYX_P; ++k) {
@@ -164,7 +165,7 @@ __device__ __forceinline__ void hyx_ggrid(hyx_ginst &s, const hyx_kargs &a, hyx_
 #pragma unroll
         for (int o = 0; o < HYX_OWN; ++o) {
             if (t.vo[o] < 0) continue;
-            const double *row = fl + HYX_XB + t.vo[o];
+            const double *row = fl + HYX_XB + t.vc[o];
             double r = row[HYX_P * HYX_NX];
             for (int k = HYX_P - 1; k >= 0; --k) r = __builtin_fma(r, dt, row[k * HYX_NX]);
             a.gout[((hyx_u64)c * HYX_N + t.vo[o]) * a.ld + i] = r;
