@@ -38,9 +38,9 @@ HBM_PEAK_GBS = 8000.0
 # Taylor coefficients on chip, so the traffic scales with the instances, not the steps. A
 # kernel whose source changed has no measured figure: the line then says null.
 MEASURED_TRAFFIC_B_PER_INSTANCE = {
-    # 65536 instances, dt=25 slices: FETCH_SIZE 27127.7 KiB (x 2.000), WRITE_SIZE 22973.2 KiB
+    # 65536 instances, dt=25 slices: FETCH_SIZE 28395.8 KiB (x 2.000), WRITE_SIZE 22984.4 KiB
     # (x 1.000) per launch.
-    ("nbody6", "group", "4d6d04093e471550"): 1206.7,
+    ("nbody6", "group", "189734a30a4ba451"): 1246.5,
 }
 
 
