@@ -390,3 +390,64 @@ def test_wave_schedule_j2var():
     assert "const double dv0 = " in src
     order10 = src.split("// order 10\n")[1].split("// order 11\n")[0]
     assert order10.count("hyx_wsync();") == 14 + 2  # levels, plus the state rows' two exchanges
+
+
+def _bank_passes(src):
+    """Serialised LDS passes per instance-step of the group form's table-driven loads: a
+    64-bit read is served 16 lanes per pass, a double at word a in banks 2a, 2a+1 of 32, so
+    lanes with distinct addresses equal mod 16 serialise."""
+    import re
+
+    def tab(name):
+        m = re.search(r"hyx_tab_%s\[\d+\] = \{([^}]*)\}" % name, src)
+        return [int(x) for x in m.group(1).split(",")] if m else None
+
+    D = {k: int(v) for k, v in re.findall(r"#define (HYX_\w+) (\d+)\n", src)}
+    G, XB, OWN, NIN, P = D["HYX_G"], D["HYX_XB"], D["HYX_OWN"], D["HYX_NIN"], D["HYX_P"]
+
+    def cost(addrs):
+        banks = {}
+        for a in addrs:
+            banks.setdefault(a % 16, set()).add(a)
+        return max(len(s) for s in banks.values()) - 1
+
+    total = 0
+    vc = tab("vc")
+    for oo in range(OWN):
+        total += cost([XB + vc[g * OWN + oo] for g in range(G)]) * (P + 3)
+    lis = tab("lis")
+    nls = len(lis) // (G * NIN)
+    for q in range(NIN * nls):
+        total += cost([XB + lis[g * NIN * nls + q] for g in range(G)]) * P
+    for oo in range(OWN):
+        for nm, off, w in (("tio%d" % oo, 0, P), ("tis%d" % oo, XB, P + 1)):
+            t = tab(nm)
+            cnt = len(t) // G if len(t) >= G else 0
+            for j in range(cnt):
+                total += cost([off + t[g * cnt + j] for g in range(G)]) * w
+    return total, D, tab
+
+
+def test_group_lds_layout_spreads_the_exchange_loads():
+    # The 6-body group form: the state columns and output slots are placed so that the
+    # 16 lanes of a group hit distinct banks; the identity layout (columns = variable
+    # indices, outputs at q * 17 + lane) serialises 257 extra passes per instance-step.
+    wl = W.nbody6(2)
+    dec = hy.taylor_decompose(wl.sys)
+    s = _lib.System(dec, 19, compact="group", source_only=True)
+    src = s.source()
+    passes, D, tab = _bank_passes(src)
+    assert passes <= 100
+    n, nx, xb, G, own, nout = D["HYX_N"], D["HYX_NX"], D["HYX_XB"], D["HYX_G"], D["HYX_OWN"], D["HYX_NOUT"]
+    vo, vc, wp = tab("vo"), tab("vc"), tab("wp")
+    # every variable has its own column; empty slots share the zero column
+    cols = {v: c for v, c in zip(vo, vc) if v >= 0}
+    assert sorted(cols) == list(range(n)) and len(set(cols.values())) == n
+    zero_cols = {c for v, c in zip(vo, vc) if v < 0}
+    assert len(zero_cols) <= 1 and not zero_cols & set(cols.values())
+    assert all(0 <= c < nx for c in vc)
+    # one lane permutation for the output rows, every slot below the x rows
+    assert sorted(wp) == list(range(G))
+    for oo in range(own):
+        t = tab("tio%d" % oo)
+        assert all(0 <= a < xb for a in t)
